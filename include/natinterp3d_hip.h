@@ -1,0 +1,111 @@
+/*
+ * natinterp3d_hip.h -- C ABI of the MI355X (gfx950) Sibson query path.
+ *
+ * The reference binds four C functions from Cython
+ * (natinterp3d_cython.pyx:17-35):
+ *
+ *     buildNewMeshAndVertices   natural.h:16            -> nat_mesh_build
+ *     freeMeshAndVertices       natural.h:18            -> nat_mesh_free
+ *     getInsertionFreeWeights   natural_insertionfree.h:76-79
+ *     getInsertionFreeWeightsParallel                    -> nat_get_weights
+ *                               natural_insertionfree.h:81-84
+ *
+ * The entry points below keep their argument meaning, ownership and error
+ * convention: queries are float64 [M,3] already centred by the caller, the
+ * caller owns rowptr[M+1], the callee malloc()s values[nnz] and cols[nnz]
+ * (release with free() or nat_free_host), and -1 means "nnz exceeds
+ * INT32_MAX" (nothing allocated).  Other negative codes are HIP or argument
+ * errors; nat_last_error() describes the last one on the calling thread.
+ *
+ * The remaining entry points have no reference counterpart; they serve
+ * callers whose data already lives in device memory (PyTorch tensors).
+ * All pointers named d_* are device pointers; `stream` is a hipStream_t
+ * (NULL = the legacy default stream).  Calls on one nat_mesh are serialised
+ * internally; different meshes may be used concurrently.
+ */
+#ifndef NATINTERP3D_HIP_H
+#define NATINTERP3D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nat_mesh nat_mesh;
+
+typedef struct {
+    int32_t n_data;          /* N data points */
+    int32_t n_tets;          /* T tets, including those touching the super simplex */
+    int32_t kd_depth;        /* depth of the k-d tree over tet centroids */
+    int32_t coplanar;        /* degeneracies the builder perturbed away */
+    int32_t cospherical;
+    int32_t edge_anomalies;  /* non-manifold edge groups met while linking tets */
+    int32_t device;          /* HIP device the mesh lives on */
+    int32_t num_cus;         /* compute units of that device */
+    int64_t device_bytes;    /* device memory held by the mesh (not workspace) */
+    int64_t workspace_bytes; /* device workspace currently held */
+} nat_mesh_info;
+
+/* Replaces buildNewMeshAndVertices (natural.c:114-118) plus the one-time
+ * packMeshSimplices / precomputeOrientSigns (natural_insertionfree.c:397-476):
+ * builds the Delaunay mesh of data_points [num_data_points,3] (float64,
+ * centred) on the host, reproducing the reference mesh exactly, then uploads
+ * it to `device` as immutable packed arrays.  device = -1 builds a host-only
+ * mesh (no GPU needed) that supports nat_mesh_get_info and nat_mesh_export
+ * only.  Returns 0, or a negative code (-2: a tet of non-positive
+ * orientation, where the reference exits). */
+int nat_mesh_build(const double *data_points, int num_data_points, int device,
+                   nat_mesh **out);
+
+/* Replaces freeMeshAndVertices (natural.c:120-123). */
+void nat_mesh_free(nat_mesh *m);
+
+int nat_mesh_get_info(const nat_mesh *m, nat_mesh_info *info);
+
+/* Copies the packed host mesh out (test and tooling access): verts[(N+4)*3]
+ * (data slots 0..N-1, super vertices N..N+3), tet_v[T*4], tet_n[T*4],
+ * kd_left[T], kd_right[T], kd_pos[T*3], kd_rect[6].  Any pointer may be NULL. */
+int nat_mesh_export(const nat_mesh *m, double *verts, int32_t *tet_v, int32_t *tet_n,
+                    int32_t *kd_left, int32_t *kd_right, double *kd_pos, double *kd_rect);
+
+/* Replaces getInsertionFreeWeights / getInsertionFreeWeightsParallel
+ * (natural_insertionfree.c:761-940): host queries in, host CSR out, same
+ * ownership and return codes.  The rows are computed on the GPU. */
+int nat_get_weights(const double *query_points, int num_query_points, nat_mesh *m,
+                    int num_data_points, double **weight_values, int **weight_col_inds,
+                    int *weight_row_ptrs);
+
+/* Device-resident form of nat_get_weights, in two steps.
+ * nat_weights_compute runs every row of d_queries [M,3] into the mesh's
+ * workspace and returns the exact nnz (int64) in *nnz_out (no INT32 limit
+ * here).  nat_weights_emit then writes the CSR into caller-owned device
+ * buffers d_vals[nnz] (float64), d_cols[nnz] (int32) and d_rowptr[M+1]
+ * (int32; returns -1 when nnz > INT32_MAX, like the reference).  The emit
+ * must follow the compute on the same mesh with no other call in between. */
+int nat_weights_compute(nat_mesh *m, const double *d_queries, int num_query_points,
+                        void *stream, int64_t *nnz_out);
+int nat_weights_emit(nat_mesh *m, double *d_vals, int32_t *d_cols, int32_t *d_rowptr,
+                     void *stream);
+
+/* W @ values on the device, accumulating each row in stored order from zero
+ * like scipy's CSR x dense product (natinterp3d.py:131-137):
+ * d_out[M,D] = W[M,N] (CSR, d_rowptr int32[M+1], d_cols int32, d_vals)
+ * x d_values[N,D].  dtype codes: 0 = float32, 1 = float64; vals_dtype is the
+ * weight type, values_dtype the value type, out_dtype the accumulation and
+ * output type (the NumPy result type of the two). */
+int nat_spmm_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
+                 const int32_t *d_cols, const void *d_vals, int vals_dtype,
+                 const void *d_values, int values_dtype, void *d_out, int out_dtype,
+                 void *stream);
+
+/* Release memory returned by nat_get_weights. */
+void nat_free_host(void *p);
+
+/* Description of the last error on the calling thread ("" if none). */
+const char *nat_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,76 @@
+"""ctypes binding of the C ABI declared in include/natinterp3d_hip.h.
+
+The library is required: there is no CPU path.  Importing the package works
+without it (so the API can be inspected), but every call that computes raises
+``ImportError`` until ``libnatinterp3d_hip.so`` has been built.
+"""
+import ctypes
+import os
+
+from ._build import LIB_PATH
+
+_lib = None
+
+EXPORTS = (
+    "nat_mesh_build", "nat_mesh_free", "nat_mesh_get_info", "nat_mesh_export",
+    "nat_get_weights", "nat_weights_compute", "nat_weights_emit", "nat_spmm_csr",
+    "nat_free_host", "nat_last_error",
+)
+
+
+class MeshInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_data", ctypes.c_int32), ("n_tets", ctypes.c_int32), ("kd_depth", ctypes.c_int32),
+        ("coplanar", ctypes.c_int32), ("cospherical", ctypes.c_int32),
+        ("edge_anomalies", ctypes.c_int32), ("device", ctypes.c_int32),
+        ("num_cus", ctypes.c_int32), ("device_bytes", ctypes.c_int64),
+        ("workspace_bytes", ctypes.c_int64),
+    ]
+
+
+def lib():
+    """The loaded HIP library; raises ImportError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"natinterp3d_amd: {LIB_PATH} is missing; build it with "
+            "`python -m natinterp3d_amd._build` (there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    L.nat_mesh_build.restype = ctypes.c_int
+    L.nat_mesh_build.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+    L.nat_mesh_free.restype = None
+    L.nat_mesh_free.argtypes = [vp]
+    L.nat_mesh_get_info.restype = ctypes.c_int
+    L.nat_mesh_get_info.argtypes = [vp, ctypes.POINTER(MeshInfo)]
+    L.nat_mesh_export.restype = ctypes.c_int
+    L.nat_mesh_export.argtypes = [vp, dp, ip, ip, ip, ip, dp, dp]
+    L.nat_get_weights.restype = ctypes.c_int
+    L.nat_get_weights.argtypes = [dp, ctypes.c_int, vp, ctypes.c_int, ctypes.POINTER(dp),
+                                  ctypes.POINTER(ip), ip]
+    L.nat_weights_compute.restype = ctypes.c_int
+    L.nat_weights_compute.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_int64)]
+    L.nat_weights_emit.restype = ctypes.c_int
+    L.nat_weights_emit.argtypes = [vp, vp, vp, vp, vp]
+    L.nat_spmm_csr.restype = ctypes.c_int
+    L.nat_spmm_csr.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp,
+                               ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, vp]
+    L.nat_free_host.restype = None
+    L.nat_free_host.argtypes = [vp]
+    L.nat_last_error.restype = ctypes.c_char_p
+    L.nat_last_error.argtypes = []
+    _lib = L
+    return L
+
+
+def last_error():
+    return lib().nat_last_error().decode(errors="replace")
+
+
+def check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc}): {last_error()}")

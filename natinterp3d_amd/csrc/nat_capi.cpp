@@ -1,0 +1,518 @@
+// nat_capi.cpp -- the C ABI (include/natinterp3d_hip.h): mesh build and
+// upload, device workspace, the capacity-level driver of the row kernel and
+// CSR assembly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "geom.h"
+#include "mesh_builder.h"
+#include "nat_internal.h"
+#include "natinterp3d_hip.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_OR_FAIL(expr)                                                               \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return fail(-10, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),      \
+                        __FILE__, __LINE__);                                            \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need, bool zero = false) {
+        if (need <= bytes && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            bytes = 0;
+        }
+        size_t alloc = std::max<size_t>(need, 256);
+        hipError_t e = hipMalloc(&p, alloc);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        bytes = alloc;
+        if (zero) return hipMemset(p, 0, alloc);
+        return hipSuccess;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+constexpr int kLevels = 4;
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+struct nat_mesh {
+    int device = 0;
+    int num_cus = 0;
+    nat::PackedMesh host;
+    nat::DevMesh dev{};
+    DevBuf verts, tet_v, tet_n, kd;
+    double bbox[6] = {0, 0, 0, 1, 1, 1};
+    std::mutex mu;
+
+    // workspace
+    nat::Caps caps[kLevels];
+    int nthr[kLevels] = {0, 0, 0, 0};
+    DevBuf scratch[kLevels];
+    DevBuf q;                    // host-path query upload
+    DevBuf order, keys, hist;    // Morton bucketing
+    DevBuf scan;                 // int64 scan scratch
+    DevBuf row_off, row_cnt, rowptr64;
+    DevBuf ovf[2];
+    DevBuf counters;             // [0..7] unsigned: work counters, ovf counts; [8] u64 pool_used
+    DevBuf pool_vals, pool_cols;
+    long long pool_cap = 0;
+    DevBuf csr_vals, csr_cols, csr_rowptr;  // host-path CSR staging
+    double nnz_per_row = 24.0;
+
+    int last_M = -1;
+    long long last_nnz = 0;
+
+    int64_t device_bytes() const {
+        return static_cast<int64_t>(verts.bytes + tet_v.bytes + tet_n.bytes + kd.bytes);
+    }
+    int64_t workspace_bytes() const {
+        size_t s = q.bytes + order.bytes + keys.bytes + hist.bytes + scan.bytes +
+                   row_off.bytes + row_cnt.bytes + rowptr64.bytes + ovf[0].bytes +
+                   ovf[1].bytes + counters.bytes + pool_vals.bytes + pool_cols.bytes +
+                   csr_vals.bytes + csr_cols.bytes + csr_rowptr.bytes;
+        for (int l = 0; l < kLevels; l++) s += scratch[l].bytes;
+        return static_cast<int64_t>(s);
+    }
+    void release_all() {
+        DevBuf *all[] = {&verts, &tet_v, &tet_n, &kd, &q, &order, &keys, &hist, &scan,
+                         &row_off, &row_cnt, &rowptr64, &ovf[0], &ovf[1], &counters,
+                         &pool_vals, &pool_cols, &csr_vals, &csr_cols, &csr_rowptr};
+        for (DevBuf *b : all) b->release();
+        for (int l = 0; l < kLevels; l++) scratch[l].release();
+    }
+};
+
+namespace {
+
+void set_levels(nat_mesh *m) {
+    const int kd = m->host.kd_depth + 2;
+    // level 0: every query; the capacities cover all rows of uniform data
+    m->caps[0] = {kd, 128, 8, 64, 160, 96};
+    // persistent grid: exactly the blocks that can be resident at once
+    int per_cu = nat::rows_blocks_per_cu();
+    m->nthr[0] = std::max(1, m->num_cus) * std::max(1, per_cu) * 256;
+    // larger levels only see the queries that overflowed the previous one
+    m->caps[1] = {kd, 2048, 12, 1024, 2560, 1536};
+    m->nthr[1] = 8192;
+    m->caps[2] = {kd, 32768, 16, 16384, 40960, 24576};
+    m->nthr[2] = 512;
+    m->caps[3] = {kd, 524288, 20, 262144, 655360, 393216};
+    m->nthr[3] = 64;
+}
+
+int upload_mesh(nat_mesh *m) {
+    const nat::PackedMesh &h = m->host;
+    const int nv = h.n_data + 4, T = h.n_tets;
+    std::vector<double4> verts(nv);
+    for (int i = 0; i < nv; i++)
+        verts[i] = make_double4(h.verts[3 * i], h.verts[3 * i + 1], h.verts[3 * i + 2], 0.0);
+    std::vector<int4> tv(T), tn(T);
+    std::vector<double4> kd(T);
+    for (int t = 0; t < T; t++) {
+        const int *p = &h.tet_v[4 * t];
+        const nat::d3 a = {h.verts[3 * p[0]], h.verts[3 * p[0] + 1], h.verts[3 * p[0] + 2]};
+        const nat::d3 b = {h.verts[3 * p[1]], h.verts[3 * p[1] + 1], h.verts[3 * p[1] + 2]};
+        const nat::d3 c = {h.verts[3 * p[2]], h.verts[3 * p[2] + 1], h.verts[3 * p[2] + 2]};
+        const nat::d3 d = {h.verts[3 * p[3]], h.verts[3 * p[3] + 1], h.verts[3 * p[3] + 2]};
+        // orientation bit (natural_insertionfree.c:457-476), stored inverted in
+        // bit 31 of p3 so that the common positive case reads unchanged
+        const bool pos = nat::orient3d(a, b, c, d) > 0;
+        tv[t] = make_int4(p[0], p[1], p[2], pos ? p[3] : (p[3] | static_cast<int>(0x80000000u)));
+        tn[t] = make_int4(h.tet_n[4 * t], h.tet_n[4 * t + 1], h.tet_n[4 * t + 2],
+                          h.tet_n[4 * t + 3]);
+        int2 lr = make_int2(h.kd_left[t], h.kd_right[t]);
+        double w;
+        std::memcpy(&w, &lr, sizeof(w));
+        kd[t] = make_double4(h.kd_pos[3 * t], h.kd_pos[3 * t + 1], h.kd_pos[3 * t + 2], w);
+    }
+    HIP_OR_FAIL(m->verts.ensure(sizeof(double4) * nv));
+    HIP_OR_FAIL(m->tet_v.ensure(sizeof(int4) * std::max(T, 1)));
+    HIP_OR_FAIL(m->tet_n.ensure(sizeof(int4) * std::max(T, 1)));
+    HIP_OR_FAIL(m->kd.ensure(sizeof(double4) * std::max(T, 1)));
+    HIP_OR_FAIL(hipMemcpy(m->verts.p, verts.data(), sizeof(double4) * nv, hipMemcpyHostToDevice));
+    if (T > 0) {
+        HIP_OR_FAIL(hipMemcpy(m->tet_v.p, tv.data(), sizeof(int4) * T, hipMemcpyHostToDevice));
+        HIP_OR_FAIL(hipMemcpy(m->tet_n.p, tn.data(), sizeof(int4) * T, hipMemcpyHostToDevice));
+        HIP_OR_FAIL(hipMemcpy(m->kd.p, kd.data(), sizeof(double4) * T, hipMemcpyHostToDevice));
+    }
+    nat::DevMesh &d = m->dev;
+    d.verts = m->verts.as<double4>();
+    d.tet_v = m->tet_v.as<int4>();
+    d.tet_n = m->tet_n.as<int4>();
+    d.kd = m->kd.as<double4>();
+    for (int k = 0; k < 3; k++) {
+        d.rmin[k] = h.kd_rect[k];
+        d.rmax[k] = h.kd_rect[3 + k];
+    }
+    d.n_data = h.n_data;
+    d.n_tets = T;
+    // bounding box of the data points: the Morton grid of the queries
+    for (int k = 0; k < 3; k++) {
+        m->bbox[k] = h.verts[k];
+        m->bbox[3 + k] = h.verts[k];
+    }
+    for (int i = 0; i < h.n_data; i++)
+        for (int k = 0; k < 3; k++) {
+            m->bbox[k] = std::min(m->bbox[k], h.verts[3 * i + k]);
+            m->bbox[3 + k] = std::max(m->bbox[3 + k], h.verts[3 * i + k]);
+        }
+    return 0;
+}
+
+int ensure_level_scratch(nat_mesh *m, int level) {
+    // each of the (at most 16) arrays is rounded up to 256 bytes when carved
+    const size_t per = nat::scratch_bytes_per_thread(m->caps[level]);
+    const size_t need = per * static_cast<size_t>(m->nthr[level]) + 16 * 256;
+    if (m->scratch[level].bytes >= need && m->scratch[level].p) return 0;
+    HIP_OR_FAIL(m->scratch[level].ensure(need, /*zero=*/true));
+    return 0;
+}
+
+int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long long *nnz_out) {
+    m->last_M = -1;
+    if (M == 0) {
+        m->last_M = 0;
+        m->last_nnz = 0;
+        *nnz_out = 0;
+        return 0;
+    }
+    const size_t Mz = static_cast<size_t>(M);
+    // Morton bucketing: 3*bits-bit cells over the data bounding box
+    int bits = 1;
+    while (bits < 7 && (1ull << (3 * (bits + 1))) <= Mz * 2) bits++;
+    const int nbuckets = 1 << (3 * bits);
+    const size_t scan_elems =
+        std::max(nat::scan_tmp_elems(M) + Mz + 1,
+                 static_cast<size_t>(nbuckets) + 1 + nat::scan_tmp_elems(nbuckets));
+    HIP_OR_FAIL(m->order.ensure(sizeof(int) * Mz));
+    HIP_OR_FAIL(m->keys.ensure(sizeof(int) * Mz));
+    HIP_OR_FAIL(m->hist.ensure(sizeof(int) * nbuckets));
+    HIP_OR_FAIL(m->scan.ensure(sizeof(long long) * scan_elems));
+    HIP_OR_FAIL(m->row_off.ensure(sizeof(long long) * Mz));
+    HIP_OR_FAIL(m->row_cnt.ensure(sizeof(int) * Mz));
+    HIP_OR_FAIL(m->rowptr64.ensure(sizeof(long long) * (Mz + 1)));
+    HIP_OR_FAIL(m->ovf[0].ensure(sizeof(int) * Mz));
+    HIP_OR_FAIL(m->ovf[1].ensure(sizeof(int) * Mz));
+    HIP_OR_FAIL(m->counters.ensure(256));
+    if (int rc = ensure_level_scratch(m, 0)) return rc;
+
+    HIP_OR_FAIL(nat::launch_morton_sort(d_q, M, m->bbox, bits, m->order.as<int>(),
+                                        m->hist.as<int>(), m->keys.as<int>(),
+                                        m->scan.as<long long>(), st));
+
+    long long want = static_cast<long long>(std::ceil(m->nnz_per_row * M)) + 1024;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        if (m->pool_cap < want) {
+            HIP_OR_FAIL(m->pool_vals.ensure(sizeof(double) * want));
+            HIP_OR_FAIL(m->pool_cols.ensure(sizeof(int) * want));
+            m->pool_cap = want;
+        }
+        unsigned *ctr = m->counters.as<unsigned>();
+        unsigned long long *pool_used = reinterpret_cast<unsigned long long *>(ctr + 16);
+        HIP_OR_FAIL(hipMemsetAsync(m->counters.p, 0, 256, st));
+        const int *items = m->order.as<int>();
+        int n_items = M;
+        for (int level = 0; level < kLevels && n_items > 0; level++) {
+            if (int rc = ensure_level_scratch(m, level)) return rc;
+            nat::Scratch sc;
+            nat::scratch_carve(m->scratch[level].p, m->caps[level], m->nthr[level], &sc);
+            nat::RowOut out;
+            out.pool_vals = m->pool_vals.as<double>();
+            out.pool_cols = m->pool_cols.as<int>();
+            out.pool_cap = m->pool_cap;
+            out.pool_used = pool_used;
+            out.row_off = m->row_off.as<long long>();
+            out.row_cnt = m->row_cnt.as<int>();
+            out.ovf_list = m->ovf[level & 1].as<int>();
+            out.ovf_count = ctr + 8 + level;
+            const int nblocks = (m->nthr[level] + 255) / 256;
+            HIP_OR_FAIL(nat::launch_rows(m->dev, m->caps[level], sc, nblocks, d_q, items,
+                                         n_items, ctr + level, out, st));
+            unsigned novf = 0;
+            HIP_OR_FAIL(hipMemcpyAsync(&novf, ctr + 8 + level, sizeof(unsigned),
+                                       hipMemcpyDeviceToHost, st));
+            HIP_OR_FAIL(hipStreamSynchronize(st));
+            n_items = static_cast<int>(novf);
+            items = m->ovf[level & 1].as<int>();
+        }
+        if (n_items > 0)
+            return fail(-4, "%d queries need more than %d cavity tets", n_items,
+                        m->caps[kLevels - 1].cav);
+        unsigned long long used = 0;
+        HIP_OR_FAIL(hipMemcpyAsync(&used, pool_used, sizeof(used), hipMemcpyDeviceToHost, st));
+        HIP_OR_FAIL(hipStreamSynchronize(st));
+        if (static_cast<long long>(used) <= m->pool_cap) {
+            m->last_M = M;
+            m->last_nnz = static_cast<long long>(used);
+            m->nnz_per_row = std::max(4.0, 1.125 * static_cast<double>(used) / M);
+            *nnz_out = m->last_nnz;
+            return 0;
+        }
+        want = static_cast<long long>(used) + 1024;
+    }
+    return fail(-5, "row pool did not converge");
+}
+
+int emit_rows(nat_mesh *m, double *d_vals, int *d_cols, int *d_rowptr, hipStream_t st) {
+    if (m->last_M < 0) return fail(-6, "nat_weights_emit without a preceding compute");
+    if (m->last_nnz > 2147483647LL) return -1;
+    const int M = m->last_M;
+    if (M == 0) {
+        HIP_OR_FAIL(hipMemsetAsync(d_rowptr, 0, sizeof(int), st));
+        return 0;
+    }
+    long long *rp64 = m->rowptr64.as<long long>();
+    HIP_OR_FAIL(nat::launch_scan_counts(m->row_cnt.as<int>(), rp64, M, m->scan.as<long long>(),
+                                        st));
+    HIP_OR_FAIL(nat::launch_gather_csr(rp64, m->row_off.as<long long>(), m->row_cnt.as<int>(),
+                                       m->pool_vals.as<double>(), m->pool_cols.as<int>(), M,
+                                       d_vals, d_cols, d_rowptr, st));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nat_mesh_build(const double *data_points, int num_data_points, int device, nat_mesh **out) {
+    g_err.clear();
+    if (!out) return fail(-1, "out is NULL");
+    *out = nullptr;
+    if (!data_points || num_data_points <= 0)
+        return fail(-1, "need at least one data point (got %d)", num_data_points);
+    if (device >= 0) {
+        int ndev = 0;
+        HIP_OR_FAIL(hipGetDeviceCount(&ndev));
+        if (device >= ndev) return fail(-1, "device %d out of range (%d)", device, ndev);
+    } else if (device != -1) {
+        return fail(-1, "device %d invalid", device);
+    }
+    nat_mesh *m = new nat_mesh();
+    m->device = device;
+    const int rc = nat::build_packed_mesh(data_points, num_data_points, &m->host);
+    if (rc) {
+        delete m;
+        if (rc == -2) return fail(-2, "mesh builder met a tet of non-positive orientation");
+        if (rc == -3) return fail(-3, "mesh builder could not locate a point");
+        return fail(rc, "mesh builder failed (%d)", rc);
+    }
+    if (device == -1) {  // host-only mesh: export and inspection, no queries
+        *out = m;
+        return 0;
+    }
+    DeviceGuard g(device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete m;
+        return fail(-10, "hipGetDeviceProperties failed");
+    }
+    m->num_cus = prop.multiProcessorCount;
+    set_levels(m);
+    const int urc = upload_mesh(m);
+    if (urc) {
+        m->release_all();
+        delete m;
+        return urc;
+    }
+    *out = m;
+    return 0;
+}
+
+void nat_mesh_free(nat_mesh *m) {
+    if (!m) return;
+    if (m->device >= 0) {
+        DeviceGuard g(m->device);
+        m->release_all();
+    }
+    delete m;
+}
+
+int nat_mesh_get_info(const nat_mesh *m, nat_mesh_info *info) {
+    if (!m || !info) return fail(-1, "NULL argument");
+    info->n_data = m->host.n_data;
+    info->n_tets = m->host.n_tets;
+    info->kd_depth = m->host.kd_depth;
+    info->coplanar = m->host.coplanar;
+    info->cospherical = m->host.cospherical;
+    info->edge_anomalies = m->host.edge_anomalies;
+    info->device = m->device;
+    info->num_cus = m->num_cus;
+    info->device_bytes = m->device_bytes();
+    info->workspace_bytes = m->workspace_bytes();
+    return 0;
+}
+
+int nat_mesh_export(const nat_mesh *m, double *verts, int32_t *tet_v, int32_t *tet_n,
+                    int32_t *kd_left, int32_t *kd_right, double *kd_pos, double *kd_rect) {
+    if (!m) return fail(-1, "NULL mesh");
+    const nat::PackedMesh &h = m->host;
+    if (verts) std::memcpy(verts, h.verts.data(), sizeof(double) * h.verts.size());
+    if (tet_v) std::memcpy(tet_v, h.tet_v.data(), sizeof(int32_t) * h.tet_v.size());
+    if (tet_n) std::memcpy(tet_n, h.tet_n.data(), sizeof(int32_t) * h.tet_n.size());
+    if (kd_left) std::memcpy(kd_left, h.kd_left.data(), sizeof(int32_t) * h.kd_left.size());
+    if (kd_right) std::memcpy(kd_right, h.kd_right.data(), sizeof(int32_t) * h.kd_right.size());
+    if (kd_pos) std::memcpy(kd_pos, h.kd_pos.data(), sizeof(double) * h.kd_pos.size());
+    if (kd_rect) std::memcpy(kd_rect, h.kd_rect, sizeof(double) * 6);
+    return 0;
+}
+
+int nat_weights_compute(nat_mesh *m, const double *d_queries, int num_query_points,
+                        void *stream, int64_t *nnz_out) {
+    g_err.clear();
+    if (!m || !nnz_out) return fail(-1, "NULL argument");
+    if (num_query_points < 0) return fail(-1, "negative query count");
+    if (num_query_points > 0 && !d_queries) return fail(-1, "NULL queries");
+    if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    std::lock_guard<std::mutex> lk(m->mu);
+    DeviceGuard g(m->device);
+    long long nnz = 0;
+    const int rc = compute_rows(m, d_queries, num_query_points,
+                                static_cast<hipStream_t>(stream), &nnz);
+    if (rc) return rc;
+    *nnz_out = nnz;
+    return 0;
+}
+
+int nat_weights_emit(nat_mesh *m, double *d_vals, int32_t *d_cols, int32_t *d_rowptr,
+                     void *stream) {
+    g_err.clear();
+    if (!m || !d_rowptr) return fail(-1, "NULL argument");
+    if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    std::lock_guard<std::mutex> lk(m->mu);
+    DeviceGuard g(m->device);
+    return emit_rows(m, d_vals, d_cols, d_rowptr, static_cast<hipStream_t>(stream));
+}
+
+int nat_get_weights(const double *query_points, int num_query_points, nat_mesh *m,
+                    int num_data_points, double **weight_values, int **weight_col_inds,
+                    int *weight_row_ptrs) {
+    g_err.clear();
+    if (!m || !weight_values || !weight_col_inds || !weight_row_ptrs)
+        return fail(-2, "NULL argument");
+    if (num_query_points < 0) return fail(-2, "negative query count");
+    if (num_data_points != m->host.n_data)
+        return fail(-2, "num_data_points %d does not match the mesh (%d)", num_data_points,
+                    m->host.n_data);
+    if (m->device < 0) return fail(-2, "host-only mesh has no device copy");
+    std::lock_guard<std::mutex> lk(m->mu);
+    DeviceGuard g(m->device);
+    const int M = num_query_points;
+    const size_t Mz = static_cast<size_t>(M);
+    hipStream_t st = nullptr;
+    if (M > 0) {
+        HIP_OR_FAIL(m->q.ensure(sizeof(double) * 3 * Mz));
+        HIP_OR_FAIL(hipMemcpy(m->q.p, query_points, sizeof(double) * 3 * Mz,
+                              hipMemcpyHostToDevice));
+    }
+    long long nnz = 0;
+    int rc = compute_rows(m, m->q.as<double>(), M, st, &nnz);
+    if (rc) return rc;
+    if (nnz > 2147483647LL) return -1;
+    const size_t nz = static_cast<size_t>(nnz);
+    HIP_OR_FAIL(m->csr_vals.ensure(sizeof(double) * std::max<size_t>(nz, 1)));
+    HIP_OR_FAIL(m->csr_cols.ensure(sizeof(int) * std::max<size_t>(nz, 1)));
+    HIP_OR_FAIL(m->csr_rowptr.ensure(sizeof(int) * (Mz + 1)));
+    rc = emit_rows(m, m->csr_vals.as<double>(), m->csr_cols.as<int>(), m->csr_rowptr.as<int>(),
+                   st);
+    if (rc) return rc;
+    double *hv = static_cast<double *>(std::malloc(sizeof(double) * std::max<size_t>(nz, 1)));
+    int *hc = static_cast<int *>(std::malloc(sizeof(int) * std::max<size_t>(nz, 1)));
+    if (!hv || !hc) {
+        std::free(hv);
+        std::free(hc);
+        return fail(-3, "host allocation of %zu entries failed", nz);
+    }
+    hipError_t e = hipMemcpy(weight_row_ptrs, m->csr_rowptr.p, sizeof(int) * (Mz + 1),
+                             hipMemcpyDeviceToHost);
+    if (e == hipSuccess && nz)
+        e = hipMemcpy(hv, m->csr_vals.p, sizeof(double) * nz, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && nz)
+        e = hipMemcpy(hc, m->csr_cols.p, sizeof(int) * nz, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        std::free(hv);
+        std::free(hc);
+        return fail(-10, "copy-out failed: %s", hipGetErrorString(e));
+    }
+    *weight_values = hv;
+    *weight_col_inds = hc;
+    return 0;
+}
+
+int nat_spmm_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
+                 const int32_t *d_cols, const void *d_vals, int vals_dtype,
+                 const void *d_values, int values_dtype, void *d_out, int out_dtype,
+                 void *stream) {
+    g_err.clear();
+    (void)num_cols;
+    if (num_rows < 0 || dim < 0) return fail(-1, "negative size");
+    if (num_rows == 0 || dim == 0) return 0;
+    if (!d_rowptr || !d_out) return fail(-1, "NULL argument");
+    if (vals_dtype < 0 || vals_dtype > 1 || values_dtype < 0 || values_dtype > 1 ||
+        out_dtype < (vals_dtype | values_dtype) || out_dtype > 1)
+        return fail(-1, "unsupported dtype combination %d %d %d", vals_dtype, values_dtype,
+                    out_dtype);
+    hipError_t e = nat::launch_spmm(num_rows, dim, d_rowptr, d_cols, d_vals, vals_dtype,
+                                    d_values, values_dtype, d_out, out_dtype,
+                                    static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail(-10, "spmm launch failed: %s", hipGetErrorString(e));
+    return 0;
+}
+
+void nat_free_host(void *p) { std::free(p); }
+
+const char *nat_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

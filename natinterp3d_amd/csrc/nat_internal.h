@@ -1,0 +1,83 @@
+// nat_internal.h -- device data layout and kernel launchers shared by the
+// C ABI (nat_capi.cpp) and the kernels (sibson_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nat {
+
+// Immutable device mesh.  All arrays are packed SoA in tet order (the
+// reference's linked-list order), 16- or 32-byte records so that one tet or
+// one vertex is one aligned vector load.
+struct DevMesh {
+    const double4 *verts;  // [N+4] x,y,z,0; slots N..N+3 are the super vertices
+    const int4 *tet_v;     // [T] vertex slots p0..p3; bit 31 of .w set when
+                           //     orient3d(p0,p1,p2,p3) <= 0
+    const int4 *tet_n;     // [T] neighbour across face 0..3, -1 = none
+    const double4 *kd;     // [T] k-d node i = tet i: centroid x,y,z and
+                           //     .w = bit pattern of int2(left, right)
+    double rmin[3], rmax[3];  // k-d root hyper-rectangle
+    int n_data;               // N
+    int n_tets;               // T
+};
+
+// Per-thread capacities of the row kernel.  A query that needs more marks
+// itself for the next, larger level.
+struct Caps {
+    int kd;        // k-d descent stack frames (>= tree depth)
+    int stack;     // cavity-search stack
+    int vis_log2;  // log2 of the visited-set hash size
+    int cav;       // cavity tets
+    int bnd;       // boundary faces
+    int nb;        // natural neighbours (row length)
+};
+
+// Per-thread scratch, interleaved: element k of thread t at [k * nthr + t].
+struct Scratch {
+    int *kd_node;     // [kd]   node | phase << 30
+    double *kd_saved; // [kd]   saved rectangle bound
+    int *stack;       // [stack]
+    int2 *vis;        // [1 << vis_log2]  (tet, stamp)
+    int *vis_pos;     // [1 << vis_log2]  cavity position, -1 = tested, outside
+    int *cav;         // [cav]
+    int *bnd;         // [bnd]  cavity position << 2 | face
+    int *nb_idx;      // [nb]
+    double *nb_vol;   // [nb]
+    int *epoch;       // [1]    per-thread stamp counter, persists across launches
+    int nthr;
+};
+
+struct RowOut {
+    double *pool_vals;           // row entries in completion order
+    int *pool_cols;
+    long long pool_cap;
+    unsigned long long *pool_used;
+    long long *row_off;          // [M] offset of each row in the pool
+    int *row_cnt;                // [M] row length
+    int *ovf_list;               // queries that exceeded the level's capacities
+    unsigned int *ovf_count;
+};
+
+size_t scratch_bytes_per_thread(const Caps &c);
+void scratch_carve(void *base, const Caps &c, int nthr, Scratch *s);
+
+hipError_t launch_morton_sort(const double *d_q, int M, const double bbox[6], int bits,
+                              int *d_order, int *d_hist, int *d_keys,
+                              long long *d_scan_tmp, hipStream_t st);
+int rows_blocks_per_cu();  // resident 256-thread blocks of the row kernel per CU
+hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
+                       const double *d_q, const int *d_items, int n_items,
+                       unsigned int *d_work, const RowOut &out, hipStream_t st);
+// exclusive scan of n int32 counts into int64 offsets; total in out[n]
+hipError_t launch_scan_counts(const int *d_in, long long *d_out, int n, long long *d_tmp,
+                              hipStream_t st);
+size_t scan_tmp_elems(int n);
+hipError_t launch_gather_csr(const long long *d_rowptr64, const long long *d_row_off,
+                             const int *d_row_cnt, const double *pool_vals,
+                             const int *pool_cols, int M, double *d_vals, int *d_cols,
+                             int *d_rowptr32, hipStream_t st);
+hipError_t launch_spmm(int M, int D, const int *rowptr, const int *cols, const void *vals,
+                       int vdt, const void *x, int xdt, void *y, int ydt, hipStream_t st);
+
+}  // namespace nat

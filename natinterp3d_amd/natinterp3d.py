@@ -1,0 +1,279 @@
+"""Public API, mirroring natinterp3d's (natinterp3d.py:9-137).
+
+``Interpolator``, ``interpolate`` and ``get_weights`` keep the reference's
+signatures, centring, dtype rules and return types.  Underneath, the mesh is
+built once on the host (exactly the reference's mesh) and uploaded to the GPU;
+every weight row and every ``W @ values`` product is computed by the gfx950
+kernels of libnatinterp3d_hip.so.  ``parallel`` and ``num_threads`` are
+accepted for compatibility and do not change any result (the reference's
+results do not depend on them either, tests/test_natinterp3d.py:347-373).
+"""
+import ctypes
+import os
+import weakref
+from typing import Optional
+
+import numpy as np
+import scipy.sparse
+
+from . import _lib
+
+_DT_CODE = {np.dtype(np.float32): 0, np.dtype(np.float64): 1}
+
+
+def _free_host(addr):
+    _lib.lib().nat_free_host(ctypes.c_void_p(addr))
+
+
+def _adopt(ptr, n, ctype, dtype):
+    """Wrap n elements of library-malloc'd memory as a NumPy array that frees
+    it when collected (the reference hands its buffers to NumPy the same way,
+    natinterp3d_cython.pyx:37-52)."""
+    addr = ctypes.cast(ptr, ctypes.c_void_p).value
+    if n == 0:
+        if addr:
+            _free_host(addr)
+        return np.zeros(0, dtype)
+    buf = (ctype * n).from_address(addr)
+    fin = weakref.finalize(buf, _free_host, addr)
+    fin.atexit = False
+    return np.frombuffer(buf, dtype=dtype)
+
+
+def _default_device():
+    env = os.environ.get("NATINTERP3D_DEVICE")
+    if env is not None:
+        return int(env)
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return torch.cuda.current_device()
+    except ImportError:
+        pass
+    return 0
+
+
+class MeshAndVerticesHIP:
+    """GPU-backed plugin object with the contract of the reference's
+    ``MeshAndVertices`` / ``MeshAndVerticesParallel``
+    (natinterp3d_cython.pyx:54-134): built from centred float64 data points,
+    ``get_weights(centred_queries)`` returns a scipy CSR matrix
+    [num_queries, num_data_points] with float64 data and int32 indices whose
+    columns are in natural-neighbour discovery order."""
+
+    def __init__(self, dataPoints, device: Optional[int] = None):
+        dataPoints = _as_c_double_2d(dataPoints)
+        self.numDataPoints = dataPoints.shape[0]
+        if dataPoints.shape[1] != 3:
+            raise ValueError("Data points must have shape (num_data_points, 3)")
+        L = _lib.lib()
+        self._m = ctypes.c_void_p()
+        self.device = _default_device() if device is None else int(device)
+        rc = L.nat_mesh_build(dataPoints.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                              self.numDataPoints, self.device, ctypes.byref(self._m))
+        _lib.check(rc, "nat_mesh_build")
+        self._finalizer = weakref.finalize(self, L.nat_mesh_free, self._m)
+
+    @property
+    def handle(self):
+        return self._m
+
+    def info(self):
+        inf = _lib.MeshInfo()
+        _lib.check(_lib.lib().nat_mesh_get_info(self._m, ctypes.byref(inf)), "nat_mesh_get_info")
+        return {f: getattr(inf, f) for f, _ in inf._fields_}
+
+    def export(self):
+        """The packed host mesh: verts [(N+4),3], tet_v/tet_n [T,4], k-d arrays."""
+        inf = self.info()
+        T, n = inf["n_tets"], inf["n_data"]
+        out = dict(verts=np.zeros((n + 4, 3)), tet_v=np.zeros((T, 4), np.int32),
+                   tet_n=np.zeros((T, 4), np.int32), kd_left=np.zeros(T, np.int32),
+                   kd_right=np.zeros(T, np.int32), kd_pos=np.zeros((T, 3)),
+                   kd_rect=np.zeros(6))
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))  # noqa: E731
+        rc = _lib.lib().nat_mesh_export(self._m, dp(out["verts"]), ip(out["tet_v"]),
+                                        ip(out["tet_n"]), ip(out["kd_left"]),
+                                        ip(out["kd_right"]), dp(out["kd_pos"]),
+                                        dp(out["kd_rect"]))
+        _lib.check(rc, "nat_mesh_export")
+        return out
+
+    def get_weights(self, queryPoints):
+        queryPoints = _as_c_double_2d(queryPoints)
+        numQueryPoints = queryPoints.shape[0]
+        if queryPoints.shape[1] != 3:
+            raise ValueError("Query points must have shape (num_query_points, 3)")
+        L = _lib.lib()
+        rowptr = np.empty(numQueryPoints + 1, dtype=np.int32)
+        vals = ctypes.POINTER(ctypes.c_double)()
+        cols = ctypes.POINTER(ctypes.c_int32)()
+        rc = L.nat_get_weights(queryPoints.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                               numQueryPoints, self._m, self.numDataPoints,
+                               ctypes.byref(vals), ctypes.byref(cols),
+                               rowptr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        if rc == -1:
+            raise OverflowError("Total number of nonzero weights exceeds int32 limit")
+        _lib.check(rc, "nat_get_weights")
+        nnz = int(rowptr[numQueryPoints])
+        v = _adopt(vals, nnz, ctypes.c_double, np.float64)
+        c = _adopt(cols, nnz, ctypes.c_int32, np.int32)
+        return scipy.sparse.csr_matrix((v, c, rowptr),
+                                       shape=(numQueryPoints, self.numDataPoints))
+
+    # ---- device-resident forms (PyTorch tensors on this mesh's GPU)
+
+    def weights_device(self, queries):
+        """CSR of centred float64 CUDA queries [M,3] as three device tensors
+        (crow int32[M+1], col int32[nnz], val float64[nnz])."""
+        import torch
+        q = queries.detach().to(dtype=torch.float64).contiguous()
+        if q.dim() != 2 or q.shape[1] != 3:
+            raise ValueError("Query points must have shape (num_query_points, 3)")
+        M = q.shape[0]
+        L = _lib.lib()
+        stream = torch.cuda.current_stream(q.device).cuda_stream
+        nnz = ctypes.c_int64()
+        rc = L.nat_weights_compute(self._m, ctypes.c_void_p(q.data_ptr()), M,
+                                   ctypes.c_void_p(stream), ctypes.byref(nnz))
+        _lib.check(rc, "nat_weights_compute")
+        n = int(nnz.value)
+        if n > 2147483647:
+            raise OverflowError("Total number of nonzero weights exceeds int32 limit")
+        val = torch.empty(n, dtype=torch.float64, device=q.device)
+        col = torch.empty(n, dtype=torch.int32, device=q.device)
+        crow = torch.empty(M + 1, dtype=torch.int32, device=q.device)
+        rc = L.nat_weights_emit(self._m, ctypes.c_void_p(val.data_ptr() if n else 0),
+                                ctypes.c_void_p(col.data_ptr() if n else 0),
+                                ctypes.c_void_p(crow.data_ptr()), ctypes.c_void_p(stream))
+        if rc == -1:
+            raise OverflowError("Total number of nonzero weights exceeds int32 limit")
+        _lib.check(rc, "nat_weights_emit")
+        return crow, col, val
+
+
+def spmm_device(crow, col, val, values, num_rows):
+    """``W @ values`` on the device with scipy's accumulation order and dtype
+    rule; values [N,D] float32/float64, val float32/float64."""
+    import torch
+    out_dt = torch.float64 if (val.dtype == torch.float64 or values.dtype == torch.float64) \
+        else torch.float32
+    D = values.shape[1]
+    out = torch.empty((num_rows, D), dtype=out_dt, device=values.device)
+    codes = {torch.float32: 0, torch.float64: 1}
+    stream = torch.cuda.current_stream(values.device).cuda_stream
+    rc = _lib.lib().nat_spmm_csr(
+        num_rows, values.shape[0], D, ctypes.c_void_p(crow.data_ptr()),
+        ctypes.c_void_p(col.data_ptr() if col.numel() else 0), codes[val.dtype],
+        ctypes.c_void_p(val.data_ptr() if val.numel() else 0),
+        codes[values.dtype], ctypes.c_void_p(values.data_ptr()), codes[out_dt],
+        ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream))
+    _lib.check(rc, "nat_spmm_csr")
+    return out
+
+
+def _as_c_double_2d(a):
+    if not isinstance(a, np.ndarray):
+        raise TypeError(f"expected a numpy.ndarray, got {type(a).__name__}")
+    if a.ndim != 2:
+        raise ValueError(f"Buffer has wrong number of dimensions (expected 2, got {a.ndim})")
+    if a.dtype != np.float64:
+        raise ValueError(f"Buffer dtype mismatch, expected 'double' but got '{a.dtype}'")
+    if not a.flags.c_contiguous:
+        raise ValueError("ndarray is not C-contiguous")
+    return a
+
+
+def interpolate(
+    queries: np.ndarray,
+    keys: np.ndarray,
+    values: np.ndarray,
+    parallel: bool = True,
+    num_threads: Optional[int] = None,
+) -> np.ndarray:
+    """Natural neighbour (Sibson) interpolation of ``values`` known at ``keys``
+    [N,3], evaluated at ``queries`` [M,3]; values [N] or [N,D] -> [M] or [M,D].
+    Same contract as natinterp3d.interpolate (natinterp3d.py:9-35)."""
+    interpolator = Interpolator(keys, parallel=parallel, num_threads=num_threads)
+    return interpolator.interpolate(queries, values)
+
+
+def get_weights(
+    queries: np.ndarray,
+    keys: np.ndarray,
+    parallel: bool = True,
+    num_threads: Optional[int] = None,
+) -> scipy.sparse.csr_matrix:
+    """Sibson coordinates of ``queries`` with respect to ``keys`` as a CSR
+    matrix [M,N].  Same contract as natinterp3d.get_weights
+    (natinterp3d.py:38-60)."""
+    interpolator = Interpolator(keys, parallel=parallel, num_threads=num_threads)
+    return interpolator.get_weights(queries)
+
+
+class Interpolator:
+    """Natural neighbour interpolator for 3D data points (natinterp3d.py:63-137).
+
+    The Delaunay mesh of ``data_points`` is built once and kept in GPU memory;
+    reuse the object for repeated queries on the same keys.  ``device``
+    selects the GPU (default: the current PyTorch device, or
+    ``$NATINTERP3D_DEVICE``)."""
+
+    def __init__(
+        self,
+        data_points: np.ndarray,
+        parallel: bool = True,
+        num_threads: Optional[int] = None,
+        device: Optional[int] = None,
+    ):
+        data_points = np.ascontiguousarray(data_points, np.float64)
+        self._centroid = data_points.mean(axis=0)
+        centered = data_points - self._centroid
+        self.parallel = parallel
+        self.num_threads = num_threads
+        self.mesh_and_vertices = MeshAndVerticesHIP(centered, device=device)
+
+    def get_weights(self, query_points: np.ndarray) -> scipy.sparse.csr_matrix:
+        """Sibson coordinates of ``query_points`` as a CSR matrix whose data has
+        the dtype of ``query_points`` (natinterp3d.py:105-117)."""
+        out_dtype = query_points.dtype
+        query_points = np.ascontiguousarray(query_points, np.float64) - self._centroid
+        result = self.mesh_and_vertices.get_weights(query_points)
+        return result.astype(out_dtype, copy=False)
+
+    def interpolate(self, query_points: np.ndarray, values: np.ndarray) -> np.ndarray:
+        """``get_weights(query_points) @ values`` (natinterp3d.py:119-137).  The
+        weights never leave the GPU: the CSR is built and multiplied there, in
+        scipy's accumulation order and result dtype."""
+        import torch
+        out_dtype = np.dtype(query_points.dtype)
+        values = np.asarray(values)
+        one_d = len(values.shape) == 1
+        vals2 = values[:, np.newaxis] if one_d else values
+        if vals2.ndim != 2:
+            raise ValueError("values must have shape (N,) or (N, D)")
+        n = self.mesh_and_vertices.numDataPoints
+        if vals2.shape[0] != n:
+            raise ValueError("dimension mismatch")
+        res_dtype = np.result_type(out_dtype, vals2.dtype)
+        if out_dtype not in _DT_CODE or res_dtype not in _DT_CODE:
+            raise TypeError(
+                f"GPU interpolation supports float32/float64 weights and results "
+                f"(query dtype {out_dtype}, values dtype {vals2.dtype})")
+        q = np.ascontiguousarray(query_points, np.float64) - self._centroid
+        if q.ndim != 2 or q.shape[1] != 3:
+            raise ValueError("Query points must have shape (num_query_points, 3)")
+        dev = torch.device("cuda", self.mesh_and_vertices.device)
+        with torch.cuda.device(dev):
+            qd = torch.from_numpy(q).to(dev)
+            crow, col, val = self.mesh_and_vertices.weights_device(qd)
+            if out_dtype == np.float32:
+                val = val.to(torch.float32)
+            vdt = vals2.dtype if vals2.dtype in _DT_CODE else res_dtype
+            xd = torch.from_numpy(np.ascontiguousarray(vals2, vdt)).to(dev)
+            y = spmm_device(crow, col, val, xd, q.shape[0])
+            out = y.cpu().numpy()
+        if out.dtype != res_dtype:
+            out = out.astype(res_dtype)
+        return np.squeeze(out, axis=1) if one_d else out
