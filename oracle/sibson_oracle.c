@@ -238,6 +238,9 @@ static inline const double *V(const orc_mesh *m, int slot) { return &m->verts[sl
 static int locate(const orc_mesh *m, const double *q, long long *kd_count, long long *tests) {
     int s = kd_nearest(m, q, kd_count);
     if (s < 0) return -1;
+    /* A Delaunay mesh never cycles; the bound only stops a corrupted mesh
+     * (on which the reference would loop forever). */
+    long long steps = 0, limit = 4LL * m->n_tets + 1024;
     for (int i = 0; i < 4; i++) {
         const int *tv = &m->tet_v[s * 4];
         (*tests)++;
@@ -246,6 +249,7 @@ static int locate(const orc_mesh *m, const double *q, long long *kd_count, long 
             if (nb < 0) return -1;
             s = nb;
             i = -1;
+            if (++steps > limit) return -1;
         }
     }
     return s;
