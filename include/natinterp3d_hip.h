@@ -99,6 +99,20 @@ int nat_spmm_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
                  const void *d_values, int values_dtype, void *d_out, int out_dtype,
                  void *stream);
 
+/* Kernel timings of the last nat_weights_compute / nat_weights_emit on a
+ * mesh, from HIP events recorded on the stream the kernels ran on (no
+ * reference counterpart; for measurement).  Off by default. */
+typedef struct {
+    double order_ms;          /* Morton bucketing of the queries */
+    double rows_ms[4];        /* row kernel, per capacity level */
+    int32_t rows_items[4];    /* queries processed per level */
+    int32_t levels;           /* levels launched */
+    double emit_ms;           /* CSR scan + gather */
+} nat_timings;
+
+int nat_mesh_set_timing(nat_mesh *m, int enable);
+int nat_mesh_get_timings(const nat_mesh *m, nat_timings *out);
+
 /* Release memory returned by nat_get_weights. */
 void nat_free_host(void *p);
 

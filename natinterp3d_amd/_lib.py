@@ -14,8 +14,16 @@ _lib = None
 EXPORTS = (
     "nat_mesh_build", "nat_mesh_free", "nat_mesh_get_info", "nat_mesh_export",
     "nat_get_weights", "nat_weights_compute", "nat_weights_emit", "nat_spmm_csr",
-    "nat_free_host", "nat_last_error",
+    "nat_mesh_set_timing", "nat_mesh_get_timings", "nat_free_host", "nat_last_error",
 )
+
+
+class Timings(ctypes.Structure):
+    _fields_ = [
+        ("order_ms", ctypes.c_double), ("rows_ms", ctypes.c_double * 4),
+        ("rows_items", ctypes.c_int32 * 4), ("levels", ctypes.c_int32),
+        ("emit_ms", ctypes.c_double),
+    ]
 
 
 class MeshInfo(ctypes.Structure):
@@ -59,6 +67,10 @@ def lib():
     L.nat_spmm_csr.restype = ctypes.c_int
     L.nat_spmm_csr.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp,
                                ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, vp]
+    L.nat_mesh_set_timing.restype = ctypes.c_int
+    L.nat_mesh_set_timing.argtypes = [vp, ctypes.c_int]
+    L.nat_mesh_get_timings.restype = ctypes.c_int
+    L.nat_mesh_get_timings.argtypes = [vp, ctypes.POINTER(Timings)]
     L.nat_free_host.restype = None
     L.nat_free_host.argtypes = [vp]
     L.nat_last_error.restype = ctypes.c_char_p

@@ -115,6 +115,11 @@ struct nat_mesh {
     int last_M = -1;
     long long last_nnz = 0;
 
+    // optional kernel timing (HIP events on the launch stream)
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    nat_timings times{};
+
     int64_t device_bytes() const {
         return static_cast<int64_t>(verts.bytes + tet_v.bytes + tet_n.bytes + kd.bytes);
     }
@@ -132,10 +137,25 @@ struct nat_mesh {
                          &pool_vals, &pool_cols, &csr_vals, &csr_cols, &csr_rowptr};
         for (DevBuf *b : all) b->release();
         for (int l = 0; l < kLevels; l++) scratch[l].release();
+        for (hipEvent_t &e : ev) {
+            if (e) (void)hipEventDestroy(e);
+            e = nullptr;
+        }
     }
 };
 
 namespace {
+
+void mark(nat_mesh *m, int i, hipStream_t st) {
+    if (m->timing) (void)hipEventRecord(m->ev[i], st);
+}
+double elapsed(nat_mesh *m, int a, int b) {
+    if (!m->timing) return 0.0;
+    float ms = 0.f;
+    if (hipEventSynchronize(m->ev[b]) != hipSuccess) return -1.0;
+    if (hipEventElapsedTime(&ms, m->ev[a], m->ev[b]) != hipSuccess) return -1.0;
+    return ms;
+}
 
 void set_levels(nat_mesh *m) {
     const int kd = m->host.kd_depth + 2;
@@ -249,12 +269,17 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
     HIP_OR_FAIL(m->counters.ensure(256));
     if (int rc = ensure_level_scratch(m, 0)) return rc;
 
+    m->times = nat_timings{};
+    mark(m, 0, st);
     HIP_OR_FAIL(nat::launch_morton_sort(d_q, M, m->bbox, bits, m->order.as<int>(),
                                         m->hist.as<int>(), m->keys.as<int>(),
                                         m->scan.as<long long>(), st));
+    mark(m, 1, st);
+    m->times.order_ms = elapsed(m, 0, 1);
 
     long long want = static_cast<long long>(std::ceil(m->nnz_per_row * M)) + 1024;
     for (int attempt = 0; attempt < 3; attempt++) {
+        m->times.levels = 0;
         if (m->pool_cap < want) {
             HIP_OR_FAIL(m->pool_vals.ensure(sizeof(double) * want));
             HIP_OR_FAIL(m->pool_cols.ensure(sizeof(int) * want));
@@ -279,12 +304,17 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
             out.ovf_list = m->ovf[level & 1].as<int>();
             out.ovf_count = ctr + 8 + level;
             const int nblocks = (m->nthr[level] + 255) / 256;
+            mark(m, 2, st);
             HIP_OR_FAIL(nat::launch_rows(m->dev, m->caps[level], sc, nblocks, d_q, items,
                                          n_items, ctr + level, out, st));
+            mark(m, 3, st);
             unsigned novf = 0;
             HIP_OR_FAIL(hipMemcpyAsync(&novf, ctr + 8 + level, sizeof(unsigned),
                                        hipMemcpyDeviceToHost, st));
             HIP_OR_FAIL(hipStreamSynchronize(st));
+            m->times.rows_ms[level] = elapsed(m, 2, 3);
+            m->times.rows_items[level] = n_items;
+            m->times.levels = level + 1;
             n_items = static_cast<int>(novf);
             items = m->ovf[level & 1].as<int>();
         }
@@ -315,11 +345,14 @@ int emit_rows(nat_mesh *m, double *d_vals, int *d_cols, int *d_rowptr, hipStream
         return 0;
     }
     long long *rp64 = m->rowptr64.as<long long>();
+    mark(m, 0, st);
     HIP_OR_FAIL(nat::launch_scan_counts(m->row_cnt.as<int>(), rp64, M, m->scan.as<long long>(),
                                         st));
     HIP_OR_FAIL(nat::launch_gather_csr(rp64, m->row_off.as<long long>(), m->row_cnt.as<int>(),
                                        m->pool_vals.as<double>(), m->pool_cols.as<int>(), M,
                                        d_vals, d_cols, d_rowptr, st));
+    mark(m, 1, st);
+    m->times.emit_ms = elapsed(m, 0, 1);
     return 0;
 }
 
@@ -508,6 +541,24 @@ int nat_spmm_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
                                     d_values, values_dtype, d_out, out_dtype,
                                     static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail(-10, "spmm launch failed: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int nat_mesh_set_timing(nat_mesh *m, int enable) {
+    if (!m) return fail(-1, "NULL mesh");
+    if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    std::lock_guard<std::mutex> lk(m->mu);
+    DeviceGuard g(m->device);
+    if (enable && !m->ev[0]) {
+        for (hipEvent_t &e : m->ev) HIP_OR_FAIL(hipEventCreate(&e));
+    }
+    m->timing = enable != 0;
+    return 0;
+}
+
+int nat_mesh_get_timings(const nat_mesh *m, nat_timings *out) {
+    if (!m || !out) return fail(-1, "NULL argument");
+    *out = m->times;
     return 0;
 }
 

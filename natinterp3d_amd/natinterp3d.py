@@ -83,6 +83,20 @@ class MeshAndVerticesHIP:
         _lib.check(_lib.lib().nat_mesh_get_info(self._m, ctypes.byref(inf)), "nat_mesh_get_info")
         return {f: getattr(inf, f) for f, _ in inf._fields_}
 
+    def set_timing(self, enable=True):
+        """Record HIP events around this mesh's kernels (see timings())."""
+        _lib.check(_lib.lib().nat_mesh_set_timing(self._m, 1 if enable else 0),
+                   "nat_mesh_set_timing")
+
+    def timings(self):
+        """Kernel times (ms) of the last weights computation on this mesh."""
+        t = _lib.Timings()
+        _lib.check(_lib.lib().nat_mesh_get_timings(self._m, ctypes.byref(t)),
+                   "nat_mesh_get_timings")
+        n = t.levels
+        return {"order_ms": t.order_ms, "rows_ms": list(t.rows_ms)[:n],
+                "rows_items": list(t.rows_items)[:n], "emit_ms": t.emit_ms}
+
     def export(self):
         """The packed host mesh: verts [(N+4),3], tet_v/tet_n [T,4], k-d arrays."""
         inf = self.info()
@@ -165,10 +179,10 @@ def spmm_device(crow, col, val, values, num_rows):
     stream = torch.cuda.current_stream(values.device).cuda_stream
     rc = _lib.lib().nat_spmm_csr(
         num_rows, values.shape[0], D, ctypes.c_void_p(crow.data_ptr()),
-        ctypes.c_void_p(col.data_ptr() if col.numel() else 0), codes[val.dtype],
-        ctypes.c_void_p(val.data_ptr() if val.numel() else 0),
-        codes[values.dtype], ctypes.c_void_p(values.data_ptr()), codes[out_dt],
-        ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream))
+        ctypes.c_void_p(col.data_ptr() if col.numel() else 0),
+        ctypes.c_void_p(val.data_ptr() if val.numel() else 0), codes[val.dtype],
+        ctypes.c_void_p(values.data_ptr()), codes[values.dtype],
+        ctypes.c_void_p(out.data_ptr()), codes[out_dt], ctypes.c_void_p(stream))
     _lib.check(rc, "nat_spmm_csr")
     return out
 

@@ -34,7 +34,8 @@ class OrcMesh(ctypes.Structure):
 
 COUNTER_FIELDS = ["queries", "kd_nodes", "walk_tests", "insphere", "cavity", "boundary",
                   "face_cc", "sub3", "sub4", "nnz", "dist_verts", "max_cavity", "max_visited",
-                  "max_stack", "max_boundary", "max_nnz", "coincident", "empty_rows"]
+                  "max_stack", "max_boundary", "max_nnz", "coincident", "empty_rows",
+                  "tets_read", "verts_read"]
 
 
 class OrcCounters(ctypes.Structure):

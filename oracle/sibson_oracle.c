@@ -234,10 +234,13 @@ static const int FACE3[4][3] = {{0, 1, 2}, {3, 1, 0}, {0, 2, 3}, {3, 2, 1}};
 
 static inline const double *V(const orc_mesh *m, int slot) { return &m->verts[slot * 3]; }
 
-/* findContainingSimplex, delaunay.c:383-407 */
-static int locate(const orc_mesh *m, const double *q, long long *kd_count, long long *tests) {
+/* findContainingSimplex, delaunay.c:383-407.  The tets the walk reads are
+ * recorded in path[0..cap) (statistics only). */
+static int locate(const orc_mesh *m, const double *q, long long *kd_count, long long *tests,
+                  int *path, int *npath, int cap) {
     int s = kd_nearest(m, q, kd_count);
     if (s < 0) return -1;
+    if (path && *npath < cap) path[(*npath)++] = s;
     /* A Delaunay mesh never cycles; the bound only stops a corrupted mesh
      * (on which the reference would loop forever). */
     long long steps = 0, limit = 4LL * m->n_tets + 1024;
@@ -249,6 +252,7 @@ static int locate(const orc_mesh *m, const double *q, long long *kd_count, long 
             if (nb < 0) return -1;
             s = nb;
             i = -1;
+            if (path && *npath < cap) path[(*npath)++] = s;
             if (++steps > limit) return -1;
         }
     }
@@ -261,6 +265,8 @@ typedef struct {
     int *vis;          /* [T] stamp: tet popped by the cavity search */
     int *cav;          /* [T] stamp: tet in the cavity */
     int *vstamp;       /* [N+4] stamp: vertex counted */
+    int *rstamp;       /* [T] stamp: tet read (statistics) */
+    int *rvstamp;      /* [N+4] stamp: vertex of a read tet (statistics) */
     int *nbr_map;      /* [N] vertex -> slot, -1 (natural_insertionfree.c:285-287) */
     int stamp;
     int *stack; int stack_cap;
@@ -286,6 +292,8 @@ static void qstate_init(qstate *s, const orc_mesh *m) {
     s->vis = (int *)calloc((size_t)m->n_tets + 1, sizeof(int));
     s->cav = (int *)calloc((size_t)m->n_tets + 1, sizeof(int));
     s->vstamp = (int *)calloc((size_t)m->n_data + 4, sizeof(int));
+    s->rstamp = (int *)calloc((size_t)m->n_tets + 1, sizeof(int));
+    s->rvstamp = (int *)calloc((size_t)m->n_data + 4, sizeof(int));
     s->nbr_map = (int *)malloc(sizeof(int) * ((size_t)m->n_data + 1));
     memset(s->nbr_map, 0xff, sizeof(int) * ((size_t)m->n_data + 1));
     s->fcap = 256;
@@ -299,6 +307,7 @@ static void qstate_init(qstate *s, const orc_mesh *m) {
 
 static void qstate_free(qstate *s) {
     free(s->vis); free(s->cav); free(s->vstamp); free(s->nbr_map);
+    free(s->rstamp); free(s->rvstamp);
     free(s->stack); free(s->cavity); free(s->bnd); free(s->nbr); free(s->stolen);
     free(s->fkey); free(s->fcc); free(s->fgen);
     free(s->av); free(s->ai);
@@ -345,6 +354,17 @@ static const double *face_cc(qstate *s, const orc_mesh *m, int va, int vb, int v
     }
 }
 
+/* statistics: a tet record and its vertices read by this query */
+static void note_read(qstate *s, const orc_mesh *m, int t, int st) {
+    if (s->rstamp[t] == st) return;
+    s->rstamp[t] = st;
+    s->c.tets_read++;
+    for (int j = 0; j < 4; j++) {
+        int p = m->tet_v[t * 4 + j];
+        if (s->rvstamp[p] != st) { s->rvstamp[p] = st; s->c.verts_read++; }
+    }
+}
+
 /* getWeightsSingleQueryIF, natural_insertionfree.c:591-738.  Appends the row
  * to the thread's arena and returns its length. */
 static int one_query(qstate *s, const orc_mesh *m, const double *q) {
@@ -356,7 +376,9 @@ static int one_query(qstate *s, const orc_mesh *m, const double *q) {
     s->c.queries++;
 
     /* findCavity, natural_insertionfree.c:486-551 */
-    int start = locate(m, q, &s->c.kd_nodes, &s->c.walk_tests);
+    int path[64], npath = 0;
+    int start = locate(m, q, &s->c.kd_nodes, &s->c.walk_tests, path, &npath, 64);
+    for (int i = 0; i < npath; i++) note_read(s, m, path[i], st);
     if (start >= 0) {
         s->stack = (int *)grow(s->stack, &s->stack_cap, 1, sizeof(int));
         s->stack[top++] = start;
@@ -365,6 +387,7 @@ static int one_query(qstate *s, const orc_mesh *m, const double *q) {
             if (s->vis[cur] == st) continue;
             s->vis[cur] = st;
             visited++;
+            note_read(s, m, cur, st);
             const int *tv = &m->tet_v[cur * 4];
             /* orientation bit, precomputeOrientSigns (natural_insertionfree.c:457-476) */
             int pos = orc_orient3d(V(m, tv[0]), V(m, tv[1]), V(m, tv[2]), V(m, tv[3])) > 0;
@@ -396,7 +419,7 @@ static int one_query(qstate *s, const orc_mesh *m, const double *q) {
     if (ncav == 0) {
         /* coincident-vertex path, natural_insertionfree.c:596-622 */
         long long dummy_k = 0, dummy_t = 0;
-        int t0 = locate(m, q, &dummy_k, &dummy_t);
+        int t0 = locate(m, q, &dummy_k, &dummy_t, NULL, NULL, 0);
         if (t0 >= 0) {
             int coin[4], nco = 0;
             for (int i = 0; i < 4; i++) {
@@ -543,6 +566,8 @@ static void add_counters(orc_counters *a, const orc_counters *b) {
     a->insphere += b->insphere; a->cavity += b->cavity; a->boundary += b->boundary;
     a->face_cc += b->face_cc; a->sub3 += b->sub3; a->sub4 += b->sub4; a->nnz += b->nnz;
     a->dist_verts += b->dist_verts;
+    a->tets_read += b->tets_read;
+    a->verts_read += b->verts_read;
     a->coincident += b->coincident; a->empty_rows += b->empty_rows;
     if (b->max_cavity > a->max_cavity) a->max_cavity = b->max_cavity;
     if (b->max_visited > a->max_visited) a->max_visited = b->max_visited;
@@ -605,7 +630,7 @@ int orc_get_weights(const orc_mesh *m, const double *q, int M, int num_threads, 
 void orc_locate(const orc_mesh *m, const double *q, int M, int *start_tet) {
     for (int i = 0; i < M; i++) {
         long long a = 0, b = 0;
-        start_tet[i] = locate(m, &q[(size_t)i * 3], &a, &b);
+        start_tet[i] = locate(m, &q[(size_t)i * 3], &a, &b, NULL, NULL, 0);
     }
 }
 

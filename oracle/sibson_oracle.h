@@ -49,6 +49,8 @@ typedef struct {
     long long max_nnz;       /* longest row */
     long long coincident;    /* rows produced by the coincident-vertex path */
     long long empty_rows;    /* rows with no entries */
+    long long tets_read;     /* distinct tets read (walk path and cavity search) */
+    long long verts_read;    /* distinct vertices of those tets */
 } orc_counters;
 
 /* Same contract as getInsertionFreeWeights[Parallel]
