@@ -99,6 +99,15 @@ int nat_spmm_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
                  const void *d_values, int values_dtype, void *d_out, int out_dtype,
                  void *stream);
 
+/* Sort the entries of every CSR row by column, in place (d_vals dtype codes
+ * as above).  The reference's Interpolator.get_weights casts W to the query
+ * dtype (natinterp3d.py:117); for any dtype other than float64 scipy's
+ * astype canonicalises the matrix, which sorts each row's columns, and
+ * W @ values then accumulates in that order.  Rows hold no duplicate
+ * columns, so sorting is the whole canonicalisation. */
+int nat_csr_sort_rows(int num_rows, const int32_t *d_rowptr, int32_t *d_cols, void *d_vals,
+                      int vals_dtype, void *stream);
+
 /* Kernel timings of the last nat_weights_compute / nat_weights_emit on a
  * mesh, from HIP events recorded on the stream the kernels ran on (no
  * reference counterpart; for measurement).  Off by default. */

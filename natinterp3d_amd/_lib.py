@@ -14,7 +14,7 @@ _lib = None
 EXPORTS = (
     "nat_mesh_build", "nat_mesh_free", "nat_mesh_get_info", "nat_mesh_export",
     "nat_get_weights", "nat_weights_compute", "nat_weights_emit", "nat_spmm_csr",
-    "nat_mesh_set_timing", "nat_mesh_get_timings", "nat_free_host", "nat_last_error",
+    "nat_csr_sort_rows", "nat_mesh_set_timing", "nat_mesh_get_timings", "nat_free_host", "nat_last_error",
 )
 
 
@@ -67,6 +67,8 @@ def lib():
     L.nat_spmm_csr.restype = ctypes.c_int
     L.nat_spmm_csr.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp,
                                ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, vp]
+    L.nat_csr_sort_rows.restype = ctypes.c_int
+    L.nat_csr_sort_rows.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int, vp]
     L.nat_mesh_set_timing.restype = ctypes.c_int
     L.nat_mesh_set_timing.argtypes = [vp, ctypes.c_int]
     L.nat_mesh_get_timings.restype = ctypes.c_int

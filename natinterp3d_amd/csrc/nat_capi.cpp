@@ -544,6 +544,18 @@ int nat_spmm_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
     return 0;
 }
 
+int nat_csr_sort_rows(int num_rows, const int32_t *d_rowptr, int32_t *d_cols, void *d_vals,
+                      int vals_dtype, void *stream) {
+    g_err.clear();
+    if (num_rows < 0) return fail(-1, "negative size");
+    if (num_rows == 0) return 0;
+    if (!d_rowptr || vals_dtype < 0 || vals_dtype > 1) return fail(-1, "bad argument");
+    hipError_t e = nat::launch_sort_rows(num_rows, d_rowptr, d_cols, d_vals, vals_dtype,
+                                         static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail(-10, "sort launch failed: %s", hipGetErrorString(e));
+    return 0;
+}
+
 int nat_mesh_set_timing(nat_mesh *m, int enable) {
     if (!m) return fail(-1, "NULL mesh");
     if (m->device < 0) return fail(-1, "host-only mesh has no device copy");

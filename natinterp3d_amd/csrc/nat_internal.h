@@ -77,6 +77,8 @@ hipError_t launch_gather_csr(const long long *d_rowptr64, const long long *d_row
                              const int *d_row_cnt, const double *pool_vals,
                              const int *pool_cols, int M, double *d_vals, int *d_cols,
                              int *d_rowptr32, hipStream_t st);
+hipError_t launch_sort_rows(int M, const int *rowptr, int *cols, void *vals, int vdt,
+                            hipStream_t st);
 hipError_t launch_spmm(int M, int D, const int *rowptr, const int *cols, const void *vals,
                        int vdt, const void *x, int xdt, void *y, int ydt, hipStream_t st);
 

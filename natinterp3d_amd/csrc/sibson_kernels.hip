@@ -738,6 +738,41 @@ hipError_t launch_gather_csr(const long long *d_rowptr64, const long long *d_row
     return hipGetLastError();
 }
 
+// ------------------------------------------------------ row canonicalisation
+
+// Insertion sort of each row by column (rows are short; columns are unique).
+template <typename TV>
+__global__ void sort_rows_kernel(int M, const int *__restrict__ rowptr, int *__restrict__ cols,
+                                 TV *__restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const int b = rowptr[i], e = rowptr[i + 1];
+    for (int j = b + 1; j < e; j++) {
+        const int c = cols[j];
+        const TV v = vals[j];
+        int k = j - 1;
+        while (k >= b && cols[k] > c) {
+            cols[k + 1] = cols[k];
+            vals[k + 1] = vals[k];
+            k--;
+        }
+        cols[k + 1] = c;
+        vals[k + 1] = v;
+    }
+}
+
+hipError_t launch_sort_rows(int M, const int *rowptr, int *cols, void *vals, int vdt,
+                            hipStream_t st) {
+    if (M <= 0) return hipSuccess;
+    if (vdt == 0)
+        hipLaunchKernelGGL(sort_rows_kernel<float>, dim3((M + 255) / 256), dim3(256), 0, st, M,
+                           rowptr, cols, static_cast<float *>(vals));
+    else
+        hipLaunchKernelGGL(sort_rows_kernel<double>, dim3((M + 255) / 256), dim3(256), 0, st, M,
+                           rowptr, cols, static_cast<double *>(vals));
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------- SpMM
 
 // y[i,d] = sum over the row's entries in stored order of w * x[col,d],

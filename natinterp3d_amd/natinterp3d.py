@@ -282,8 +282,16 @@ class Interpolator:
         with torch.cuda.device(dev):
             qd = torch.from_numpy(q).to(dev)
             crow, col, val = self.mesh_and_vertices.weights_device(qd)
-            if out_dtype == np.float32:
+            if out_dtype != np.float64:
+                # get_weights' astype(out_dtype) canonicalises W (rows sorted
+                # by column) before the product; do the same on the device
                 val = val.to(torch.float32)
+                stream = torch.cuda.current_stream(dev).cuda_stream
+                _lib.check(_lib.lib().nat_csr_sort_rows(
+                    q.shape[0], ctypes.c_void_p(crow.data_ptr()),
+                    ctypes.c_void_p(col.data_ptr() if col.numel() else 0),
+                    ctypes.c_void_p(val.data_ptr() if val.numel() else 0), 0,
+                    ctypes.c_void_p(stream)), "nat_csr_sort_rows")
             vdt = vals2.dtype if vals2.dtype in _DT_CODE else res_dtype
             xd = torch.from_numpy(np.ascontiguousarray(vals2, vdt)).to(dev)
             y = spmm_device(crow, col, val, xd, q.shape[0])
