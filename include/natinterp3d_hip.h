@@ -113,7 +113,9 @@ int nat_csr_sort_rows(int num_rows, const int32_t *d_rowptr, int32_t *d_cols, vo
  * reference counterpart; for measurement).  Off by default. */
 typedef struct {
     double order_ms;          /* Morton bucketing of the queries */
-    double rows_ms[4];        /* row kernel, per capacity level */
+    double topo_ms;           /* level 0: start tets, cavities, boundaries */
+    double geom_ms;           /* level 0: circumcentres, volumes, rows */
+    double rows_ms[4];        /* per capacity level ([0] = topo + geom) */
     int32_t rows_items[4];    /* queries processed per level */
     int32_t levels;           /* levels launched */
     double emit_ms;           /* CSR scan + gather */

@@ -1,0 +1,285 @@
+// csr_kernels.hip -- gfx950 kernels around the row computation:
+//   morton_*        spatial bucketing of the queries (order only; no result
+//                   depends on it)                            natural.c:18-96
+//   scan/gather     CSR assembly (count, scan, fill)   natural_insertionfree.c:895-928
+//   sort_rows       row canonicalisation done by scipy's astype   natinterp3d.py:117
+//   spmm            W @ values in stored order                 natinterp3d.py:131-137
+#include <hip/hip_runtime.h>
+
+#include "nat_internal.h"
+
+namespace nat {
+
+// --------------------------------------------------------- Morton bucketing
+
+__device__ __forceinline__ unsigned spread3(unsigned v) {
+    v &= 0x3ff;
+    v = (v | (v << 16)) & 0x030000ff;
+    v = (v | (v << 8)) & 0x0300f00f;
+    v = (v | (v << 4)) & 0x030c30c3;
+    v = (v | (v << 2)) & 0x09249249;
+    return v;
+}
+
+__global__ void morton_key_kernel(const double *__restrict__ q, int M, double x0, double y0,
+                                  double z0, double sx, double sy, double sz, int bits,
+                                  int *__restrict__ keys, int *__restrict__ hist) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const int lim = (1 << bits) - 1;
+    auto cell = [&](double v, double o, double s) {
+        double c = (v - o) * s;
+        int k = (c >= 0.0 && c < 2147483647.0) ? static_cast<int>(c) : 0;  // NaN -> 0
+        if (c >= 2147483647.0) k = lim;
+        return k > lim ? lim : k;
+    };
+    const unsigned ix = cell(q[3 * static_cast<size_t>(i)], x0, sx);
+    const unsigned iy = cell(q[3 * static_cast<size_t>(i) + 1], y0, sy);
+    const unsigned iz = cell(q[3 * static_cast<size_t>(i) + 2], z0, sz);
+    const int key = static_cast<int>(spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2));
+    keys[i] = key;
+    atomicAdd(&hist[key], 1);
+}
+
+__global__ void morton_scatter_kernel(const int *__restrict__ keys, int M,
+                                      const long long *__restrict__ start, int *cursor,
+                                      int *__restrict__ order) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const int key = keys[i];
+    const int k = atomicAdd(&cursor[key], 1);
+    order[start[key] + k] = i;
+}
+
+hipError_t launch_morton_sort(const double *d_q, int M, const double bbox[6], int bits,
+                              int *d_order, int *d_hist, int *d_keys, long long *d_scan_tmp,
+                              hipStream_t st) {
+    const int nb = 1 << (3 * bits);
+    hipError_t e = hipMemsetAsync(d_hist, 0, sizeof(int) * nb, st);
+    if (e != hipSuccess) return e;
+    auto scale = [&](int d) {
+        const double r = bbox[3 + d] - bbox[d];
+        return r > 0 ? static_cast<double>(1 << bits) / r : 1.0;
+    };
+    hipLaunchKernelGGL(morton_key_kernel, dim3((M + 255) / 256), dim3(256), 0, st, d_q, M,
+                       bbox[0], bbox[1], bbox[2], scale(0), scale(1), scale(2), bits, d_keys,
+                       d_hist);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    long long *starts = d_scan_tmp;  // [nb + 1]
+    long long *tmp = d_scan_tmp + nb + 1;
+    e = launch_scan_counts(d_hist, starts, nb, tmp, st);
+    if (e != hipSuccess) return e;
+    e = hipMemsetAsync(d_hist, 0, sizeof(int) * nb, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(morton_scatter_kernel, dim3((M + 255) / 256), dim3(256), 0, st, d_keys,
+                       M, starts, d_hist, d_order);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- scanning
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ long long block_excl_scan(long long v, long long &total) {
+    __shared__ long long wsum[kScanBlock / 64];
+    const int lane = __lane_id();
+    const int w = threadIdx.x >> 6;
+    long long x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const long long y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    long long off = 0, tot = 0;
+#pragma unroll
+    for (int j = 0; j < kScanBlock / 64; j++) {
+        const long long s = wsum[j];
+        if (j < w) off += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return off + x - v;
+}
+
+__global__ void scan_tiles_kernel(const int *__restrict__ in, int n, long long *__restrict__ tsum) {
+    const int base = blockIdx.x * kScanTile;
+    long long s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int i = base + k * kScanBlock + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    long long total;
+    block_excl_scan(s, total);
+    if (threadIdx.x == 0) tsum[blockIdx.x] = total;
+}
+
+__global__ void scan_sums_kernel(long long *tsum, int ntiles, long long *grand) {
+    long long carry = 0;
+    for (int b = 0; b < ntiles; b += kScanBlock) {
+        const int i = b + threadIdx.x;
+        const long long v = i < ntiles ? tsum[i] : 0;
+        long long total;
+        const long long e = block_excl_scan(v, total);
+        if (i < ntiles) tsum[i] = carry + e;
+        carry += total;
+    }
+    if (threadIdx.x == 0) *grand = carry;
+}
+
+__global__ void scan_apply_kernel(const int *__restrict__ in, int n,
+                                  const long long *__restrict__ tsum, long long *__restrict__ out) {
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    long long v[kScanItems];
+    long long s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int i = base + k;
+        v[k] = i < n ? in[i] : 0;
+        s += v[k];
+    }
+    long long total;
+    long long off = block_excl_scan(s, total) + tsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int i = base + k;
+        if (i < n) out[i] = off;
+        off += v[k];
+    }
+}
+
+size_t scan_tmp_elems(int n) { return static_cast<size_t>((n + kScanTile - 1) / kScanTile) + 1; }
+
+hipError_t launch_scan_counts(const int *d_in, long long *d_out, int n, long long *d_tmp,
+                              hipStream_t st) {
+    const int ntiles = (n + kScanTile - 1) / kScanTile;
+    if (ntiles > 0) {
+        hipLaunchKernelGGL(scan_tiles_kernel, dim3(ntiles), dim3(kScanBlock), 0, st, d_in, n,
+                           d_tmp);
+    }
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, st, d_tmp, ntiles,
+                       d_out + n);
+    if (ntiles > 0) {
+        hipLaunchKernelGGL(scan_apply_kernel, dim3(ntiles), dim3(kScanBlock), 0, st, d_in, n,
+                           d_tmp, d_out);
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ CSR gather
+
+__global__ void gather_csr_kernel(const long long *__restrict__ rowptr64,
+                                  const long long *__restrict__ row_off,
+                                  const int *__restrict__ row_cnt,
+                                  const double *__restrict__ pv, const int *__restrict__ pc,
+                                  int M, double *__restrict__ vals, int *__restrict__ cols,
+                                  int *__restrict__ rowptr32) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > M) return;
+    rowptr32[i] = static_cast<int>(rowptr64[i]);
+    if (i == M) return;
+    const long long dst = rowptr64[i], src = row_off[i];
+    const int c = row_cnt[i];
+    for (int j = 0; j < c; j++) {
+        vals[dst + j] = pv[src + j];
+        cols[dst + j] = pc[src + j];
+    }
+}
+
+hipError_t launch_gather_csr(const long long *d_rowptr64, const long long *d_row_off,
+                             const int *d_row_cnt, const double *pool_vals,
+                             const int *pool_cols, int M, double *d_vals, int *d_cols,
+                             int *d_rowptr32, hipStream_t st) {
+    hipLaunchKernelGGL(gather_csr_kernel, dim3((M + 1 + 255) / 256), dim3(256), 0, st,
+                       d_rowptr64, d_row_off, d_row_cnt, pool_vals, pool_cols, M, d_vals,
+                       d_cols, d_rowptr32);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ row canonicalisation
+
+// Insertion sort of each row by column (rows are short; columns are unique).
+template <typename TV>
+__global__ void sort_rows_kernel(int M, const int *__restrict__ rowptr, int *__restrict__ cols,
+                                 TV *__restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const int b = rowptr[i], e = rowptr[i + 1];
+    for (int j = b + 1; j < e; j++) {
+        const int c = cols[j];
+        const TV v = vals[j];
+        int k = j - 1;
+        while (k >= b && cols[k] > c) {
+            cols[k + 1] = cols[k];
+            vals[k + 1] = vals[k];
+            k--;
+        }
+        cols[k + 1] = c;
+        vals[k + 1] = v;
+    }
+}
+
+hipError_t launch_sort_rows(int M, const int *rowptr, int *cols, void *vals, int vdt,
+                            hipStream_t st) {
+    if (M <= 0) return hipSuccess;
+    if (vdt == 0)
+        hipLaunchKernelGGL(sort_rows_kernel<float>, dim3((M + 255) / 256), dim3(256), 0, st, M,
+                           rowptr, cols, static_cast<float *>(vals));
+    else
+        hipLaunchKernelGGL(sort_rows_kernel<double>, dim3((M + 255) / 256), dim3(256), 0, st, M,
+                           rowptr, cols, static_cast<double *>(vals));
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------- SpMM
+
+// y[i,d] = sum over the row's entries in stored order of w * x[col,d],
+// starting from zero, in the output type (scipy csr_matvec(s)).
+template <typename TW, typename TX, typename TY>
+__global__ void spmm_kernel(int M, int D, const int *__restrict__ rowptr,
+                            const int *__restrict__ cols, const TW *__restrict__ w,
+                            const TX *__restrict__ x, TY *__restrict__ y) {
+    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= static_cast<long long>(M) * D) return;
+    const int i = static_cast<int>(gid / D), d = static_cast<int>(gid % D);
+    TY acc = 0;
+    const int b = rowptr[i], e = rowptr[i + 1];
+    for (int j = b; j < e; j++) {
+        const TY a = static_cast<TY>(w[j]);
+        const TY v = static_cast<TY>(x[static_cast<size_t>(cols[j]) * D + d]);
+        acc += a * v;
+    }
+    y[gid] = acc;
+}
+
+template <typename TW, typename TX, typename TY>
+static hipError_t spmm_t(int M, int D, const int *rp, const int *c, const void *w, const void *x,
+                         void *y, hipStream_t st) {
+    const long long n = static_cast<long long>(M) * D;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL((spmm_kernel<TW, TX, TY>), dim3(static_cast<unsigned>((n + 255) / 256)),
+                       dim3(256), 0, st, M, D, rp, c, static_cast<const TW *>(w),
+                       static_cast<const TX *>(x), static_cast<TY *>(y));
+    return hipGetLastError();
+}
+
+hipError_t launch_spmm(int M, int D, const int *rp, const int *c, const void *w, int vdt,
+                       const void *x, int xdt, void *y, int ydt, hipStream_t st) {
+    const int code = vdt * 4 + xdt * 2 + ydt;
+    switch (code) {
+        case 0: return spmm_t<float, float, float>(M, D, rp, c, w, x, y, st);
+        case 1: return spmm_t<float, float, double>(M, D, rp, c, w, x, y, st);
+        case 3: return spmm_t<float, double, double>(M, D, rp, c, w, x, y, st);
+        case 5: return spmm_t<double, float, double>(M, D, rp, c, w, x, y, st);
+        case 7: return spmm_t<double, double, double>(M, D, rp, c, w, x, y, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace nat

@@ -93,14 +93,19 @@ struct nat_mesh {
     int num_cus = 0;
     nat::PackedMesh host;
     nat::DevMesh dev{};
-    DevBuf verts, tet_v, tet_n, kd;
+    DevBuf verts, tet_v, tet_n, kd, cell_start, cell_pts;
     double bbox[6] = {0, 0, 0, 1, 1, 1};
     std::mutex mu;
 
     // workspace
     nat::Caps caps[kLevels];
-    int nthr[kLevels] = {0, 0, 0, 0};
+    int nthr[kLevels] = {0, 0, 0, 0};  // [0]: topology kernel threads
+    int nthr_geom = 0;
     DevBuf scratch[kLevels];
+    DevBuf geom_vol;                   // geometry kernel: per-thread row accumulators
+    DevBuf rec, rec_off;               // topology records
+    long long rec_cap = 0;
+    double units_per_row = 40.0;
     DevBuf q;                    // host-path query upload
     DevBuf order, keys, hist;    // Morton bucketing
     DevBuf scan;                 // int64 scan scratch
@@ -121,20 +126,23 @@ struct nat_mesh {
     nat_timings times{};
 
     int64_t device_bytes() const {
-        return static_cast<int64_t>(verts.bytes + tet_v.bytes + tet_n.bytes + kd.bytes);
+        return static_cast<int64_t>(verts.bytes + tet_v.bytes + tet_n.bytes + kd.bytes +
+                                    cell_start.bytes + cell_pts.bytes);
     }
     int64_t workspace_bytes() const {
         size_t s = q.bytes + order.bytes + keys.bytes + hist.bytes + scan.bytes +
                    row_off.bytes + row_cnt.bytes + rowptr64.bytes + ovf[0].bytes +
                    ovf[1].bytes + counters.bytes + pool_vals.bytes + pool_cols.bytes +
-                   csr_vals.bytes + csr_cols.bytes + csr_rowptr.bytes;
+                   csr_vals.bytes + csr_cols.bytes + csr_rowptr.bytes + geom_vol.bytes +
+                   rec.bytes + rec_off.bytes;
         for (int l = 0; l < kLevels; l++) s += scratch[l].bytes;
         return static_cast<int64_t>(s);
     }
     void release_all() {
-        DevBuf *all[] = {&verts, &tet_v, &tet_n, &kd, &q, &order, &keys, &hist, &scan,
-                         &row_off, &row_cnt, &rowptr64, &ovf[0], &ovf[1], &counters,
-                         &pool_vals, &pool_cols, &csr_vals, &csr_cols, &csr_rowptr};
+        DevBuf *all[] = {&verts, &tet_v, &tet_n, &kd, &cell_start, &cell_pts, &q, &order,
+                         &keys, &hist, &scan, &row_off, &row_cnt, &rowptr64, &ovf[0],
+                         &ovf[1], &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,
+                         &csr_rowptr, &geom_vol, &rec, &rec_off};
         for (DevBuf *b : all) b->release();
         for (int l = 0; l < kLevels; l++) scratch[l].release();
         for (hipEvent_t &e : ev) {
@@ -159,18 +167,83 @@ double elapsed(nat_mesh *m, int a, int b) {
 
 void set_levels(nat_mesh *m) {
     const int kd = m->host.kd_depth + 2;
-    // level 0: every query; the capacities cover all rows of uniform data
-    m->caps[0] = {kd, 128, 8, 64, 160, 96};
-    // persistent grid: exactly the blocks that can be resident at once
-    int per_cu = nat::rows_blocks_per_cu();
-    m->nthr[0] = std::max(1, m->num_cus) * std::max(1, per_cu) * 256;
-    // larger levels only see the queries that overflowed the previous one
-    m->caps[1] = {kd, 2048, 12, 1024, 2560, 1536};
+    const int cus = std::max(1, m->num_cus);
+    // level 0 (topology + geometry kernels) sees every query; its capacities
+    // cover all but a handful of rows of uniform data.  Persistent grids:
+    // exactly the blocks that can be resident at once.
+    m->caps[0] = {kd, 128, 8, 64, 160, 96, 8};
+    m->nthr[0] = cus * nat::topo_blocks_per_cu() * 256;
+    m->nthr_geom = cus * nat::geom_blocks_per_cu() * 256;
+    // larger levels (fused kernel) only see the queries that overflowed
+    m->caps[1] = {kd, 2048, 12, 1024, 2560, 1536, 12};
     m->nthr[1] = 8192;
-    m->caps[2] = {kd, 32768, 16, 16384, 40960, 24576};
+    m->caps[2] = {kd, 32768, 16, 16384, 40960, 24576, 16};
     m->nthr[2] = 512;
-    m->caps[3] = {kd, 524288, 20, 262144, 655360, 393216};
+    m->caps[3] = {kd, 524288, 20, 262144, 655360, 393216, 20};
     m->nthr[3] = 64;
+}
+
+// Uniform grid over the tet centroids (the k-d tree's points) for the exact
+// nearest-centroid search: the data bounding box padded by 0.1%, about two
+// centroids per cell; centroids outside the box are filed into the nearest
+// edge cell.
+void build_grid(const nat::PackedMesh &h, double g0[3], double &gh, int gn[3],
+                std::vector<int> &start, std::vector<double4> &pts) {
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; k++) lo[k] = hi[k] = h.verts[k];
+    for (int i = 0; i < h.n_data; i++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], h.verts[3 * i + k]);
+            hi[k] = std::max(hi[k], h.verts[3 * i + k]);
+        }
+    double ext = 0;
+    for (int k = 0; k < 3; k++) ext = std::max(ext, hi[k] - lo[k]);
+    if (!(ext > 0)) ext = 1.0;
+    double vol = 1.0;
+    for (int k = 0; k < 3; k++) {
+        const double pad = 1e-3 * ext;
+        lo[k] -= pad;
+        hi[k] += pad;
+        vol *= std::max(hi[k] - lo[k], 1e-3 * ext);
+    }
+    const int T = h.n_tets;
+    double target = std::max(1.0, std::min(T / 2.0, 1.0 * (1 << 24)));
+    gh = std::cbrt(vol / target);
+    for (;;) {
+        long long total = 1;
+        for (int k = 0; k < 3; k++) {
+            gn[k] = std::max(1, static_cast<int>(std::ceil((hi[k] - lo[k]) / gh)));
+            total *= gn[k];
+        }
+        if (total <= (1 << 25) && gn[0] <= 4096 && gn[1] <= 4096 && gn[2] <= 4096) break;
+        gh *= 1.25;
+    }
+    for (int k = 0; k < 3; k++) g0[k] = lo[k];
+    const double inv = 1.0 / gh;
+    const long long ncell = static_cast<long long>(gn[0]) * gn[1] * gn[2];
+    std::vector<int> cell(T);
+    start.assign(ncell + 1, 0);
+    for (int t = 0; t < T; t++) {
+        int c[3];
+        for (int k = 0; k < 3; k++) {
+            const double f = (h.kd_pos[3 * t + k] - g0[k]) * inv;
+            int i = f >= 0.0 ? (f < gn[k] ? static_cast<int>(f) : gn[k] - 1) : 0;
+            if (!(f == f)) i = 0;
+            c[k] = i;
+        }
+        cell[t] = (c[2] * gn[1] + c[1]) * gn[0] + c[0];
+        start[cell[t] + 1]++;
+    }
+    for (long long c = 0; c < ncell; c++) start[c + 1] += start[c];
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    pts.resize(std::max(T, 1));
+    for (int t = 0; t < T; t++) {
+        const long long id = t;
+        double w;
+        std::memcpy(&w, &id, sizeof(w));
+        pts[fill[cell[t]]++] =
+            make_double4(h.kd_pos[3 * t], h.kd_pos[3 * t + 1], h.kd_pos[3 * t + 2], w);
+    }
 }
 
 int upload_mesh(nat_mesh *m) {
@@ -219,6 +292,27 @@ int upload_mesh(nat_mesh *m) {
     }
     d.n_data = h.n_data;
     d.n_tets = T;
+    {
+        std::vector<int> start;
+        std::vector<double4> pts;
+        double g0[3], gh;
+        int gn[3];
+        build_grid(h, g0, gh, gn, start, pts);
+        HIP_OR_FAIL(m->cell_start.ensure(sizeof(int) * start.size()));
+        HIP_OR_FAIL(m->cell_pts.ensure(sizeof(double4) * pts.size()));
+        HIP_OR_FAIL(hipMemcpy(m->cell_start.p, start.data(), sizeof(int) * start.size(),
+                              hipMemcpyHostToDevice));
+        HIP_OR_FAIL(hipMemcpy(m->cell_pts.p, pts.data(), sizeof(double4) * pts.size(),
+                              hipMemcpyHostToDevice));
+        d.cell_start = m->cell_start.as<int>();
+        d.cell_pts = m->cell_pts.as<double4>();
+        for (int k = 0; k < 3; k++) {
+            d.g0[k] = g0[k];
+            d.gn[k] = gn[k];
+        }
+        d.gh = gh;
+        d.ginv = 1.0 / gh;
+    }
     // bounding box of the data points: the Morton grid of the queries
     for (int k = 0; k < 3; k++) {
         m->bbox[k] = h.verts[k];
@@ -277,30 +371,80 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
     mark(m, 1, st);
     m->times.order_ms = elapsed(m, 0, 1);
 
+    HIP_OR_FAIL(m->rec_off.ensure(sizeof(long long) * Mz));
+    HIP_OR_FAIL(m->geom_vol.ensure(sizeof(double) * static_cast<size_t>(m->caps[0].nb) *
+                                   static_cast<size_t>(std::max(m->nthr_geom, 64))));
     long long want = static_cast<long long>(std::ceil(m->nnz_per_row * M)) + 1024;
-    for (int attempt = 0; attempt < 3; attempt++) {
+    long long want_rec = static_cast<long long>(std::ceil(m->units_per_row * M)) + 1024;
+    for (int attempt = 0; attempt < 4; attempt++) {
         m->times.levels = 0;
         if (m->pool_cap < want) {
             HIP_OR_FAIL(m->pool_vals.ensure(sizeof(double) * want));
             HIP_OR_FAIL(m->pool_cols.ensure(sizeof(int) * want));
             m->pool_cap = want;
         }
+        if (m->rec_cap < want_rec) {
+            HIP_OR_FAIL(m->rec.ensure(sizeof(int4) * want_rec));
+            m->rec_cap = want_rec;
+        }
+        // counters: [0..3] fused-level work, [4] topology work, [5] geometry
+        // work, [8..11] overflow counts, u64 pool use at [16], records at [18]
         unsigned *ctr = m->counters.as<unsigned>();
         unsigned long long *pool_used = reinterpret_cast<unsigned long long *>(ctr + 16);
+        unsigned long long *rec_used = reinterpret_cast<unsigned long long *>(ctr + 18);
         HIP_OR_FAIL(hipMemsetAsync(m->counters.p, 0, 256, st));
-        const int *items = m->order.as<int>();
-        int n_items = M;
-        for (int level = 0; level < kLevels && n_items > 0; level++) {
+        nat::RowOut out;
+        out.pool_vals = m->pool_vals.as<double>();
+        out.pool_cols = m->pool_cols.as<int>();
+        out.pool_cap = m->pool_cap;
+        out.pool_used = pool_used;
+        out.row_off = m->row_off.as<long long>();
+        out.row_cnt = m->row_cnt.as<int>();
+
+        // level 0: topology records, then rows from them
+        nat::Scratch sc0;
+        nat::scratch_carve(m->scratch[0].p, m->caps[0], m->nthr[0], &sc0);
+        nat::Topo topo;
+        topo.rec = m->rec.as<int4>();
+        topo.rec_cap = m->rec_cap;
+        topo.rec_used = rec_used;
+        topo.rec_off = m->rec_off.as<long long>();
+        mark(m, 2, st);
+        HIP_OR_FAIL(nat::launch_topo(m->dev, m->caps[0], sc0, m->nthr[0] / 256, d_q,
+                                     m->order.as<int>(), M, ctr + 4, topo, ctr + 8,
+                                     m->ovf[0].as<int>(), st));
+        mark(m, 3, st);
+        unsigned long long meta[2] = {0, 0};
+        HIP_OR_FAIL(hipMemcpyAsync(meta, ctr + 16, sizeof(meta), hipMemcpyDeviceToHost, st));
+        unsigned novf = 0;
+        HIP_OR_FAIL(hipMemcpyAsync(&novf, ctr + 8, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        HIP_OR_FAIL(hipStreamSynchronize(st));
+        m->times.topo_ms = elapsed(m, 2, 3);
+        const long long units = static_cast<long long>(meta[1]);
+        if (units > m->rec_cap) {
+            want_rec = units + units / 8 + 1024;
+            continue;
+        }
+        m->units_per_row = std::max(8.0, 1.125 * static_cast<double>(units) / M);
+        nat::Scratch scg{};
+        scg.nb_vol = m->geom_vol.as<double>();
+        scg.nthr = m->nthr_geom;
+        mark(m, 2, st);
+        HIP_OR_FAIL(nat::launch_geom(m->dev, scg, m->nthr_geom / 256, d_q, m->order.as<int>(),
+                                     M, ctr + 5, topo, out, st));
+        mark(m, 3, st);
+        m->times.geom_ms = elapsed(m, 2, 3);
+        m->times.rows_ms[0] = m->times.topo_ms + m->times.geom_ms;
+        m->times.rows_items[0] = M;
+        m->times.levels = 1;
+
+        // larger levels: the queries that overflowed the previous one
+        const int *items = m->ovf[0].as<int>();
+        int n_items = static_cast<int>(novf);
+        for (int level = 1; level < kLevels && n_items > 0; level++) {
             if (int rc = ensure_level_scratch(m, level)) return rc;
             nat::Scratch sc;
             nat::scratch_carve(m->scratch[level].p, m->caps[level], m->nthr[level], &sc);
-            nat::RowOut out;
-            out.pool_vals = m->pool_vals.as<double>();
-            out.pool_cols = m->pool_cols.as<int>();
-            out.pool_cap = m->pool_cap;
-            out.pool_used = pool_used;
-            out.row_off = m->row_off.as<long long>();
-            out.row_cnt = m->row_cnt.as<int>();
             out.ovf_list = m->ovf[level & 1].as<int>();
             out.ovf_count = ctr + 8 + level;
             const int nblocks = (m->nthr[level] + 255) / 256;
@@ -308,14 +452,14 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
             HIP_OR_FAIL(nat::launch_rows(m->dev, m->caps[level], sc, nblocks, d_q, items,
                                          n_items, ctr + level, out, st));
             mark(m, 3, st);
-            unsigned novf = 0;
-            HIP_OR_FAIL(hipMemcpyAsync(&novf, ctr + 8 + level, sizeof(unsigned),
+            unsigned nnext = 0;
+            HIP_OR_FAIL(hipMemcpyAsync(&nnext, ctr + 8 + level, sizeof(unsigned),
                                        hipMemcpyDeviceToHost, st));
             HIP_OR_FAIL(hipStreamSynchronize(st));
             m->times.rows_ms[level] = elapsed(m, 2, 3);
             m->times.rows_items[level] = n_items;
             m->times.levels = level + 1;
-            n_items = static_cast<int>(novf);
+            n_items = static_cast<int>(nnext);
             items = m->ovf[level & 1].as<int>();
         }
         if (n_items > 0)

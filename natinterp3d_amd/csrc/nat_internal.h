@@ -20,20 +20,31 @@ struct DevMesh {
     double rmin[3], rmax[3];  // k-d root hyper-rectangle
     int n_data;               // N
     int n_tets;               // T
+    // Uniform grid over the same tet centroids, for the exact nearest-centroid
+    // search that replaces the k-d descent whenever its answer is certified.
+    const int *cell_start;    // [ncell + 1] records of cell c: [cell_start[c], cell_start[c+1])
+    const double4 *cell_pts;  // [T] centroid x,y,z; .w = bit pattern of the tet index (int64)
+    double g0[3];             // grid origin
+    double gh, ginv;          // cell edge and its inverse
+    int gn[3];                // cells per axis
 };
 
-// Per-thread capacities of the row kernel.  A query that needs more marks
-// itself for the next, larger level.
+// Per-thread capacities.  A query that needs more marks itself for the next,
+// larger level.
 struct Caps {
-    int kd;        // k-d descent stack frames (>= tree depth)
-    int stack;     // cavity-search stack
-    int vis_log2;  // log2 of the visited-set hash size
-    int cav;       // cavity tets
-    int bnd;       // boundary faces
-    int nb;        // natural neighbours (row length)
+    int kd;          // k-d descent stack frames (>= tree depth)
+    int stack;       // cavity-search stack
+    int vis_log2;    // log2 of the visited-set hash size
+    int cav;         // cavity tets
+    int bnd;         // boundary faces
+    int nb;          // natural neighbours (row length)
+    int nbmap_log2;  // log2 of the vertex -> slot hash size
 };
 
-// Per-thread scratch, interleaved: element k of thread t at [k * nthr + t].
+// Per-thread scratch.  Stacks and lists are interleaved (element k of thread
+// t at [k * nthr + t]); hash tables are contiguous per thread (slot h of
+// thread t at [t << log2 | h]) so that one query's probes stay on a few
+// cache lines.
 struct Scratch {
     int *kd_node;     // [kd]   node | phase << 30
     double *kd_saved; // [kd]   saved rectangle bound
@@ -44,6 +55,8 @@ struct Scratch {
     int *bnd;         // [bnd]  cavity position << 2 | face
     int *nb_idx;      // [nb]
     double *nb_vol;   // [nb]
+    int2 *nbmap;      // [1 << nbmap_log2]  (vertex, stamp)
+    int *nbslot;      // [1 << nbmap_log2]  slot of the vertex
     int *epoch;       // [1]    per-thread stamp counter, persists across launches
     int nthr;
 };
@@ -59,16 +72,36 @@ struct RowOut {
     unsigned int *ovf_count;
 };
 
+// Per-query topology records written by the topology kernel and consumed by
+// the geometry kernel (16-byte units, one contiguous record per query).
+struct Topo {
+    int4 *rec;                   // record pool
+    long long rec_cap;           // in int4 units
+    unsigned long long *rec_used;
+    long long *rec_off;          // [M] record offset, -1 = row made by another level
+};
+
 size_t scratch_bytes_per_thread(const Caps &c);
 void scratch_carve(void *base, const Caps &c, int nthr, Scratch *s);
 
 hipError_t launch_morton_sort(const double *d_q, int M, const double bbox[6], int bits,
                               int *d_order, int *d_hist, int *d_keys,
                               long long *d_scan_tmp, hipStream_t st);
-int rows_blocks_per_cu();  // resident 256-thread blocks of the row kernel per CU
+int rows_blocks_per_cu();  // resident 256-thread blocks of the fused row kernel per CU
+int topo_blocks_per_cu();
+int geom_blocks_per_cu();
+// fused row kernel: used for the larger capacity levels
 hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
                        const double *d_q, const int *d_items, int n_items,
                        unsigned int *d_work, const RowOut &out, hipStream_t st);
+// level 0: topology records, then rows from them
+hipError_t launch_topo(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
+                       const double *d_q, const int *d_items, int n_items,
+                       unsigned int *d_work, const Topo &topo, unsigned int *ovf_count,
+                       int *ovf_list, hipStream_t st);
+hipError_t launch_geom(const DevMesh &m, const Scratch &sc, int nblocks, const double *d_q,
+                       const int *d_items, int n_items, unsigned int *d_work,
+                       const Topo &topo, const RowOut &out, hipStream_t st);
 // exclusive scan of n int32 counts into int64 offsets; total in out[n]
 hipError_t launch_scan_counts(const int *d_in, long long *d_out, int n, long long *d_tmp,
                               hipStream_t st);

@@ -1,15 +1,20 @@
-// sibson_kernels.hip -- gfx950 kernels of the Sibson query path.
+// sibson_kernels.hip -- gfx950 kernels of the Sibson row computation.
 //
-//   morton_*        spatial bucketing of the queries (order only; no result
-//                   depends on it)                         natural.c:18-96
-//   sibson_rows     one thread per query: k-d nearest centroid, visibility
-//                   walk, cavity search, boundary/neighbour order, stolen
-//                   volumes, normalisation         natural_insertionfree.c:486-738
-//   scan/gather     CSR assembly (count, scan, fill)  natural_insertionfree.c:895-928
-//   spmm            W @ values in stored order        natinterp3d.py:131-137
+// Level 0 (every query) runs as two kernels:
+//   topo_kernel   one thread per query: start tet (exact nearest centroid,
+//                 then the visibility walk), cavity search in the reference's
+//                 LIFO order, boundary faces and neighbour first-appearance
+//                 order; writes a compact record per query
+//                                             natural_insertionfree.c:486-584
+//   geom_kernel   one thread per query: circumcentres and Voronoi subcell
+//                 volumes from the record (steps 3-5), row into the pool
+//                                             natural_insertionfree.c:628-737
+// Queries whose cavity or row exceeds the level-0 capacities are finished by
+// sibson_rows_kernel, which does all of it in one thread with larger scratch.
 //
 // Every FP64 decision and value is bit-identical to the reference: the same
-// expressions (geom.h), the same visiting orders, -ffp-contract=off.
+// expressions (geom.h), the same visiting and accumulation orders, built
+// with -ffp-contract=off.
 #include <hip/hip_runtime.h>
 
 #include "geom.h"
@@ -35,8 +40,11 @@ __device__ __forceinline__ int4 ld_tet_v(const DevMesh &m, int t, bool &neg) {
 __device__ __forceinline__ double seld(const d3 &v, int i) {
     return i == 0 ? v.x : (i == 1 ? v.y : v.z);
 }
-__device__ __forceinline__ unsigned hash_tet(int t, int log2) {
-    return (static_cast<unsigned>(t) * 2654435761u) >> (32 - log2);
+__device__ __forceinline__ unsigned hash_key(int k, int log2) {
+    return (static_cast<unsigned>(k) * 2654435761u) >> (32 - log2);
+}
+__device__ __forceinline__ size_t il(int k, size_t S, int tid) {  // interleaved index
+    return static_cast<size_t>(k) * S + tid;
 }
 
 // ------------------------------------------------------------- k-d nearest
@@ -61,10 +69,10 @@ __device__ int kd_nearest(const DevMesh &m, const d3 &q, const Scratch &sc, int 
     int sp = 1;
     sc.kd_node[tid] = 0;
     while (sp > 0) {
-        const size_t fi = static_cast<size_t>(sp - 1) * S + tid;
+        const size_t fi = il(sp - 1, S, tid);
         const int fr = sc.kd_node[fi];
         const int node = fr & 0x3fffffff;
-        int phase = fr >> 30;
+        const int phase = fr >> 30;
         const int dir = (sp - 1) % 3;
         const double4 nd = m.kd[node];
         const int2 lr = *reinterpret_cast<const int2 *>(&nd.w);
@@ -84,7 +92,7 @@ __device__ int kd_nearest(const DevMesh &m, const d3 &q, const Scratch &sc, int 
                 sc.kd_saved[fi] = c;
                 c = npd;
                 sc.kd_node[fi] = node | (1 << 30);
-                sc.kd_node[static_cast<size_t>(sp) * S + tid] = nearer;
+                sc.kd_node[il(sp, S, tid)] = nearer;
                 sp++;
                 continue;
             }
@@ -97,7 +105,6 @@ __device__ int kd_nearest(const DevMesh &m, const d3 &q, const Scratch &sc, int 
             sp--;
             continue;
         }
-        // this node
         {
             double d = 0;
             const double dx = np.x - q.x, dy = np.y - q.y, dz = np.z - q.z;
@@ -127,7 +134,7 @@ __device__ int kd_nearest(const DevMesh &m, const d3 &q, const Scratch &sc, int 
                 }
                 sc.kd_saved[fi] = saved;
                 sc.kd_node[fi] = node | (2 << 30);
-                sc.kd_node[static_cast<size_t>(sp) * S + tid] = farther;
+                sc.kd_node[il(sp, S, tid)] = farther;
                 sp++;
                 continue;
             }
@@ -136,6 +143,82 @@ __device__ int kd_nearest(const DevMesh &m, const d3 &q, const Scratch &sc, int 
         sp--;
     }
     return best;
+}
+
+// ------------------------------------------------- grid nearest centroid
+
+// The k-d search above returns the centroid of least squared distance
+// (dx^2 + dy^2) + dz^2 -- it never prunes a subtree holding a strictly
+// closer centroid -- and among equal distances the first one it meets.  This
+// search finds the same least distance over a uniform grid of the centroids
+// and returns its tet when the answer is certain: no unexamined centroid can
+// be as close (distance certificate) and no examined one ties.  Otherwise it
+// returns -1 and the caller runs the k-d replica.
+__device__ int grid_nearest(const DevMesh &m, const d3 &q) {
+    const double fx = (q.x - m.g0[0]) * m.ginv;
+    const double fy = (q.y - m.g0[1]) * m.ginv;
+    const double fz = (q.z - m.g0[2]) * m.ginv;
+    if (!(fx >= 0.0 && fx < m.gn[0] && fy >= 0.0 && fy < m.gn[1] && fz >= 0.0 &&
+          fz < m.gn[2]))
+        return -1;
+    const int ix = static_cast<int>(fx), iy = static_cast<int>(fy), iz = static_cast<int>(fz);
+    const int ii[3] = {ix, iy, iz};
+    double best_d = __builtin_huge_val();
+    int best = -1;
+    bool tie = false;
+    const double h = m.gh;
+    constexpr double kSafe = 1.0 - 1e-9;
+    for (int r = 0; r <= 2; r++) {
+        for (int dz = -r; dz <= r; dz++) {
+            const int z = iz + dz;
+            if (z < 0 || z >= m.gn[2]) continue;
+            for (int dy = -r; dy <= r; dy++) {
+                const int y = iy + dy;
+                if (y < 0 || y >= m.gn[1]) continue;
+                const bool shell = (dz == -r || dz == r || dy == -r || dy == r);
+                const int step = shell ? 1 : (r > 0 ? 2 * r : 1);
+                for (int dx = -r; dx <= r; dx += step) {
+                    const int x = ix + dx;
+                    if (x < 0 || x >= m.gn[0]) continue;
+                    // squared distance from q to the cell box, rounded down
+                    const double lx = m.g0[0] + x * h, ly = m.g0[1] + y * h,
+                                 lz = m.g0[2] + z * h;
+                    const double ex = fmax(fmax(lx - q.x, q.x - (lx + h)), 0.0);
+                    const double ey = fmax(fmax(ly - q.y, q.y - (ly + h)), 0.0);
+                    const double ez = fmax(fmax(lz - q.z, q.z - (lz + h)), 0.0);
+                    const double cd = (ex * ex + ey * ey + ez * ez) * kSafe;
+                    if (cd > best_d) continue;
+                    const int c = (z * m.gn[1] + y) * m.gn[0] + x;
+                    const int b = m.cell_start[c], e = m.cell_start[c + 1];
+                    for (int j = b; j < e; j++) {
+                        const double4 p = m.cell_pts[j];
+                        const double ddx = p.x - q.x, ddy = p.y - q.y, ddz = p.z - q.z;
+                        const double d = ddx * ddx + ddy * ddy + ddz * ddz;
+                        if (d < best_d) {
+                            best_d = d;
+                            best = static_cast<int>(__double_as_longlong(p.w));
+                            tie = false;
+                        } else if (d == best_d) {
+                            tie = true;
+                        }
+                    }
+                }
+            }
+        }
+        // distance from q to the outside of the searched block; a side that
+        // reaches the grid's edge holds every centroid beyond it (they are
+        // filed into the edge cells) and does not limit the certificate
+        double rho = __builtin_huge_val();
+        const double qq[3] = {q.x, q.y, q.z};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            if (ii[a] - r > 0) rho = fmin(rho, qq[a] - (m.g0[a] + (ii[a] - r) * h));
+            if (ii[a] + r < m.gn[a] - 1)
+                rho = fmin(rho, (m.g0[a] + (ii[a] + r + 1) * h) - qq[a]);
+        }
+        if (best >= 0 && best_d < rho * rho * kSafe) return tie ? -1 : best;
+    }
+    return -1;
 }
 
 // findContainingSimplex (delaunay.c:383-407): leave through the first face
@@ -166,95 +249,83 @@ __device__ int walk(const DevMesh &m, int s, const d3 &q, bool &ovf) {
     return s;
 }
 
-// Visited-set probes (set semantics of natural_insertionfree.c:152-211).
-struct Visited {
+// Start tet of the cavity search (findContainingSimplex with the k-d seed).
+__device__ int locate(const DevMesh &m, const d3 &q, const Scratch &sc, int tid,
+                      const Caps &caps, bool &ovf) {
+    int s = grid_nearest(m, q);
+    if (s < 0) s = kd_nearest(m, q, sc, tid, caps.kd, ovf);
+    if (ovf || s < 0) return -1;
+    return walk(m, s, q, ovf);
+}
+
+// ----------------------------------------------------------- hash tables
+
+// Open-addressing table of one thread, entries (key, stamp) plus an int
+// value; entries of other stamps read as empty, so a new query only needs a
+// new stamp.  Set semantics of natural_insertionfree.c:152-211 and of the
+// neighbour map natural_insertionfree.c:373-382.
+struct HashTab {
     int2 *tab;
-    int *pos;
-    size_t S;
-    int tid, log2, stamp;
-    __device__ size_t at(unsigned h) const { return static_cast<size_t>(h) * S + tid; }
-    // returns slot index of t, inserting it when absent; found = already present
-    __device__ unsigned insert(int t, bool &found) const {
-        const unsigned mask = (1u << log2) - 1;
-        unsigned h = hash_tet(t, log2);
+    int *val;
+    size_t base;
+    unsigned mask;
+    int log2, stamp;
+    __device__ HashTab(int2 *t, int *v, int tid, int lg, int st)
+        : tab(t), val(v), base(static_cast<size_t>(tid) << lg), mask((1u << lg) - 1),
+          log2(lg), stamp(st) {}
+    // slot of key, inserting it (value -1) when absent; found = was present
+    __device__ unsigned insert(int key, bool &found) const {
+        unsigned h = hash_key(key, log2);
         for (;;) {
-            const int2 e = tab[at(h)];
+            const int2 e = tab[base + h];
             if (e.y != stamp) {
-                tab[at(h)] = make_int2(t, stamp);
-                pos[at(h)] = -1;
+                tab[base + h] = make_int2(key, stamp);
+                val[base + h] = -1;
                 found = false;
                 return h;
             }
-            if (e.x == t) {
+            if (e.x == key) {
                 found = true;
                 return h;
             }
             h = (h + 1) & mask;
         }
     }
-    __device__ bool contains(int t) const {
-        const unsigned mask = (1u << log2) - 1;
-        unsigned h = hash_tet(t, log2);
+    __device__ bool contains(int key) const {
+        unsigned h = hash_key(key, log2);
         for (;;) {
-            const int2 e = tab[at(h)];
+            const int2 e = tab[base + h];
             if (e.y != stamp) return false;
-            if (e.x == t) return true;
+            if (e.x == key) return true;
             h = (h + 1) & mask;
         }
     }
-    // cavity position of t, -1 when t is not in the cavity
-    __device__ int cavity_pos(int t) const {
-        const unsigned mask = (1u << log2) - 1;
-        unsigned h = hash_tet(t, log2);
+    // value of key, -1 when absent
+    __device__ int get(int key) const {
+        unsigned h = hash_key(key, log2);
         for (;;) {
-            const int2 e = tab[at(h)];
+            const int2 e = tab[base + h];
             if (e.y != stamp) return -1;
-            if (e.x == t) return pos[at(h)];
+            if (e.x == key) return val[base + h];
             h = (h + 1) & mask;
         }
     }
+    __device__ void set(unsigned h, int v) const { val[base + h] = v; }
 };
 
-__device__ __forceinline__ int find_nb(const Scratch &sc, size_t S, int tid, int n, int v) {
-    for (int j = 0; j < n; j++)
-        if (sc.nb_idx[static_cast<size_t>(j) * S + tid] == v) return j;
-    return -1;
-}
-
-// Circumcentre of the face of tet `t` opposite its vertex slot k, vertices in
-// ascending slot order -- the order faceCCLookup is first called with
-// (natural_insertionfree.c:649-652).
-__device__ __forceinline__ d3 face_cc_own(const d3 P[4], int k) {
-    const int a = k == 0 ? 1 : 0;
-    const int b = k <= 1 ? 2 : 1;
-    const int c = k <= 2 ? 3 : 2;
-    return cc_tri(P[a], P[b], P[c]);
-}
-
-// One query.  Returns the row length (the row is left in nb_idx/nb_vol), or
-// -1 when a capacity of this level was exceeded.
-__device__ int sibson_row(const DevMesh &m, const Caps &caps, const Scratch &sc, int tid,
-                          int stamp, const d3 &q) {
+// findCavity (natural_insertionfree.c:498-530): LIFO search from `start`;
+// a tet is marked visited when popped, pushed neighbours are those not yet
+// visited at push time.  Returns the cavity size (tets in sc.cav in cavity
+// order, cavity positions in the visited table) or -1 on overflow.
+__device__ int cavity_search(const DevMesh &m, const Caps &caps, const Scratch &sc, int tid,
+                             const HashTab &vis, int start, const d3 &q) {
     const size_t S = static_cast<size_t>(sc.nthr);
-    const int N = m.n_data;
-    bool ovf = false;
-
-    int start = kd_nearest(m, q, sc, tid, caps.kd, ovf);
-    if (ovf) return -1;
-    if (start < 0) return 0;
-    start = walk(m, start, q, ovf);
-    if (ovf) return -1;
-    if (start < 0) return 0;  // outside the super simplex: empty row
-
-    // ---- cavity search, natural_insertionfree.c:498-530
-    Visited vis{sc.vis, sc.vis_pos, S, tid, caps.vis_log2, stamp};
     const int vis_max = (3 << caps.vis_log2) / 4;
-    int nvis = 0, ncav = 0, top = 0;
+    int nvis = 0, ncav = 0, top = 1;
     sc.stack[tid] = start;
-    top = 1;
     while (top > 0) {
         top--;
-        const int cur = sc.stack[static_cast<size_t>(top) * S + tid];
+        const int cur = sc.stack[il(top, S, tid)];
         bool found;
         const unsigned h = vis.insert(cur, found);
         if (found) continue;
@@ -266,8 +337,8 @@ __device__ int sibson_row(const DevMesh &m, const Caps &caps, const Scratch &sc,
         const double ins = neg ? insphere(p1, p0, p2, p3, q) : insphere(p0, p1, p2, p3, q);
         if (ins > 0) {
             if (ncav >= caps.cav) return -1;
-            sc.cav[static_cast<size_t>(ncav) * S + tid] = cur;
-            sc.vis_pos[vis.at(h)] = ncav;
+            sc.cav[il(ncav, S, tid)] = cur;
+            vis.set(h, ncav);
             ncav++;
             const int4 tn = m.tet_n[cur];
 #pragma unroll
@@ -275,162 +346,124 @@ __device__ int sibson_row(const DevMesh &m, const Caps &caps, const Scratch &sc,
                 const int nb = sel4(tn, i);
                 if (nb >= 0 && !vis.contains(nb)) {
                     if (top >= caps.stack) return -1;
-                    sc.stack[static_cast<size_t>(top) * S + tid] = nb;
+                    sc.stack[il(top, S, tid)] = nb;
                     top++;
                 }
             }
         }
     }
+    return ncav;
+}
 
-    int nnb = 0;
-    if (ncav == 0) {
-        // coincident-vertex path, natural_insertionfree.c:596-622: the re-walk
-        // of the reference ends in the same tet `start`.
-        bool neg;
-        const int4 tv = ld_tet_v(m, start, neg);
-        int coin[4];
-        int nco = 0;
+// Coincident-vertex path (natural_insertionfree.c:596-622): vertices of the
+// start tet equal to the query (the reference's re-walk ends in the same
+// tet).  Writes their indices to out[0..n) and returns n.
+__device__ int coincident(const DevMesh &m, int start, const d3 &q, int out[4]) {
+    bool neg;
+    const int4 tv = ld_tet_v(m, start, neg);
+    int n = 0;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int p = sel4(tv, i);
-            const d3 v = ld_vert(m, p);
-            if (p < N && v.x == q.x && v.y == q.y && v.z == q.z) coin[nco++] = p;
-        }
-        if (nco > 0) {
-            const double w = 1.0 / nco;
-            for (int i = 0; i < nco; i++) {
-                sc.nb_idx[static_cast<size_t>(i) * S + tid] = coin[i];
-                sc.nb_vol[static_cast<size_t>(i) * S + tid] = w;
-            }
-        }
-        return nco;
+    for (int i = 0; i < 4; i++) {
+        const int p = sel4(tv, i);
+        const d3 v = ld_vert(m, p);
+        if (p < m.n_data && v.x == q.x && v.y == q.y && v.z == q.z) out[n++] = p;
     }
+    return n;
+}
 
-    // ---- boundary faces and neighbour order, natural_insertionfree.c:559-584
-    int nbnd = 0;
-    for (int ci = 0; ci < ncav; ci++) {
-        const int t = sc.cav[static_cast<size_t>(ci) * S + tid];
-        const int4 tn = m.tet_n[t];
-        bool neg;
-        const int4 tv = ld_tet_v(m, t, neg);
-        for (int fi = 0; fi < 4; fi++) {
-            const int nb = sel4(tn, fi);
-            if (nb >= 0 && vis.cavity_pos(nb) >= 0) continue;
-            if (nbnd >= caps.bnd) return -1;
-            sc.bnd[static_cast<size_t>(nbnd) * S + tid] = (ci << 2) | fi;
-            nbnd++;
-            for (int k = 0; k < 3; k++) {
-                const int v = sel4(tv, face_slot(fi, k));
-                if (v >= N) continue;
-                if (find_nb(sc, S, tid, nnb, v) >= 0) continue;
-                if (nnb >= caps.nb) return -1;
-                sc.nb_idx[static_cast<size_t>(nnb) * S + tid] = v;
-                sc.nb_vol[static_cast<size_t>(nnb) * S + tid] = 0.0;
-                nnb++;
-            }
-        }
+// Circumcentre of the face of a tet opposite its vertex k, vertices in
+// ascending slot order -- the order faceCCLookup is first called with
+// (natural_insertionfree.c:649-652).
+__device__ __forceinline__ d3 face_cc_own(const d3 P[4], int k) {
+    const int a = k == 0 ? 1 : 0;
+    const int b = k <= 1 ? 2 : 1;
+    const int c = k <= 2 ? 3 : 2;
+    return cc_tri(P[a], P[b], P[c]);
+}
+
+// Circumcentre of face k of tet t (opposite its vertex k) evaluated in the
+// ascending-slot order of neighbour `nb`, which holds the same face.
+__device__ d3 face_cc_from_neighbour(const DevMesh &m, const int4 &tv, int k, int nb) {
+    bool nneg;
+    const int4 nv = ld_tet_v(m, nb, nneg);
+    const int fa = sel4(tv, k == 0 ? 1 : 0), fb = sel4(tv, k <= 1 ? 2 : 1),
+              fc = sel4(tv, k <= 2 ? 3 : 2);
+    int kk = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int x = sel4(nv, j);
+        if (x != fa && x != fb && x != fc) kk = j;
     }
-    if (nnb == 0) return 0;
+    const int ia = sel4(nv, kk == 0 ? 1 : 0), ib = sel4(nv, kk <= 1 ? 2 : 1),
+              ic = sel4(nv, kk <= 2 ? 3 : 2);
+    return cc_tri(ld_vert(m, ia), ld_vert(m, ib), ld_vert(m, ic));
+}
 
-    // ---- step 3: old contributions, natural_insertionfree.c:642-676
-    for (int ci = 0; ci < ncav; ci++) {
-        const int t = sc.cav[static_cast<size_t>(ci) * S + tid];
-        bool neg;
-        const int4 tv = ld_tet_v(m, t, neg);
-        const int4 tn = m.tet_n[t];
-        d3 P[4];
-        P[0] = ld_vert(m, tv.x);
-        P[1] = ld_vert(m, tv.y);
-        P[2] = ld_vert(m, tv.z);
-        P[3] = ld_vert(m, tv.w);
-        const d3 ct = cc_tet(P[0], P[1], P[2], P[3]);
-        // fcc[k]: circumcentre of the face opposite p[k], evaluated in the
-        // vertex order of the first cavity tet (in cavity order) holding it.
-        d3 fcc[4];
+// Step 3 for one cavity tet (natural_insertionfree.c:642-676): calls
+// add(slot, vol) for each vertex with a neighbour slot, in vertex order.
+// fcmask bit k: face opposite p[k] is held by an earlier cavity tet.
+template <typename SlotOf, typename Add>
+__device__ __forceinline__ void old_contributions(const DevMesh &m, int t, unsigned fcmask,
+                                                  SlotOf slot_of, Add add) {
+    bool neg;
+    const int4 tv = ld_tet_v(m, t, neg);
+    d3 P[4];
+    P[0] = ld_vert(m, tv.x);
+    P[1] = ld_vert(m, tv.y);
+    P[2] = ld_vert(m, tv.z);
+    P[3] = ld_vert(m, tv.w);
+    const d3 ct = cc_tet(P[0], P[1], P[2], P[3]);
+    d3 fcc[4];
+    int4 tn = make_int4(-1, -1, -1, -1);
+    if (fcmask) tn = m.tet_n[t];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int nb = sel4(tn, 3 - k);
-            int cj = -1;
-            if (nb >= 0) cj = vis.cavity_pos(nb);
-            if (cj >= 0 && cj < ci) {
-                bool nneg;
-                const int4 nv = ld_tet_v(m, nb, nneg);
-                // the neighbour's vertex not on this face
-                const int fa = sel4(tv, k == 0 ? 1 : 0), fb = sel4(tv, k <= 1 ? 2 : 1),
-                          fc = sel4(tv, k <= 2 ? 3 : 2);
-                int kk = 0;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int x = sel4(nv, j);
-                    if (x != fa && x != fb && x != fc) kk = j;
-                }
-                const int ia = sel4(nv, kk == 0 ? 1 : 0), ib = sel4(nv, kk <= 1 ? 2 : 1),
-                          ic = sel4(nv, kk <= 2 ? 3 : 2);
-                fcc[k] = cc_tri(ld_vert(m, ia), ld_vert(m, ib), ld_vert(m, ic));
-            } else {
-                fcc[k] = face_cc_own(P, k);
-            }
-        }
-        const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
-#pragma unroll
-        for (int vi = 0; vi < 4; vi++) {
-            const int pk = vid[vi];
-            if (pk >= N) continue;
-            const int slot = find_nb(sc, S, tid, nnb, pk);
-            if (slot < 0) continue;
-            int o0 = (vi + 1) & 3, o1 = (vi + 2) & 3, o2 = (vi + 3) & 3, tmp;
-            if (o2 < o0) { tmp = o0; o0 = o2; o2 = tmp; }
-            if (o1 > o2) { tmp = o1; o1 = o2; o2 = tmp; }
-            if (o0 > o1) { tmp = o0; o0 = o1; o1 = tmp; }
-            const double vol =
-                subcell_volume(P[vi], P[o0], P[o1], P[o2], ct, fcc[o2], fcc[o1], fcc[o0]);
-            double &acc = sc.nb_vol[static_cast<size_t>(slot) * S + tid];
-            acc += vol;
-        }
+    for (int k = 0; k < 4; k++) {
+        if ((fcmask >> k) & 1u)
+            fcc[k] = face_cc_from_neighbour(m, tv, k, sel4(tn, 3 - k));
+        else
+            fcc[k] = face_cc_own(P, k);
     }
-
-    // ---- step 4: new contributions, natural_insertionfree.c:681-719
-    for (int bi = 0; bi < nbnd; bi++) {
-        const int code = sc.bnd[static_cast<size_t>(bi) * S + tid];
-        const int ci = code >> 2, fi = code & 3;
-        const int t = sc.cav[static_cast<size_t>(ci) * S + tid];
-        bool neg;
-        const int4 tv = ld_tet_v(m, t, neg);
-        d3 P[4];
-        P[0] = ld_vert(m, tv.x);
-        P[1] = ld_vert(m, tv.y);
-        P[2] = ld_vert(m, tv.z);
-        P[3] = ld_vert(m, tv.w);
-        const int s0 = face_slot(fi, 0), s1 = face_slot(fi, 1), s2 = face_slot(fi, 2);
-        const int fv[3] = {sel4(tv, s0), sel4(tv, s1), sel4(tv, s2)};
-        const d3 F[3] = {s0 == 0 ? P[0] : (s0 == 1 ? P[1] : (s0 == 2 ? P[2] : P[3])),
-                         s1 == 0 ? P[0] : (s1 == 1 ? P[1] : (s1 == 2 ? P[2] : P[3])),
-                         s2 == 0 ? P[0] : (s2 == 1 ? P[1] : (s2 == 2 ? P[2] : P[3]))};
-        const d3 cv = cc_tet(q, F[0], F[1], F[2]);
-        const d3 cf = face_cc_own(P, 3 - fi);
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const int pk = fv[k];
-            if (pk >= N) continue;
-            const int slot = find_nb(sc, S, tid, nnb, pk);
-            if (slot < 0) continue;
-            const d3 &A = F[(k + 1) % 3];
-            const d3 &B = F[(k + 2) % 3];
-            const d3 ca = cc_tri(F[k], q, A);
-            const d3 cb = cc_tri(F[k], q, B);
-            const double vol = subcell_volume(F[k], q, A, B, cv, ca, cb, cf);
-            double &acc = sc.nb_vol[static_cast<size_t>(slot) * S + tid];
-            acc -= vol;
-        }
+    for (int vi = 0; vi < 4; vi++) {
+        const int slot = slot_of(vi, sel4(tv, vi));
+        if (slot < 0) continue;
+        int o0 = (vi + 1) & 3, o1 = (vi + 2) & 3, o2 = (vi + 3) & 3, tmp;
+        if (o2 < o0) { tmp = o0; o0 = o2; o2 = tmp; }
+        if (o1 > o2) { tmp = o1; o1 = o2; o2 = tmp; }
+        if (o0 > o1) { tmp = o0; o0 = o1; o1 = tmp; }
+        add(slot, subcell_volume(P[vi], P[o0], P[o1], P[o2], ct, fcc[o2], fcc[o1], fcc[o0]));
     }
+}
 
-    // ---- step 5: normalise, natural_insertionfree.c:721-737
-    double total = 0.0;
-    for (int j = 0; j < nnb; j++) total += sc.nb_vol[static_cast<size_t>(j) * S + tid];
-    if (!(total > 1e-30)) return 0;
-    const double inv = 1.0 / total;
-    for (int j = 0; j < nnb; j++) sc.nb_vol[static_cast<size_t>(j) * S + tid] *= inv;
-    return nnb;
+// Step 4 for one boundary face, face fi of cavity tet t
+// (natural_insertionfree.c:681-719): calls sub(slot, vol) per face vertex.
+template <typename SlotOf, typename Sub>
+__device__ __forceinline__ void new_contributions(const DevMesh &m, int t, int fi, const d3 &q,
+                                                  SlotOf slot_of, Sub sub) {
+    bool neg;
+    const int4 tv = ld_tet_v(m, t, neg);
+    d3 P[4];
+    P[0] = ld_vert(m, tv.x);
+    P[1] = ld_vert(m, tv.y);
+    P[2] = ld_vert(m, tv.z);
+    P[3] = ld_vert(m, tv.w);
+    const int s[3] = {face_slot(fi, 0), face_slot(fi, 1), face_slot(fi, 2)};
+    d3 F[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        F[k] = s[k] == 0 ? P[0] : (s[k] == 1 ? P[1] : (s[k] == 2 ? P[2] : P[3]));
+    const d3 cv = cc_tet(q, F[0], F[1], F[2]);
+    const d3 cf = face_cc_own(P, 3 - fi);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int slot = slot_of(s[k], sel4(tv, s[k]));
+        if (slot < 0) continue;
+        const d3 &A = F[(k + 1) % 3];
+        const d3 &B = F[(k + 2) % 3];
+        const d3 ca = cc_tri(F[k], q, A);
+        const d3 cb = cc_tri(F[k], q, B);
+        sub(slot, subcell_volume(F[k], q, A, B, cv, ca, cb, cf));
+    }
 }
 
 __device__ __forceinline__ int wave_excl_scan(int v, int &total) {
@@ -445,55 +478,143 @@ __device__ __forceinline__ int wave_excl_scan(int v, int &total) {
     return x - v;
 }
 
-// Persistent row kernel: each wave takes 64 consecutive work items at a time
-// from a global counter, computes one row per lane, then reserves one
-// contiguous pool range for the wave's rows.
+// Reserves one pool range for the wave's rows; returns this lane's offset.
+__device__ __forceinline__ long long wave_reserve(unsigned long long *counter, int cnt) {
+    int total;
+    const int pre = wave_excl_scan(cnt, total);
+    unsigned long long base = 0;
+    if (__lane_id() == 0 && total > 0)
+        base = atomicAdd(counter, static_cast<unsigned long long>(total));
+    base = __shfl(base, 0, 64);
+    return static_cast<long long>(base) + pre;
+}
+
+// The wave's next 64 work items; returns false when the list is exhausted.
+__device__ __forceinline__ bool next_batch(unsigned int *work, int n_items, int &item) {
+    unsigned base = 0;
+    if (__lane_id() == 0) base = atomicAdd(work, 64u);
+    base = __shfl(base, 0, 64);
+    item = static_cast<int>(base) + __lane_id();
+    return base < static_cast<unsigned>(n_items);
+}
+
+// --------------------------------------------------------- fused row kernel
+
+// One query with this level's capacities: the whole of
+// getWeightsSingleQueryIF (natural_insertionfree.c:591-738).  Returns the
+// row length (row in nb_idx / nb_vol) or -1 when a capacity was exceeded.
+__device__ int fused_row(const DevMesh &m, const Caps &caps, const Scratch &sc, int tid,
+                         int stamp, const d3 &q) {
+    const size_t S = static_cast<size_t>(sc.nthr);
+    const int N = m.n_data;
+    bool ovf = false;
+    const int start = locate(m, q, sc, tid, caps, ovf);
+    if (ovf) return -1;
+    if (start < 0) return 0;  // outside the super simplex: empty row
+    const HashTab vis(sc.vis, sc.vis_pos, tid, caps.vis_log2, stamp);
+    const int ncav = cavity_search(m, caps, sc, tid, vis, start, q);
+    if (ncav < 0) return -1;
+    if (ncav == 0) {
+        int coin[4];
+        const int nco = coincident(m, start, q, coin);
+        if (nco > 0) {
+            const double w = 1.0 / nco;
+            for (int i = 0; i < nco; i++) {
+                sc.nb_idx[il(i, S, tid)] = coin[i];
+                sc.nb_vol[il(i, S, tid)] = w;
+            }
+        }
+        return nco;
+    }
+    // boundary faces and neighbour order (natural_insertionfree.c:559-584)
+    const HashTab nbm(sc.nbmap, sc.nbslot, tid, caps.nbmap_log2, stamp);
+    int nnb = 0, nbnd = 0;
+    for (int ci = 0; ci < ncav; ci++) {
+        const int t = sc.cav[il(ci, S, tid)];
+        const int4 tn = m.tet_n[t];
+        bool neg;
+        const int4 tv = ld_tet_v(m, t, neg);
+        for (int fi = 0; fi < 4; fi++) {
+            const int nb = sel4(tn, fi);
+            if (nb >= 0 && vis.get(nb) >= 0) continue;
+            if (nbnd >= caps.bnd) return -1;
+            sc.bnd[il(nbnd, S, tid)] = (ci << 2) | fi;
+            nbnd++;
+            for (int k = 0; k < 3; k++) {
+                const int v = sel4(tv, face_slot(fi, k));
+                if (v >= N) continue;
+                bool found;
+                const unsigned h = nbm.insert(v, found);
+                if (found) continue;
+                if (nnb >= caps.nb) return -1;
+                nbm.set(h, nnb);
+                sc.nb_idx[il(nnb, S, tid)] = v;
+                sc.nb_vol[il(nnb, S, tid)] = 0.0;
+                nnb++;
+            }
+        }
+    }
+    if (nnb == 0) return 0;
+    auto slot_of = [&](int, int v) { return v < N ? nbm.get(v) : -1; };
+    auto add = [&](int slot, double vol) { sc.nb_vol[il(slot, S, tid)] += vol; };
+    auto sub = [&](int slot, double vol) { sc.nb_vol[il(slot, S, tid)] -= vol; };
+    for (int ci = 0; ci < ncav; ci++) {
+        const int t = sc.cav[il(ci, S, tid)];
+        const int4 tn = m.tet_n[t];
+        unsigned mask = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int nb = sel4(tn, 3 - k);
+            if (nb < 0) continue;
+            const int cj = vis.get(nb);
+            if (cj >= 0 && cj < ci) mask |= 1u << k;
+        }
+        old_contributions(m, t, mask, slot_of, add);
+    }
+    for (int bi = 0; bi < nbnd; bi++) {
+        const int code = sc.bnd[il(bi, S, tid)];
+        new_contributions(m, sc.cav[il(code >> 2, S, tid)], code & 3, q, slot_of, sub);
+    }
+    // step 5, natural_insertionfree.c:721-737
+    double total = 0.0;
+    for (int j = 0; j < nnb; j++) total += sc.nb_vol[il(j, S, tid)];
+    if (!(total > 1e-30)) return 0;
+    const double inv = 1.0 / total;
+    for (int j = 0; j < nnb; j++) sc.nb_vol[il(j, S, tid)] *= inv;
+    return nnb;
+}
+
 __global__ void __launch_bounds__(256) sibson_rows_kernel(DevMesh m, Caps caps, Scratch sc,
                                                           const double *__restrict__ q,
                                                           const int *__restrict__ items,
                                                           int n_items, unsigned int *work,
                                                           RowOut out) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    // nthr is a multiple of 64, so whole waves leave here
-    if (tid >= sc.nthr) return;
-    const int lane = __lane_id();
+    if (tid >= sc.nthr) return;  // nthr is a multiple of 64: whole waves leave
     const size_t S = static_cast<size_t>(sc.nthr);
     int epoch = sc.epoch[tid];
-    for (;;) {
-        unsigned base = 0;
-        if (lane == 0) base = atomicAdd(work, 64u);
-        base = __shfl(base, 0, 64);
-        if (base >= static_cast<unsigned>(n_items)) break;
-        const int item = static_cast<int>(base) + lane;
-        int qi = -1;
-        int cnt = 0;
+    int item;
+    while (next_batch(work, n_items, item)) {
+        int qi = -1, cnt = 0;
         if (item < n_items) {
             qi = items ? items[item] : item;
-            epoch++;
-            if (epoch == 0x7fffffff) epoch = 1;
+            epoch = epoch == 0x7ffffffe ? 1 : epoch + 1;
             const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                            q[3 * static_cast<size_t>(qi) + 2]};
-            cnt = sibson_row(m, caps, sc, tid, epoch, qq);
+            cnt = fused_row(m, caps, sc, tid, epoch, qq);
             if (cnt < 0) {
-                const unsigned k = atomicAdd(out.ovf_count, 1u);
-                out.ovf_list[k] = qi;
+                out.ovf_list[atomicAdd(out.ovf_count, 1u)] = qi;
                 cnt = 0;
             }
         }
-        int total;
-        const int pre = wave_excl_scan(cnt, total);
-        unsigned long long wbase = 0;
-        if (lane == 0 && total > 0)
-            wbase = atomicAdd(out.pool_used, static_cast<unsigned long long>(total));
-        wbase = __shfl(wbase, 0, 64);
+        const long long off = wave_reserve(out.pool_used, cnt);
         if (qi >= 0) {
-            const long long off = static_cast<long long>(wbase) + pre;
             out.row_off[qi] = off;
             out.row_cnt[qi] = cnt;
             if (off + cnt <= out.pool_cap) {
                 for (int j = 0; j < cnt; j++) {
-                    out.pool_cols[off + j] = sc.nb_idx[static_cast<size_t>(j) * S + tid];
-                    out.pool_vals[off + j] = sc.nb_vol[static_cast<size_t>(j) * S + tid];
+                    out.pool_cols[off + j] = sc.nb_idx[il(j, S, tid)];
+                    out.pool_vals[off + j] = sc.nb_vol[il(j, S, tid)];
                 }
             }
         }
@@ -501,10 +622,224 @@ __global__ void __launch_bounds__(256) sibson_rows_kernel(DevMesh m, Caps caps, 
     sc.epoch[tid] = epoch;
 }
 
+// ----------------------------------------------------------- topology kernel
+
+// Record of one query, in 16-byte units at topo.rec[rec_off[qi]]:
+//   [0]            (C, B, NB, 0): cavity tets, boundary faces, neighbours
+//   [1 .. C]       per cavity tet: (tet, slot(p0) | slot(p1) << 16,
+//                  slot(p2) | slot(p3) << 16, fcmask) -- slot 0xffff = none;
+//                  fcmask bit k: face opposite p[k] is held by an earlier
+//                  cavity tet (its circumcentre uses that tet's vertex order)
+//   next ceil(B/4) boundary faces, cavity position << 2 | face, 4 per unit
+//   next ceil(NB/4) neighbour vertex indices in slot order, 4 per unit
+// C = 0 rows: NB = vertices coinciding with the query (weight 1/NB each),
+// or NB = 0 for an empty row.
+__global__ void __launch_bounds__(256) topo_kernel(DevMesh m, Caps caps, Scratch sc,
+                                                   const double *__restrict__ q,
+                                                   const int *__restrict__ items, int n_items,
+                                                   unsigned int *work, Topo topo,
+                                                   unsigned int *ovf_count, int *ovf_list) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= sc.nthr) return;
+    const size_t S = static_cast<size_t>(sc.nthr);
+    const int N = m.n_data;
+    int epoch = sc.epoch[tid];
+    int item;
+    while (next_batch(work, n_items, item)) {
+        int qi = -1, ncav = 0, nbnd = 0, nnb = 0, coin_n = 0;
+        int coin[4] = {0, 0, 0, 0};
+        bool ok = false;
+        d3 qq = {0, 0, 0};
+        if (item < n_items) {
+            qi = items ? items[item] : item;
+            epoch = epoch == 0x7ffffffe ? 1 : epoch + 1;
+            qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
+                  q[3 * static_cast<size_t>(qi) + 2]};
+            bool ovf = false;
+            const int start = locate(m, qq, sc, tid, caps, ovf);
+            ok = !ovf;
+            if (ok && start >= 0) {
+                const HashTab vis(sc.vis, sc.vis_pos, tid, caps.vis_log2, epoch);
+                ncav = cavity_search(m, caps, sc, tid, vis, start, qq);
+                if (ncav < 0) {
+                    ok = false;
+                } else if (ncav == 0) {
+                    coin_n = coincident(m, start, qq, coin);
+                    nnb = coin_n;
+                } else {
+                    const HashTab nbm(sc.nbmap, sc.nbslot, tid, caps.nbmap_log2, epoch);
+                    for (int ci = 0; ci < ncav && ok; ci++) {
+                        const int t = sc.cav[il(ci, S, tid)];
+                        const int4 tn = m.tet_n[t];
+                        bool neg;
+                        const int4 tv = ld_tet_v(m, t, neg);
+                        for (int fi = 0; fi < 4; fi++) {
+                            const int nb = sel4(tn, fi);
+                            const int cj = nb >= 0 ? vis.get(nb) : -1;
+                            if (cj >= 0) continue;
+                            if (nbnd >= caps.bnd) {
+                                ok = false;
+                                break;
+                            }
+                            sc.bnd[il(nbnd, S, tid)] = (ci << 2) | fi;
+                            nbnd++;
+                            for (int k = 0; k < 3; k++) {
+                                const int v = sel4(tv, face_slot(fi, k));
+                                if (v >= N) continue;
+                                bool found;
+                                const unsigned h = nbm.insert(v, found);
+                                if (found) continue;
+                                if (nnb >= caps.nb) {
+                                    ok = false;
+                                    break;
+                                }
+                                nbm.set(h, nnb);
+                                sc.nb_idx[il(nnb, S, tid)] = v;
+                                nnb++;
+                            }
+                            if (!ok) break;
+                        }
+                    }
+                    if (ok && nnb == 0) ncav = nbnd = 0;  // no data vertex: empty row
+                }
+            }
+            if (!ok) ovf_list[atomicAdd(ovf_count, 1u)] = qi;
+        }
+        int units = 0;
+        if (qi >= 0 && ok) units = 1 + ncav + (nbnd + 3) / 4 + (nnb + 3) / 4;
+        const long long off = wave_reserve(topo.rec_used, units);
+        if (qi < 0) continue;
+        if (!ok || off + units > topo.rec_cap) {
+            topo.rec_off[qi] = -1;
+            continue;
+        }
+        topo.rec_off[qi] = off;
+        int4 *r = topo.rec + off;
+        r[0] = make_int4(ncav, nbnd, nnb, 0);
+        if (ncav == 0) {
+            if (nnb > 0) r[1] = make_int4(coin[0], coin[1], coin[2], coin[3]);
+            continue;
+        }
+        const HashTab vis(sc.vis, sc.vis_pos, tid, caps.vis_log2, epoch);
+        const HashTab nbm(sc.nbmap, sc.nbslot, tid, caps.nbmap_log2, epoch);
+        for (int ci = 0; ci < ncav; ci++) {
+            const int t = sc.cav[il(ci, S, tid)];
+            bool neg;
+            const int4 tv = ld_tet_v(m, t, neg);
+            const int4 tn = m.tet_n[t];
+            unsigned mask = 0;
+            int sl[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int nb = sel4(tn, 3 - k);
+                const int cj = nb >= 0 ? vis.get(nb) : -1;
+                if (cj >= 0 && cj < ci) mask |= 1u << k;
+                const int v = sel4(tv, k);
+                const int s = v < N ? nbm.get(v) : -1;
+                sl[k] = s < 0 ? 0xffff : s;
+            }
+            r[1 + ci] = make_int4(t, sl[0] | (sl[1] << 16), sl[2] | (sl[3] << 16),
+                                  static_cast<int>(mask));
+        }
+        int *rf = reinterpret_cast<int *>(r + 1 + ncav);
+        for (int bi = 0; bi < nbnd; bi++) rf[bi] = sc.bnd[il(bi, S, tid)];
+        int *rn = reinterpret_cast<int *>(r + 1 + ncav + (nbnd + 3) / 4);
+        for (int j = 0; j < nnb; j++) rn[j] = sc.nb_idx[il(j, S, tid)];
+    }
+    sc.epoch[tid] = epoch;
+}
+
+// ----------------------------------------------------------- geometry kernel
+
+__device__ __forceinline__ int rec_slot(const int4 &cr, int vi) {
+    const int w = vi < 2 ? cr.y : cr.z;
+    const int s = (vi & 1) ? ((w >> 16) & 0xffff) : (w & 0xffff);
+    return s == 0xffff ? -1 : s;
+}
+
+__global__ void __launch_bounds__(256) geom_kernel(DevMesh m, double *nb_vol, int nthr,
+                                                   const double *__restrict__ q,
+                                                   const int *__restrict__ items, int n_items,
+                                                   unsigned int *work, Topo topo, RowOut out) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nthr) return;
+    const size_t S = static_cast<size_t>(nthr);
+    int item;
+    while (next_batch(work, n_items, item)) {
+        int qi = -1, cnt = 0, nnb = 0;
+        long long roff = -1;
+        if (item < n_items) {
+            qi = items ? items[item] : item;
+            roff = topo.rec_off[qi];
+        }
+        const int4 *r = roff >= 0 ? topo.rec + roff : nullptr;
+        if (r) {
+            const int4 hd = r[0];
+            const int ncav = hd.x, nbnd = hd.y;
+            nnb = hd.z;
+            if (ncav == 0) {
+                cnt = nnb;  // coincident vertices (or an empty row)
+            } else {
+                const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
+                               q[3 * static_cast<size_t>(qi) + 2]};
+                for (int j = 0; j < nnb; j++) nb_vol[il(j, S, tid)] = 0.0;
+                for (int ci = 0; ci < ncav; ci++) {
+                    const int4 cr = r[1 + ci];
+                    old_contributions(
+                        m, cr.x, static_cast<unsigned>(cr.w),
+                        [&](int vi, int) { return rec_slot(cr, vi); },
+                        [&](int slot, double vol) { nb_vol[il(slot, S, tid)] += vol; });
+                }
+                const int *rf = reinterpret_cast<const int *>(r + 1 + ncav);
+                for (int bi = 0; bi < nbnd; bi++) {
+                    const int code = rf[bi];
+                    const int4 cr = r[1 + (code >> 2)];
+                    new_contributions(
+                        m, cr.x, code & 3, qq, [&](int vi, int) { return rec_slot(cr, vi); },
+                        [&](int slot, double vol) { nb_vol[il(slot, S, tid)] -= vol; });
+                }
+                double total = 0.0;
+                for (int j = 0; j < nnb; j++) total += nb_vol[il(j, S, tid)];
+                if (total > 1e-30) {
+                    const double inv = 1.0 / total;
+                    for (int j = 0; j < nnb; j++) nb_vol[il(j, S, tid)] *= inv;
+                    cnt = nnb;
+                } else {
+                    cnt = 0;
+                }
+            }
+        }
+        const long long off = wave_reserve(out.pool_used, cnt);
+        if (!r) continue;  // another level makes this row
+        out.row_off[qi] = off;
+        out.row_cnt[qi] = cnt;
+        if (cnt == 0 || off + cnt > out.pool_cap) continue;
+        const int4 hd = r[0];
+        if (hd.x == 0) {
+            const int4 co = r[1];
+            const double w = 1.0 / cnt;
+            for (int j = 0; j < cnt; j++) {
+                out.pool_cols[off + j] = sel4(co, j);
+                out.pool_vals[off + j] = w;
+            }
+        } else {
+            const int *rn = reinterpret_cast<const int *>(r + 1 + hd.x + (hd.y + 3) / 4);
+            for (int j = 0; j < cnt; j++) {
+                out.pool_cols[off + j] = rn[j];
+                out.pool_vals[off + j] = nb_vol[il(j, S, tid)];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ host helpers
+
 size_t scratch_bytes_per_thread(const Caps &c) {
+    const size_t vis = static_cast<size_t>(1) << c.vis_log2;
+    const size_t nbm = static_cast<size_t>(1) << c.nbmap_log2;
     return static_cast<size_t>(c.kd) * (4 + 8) + static_cast<size_t>(c.stack) * 4 +
-           (static_cast<size_t>(1) << c.vis_log2) * (8 + 4) + static_cast<size_t>(c.cav) * 4 +
-           static_cast<size_t>(c.bnd) * 4 + static_cast<size_t>(c.nb) * (4 + 8) + 4;
+           vis * (8 + 4) + static_cast<size_t>(c.cav) * 4 + static_cast<size_t>(c.bnd) * 4 +
+           static_cast<size_t>(c.nb) * (4 + 8) + nbm * (8 + 4) + 4;
 }
 
 void scratch_carve(void *base, const Caps &c, int nthr, Scratch *s) {
@@ -517,12 +852,16 @@ void scratch_carve(void *base, const Caps &c, int nthr, Scratch *s) {
         p += bytes;
         return static_cast<void *>(r);
     };
+    const size_t vis = static_cast<size_t>(1) << c.vis_log2;
+    const size_t nbm = static_cast<size_t>(1) << c.nbmap_log2;
     s->kd_saved = static_cast<double *>(take(c.kd, 8));
     s->nb_vol = static_cast<double *>(take(c.nb, 8));
-    s->vis = static_cast<int2 *>(take(static_cast<size_t>(1) << c.vis_log2, 8));
+    s->vis = static_cast<int2 *>(take(vis, 8));
+    s->nbmap = static_cast<int2 *>(take(nbm, 8));
     s->kd_node = static_cast<int *>(take(c.kd, 4));
     s->stack = static_cast<int *>(take(c.stack, 4));
-    s->vis_pos = static_cast<int *>(take(static_cast<size_t>(1) << c.vis_log2, 4));
+    s->vis_pos = static_cast<int *>(take(vis, 4));
+    s->nbslot = static_cast<int *>(take(nbm, 4));
     s->cav = static_cast<int *>(take(c.cav, 4));
     s->bnd = static_cast<int *>(take(c.bnd, 4));
     s->nb_idx = static_cast<int *>(take(c.nb, 4));
@@ -530,13 +869,15 @@ void scratch_carve(void *base, const Caps &c, int nthr, Scratch *s) {
     s->nthr = nthr;
 }
 
-int rows_blocks_per_cu() {
+template <typename K>
+static int blocks_per_cu(K kernel) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sibson_rows_kernel, 256, 0) !=
-        hipSuccess)
-        return 1;
-    return n;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, 256, 0) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
 }
+int rows_blocks_per_cu() { return blocks_per_cu(sibson_rows_kernel); }
+int topo_blocks_per_cu() { return blocks_per_cu(topo_kernel); }
+int geom_blocks_per_cu() { return blocks_per_cu(geom_kernel); }
 
 hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
                        const double *d_q, const int *d_items, int n_items,
@@ -546,276 +887,21 @@ hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, in
     return hipGetLastError();
 }
 
-// --------------------------------------------------------- Morton bucketing
-
-__device__ __forceinline__ unsigned spread3(unsigned v) {
-    v &= 0x3ff;
-    v = (v | (v << 16)) & 0x030000ff;
-    v = (v | (v << 8)) & 0x0300f00f;
-    v = (v | (v << 4)) & 0x030c30c3;
-    v = (v | (v << 2)) & 0x09249249;
-    return v;
-}
-
-__global__ void morton_key_kernel(const double *__restrict__ q, int M, double x0, double y0,
-                                  double z0, double sx, double sy, double sz, int bits,
-                                  int *__restrict__ keys, int *__restrict__ hist) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    const int lim = (1 << bits) - 1;
-    auto cell = [&](double v, double o, double s) {
-        double c = (v - o) * s;
-        int k = (c >= 0.0 && c < 2147483647.0) ? static_cast<int>(c) : 0;  // NaN -> 0
-        if (c >= 2147483647.0) k = lim;
-        return k > lim ? lim : k;
-    };
-    const unsigned ix = cell(q[3 * static_cast<size_t>(i)], x0, sx);
-    const unsigned iy = cell(q[3 * static_cast<size_t>(i) + 1], y0, sy);
-    const unsigned iz = cell(q[3 * static_cast<size_t>(i) + 2], z0, sz);
-    const int key = static_cast<int>(spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2));
-    keys[i] = key;
-    atomicAdd(&hist[key], 1);
-}
-
-__global__ void morton_scatter_kernel(const int *__restrict__ keys, int M,
-                                      const long long *__restrict__ start, int *cursor,
-                                      int *__restrict__ order) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    const int key = keys[i];
-    const int k = atomicAdd(&cursor[key], 1);
-    order[start[key] + k] = i;
-}
-
-hipError_t launch_morton_sort(const double *d_q, int M, const double bbox[6], int bits,
-                              int *d_order, int *d_hist, int *d_keys, long long *d_scan_tmp,
-                              hipStream_t st) {
-    const int nb = 1 << (3 * bits);
-    hipError_t e = hipMemsetAsync(d_hist, 0, sizeof(int) * nb, st);
-    if (e != hipSuccess) return e;
-    auto scale = [&](int d) {
-        const double r = bbox[3 + d] - bbox[d];
-        return r > 0 ? static_cast<double>(1 << bits) / r : 1.0;
-    };
-    hipLaunchKernelGGL(morton_key_kernel, dim3((M + 255) / 256), dim3(256), 0, st, d_q, M,
-                       bbox[0], bbox[1], bbox[2], scale(0), scale(1), scale(2), bits, d_keys,
-                       d_hist);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    long long *starts = d_scan_tmp;  // [nb + 1]
-    long long *tmp = d_scan_tmp + nb + 1;
-    e = launch_scan_counts(d_hist, starts, nb, tmp, st);
-    if (e != hipSuccess) return e;
-    e = hipMemsetAsync(d_hist, 0, sizeof(int) * nb, st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(morton_scatter_kernel, dim3((M + 255) / 256), dim3(256), 0, st, d_keys,
-                       M, starts, d_hist, d_order);
+hipError_t launch_topo(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
+                       const double *d_q, const int *d_items, int n_items,
+                       unsigned int *d_work, const Topo &topo, unsigned int *ovf_count,
+                       int *ovf_list, hipStream_t st) {
+    hipLaunchKernelGGL(topo_kernel, dim3(nblocks), dim3(256), 0, st, m, caps, sc, d_q, d_items,
+                       n_items, d_work, topo, ovf_count, ovf_list);
     return hipGetLastError();
 }
 
-// ---------------------------------------------------------------- scanning
-
-constexpr int kScanBlock = 256;
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanBlock * kScanItems;
-
-__device__ __forceinline__ long long block_excl_scan(long long v, long long &total) {
-    __shared__ long long wsum[kScanBlock / 64];
-    const int lane = __lane_id();
-    const int w = threadIdx.x >> 6;
-    long long x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const long long y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63) wsum[w] = x;
-    __syncthreads();
-    long long off = 0, tot = 0;
-#pragma unroll
-    for (int j = 0; j < kScanBlock / 64; j++) {
-        const long long s = wsum[j];
-        if (j < w) off += s;
-        tot += s;
-    }
-    __syncthreads();
-    total = tot;
-    return off + x - v;
-}
-
-__global__ void scan_tiles_kernel(const int *__restrict__ in, int n, long long *__restrict__ tsum) {
-    const int base = blockIdx.x * kScanTile;
-    long long s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; k++) {
-        const int i = base + k * kScanBlock + threadIdx.x;
-        if (i < n) s += in[i];
-    }
-    long long total;
-    block_excl_scan(s, total);
-    if (threadIdx.x == 0) tsum[blockIdx.x] = total;
-}
-
-__global__ void scan_sums_kernel(long long *tsum, int ntiles, long long *grand) {
-    long long carry = 0;
-    for (int b = 0; b < ntiles; b += kScanBlock) {
-        const int i = b + threadIdx.x;
-        const long long v = i < ntiles ? tsum[i] : 0;
-        long long total;
-        const long long e = block_excl_scan(v, total);
-        if (i < ntiles) tsum[i] = carry + e;
-        carry += total;
-    }
-    if (threadIdx.x == 0) *grand = carry;
-}
-
-__global__ void scan_apply_kernel(const int *__restrict__ in, int n,
-                                  const long long *__restrict__ tsum, long long *__restrict__ out) {
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    long long v[kScanItems];
-    long long s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; k++) {
-        const int i = base + k;
-        v[k] = i < n ? in[i] : 0;
-        s += v[k];
-    }
-    long long total;
-    long long off = block_excl_scan(s, total) + tsum[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < kScanItems; k++) {
-        const int i = base + k;
-        if (i < n) out[i] = off;
-        off += v[k];
-    }
-}
-
-size_t scan_tmp_elems(int n) { return static_cast<size_t>((n + kScanTile - 1) / kScanTile) + 1; }
-
-hipError_t launch_scan_counts(const int *d_in, long long *d_out, int n, long long *d_tmp,
-                              hipStream_t st) {
-    const int ntiles = (n + kScanTile - 1) / kScanTile;
-    if (ntiles > 0) {
-        hipLaunchKernelGGL(scan_tiles_kernel, dim3(ntiles), dim3(kScanBlock), 0, st, d_in, n,
-                           d_tmp);
-    }
-    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, st, d_tmp, ntiles,
-                       d_out + n);
-    if (ntiles > 0) {
-        hipLaunchKernelGGL(scan_apply_kernel, dim3(ntiles), dim3(kScanBlock), 0, st, d_in, n,
-                           d_tmp, d_out);
-    }
+hipError_t launch_geom(const DevMesh &m, const Scratch &sc, int nblocks, const double *d_q,
+                       const int *d_items, int n_items, unsigned int *d_work,
+                       const Topo &topo, const RowOut &out, hipStream_t st) {
+    hipLaunchKernelGGL(geom_kernel, dim3(nblocks), dim3(256), 0, st, m, sc.nb_vol, sc.nthr, d_q,
+                       d_items, n_items, d_work, topo, out);
     return hipGetLastError();
-}
-
-// ------------------------------------------------------------ CSR gather
-
-__global__ void gather_csr_kernel(const long long *__restrict__ rowptr64,
-                                  const long long *__restrict__ row_off,
-                                  const int *__restrict__ row_cnt,
-                                  const double *__restrict__ pv, const int *__restrict__ pc,
-                                  int M, double *__restrict__ vals, int *__restrict__ cols,
-                                  int *__restrict__ rowptr32) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > M) return;
-    rowptr32[i] = static_cast<int>(rowptr64[i]);
-    if (i == M) return;
-    const long long dst = rowptr64[i], src = row_off[i];
-    const int c = row_cnt[i];
-    for (int j = 0; j < c; j++) {
-        vals[dst + j] = pv[src + j];
-        cols[dst + j] = pc[src + j];
-    }
-}
-
-hipError_t launch_gather_csr(const long long *d_rowptr64, const long long *d_row_off,
-                             const int *d_row_cnt, const double *pool_vals,
-                             const int *pool_cols, int M, double *d_vals, int *d_cols,
-                             int *d_rowptr32, hipStream_t st) {
-    hipLaunchKernelGGL(gather_csr_kernel, dim3((M + 1 + 255) / 256), dim3(256), 0, st,
-                       d_rowptr64, d_row_off, d_row_cnt, pool_vals, pool_cols, M, d_vals,
-                       d_cols, d_rowptr32);
-    return hipGetLastError();
-}
-
-// ------------------------------------------------------ row canonicalisation
-
-// Insertion sort of each row by column (rows are short; columns are unique).
-template <typename TV>
-__global__ void sort_rows_kernel(int M, const int *__restrict__ rowptr, int *__restrict__ cols,
-                                 TV *__restrict__ vals) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    const int b = rowptr[i], e = rowptr[i + 1];
-    for (int j = b + 1; j < e; j++) {
-        const int c = cols[j];
-        const TV v = vals[j];
-        int k = j - 1;
-        while (k >= b && cols[k] > c) {
-            cols[k + 1] = cols[k];
-            vals[k + 1] = vals[k];
-            k--;
-        }
-        cols[k + 1] = c;
-        vals[k + 1] = v;
-    }
-}
-
-hipError_t launch_sort_rows(int M, const int *rowptr, int *cols, void *vals, int vdt,
-                            hipStream_t st) {
-    if (M <= 0) return hipSuccess;
-    if (vdt == 0)
-        hipLaunchKernelGGL(sort_rows_kernel<float>, dim3((M + 255) / 256), dim3(256), 0, st, M,
-                           rowptr, cols, static_cast<float *>(vals));
-    else
-        hipLaunchKernelGGL(sort_rows_kernel<double>, dim3((M + 255) / 256), dim3(256), 0, st, M,
-                           rowptr, cols, static_cast<double *>(vals));
-    return hipGetLastError();
-}
-
-// ------------------------------------------------------------------- SpMM
-
-// y[i,d] = sum over the row's entries in stored order of w * x[col,d],
-// starting from zero, in the output type (scipy csr_matvec(s)).
-template <typename TW, typename TX, typename TY>
-__global__ void spmm_kernel(int M, int D, const int *__restrict__ rowptr,
-                            const int *__restrict__ cols, const TW *__restrict__ w,
-                            const TX *__restrict__ x, TY *__restrict__ y) {
-    const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gid >= static_cast<long long>(M) * D) return;
-    const int i = static_cast<int>(gid / D), d = static_cast<int>(gid % D);
-    TY acc = 0;
-    const int b = rowptr[i], e = rowptr[i + 1];
-    for (int j = b; j < e; j++) {
-        const TY a = static_cast<TY>(w[j]);
-        const TY v = static_cast<TY>(x[static_cast<size_t>(cols[j]) * D + d]);
-        acc += a * v;
-    }
-    y[gid] = acc;
-}
-
-template <typename TW, typename TX, typename TY>
-static hipError_t spmm_t(int M, int D, const int *rp, const int *c, const void *w, const void *x,
-                         void *y, hipStream_t st) {
-    const long long n = static_cast<long long>(M) * D;
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL((spmm_kernel<TW, TX, TY>), dim3(static_cast<unsigned>((n + 255) / 256)),
-                       dim3(256), 0, st, M, D, rp, c, static_cast<const TW *>(w),
-                       static_cast<const TX *>(x), static_cast<TY *>(y));
-    return hipGetLastError();
-}
-
-hipError_t launch_spmm(int M, int D, const int *rp, const int *c, const void *w, int vdt,
-                       const void *x, int xdt, void *y, int ydt, hipStream_t st) {
-    const int code = vdt * 4 + xdt * 2 + ydt;
-    switch (code) {
-        case 0: return spmm_t<float, float, float>(M, D, rp, c, w, x, y, st);
-        case 1: return spmm_t<float, float, double>(M, D, rp, c, w, x, y, st);
-        case 3: return spmm_t<float, double, double>(M, D, rp, c, w, x, y, st);
-        case 5: return spmm_t<double, float, double>(M, D, rp, c, w, x, y, st);
-        case 7: return spmm_t<double, double, double>(M, D, rp, c, w, x, y, st);
-        default: return hipErrorInvalidValue;
-    }
 }
 
 }  // namespace nat
