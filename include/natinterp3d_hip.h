@@ -113,7 +113,8 @@ int nat_csr_sort_rows(int num_rows, const int32_t *d_rowptr, int32_t *d_cols, vo
  * reference counterpart; for measurement).  Off by default. */
 typedef struct {
     double order_ms;          /* Morton bucketing of the queries */
-    double topo_ms;           /* level 0: start tets, cavities, boundaries */
+    double locate_ms;         /* level 0: start tets */
+    double topo_ms;           /* level 0: cavities, boundaries, neighbour order */
     double geom_ms;           /* level 0: circumcentres, volumes, rows */
     double rows_ms[4];        /* per capacity level ([0] = topo + geom) */
     int32_t rows_items[4];    /* queries processed per level */

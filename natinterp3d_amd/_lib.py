@@ -20,7 +20,8 @@ EXPORTS = (
 
 class Timings(ctypes.Structure):
     _fields_ = [
-        ("order_ms", ctypes.c_double), ("topo_ms", ctypes.c_double),
+        ("order_ms", ctypes.c_double), ("locate_ms", ctypes.c_double),
+        ("topo_ms", ctypes.c_double),
         ("geom_ms", ctypes.c_double), ("rows_ms", ctypes.c_double * 4),
         ("rows_items", ctypes.c_int32 * 4), ("levels", ctypes.c_int32),
         ("emit_ms", ctypes.c_double),

@@ -99,9 +99,11 @@ struct nat_mesh {
 
     // workspace
     nat::Caps caps[kLevels];
-    int nthr[kLevels] = {0, 0, 0, 0};  // [0]: topology kernel threads
+    int nthr[kLevels] = {0, 0, 0, 0};  // [0]: locate kernel threads
     int nthr_geom = 0;
+    int topo_blocks = 0;
     DevBuf scratch[kLevels];
+    DevBuf start;                      // start tet per query
     DevBuf geom_vol;                   // geometry kernel: per-thread row accumulators
     DevBuf rec, rec_off;               // topology records
     long long rec_cap = 0;
@@ -134,7 +136,7 @@ struct nat_mesh {
                    row_off.bytes + row_cnt.bytes + rowptr64.bytes + ovf[0].bytes +
                    ovf[1].bytes + counters.bytes + pool_vals.bytes + pool_cols.bytes +
                    csr_vals.bytes + csr_cols.bytes + csr_rowptr.bytes + geom_vol.bytes +
-                   rec.bytes + rec_off.bytes;
+                   rec.bytes + rec_off.bytes + start.bytes;
         for (int l = 0; l < kLevels; l++) s += scratch[l].bytes;
         return static_cast<int64_t>(s);
     }
@@ -142,7 +144,7 @@ struct nat_mesh {
         DevBuf *all[] = {&verts, &tet_v, &tet_n, &kd, &cell_start, &cell_pts, &q, &order,
                          &keys, &hist, &scan, &row_off, &row_cnt, &rowptr64, &ovf[0],
                          &ovf[1], &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,
-                         &csr_rowptr, &geom_vol, &rec, &rec_off};
+                         &csr_rowptr, &geom_vol, &rec, &rec_off, &start};
         for (DevBuf *b : all) b->release();
         for (int l = 0; l < kLevels; l++) scratch[l].release();
         for (hipEvent_t &e : ev) {
@@ -168,11 +170,13 @@ double elapsed(nat_mesh *m, int a, int b) {
 void set_levels(nat_mesh *m) {
     const int kd = m->host.kd_depth + 2;
     const int cus = std::max(1, m->num_cus);
-    // level 0 (topology + geometry kernels) sees every query; its capacities
-    // cover all but a handful of rows of uniform data.  Persistent grids:
-    // exactly the blocks that can be resident at once.
-    m->caps[0] = {kd, 128, 8, 64, 160, 96, 8};
-    m->nthr[0] = cus * nat::topo_blocks_per_cu() * 256;
+    // level 0 (locate, topology and geometry kernels) sees every query; the
+    // topology kernel's LDS capacities cover all but a handful of rows of
+    // uniform data.  Level-0 scratch only holds the k-d fallback stacks of
+    // the locate kernel.  Persistent grids: exactly the resident blocks.
+    m->caps[0] = {kd, 1, 1, 1, 1, 1, 1};
+    m->nthr[0] = cus * nat::locate_blocks_per_cu() * 256;
+    m->topo_blocks = cus * nat::topo_blocks_per_cu();
     m->nthr_geom = cus * nat::geom_blocks_per_cu() * 256;
     // larger levels (fused kernel) only see the queries that overflowed
     m->caps[1] = {kd, 2048, 12, 1024, 2560, 1536, 12};
@@ -372,7 +376,9 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
     m->times.order_ms = elapsed(m, 0, 1);
 
     HIP_OR_FAIL(m->rec_off.ensure(sizeof(long long) * Mz));
-    HIP_OR_FAIL(m->geom_vol.ensure(sizeof(double) * static_cast<size_t>(m->caps[0].nb) *
+    HIP_OR_FAIL(m->start.ensure(sizeof(int) * Mz));
+    // the geometry kernel accumulates up to kLevel0Row slots per thread
+    HIP_OR_FAIL(m->geom_vol.ensure(sizeof(double) * static_cast<size_t>(nat::kLevel0Row) *
                                    static_cast<size_t>(std::max(m->nthr_geom, 64))));
     long long want = static_cast<long long>(std::ceil(m->nnz_per_row * M)) + 1024;
     long long want_rec = static_cast<long long>(std::ceil(m->units_per_row * M)) + 1024;
@@ -401,7 +407,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
         out.row_off = m->row_off.as<long long>();
         out.row_cnt = m->row_cnt.as<int>();
 
-        // level 0: topology records, then rows from them
+        // level 0: start tets, topology records, then rows from them
         nat::Scratch sc0;
         nat::scratch_carve(m->scratch[0].p, m->caps[0], m->nthr[0], &sc0);
         nat::Topo topo;
@@ -410,8 +416,13 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
         topo.rec_used = rec_used;
         topo.rec_off = m->rec_off.as<long long>();
         mark(m, 2, st);
-        HIP_OR_FAIL(nat::launch_topo(m->dev, m->caps[0], sc0, m->nthr[0] / 256, d_q,
-                                     m->order.as<int>(), M, ctr + 4, topo, ctr + 8,
+        HIP_OR_FAIL(nat::launch_locate(m->dev, m->caps[0], sc0, m->nthr[0] / 256, d_q,
+                                       m->order.as<int>(), M, m->start.as<int>(), st));
+        mark(m, 3, st);
+        m->times.locate_ms = elapsed(m, 2, 3);
+        mark(m, 2, st);
+        HIP_OR_FAIL(nat::launch_topo(m->dev, m->topo_blocks, d_q, m->order.as<int>(), M,
+                                     m->start.as<int>(), ctr + 4, topo, ctr + 8,
                                      m->ovf[0].as<int>(), st));
         mark(m, 3, st);
         unsigned long long meta[2] = {0, 0};

@@ -29,6 +29,10 @@ struct DevMesh {
     int gn[3];                // cells per axis
 };
 
+// Longest row the level-0 kernels handle (topology LDS neighbour list and
+// the geometry kernel's per-thread accumulators); longer rows go to level 1.
+constexpr int kLevel0Row = 96;
+
 // Per-thread capacities.  A query that needs more marks itself for the next,
 // larger level.
 struct Caps {
@@ -88,17 +92,21 @@ hipError_t launch_morton_sort(const double *d_q, int M, const double bbox[6], in
                               int *d_order, int *d_hist, int *d_keys,
                               long long *d_scan_tmp, hipStream_t st);
 int rows_blocks_per_cu();  // resident 256-thread blocks of the fused row kernel per CU
-int topo_blocks_per_cu();
+int topo_blocks_per_cu();   // 128-thread blocks of the topology kernel
 int geom_blocks_per_cu();
 // fused row kernel: used for the larger capacity levels
 hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
                        const double *d_q, const int *d_items, int n_items,
                        unsigned int *d_work, const RowOut &out, hipStream_t st);
-// level 0: topology records, then rows from them
-hipError_t launch_topo(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
-                       const double *d_q, const int *d_items, int n_items,
-                       unsigned int *d_work, const Topo &topo, unsigned int *ovf_count,
-                       int *ovf_list, hipStream_t st);
+// level 0: start tets, topology records, then rows from them
+int locate_blocks_per_cu();  // 256-thread blocks
+hipError_t launch_locate(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
+                         const double *d_q, const int *d_items, int n_items, int *d_start,
+                         hipStream_t st);
+// topology: 128-thread blocks of 8 queries (16 lanes each)
+hipError_t launch_topo(const DevMesh &m, int nblocks, const double *d_q, const int *d_items,
+                       int n_items, const int *d_start, unsigned int *d_work, const Topo &topo,
+                       unsigned int *ovf_count, int *ovf_list, hipStream_t st);
 hipError_t launch_geom(const DevMesh &m, const Scratch &sc, int nblocks, const double *d_q,
                        const int *d_items, int n_items, unsigned int *d_work,
                        const Topo &topo, const RowOut &out, hipStream_t st);

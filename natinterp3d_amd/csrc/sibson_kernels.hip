@@ -1,10 +1,11 @@
 // sibson_kernels.hip -- gfx950 kernels of the Sibson row computation.
 //
 // Level 0 (every query) runs as two kernels:
-//   topo_kernel   one thread per query: start tet (exact nearest centroid,
-//                 then the visibility walk), cavity search in the reference's
-//                 LIFO order, boundary faces and neighbour first-appearance
-//                 order; writes a compact record per query
+//   locate_kernel      one thread per query: start tet (exact nearest
+//                      centroid, then the visibility walk)  delaunay.c:383-407
+//   topo_group_kernel  16 lanes per query, state in LDS: cavity in the
+//                      reference's LIFO order, boundary faces, neighbour
+//                      first-appearance order; a compact record per query
 //                                             natural_insertionfree.c:486-584
 //   geom_kernel   one thread per query: circumcentres and Voronoi subcell
 //                 volumes from the record (steps 3-5), row into the pool
@@ -634,119 +635,380 @@ __global__ void __launch_bounds__(256) sibson_rows_kernel(DevMesh m, Caps caps, 
 //   next ceil(NB/4) neighbour vertex indices in slot order, 4 per unit
 // C = 0 rows: NB = vertices coinciding with the query (weight 1/NB each),
 // or NB = 0 for an empty row.
-__global__ void __launch_bounds__(256) topo_kernel(DevMesh m, Caps caps, Scratch sc,
-                                                   const double *__restrict__ q,
-                                                   const int *__restrict__ items, int n_items,
-                                                   unsigned int *work, Topo topo,
-                                                   unsigned int *ovf_count, int *ovf_list) {
+// Start tet of every work item: exact nearest centroid (grid; the k-d
+// replica when the grid cannot certify it), then the visibility walk.
+// start[qi] = tet, -1 outside the super simplex, -2 k-d stack overflow.
+__global__ void __launch_bounds__(256) locate_kernel(DevMesh m, Caps caps, Scratch sc,
+                                                     const double *__restrict__ q,
+                                                     const int *__restrict__ items, int n_items,
+                                                     int *__restrict__ start) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= sc.nthr) return;
-    const size_t S = static_cast<size_t>(sc.nthr);
+    for (int i = tid; i < n_items; i += sc.nthr) {
+        const int qi = items ? items[i] : i;
+        const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
+                       q[3 * static_cast<size_t>(qi) + 2]};
+        bool ovf = false;
+        const int s = locate(m, qq, sc, tid, caps, ovf);
+        start[qi] = ovf ? -2 : s;
+    }
+}
+
+// Wave-cooperative topology: 16 lanes per query, 4 queries per wave, all
+// per-query state in LDS.
+namespace grp {
+constexpr int G = 16;         // lanes per query
+constexpr int GPB = 8;        // queries per 128-thread block
+constexpr int VT = 128;       // visited-table slots
+constexpr int VMAX = 96;      // visited tets
+constexpr int CMAX = 64;      // cavity tets
+constexpr int BMAX = 160;     // boundary faces
+constexpr int NT = 128;       // neighbour-table slots
+constexpr int NMAX = kLevel0Row;  // neighbours
+constexpr int SMAX = 256;     // order-replay stack
+}  // namespace grp
+
+struct GroupMem {
+    int vkey[grp::VT];             // tet index, -1 = empty slot
+    unsigned vnb[grp::VT];         // visited slots of the 4 neighbours, 8 bits each (0xff none)
+    unsigned char vflag[grp::VT];  // 1 tested, 2 inside the circumsphere, 4 popped by the replay
+    unsigned char vpos[grp::VT];   // cavity position
+    unsigned char front[2][grp::VMAX];
+    unsigned char stk[grp::SMAX];
+    unsigned char cav[grp::CMAX];  // visited slots in cavity order
+    unsigned short bnd[grp::BMAX]; // boundary faces, cavity position << 2 | face
+    int nkey[grp::NT];             // vertex index, -1 = empty slot
+    int nmin[grp::NT];             // first appearance (face-vertex sequence number)
+    int nslot[grp::NT];            // neighbour slot
+    int nbv[grp::NMAX];            // neighbours in slot order
+    int cnt[4];                    // frontier sizes, visited count, overflow flag
+};
+
+__device__ __forceinline__ void group_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// visited-table insert: slot of t, -1 when the table is full
+__device__ __forceinline__ int vis_insert(GroupMem &g, int t, bool &isnew) {
+    unsigned h = hash_key(t, 7);
+    for (int n = 0; n < grp::VT; n++) {
+        const int old = atomicCAS(&g.vkey[h], -1, t);
+        if (old == -1) {
+            isnew = true;
+            if (atomicAdd(&g.cnt[2], 1) >= grp::VMAX) g.cnt[3] = 1;
+            return static_cast<int>(h);
+        }
+        if (old == t) {
+            isnew = false;
+            return static_cast<int>(h);
+        }
+        h = (h + 1) & (grp::VT - 1);
+    }
+    isnew = false;
+    return -1;
+}
+
+__device__ __forceinline__ int nb_insert(GroupMem &g, int v) {
+    unsigned h = hash_key(v, 7);
+    for (int n = 0; n < grp::NT; n++) {
+        const int old = atomicCAS(&g.nkey[h], -1, v);
+        if (old == -1 || old == v) return static_cast<int>(h);
+        h = (h + 1) & (grp::NT - 1);
+    }
+    return -1;
+}
+
+__device__ __forceinline__ int nb_find(const GroupMem &g, int v) {
+    unsigned h = hash_key(v, 7);
+    for (int n = 0; n < grp::NT; n++) {
+        const int k = g.nkey[h];
+        if (k == v) return static_cast<int>(h);
+        if (k == -1) return -1;
+        h = (h + 1) & (grp::NT - 1);
+    }
+    return -1;
+}
+
+// Vertex of entry e of the face-vertex sequence (boundary face e / 3,
+// vertex e % 3 in getFaceVerticies3 order).
+__device__ __forceinline__ int face_vertex(const DevMesh &m, const GroupMem &g, int e) {
+    const int j = g.bnd[e / 3];
+    const int t = g.vkey[g.cav[j >> 2]];
+    bool neg;
+    const int4 tv = ld_tet_v(m, t, neg);
+    return sel4(tv, face_slot(j & 3, e % 3));
+}
+
+// Record of one query, in 16-byte units at topo.rec[rec_off[qi]]:
+//   [0]            (C, B, NB, 0): cavity tets, boundary faces, neighbours
+//   [1 .. C]       per cavity tet: (tet, slot(p0) | slot(p1) << 16,
+//                  slot(p2) | slot(p3) << 16, fcmask) -- slot 0xffff = none;
+//                  fcmask bit k: face opposite p[k] is held by an earlier
+//                  cavity tet (its circumcentre uses that tet's vertex order)
+//   next ceil(B/4) boundary faces, cavity position << 2 | face, 4 per unit
+//   next ceil(NB/4) neighbour vertex indices in slot order, 4 per unit
+// C = 0 rows: NB = vertices coinciding with the query (weight 1/NB each),
+// or NB = 0 for an empty row.
+//
+// Phases per query (16 lanes):
+//   B  breadth-first expansion: every tet the reference's search pops -- the
+//      start tet and every neighbour of a tet whose circumsphere holds q --
+//      is tested once, the tests of one front in parallel
+//   C  one lane replays the reference's LIFO order over the tested set
+//      (natural_insertionfree.c:498-530) from LDS alone: cavity order
+//   D  boundary faces in (cavity position, face) order by ballot compaction;
+//      neighbour slots by least first appearance, ranked in sequence order
+//      (natural_insertionfree.c:559-584)
+//   E  the record, written by all 16 lanes
+__global__ void __launch_bounds__(128) topo_group_kernel(DevMesh m, const double *__restrict__ q,
+                                                         const int *__restrict__ items,
+                                                         int n_items,
+                                                         const int *__restrict__ start_of,
+                                                         unsigned int *work, Topo topo,
+                                                         unsigned int *ovf_count,
+                                                         int *ovf_list) {
+    __shared__ GroupMem gmem[grp::GPB];
+    const int lane = __lane_id();
+    const int gl = lane & (grp::G - 1);
+    const int gw = lane >> 4;
+    GroupMem &g = gmem[threadIdx.x >> 4];
     const int N = m.n_data;
-    int epoch = sc.epoch[tid];
-    int item;
-    while (next_batch(work, n_items, item)) {
-        int qi = -1, ncav = 0, nbnd = 0, nnb = 0, coin_n = 0;
-        int coin[4] = {0, 0, 0, 0};
-        bool ok = false;
+    for (;;) {
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(work, 4u);
+        base = __shfl(base, 0, 64);
+        if (base >= static_cast<unsigned>(n_items)) break;
+        const int item = static_cast<int>(base) + gw;
+        const bool active = item < n_items;
+        const int qi = active ? items[item] : -1;
+        int st = -1;
         d3 qq = {0, 0, 0};
-        if (item < n_items) {
-            qi = items ? items[item] : item;
-            epoch = epoch == 0x7ffffffe ? 1 : epoch + 1;
+        if (active) {
+            st = start_of[qi];
             qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                   q[3 * static_cast<size_t>(qi) + 2]};
-            bool ovf = false;
-            const int start = locate(m, qq, sc, tid, caps, ovf);
-            ok = !ovf;
-            if (ok && start >= 0) {
-                const HashTab vis(sc.vis, sc.vis_pos, tid, caps.vis_log2, epoch);
-                ncav = cavity_search(m, caps, sc, tid, vis, start, qq);
-                if (ncav < 0) {
-                    ok = false;
-                } else if (ncav == 0) {
-                    coin_n = coincident(m, start, qq, coin);
-                    nnb = coin_n;
-                } else {
-                    const HashTab nbm(sc.nbmap, sc.nbslot, tid, caps.nbmap_log2, epoch);
-                    for (int ci = 0; ci < ncav && ok; ci++) {
-                        const int t = sc.cav[il(ci, S, tid)];
+        }
+        bool ok = st != -2;
+        int ncav = 0, nbnd = 0, nnb = 0;
+        int coin[4] = {0, 0, 0, 0};
+        if (active && st >= 0) {
+            // ---- phase 0: empty tables
+            for (int i = gl; i < grp::VT; i += grp::G) {
+                g.vkey[i] = -1;
+                g.vflag[i] = 0;
+            }
+            for (int i = gl; i < grp::NT; i += grp::G) {
+                g.nkey[i] = -1;
+                g.nmin[i] = 0x7fffffff;
+            }
+            if (gl == 0) g.cnt[0] = g.cnt[1] = g.cnt[2] = g.cnt[3] = 0;
+            group_sync();
+            // ---- phase B: test every tet the search pops, front by front
+            int s0 = 0;
+            if (gl == 0) {
+                bool isnew;
+                s0 = vis_insert(g, st, isnew);
+                g.front[0][0] = static_cast<unsigned char>(s0);
+                g.cnt[0] = 1;
+            }
+            group_sync();
+            int cur = 0;
+            int nf = g.cnt[0];
+            while (nf > 0) {
+                if (gl == 0) g.cnt[cur ^ 1] = 0;
+                group_sync();
+                for (int i = gl; i < nf; i += grp::G) {
+                    const int slot = g.front[cur][i];
+                    const int t = g.vkey[slot];
+                    bool neg;
+                    const int4 tv = ld_tet_v(m, t, neg);
+                    const d3 p0 = ld_vert(m, tv.x), p1 = ld_vert(m, tv.y);
+                    const d3 p2 = ld_vert(m, tv.z), p3 = ld_vert(m, tv.w);
+                    const double ins =
+                        neg ? insphere(p1, p0, p2, p3, qq) : insphere(p0, p1, p2, p3, qq);
+                    unsigned packed = 0xffffffffu;
+                    if (ins > 0) {
                         const int4 tn = m.tet_n[t];
-                        bool neg;
-                        const int4 tv = ld_tet_v(m, t, neg);
-                        for (int fi = 0; fi < 4; fi++) {
-                            const int nb = sel4(tn, fi);
-                            const int cj = nb >= 0 ? vis.get(nb) : -1;
-                            if (cj >= 0) continue;
-                            if (nbnd >= caps.bnd) {
-                                ok = false;
-                                break;
-                            }
-                            sc.bnd[il(nbnd, S, tid)] = (ci << 2) | fi;
-                            nbnd++;
-                            for (int k = 0; k < 3; k++) {
-                                const int v = sel4(tv, face_slot(fi, k));
-                                if (v >= N) continue;
-                                bool found;
-                                const unsigned h = nbm.insert(v, found);
-                                if (found) continue;
-                                if (nnb >= caps.nb) {
-                                    ok = false;
-                                    break;
+                        packed = 0;
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            const int nb = sel4(tn, k);
+                            unsigned ns = 0xff;
+                            if (nb >= 0) {
+                                bool isnew;
+                                const int s2 = vis_insert(g, nb, isnew);
+                                if (s2 < 0) {
+                                    g.cnt[3] = 1;
+                                } else {
+                                    ns = static_cast<unsigned>(s2);
+                                    if (isnew) {
+                                        const int p = atomicAdd(&g.cnt[cur ^ 1], 1);
+                                        if (p < grp::VMAX)
+                                            g.front[cur ^ 1][p] = static_cast<unsigned char>(s2);
+                                        else
+                                            g.cnt[3] = 1;
+                                    }
                                 }
-                                nbm.set(h, nnb);
-                                sc.nb_idx[il(nnb, S, tid)] = v;
-                                nnb++;
                             }
-                            if (!ok) break;
+                            packed |= ns << (8 * k);
                         }
                     }
-                    if (ok && nnb == 0) ncav = nbnd = 0;  // no data vertex: empty row
+                    g.vnb[slot] = packed;
+                    g.vflag[slot] = ins > 0 ? 3 : 1;
+                }
+                group_sync();
+                nf = g.cnt[3] ? 0 : g.cnt[cur ^ 1];
+                cur ^= 1;
+            }
+            ok = g.cnt[3] == 0;
+            // ---- phase C: the reference's cavity order
+            int nc = 0;
+            if (ok && gl == 0) {
+                int top = 0;
+                g.stk[top++] = static_cast<unsigned char>(s0);
+                while (top > 0) {
+                    const int s = g.stk[--top];
+                    const unsigned f = g.vflag[s];
+                    if (f & 4u) continue;
+                    g.vflag[s] = static_cast<unsigned char>(f | 4u);
+                    if (!(f & 2u)) continue;
+                    if (nc >= grp::CMAX) {
+                        nc = -1;
+                        break;
+                    }
+                    g.cav[nc] = static_cast<unsigned char>(s);
+                    g.vpos[s] = static_cast<unsigned char>(nc);
+                    nc++;
+                    const unsigned nbs = g.vnb[s];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const unsigned ns = (nbs >> (8 * k)) & 0xffu;
+                        if (ns == 0xffu || (g.vflag[ns] & 4u)) continue;
+                        if (top >= grp::SMAX) {
+                            nc = -1;
+                            top = 0;
+                            break;
+                        }
+                        g.stk[top++] = static_cast<unsigned char>(ns);
+                    }
+                    if (nc < 0) break;
                 }
             }
-            if (!ok) ovf_list[atomicAdd(ovf_count, 1u)] = qi;
+            nc = __shfl(nc, 0, grp::G);
+            group_sync();
+            if (ok && nc < 0) ok = false;
+            ncav = ok ? nc : 0;
+            if (ok && ncav == 0) {
+                // coincident-vertex path, natural_insertionfree.c:596-622
+                nnb = coincident(m, st, qq, coin);
+            } else if (ok) {
+                // ---- phase D: boundary faces
+                for (int c0 = 0; c0 < 4 * ncav; c0 += grp::G) {
+                    const int j = c0 + gl;
+                    bool isb = false;
+                    if (j < 4 * ncav) {
+                        const unsigned ns = (g.vnb[g.cav[j >> 2]] >> (8 * (j & 3))) & 0xffu;
+                        isb = ns == 0xffu || !(g.vflag[ns] & 2u);
+                    }
+                    const unsigned mine =
+                        static_cast<unsigned>(__ballot(isb) >> (16 * gw)) & 0xffffu;
+                    if (isb) {
+                        const int p = nbnd + __popc(mine & ((1u << gl) - 1u));
+                        if (p < grp::BMAX) g.bnd[p] = static_cast<unsigned short>(j);
+                    }
+                    nbnd += __popc(mine);
+                }
+                if (nbnd > grp::BMAX) ok = false;
+                group_sync();
+                // neighbours: least sequence number of each data vertex
+                const int nent = ok ? 3 * nbnd : 0;
+                bool full = false;
+                for (int e = gl; e < nent; e += grp::G) {
+                    const int v = face_vertex(m, g, e);
+                    if (v >= N) continue;
+                    const int h = nb_insert(g, v);
+                    if (h < 0) {
+                        full = true;
+                        continue;
+                    }
+                    atomicMin(&g.nmin[h], e);
+                }
+                if (__ballot(full) >> (16 * gw) & 0xffffull) ok = false;
+                group_sync();
+                // slots in order of first appearance
+                for (int c0 = 0; c0 < nent && ok; c0 += grp::G) {
+                    const int e = c0 + gl;
+                    bool first = false;
+                    int v = -1, h = -1;
+                    if (e < nent) {
+                        v = face_vertex(m, g, e);
+                        if (v < N) {
+                            h = nb_find(g, v);
+                            first = h >= 0 && g.nmin[h] == e;
+                        }
+                    }
+                    const unsigned mine =
+                        static_cast<unsigned>(__ballot(first) >> (16 * gw)) & 0xffffu;
+                    if (first) {
+                        const int slot = nnb + __popc(mine & ((1u << gl) - 1u));
+                        g.nslot[h] = slot;
+                        if (slot < grp::NMAX) g.nbv[slot] = v;
+                    }
+                    nnb += __popc(mine);
+                }
+                if (nnb > grp::NMAX) ok = false;
+                if (ok && nnb == 0) ncav = nbnd = 0;  // no data vertex: empty row
+                group_sync();
+            }
         }
+        // ---- phase E: reserve and write the record
         int units = 0;
-        if (qi >= 0 && ok) units = 1 + ncav + (nbnd + 3) / 4 + (nnb + 3) / 4;
-        const long long off = wave_reserve(topo.rec_used, units);
-        if (qi < 0) continue;
+        if (active && ok) units = 1 + ncav + (nbnd + 3) / 4 + (nnb + 3) / 4;
+        const int lead_units = gl == 0 ? units : 0;
+        const long long off0 = wave_reserve(topo.rec_used, lead_units);
+        const long long off = __shfl(off0, 0, grp::G);
+        if (!active) continue;
         if (!ok || off + units > topo.rec_cap) {
-            topo.rec_off[qi] = -1;
+            if (gl == 0) {
+                topo.rec_off[qi] = -1;
+                if (!ok) ovf_list[atomicAdd(ovf_count, 1u)] = qi;
+            }
             continue;
         }
-        topo.rec_off[qi] = off;
         int4 *r = topo.rec + off;
-        r[0] = make_int4(ncav, nbnd, nnb, 0);
-        if (ncav == 0) {
-            if (nnb > 0) r[1] = make_int4(coin[0], coin[1], coin[2], coin[3]);
-            continue;
+        if (gl == 0) {
+            topo.rec_off[qi] = off;
+            r[0] = make_int4(ncav, nbnd, nnb, 0);
+            if (ncav == 0 && nnb > 0) r[1] = make_int4(coin[0], coin[1], coin[2], coin[3]);
         }
-        const HashTab vis(sc.vis, sc.vis_pos, tid, caps.vis_log2, epoch);
-        const HashTab nbm(sc.nbmap, sc.nbslot, tid, caps.nbmap_log2, epoch);
-        for (int ci = 0; ci < ncav; ci++) {
-            const int t = sc.cav[il(ci, S, tid)];
+        if (ncav == 0) continue;
+        for (int ci = gl; ci < ncav; ci += grp::G) {
+            const int s = g.cav[ci];
+            const int t = g.vkey[s];
             bool neg;
             const int4 tv = ld_tet_v(m, t, neg);
-            const int4 tn = m.tet_n[t];
+            const unsigned nbs = g.vnb[s];
             unsigned mask = 0;
             int sl[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const int nb = sel4(tn, 3 - k);
-                const int cj = nb >= 0 ? vis.get(nb) : -1;
-                if (cj >= 0 && cj < ci) mask |= 1u << k;
+                const unsigned ns = (nbs >> (8 * (3 - k))) & 0xffu;
+                if (ns != 0xffu && (g.vflag[ns] & 2u) && g.vpos[ns] < ci) mask |= 1u << k;
                 const int v = sel4(tv, k);
-                const int s = v < N ? nbm.get(v) : -1;
-                sl[k] = s < 0 ? 0xffff : s;
+                const int h = v < N ? nb_find(g, v) : -1;
+                sl[k] = h < 0 ? 0xffff : g.nslot[h];
             }
             r[1 + ci] = make_int4(t, sl[0] | (sl[1] << 16), sl[2] | (sl[3] << 16),
                                   static_cast<int>(mask));
         }
         int *rf = reinterpret_cast<int *>(r + 1 + ncav);
-        for (int bi = 0; bi < nbnd; bi++) rf[bi] = sc.bnd[il(bi, S, tid)];
+        for (int bi = gl; bi < nbnd; bi += grp::G) rf[bi] = g.bnd[bi];
         int *rn = reinterpret_cast<int *>(r + 1 + ncav + (nbnd + 3) / 4);
-        for (int j = 0; j < nnb; j++) rn[j] = sc.nb_idx[il(j, S, tid)];
+        for (int j = gl; j < nnb; j += grp::G) rn[j] = g.nbv[j];
+        group_sync();
     }
-    sc.epoch[tid] = epoch;
 }
 
 // ----------------------------------------------------------- geometry kernel
@@ -876,7 +1138,14 @@ static int blocks_per_cu(K kernel) {
     return n > 0 ? n : 1;
 }
 int rows_blocks_per_cu() { return blocks_per_cu(sibson_rows_kernel); }
-int topo_blocks_per_cu() { return blocks_per_cu(topo_kernel); }
+int topo_blocks_per_cu() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, topo_group_kernel, 128, 0) !=
+        hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
+}
+int locate_blocks_per_cu() { return blocks_per_cu(locate_kernel); }
 int geom_blocks_per_cu() { return blocks_per_cu(geom_kernel); }
 
 hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
@@ -887,12 +1156,19 @@ hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, in
     return hipGetLastError();
 }
 
-hipError_t launch_topo(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
-                       const double *d_q, const int *d_items, int n_items,
-                       unsigned int *d_work, const Topo &topo, unsigned int *ovf_count,
-                       int *ovf_list, hipStream_t st) {
-    hipLaunchKernelGGL(topo_kernel, dim3(nblocks), dim3(256), 0, st, m, caps, sc, d_q, d_items,
-                       n_items, d_work, topo, ovf_count, ovf_list);
+hipError_t launch_locate(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
+                         const double *d_q, const int *d_items, int n_items, int *d_start,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(locate_kernel, dim3(nblocks), dim3(256), 0, st, m, caps, sc, d_q, d_items,
+                       n_items, d_start);
+    return hipGetLastError();
+}
+
+hipError_t launch_topo(const DevMesh &m, int nblocks, const double *d_q, const int *d_items,
+                       int n_items, const int *d_start, unsigned int *d_work, const Topo &topo,
+                       unsigned int *ovf_count, int *ovf_list, hipStream_t st) {
+    hipLaunchKernelGGL(topo_group_kernel, dim3(nblocks), dim3(128), 0, st, m, d_q, d_items,
+                       n_items, d_start, d_work, topo, ovf_count, ovf_list);
     return hipGetLastError();
 }
 

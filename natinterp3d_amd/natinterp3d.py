@@ -94,7 +94,8 @@ class MeshAndVerticesHIP:
         _lib.check(_lib.lib().nat_mesh_get_timings(self._m, ctypes.byref(t)),
                    "nat_mesh_get_timings")
         n = t.levels
-        return {"order_ms": t.order_ms, "topo_ms": t.topo_ms, "geom_ms": t.geom_ms,
+        return {"order_ms": t.order_ms, "locate_ms": t.locate_ms, "topo_ms": t.topo_ms,
+                "geom_ms": t.geom_ms,
                 "rows_ms": list(t.rows_ms)[:n],
                 "rows_items": list(t.rows_items)[:n], "emit_ms": t.emit_ms}
 
