@@ -1,23 +1,28 @@
 #!/usr/bin/env python3
-"""Sibson query throughput on one MI355X.
+"""Sibson query throughput, forward + backward, on one MI355X.
 
 Workload (the plain-at-scale configuration): 1,000,000 keys (90% uniform in
 [0,1]^3, 10% in 20 isotropic Gaussian blobs of sigma 0.01), 10,000,000
 queries uniform in [0.02,0.98]^3, values float32 [N,3].  The mesh is built
-once on the host (untimed, reported on stderr).  One step is the forward pass
-over all queries with inputs resident in HBM: every weight row computed on
-the GPU, the CSR assembled in device memory, and W @ values.
+once on the host (untimed, reported on stderr).  One step is one forward and
+backward pass of the differentiable interpolation over all queries, inputs
+resident in HBM: every weight row on the GPU (float64, bit-exact CSR in
+device memory), y = W @ values, then dL/dvalues = W^T g and dL/dqueries
+from the cached cavities.  The plain forward (rows + W @ values) is timed
+beside it for the fwd+bwd / fwd ratio.
 
-Prints one JSON line on stdout.  ``roofline`` is the row kernel's achieved
-algorithmic bandwidth (bytes per query from the oracle's work counters of this
-workload, times queries per launch, over the launch time measured with HIP
-events on the launch stream) against MI355X's 8 TB/s.  ``cpu_baseline`` is the
-compiled reference (oracle/_ref) on the host's cores over a bounded sample of
-the same queries.
+Prints one JSON line on stdout.  ``roofline`` is the dominant kernel's
+achieved algorithmic bandwidth (bytes per query from the oracle's work
+counters of this workload, times queries per launch, over the launch time
+measured with HIP events on the launch stream) against MI355X's 8 TB/s.
+``cpu_baseline`` is the compiled reference (oracle/_ref) on the host's cores
+over a bounded sample of the same queries (forward only: the reference has no
+backward).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -46,40 +51,46 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def plain_at_scale(n_keys=1_000_000, n_queries=10_000_000, seed=1):
-    """Keys, queries and values of the plain-at-scale workload (seeded)."""
-    rng = np.random.default_rng(seed)
-    n_u = n_keys * 9 // 10
-    n_b = n_keys - n_u
-    a = rng.random((n_u, 3))
-    centers = rng.random((20, 3))
-    b = centers[rng.integers(0, 20, n_b)] + 0.01 * rng.standard_normal((n_b, 3))
-    keys = np.vstack([a, b])
-    keys = keys[rng.permutation(len(keys))]
-    queries = 0.02 + 0.96 * rng.random((n_queries, 3))
-    values = rng.standard_normal((n_keys, 3)).astype(np.float32)
-    return keys, queries, values
+def record_bytes(units, M):
+    """Mean topology-record bytes per query (16-byte units) of the run."""
+    return 16.0 * units / M
 
 
-def bytes_per_query(w):
-    """Algorithmic HBM bytes of the row kernel per query: the query (24 B),
-    each k-d node entered (centroid + children, 32 B), each tet record read
-    (4 vertex + 4 neighbour indices, 32 B), each distinct vertex read (24 B),
-    the row written (index + weight, 12 B per entry) and its offset/length
-    (12 B)."""
-    return (24 + 32 * w["kd_nodes"] + 32 * w["tets_read"] + 24 * w["verts_read"]
-            + 12 * w["nnz"] + 12)
+def kernel_bytes_per_query(w, rec_q, D):
+    """Algorithmic HBM bytes per query of each level-0 kernel, from the mean
+    work counters w (cavity tets C, insphere-tested tets T, distinct vertices
+    V, row entries nnz) and the mean record size rec_q:
+
+    topo_group_kernel: order + start (8), query (24), tet vertex ids of every
+      tested tet (16 T), their vertices (24 V), neighbour ids of cavity tets
+      (16 C), the record written (rec_q) and its offset (8);
+    geom_kernel: order + record offset (12), query (24), record (rec_q), tet
+      vertex ids of cavity tets (16 C), vertices (24 V), row written (12 nnz)
+      and its offset/length (12);
+    grad_geom_kernel: as geom_kernel without the row, plus dL/dy (8 D), the
+      neighbours' values (8 D nnz) and dL/dq written (24)."""
+    geo = 12 + 24 + rec_q + 16 * w["cavity"] + 24 * w["verts_read"]
+    return {
+        "topo_group_kernel": 8 + 24 + 16 * w["tets_read"] + 24 * w["verts_read"]
+                             + 16 * w["cavity"] + rec_q + 8,
+        "geom_kernel": geo + 12 * w["nnz"] + 12,
+        "grad_geom_kernel": geo + 8 * D + 8 * D * w["nnz"] + 24,
+    }
 
 
-def fp64_ops_per_query(w):
-    """Non-fused FP64 operations per query: k-d distance and rectangle tests
-    (~19 per node), walk orient3d (23), insphere (77), tet circumcentres for
-    cavity tets and virtual tets (79 each), face circumcentres (56 each:
-    distinct faces plus two per new subcell), subcell volumes (89 each),
-    normalisation (2 per entry)."""
-    return (19 * w["kd_nodes"] + 23 * w["walk_tests"] + 77 * w["insphere"]
-            + 79 * (w["cavity"] + w["boundary"]) + 56 * (w["face_cc"] + 2 * w["sub4"])
-            + 89 * (w["sub3"] + w["sub4"]) + 2 * w["nnz"] + 1)
+def kernel_fp64_ops_per_query(w):
+    """Non-fused FP64 operations per query of each level-0 kernel: insphere
+    77; tet circumcentre 79 (cavity and virtual tets); face circumcentre 56
+    (distinct faces plus two per new subcell); subcell volume 89;
+    normalisation 2 per entry; facet moment ~90 per new subcell and the
+    gradient sums ~15 per volume."""
+    geo = (79 * (w["cavity"] + w["boundary"]) + 56 * (w["face_cc"] + 2 * w["sub4"])
+           + 89 * (w["sub3"] + w["sub4"]))
+    return {
+        "topo_group_kernel": 77 * w["insphere"],
+        "geom_kernel": geo + 2 * w["nnz"] + 1,
+        "grad_geom_kernel": geo + 90 * w["sub4"] + 15 * (w["sub3"] + w["sub4"]),
+    }
 
 
 def cpu_threads():
@@ -151,10 +162,12 @@ def main():
     import torch
 
     import natinterp3d_amd
+    from natinterp3d_amd import workloads
+    from natinterp3d_amd.autograd import SibsonInterpolation
     from natinterp3d_amd.natinterp3d import spmm_device
 
     log(f"workload: {args.keys:,} keys, {args.queries:,} queries")
-    keys, queries, values = plain_at_scale(args.keys, args.queries)
+    keys, queries, values = workloads.plain_at_scale(n_keys=args.keys, n_queries=args.queries)
     keys = np.ascontiguousarray(keys, np.float64)
     centroid = keys.mean(axis=0)
     kc = np.ascontiguousarray(keys - centroid)
@@ -176,40 +189,72 @@ def main():
     log(f"mesh: {info['n_tets']:,} tets, k-d depth {info['kd_depth']}, built in {build_s:.1f} s")
 
     dev = torch.device("cuda", 0)
-    qd = torch.from_numpy(qc).to(dev)
-    xd = torch.from_numpy(values).to(dev)
-    M = qc.shape[0]
+    M, D = qc.shape[0], values.shape[1]
+    qd = torch.from_numpy(qc).to(dev).requires_grad_(True)
+    xd = torch.from_numpy(values).to(dev).requires_grad_(True)
+    gy = torch.randn((M, D), dtype=torch.float64, device=dev,
+                     generator=torch.Generator(device=dev).manual_seed(5))
     mesh.set_timing(True)
 
     def step():
-        crow, col, val = mesh.weights_device(qd)
-        return spmm_device(crow, col, val, xd, M), col.numel()
+        """one forward + backward pass over all queries"""
+        qd.grad = None
+        xd.grad = None
+        y = SibsonInterpolation.apply(mesh, qd, xd)
+        y.backward(gy)
+        return y
+
+    def plain_forward():
+        with torch.no_grad():
+            crow, col, val = mesh.weights_device(qd.detach())
+            return spmm_device(crow, col, val, xd.detach(), M), col.numel()
 
     for i in range(args.warmup):
-        y, nnz = step()
+        y = step()
         torch.cuda.synchronize()
-        log(f"warmup {i + 1}/{args.warmup}: nnz {nnz:,}")
+        log(f"warmup {i + 1}/{args.warmup}: timings {mesh.timings()}")
 
-    rows_ms, rows_items, levels = [], [], []
+    per_step = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        y, nnz = step()
-        tm = mesh.timings()
-        rows_ms.append(tm["rows_ms"][0])
-        rows_items.append(tm["rows_items"][0])
-        levels.append(len(tm["rows_ms"]))
+        y = step()
+        per_step.append(mesh.timings())  # the backward's calls synchronise the stream
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     qps = M / (ms_per_step / 1e3)
-    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {qps / 1e6:.1f} M queries/s; "
-        f"row kernel {np.mean(rows_ms):.2f} ms, levels {max(levels)}, last timings {tm}")
+    assert torch.isfinite(qd.grad).all() and torch.isfinite(xd.grad).all()
 
-    bq = bytes_per_query(WORK_PER_QUERY)
-    avg_rows_ms = float(np.mean(rows_ms))
-    achieved = bq * float(np.mean(rows_items)) / (avg_rows_ms / 1e3) / 1e9
-    ops = fp64_ops_per_query(WORK_PER_QUERY) * float(np.mean(rows_items)) / (avg_rows_ms / 1e3)
+    # the plain forward, for the fwd+bwd / fwd ratio
+    for _ in range(max(1, args.warmup // 2)):
+        plain_forward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        _, nnz = plain_forward()
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t2) * 1e3 / args.steps
+    log(f"timed {args.steps} steps: fwd+bwd {ms_per_step:.2f} ms/step "
+        f"({qps / 1e6:.1f} M queries/s), plain fwd {fwd_ms:.2f} ms/step, "
+        f"ratio {ms_per_step / fwd_ms:.3f}; last timings {per_step[-1]}")
+
+    nbytes = ctypes.c_int64()
+    natinterp3d_amd._lib.lib().nat_topology_save(mesh.handle, None, 0, ctypes.byref(nbytes),
+                                                 None)
+    units = (int(nbytes.value) - 32 - 12 * M) / 16.0
+    rec_q = record_bytes(units, M)
+    kb = kernel_bytes_per_query(WORK_PER_QUERY, rec_q, D)
+    kops = kernel_fp64_ops_per_query(WORK_PER_QUERY)
+    kms = {"topo_group_kernel": float(np.mean([t["topo_ms"] for t in per_step])),
+           "geom_kernel": float(np.mean([t["geom_ms"] for t in per_step])),
+           "grad_geom_kernel": float(np.mean([t["grad_ms"][0] for t in per_step]))}
+    dom = max(kms, key=kms.get)
+    launch_ms = kms[dom]
+    achieved = kb[dom] * M / (launch_ms / 1e3) / 1e9
+    ops = kops[dom] * M / (launch_ms / 1e3)
+    log(f"kernels (ms): {kms}; dominant {dom}: {kb[dom]:.0f} B/query -> "
+        f"{achieved:.1f} GB/s, {ops / 1e12:.2f} T FP64 op/s")
 
     base = None
     if refbuild is not None:
@@ -236,14 +281,21 @@ def main():
                         "Gaussian blobs sigma 0.01), 10,000,000 queries uniform "
                         "[0.02,0.98]^3, values float32 [N,3]",
             "keys": args.keys, "queries": M, "nnz": int(nnz),
-            "pass": "forward: all weight rows (FP64, bit-exact CSR in HBM) + W @ values",
+            "pass": "forward (all weight rows, FP64 bit-exact CSR in HBM, W @ values) + "
+                    "backward (dL/dvalues = W^T g, dL/dqueries from cached cavities)",
+            "plain_fwd_ms": fwd_ms,
+            "fwd_bwd_over_fwd": ms_per_step / fwd_ms,
+            "kernel_ms": {**kms, "locate_kernel": per_step[-1]["locate_ms"],
+                          "larger_levels_fwd": float(sum(per_step[-1]["rows_ms"][1:])),
+                          "larger_levels_bwd": float(sum(per_step[-1]["grad_ms"][1:])),
+                          "emit": per_step[-1]["emit_ms"]},
             "mesh_build_s": round(build_s, 1),
         },
         "roofline": {
-            "kernel": "sibson_rows_kernel",
+            "kernel": dom,
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "bytes_per_query": bq, "launch_ms": avg_rows_ms,
+            "bytes_per_query": kb[dom], "queries_per_launch": M, "launch_ms": launch_ms,
             "fp64_ops_per_s": ops, "fp64_frac": ops / FP64_OPS_PEAK,
         },
         "cpu_baseline": base,

@@ -108,6 +108,35 @@ int nat_spmm_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
 int nat_csr_sort_rows(int num_rows, const int32_t *d_rowptr, int32_t *d_cols, void *d_vals,
                       int vals_dtype, void *stream);
 
+/* ---- backward (no reference counterpart: the reference has no autograd;
+ * these serve the torch.autograd.Function of natinterp3d_amd).
+ *
+ * nat_topology_save copies what the last nat_weights_compute on this mesh
+ * learned about its queries -- the cavity, boundary faces and neighbour
+ * order of every row the level-0 kernels made, and the query order -- into
+ * caller-owned device memory, so that the backward pass recomputes the
+ * geometry from it instead of searching again.  With d_dst NULL it only
+ * reports the size in *bytes.  Returns -7 when capacity is too small. */
+int nat_topology_save(nat_mesh *m, void *d_dst, int64_t capacity, int64_t *bytes,
+                      void *stream);
+
+/* dL/dq [M,3] float64 of y = W(q) @ values, from dL/dy [M,D] float64:
+ * dL/dq_i = sum_j <dL/dy_i, values_j> d W_ij / d q_i, with the Sibson
+ * coordinate derivative from the Voronoi facets between q_i and its
+ * neighbours (d sigma_j/dq = facet moment about q / |p_j - q|).  d_topo is a
+ * nat_topology_save of the forward call on the same centred queries; rows it
+ * lacks are recomputed.  Empty and coincident rows get 0. */
+int nat_grad_queries(nat_mesh *m, const void *d_topo, const double *d_queries,
+                     int num_query_points, const double *d_gy, const double *d_values,
+                     int dim, double *d_dq, void *stream);
+
+/* dL/dvalues = W^T @ dL/dy: d_out [N,D] float64 (overwritten) from the
+ * device CSR of nat_weights_emit and d_gy [M,D] float64.  Float64 atomic
+ * accumulation: the order of the additions into one output is not fixed. */
+int nat_spmm_t_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
+                   const int32_t *d_cols, const double *d_vals, const double *d_gy,
+                   double *d_out, void *stream);
+
 /* Kernel timings of the last nat_weights_compute / nat_weights_emit on a
  * mesh, from HIP events recorded on the stream the kernels ran on (no
  * reference counterpart; for measurement).  Off by default. */
@@ -120,6 +149,9 @@ typedef struct {
     int32_t rows_items[4];    /* queries processed per level */
     int32_t levels;           /* levels launched */
     double emit_ms;           /* CSR scan + gather */
+    double grad_ms[4];        /* nat_grad_queries: [0] from the saved records,
+                                 [1..3] recomputed rows per capacity level */
+    int32_t grad_items[4];
 } nat_timings;
 
 int nat_mesh_set_timing(nat_mesh *m, int enable);

@@ -14,7 +14,8 @@ _lib = None
 EXPORTS = (
     "nat_mesh_build", "nat_mesh_free", "nat_mesh_get_info", "nat_mesh_export",
     "nat_get_weights", "nat_weights_compute", "nat_weights_emit", "nat_spmm_csr",
-    "nat_csr_sort_rows", "nat_mesh_set_timing", "nat_mesh_get_timings", "nat_free_host", "nat_last_error",
+    "nat_csr_sort_rows", "nat_topology_save", "nat_grad_queries", "nat_spmm_t_csr",
+    "nat_mesh_set_timing", "nat_mesh_get_timings", "nat_free_host", "nat_last_error",
 )
 
 
@@ -24,7 +25,8 @@ class Timings(ctypes.Structure):
         ("topo_ms", ctypes.c_double),
         ("geom_ms", ctypes.c_double), ("rows_ms", ctypes.c_double * 4),
         ("rows_items", ctypes.c_int32 * 4), ("levels", ctypes.c_int32),
-        ("emit_ms", ctypes.c_double),
+        ("emit_ms", ctypes.c_double), ("grad_ms", ctypes.c_double * 4),
+        ("grad_items", ctypes.c_int32 * 4),
     ]
 
 
@@ -71,6 +73,13 @@ def lib():
                                ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, vp]
     L.nat_csr_sort_rows.restype = ctypes.c_int
     L.nat_csr_sort_rows.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int, vp]
+    L.nat_topology_save.restype = ctypes.c_int
+    L.nat_topology_save.argtypes = [vp, vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), vp]
+    L.nat_grad_queries.restype = ctypes.c_int
+    L.nat_grad_queries.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, ctypes.c_int, vp, vp]
+    L.nat_spmm_t_csr.restype = ctypes.c_int
+    L.nat_spmm_t_csr.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp,
+                                 vp]
     L.nat_mesh_set_timing.restype = ctypes.c_int
     L.nat_mesh_set_timing.argtypes = [vp, ctypes.c_int]
     L.nat_mesh_get_timings.restype = ctypes.c_int

@@ -1,0 +1,86 @@
+"""Differentiable Sibson interpolation on the GPU (no reference counterpart:
+natinterp3d has no gradients).
+
+``SibsonInterpolation.apply(mesh, queries, values)`` computes
+``y = W(queries) @ values`` for centred float64 CUDA queries [M,3] and values
+[N,D]; its backward gives
+
+* dL/dvalues = W^T dL/dy  (float64 atomics, nat_spmm_t_csr), and
+* dL/dqueries from the analytic derivative of the Sibson coordinates: with
+  sigma_j the volume query i steals from neighbour j, S = sum_j sigma_j and
+  lambda_j = sigma_j / S,  d sigma_j / dq = (moment about q of the Voronoi
+  facet shared by q and p_j) / |p_j - q|  and
+  d lambda_j / dq = (d sigma_j/dq - lambda_j sum_k d sigma_k/dq) / S.
+
+The forward pass keeps each row's cavity, boundary faces and neighbour order
+(nat_topology_save); the backward pass recomputes the circumcentres and
+volumes from them rather than storing per-entry derivatives.  Empty rows,
+rows of queries that coincide with a data point and rows outside the hull's
+super simplex have zero query gradient.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from .natinterp3d import spmm_device
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr() if t is not None and t.numel() else 0)
+
+
+class SibsonInterpolation(torch.autograd.Function):
+    """y = W(queries) @ values with gradients to queries and values."""
+
+    @staticmethod
+    def forward(ctx, mesh, queries, values):
+        if not queries.is_cuda or not values.is_cuda:
+            raise ValueError("queries and values must be CUDA tensors")
+        if queries.dim() != 2 or queries.shape[1] != 3:
+            raise ValueError("Query points must have shape (num_query_points, 3)")
+        if values.dim() != 2 or values.shape[0] != mesh.numDataPoints:
+            raise ValueError("values must have shape (num_data_points, D)")
+        q = queries.detach().to(torch.float64).contiguous()
+        v = values.detach().to(torch.float64).contiguous()
+        crow, col, val = mesh.weights_device(q)
+        topo = None
+        if ctx.needs_input_grad[1]:
+            L = _lib.lib()
+            nbytes = ctypes.c_int64()
+            _lib.check(L.nat_topology_save(mesh.handle, None, 0, ctypes.byref(nbytes), None),
+                       "nat_topology_save")
+            topo = torch.empty(int(nbytes.value), dtype=torch.uint8, device=q.device)
+            stream = torch.cuda.current_stream(q.device).cuda_stream
+            _lib.check(L.nat_topology_save(mesh.handle, ctypes.c_void_p(topo.data_ptr()),
+                                           topo.numel(), ctypes.byref(nbytes),
+                                           ctypes.c_void_p(stream)), "nat_topology_save")
+        y = spmm_device(crow, col, val, v, q.shape[0])
+        ctx.mesh = mesh
+        ctx.values_dtype = values.dtype
+        ctx.queries_dtype = queries.dtype
+        ctx.save_for_backward(q, v, crow, col, val, topo)
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        q, v, crow, col, val, topo = ctx.saved_tensors
+        mesh = ctx.mesh
+        gy = gy.to(torch.float64).contiguous()
+        M, D = gy.shape
+        L = _lib.lib()
+        stream = ctypes.c_void_p(torch.cuda.current_stream(gy.device).cuda_stream)
+        dq = dv = None
+        if ctx.needs_input_grad[1]:
+            dq = torch.empty((M, 3), dtype=torch.float64, device=gy.device)
+            if M:
+                _lib.check(L.nat_grad_queries(mesh.handle, ctypes.c_void_p(topo.data_ptr()),
+                                              _vp(q), M, _vp(gy), _vp(v), D, _vp(dq), stream),
+                           "nat_grad_queries")
+            dq = dq.to(ctx.queries_dtype)
+        if ctx.needs_input_grad[2]:
+            dv = torch.empty(v.shape, dtype=torch.float64, device=gy.device)
+            _lib.check(L.nat_spmm_t_csr(M, v.shape[0], D, _vp(crow), _vp(col), _vp(val),
+                                        _vp(gy), _vp(dv), stream), "nat_spmm_t_csr")
+            dv = dv.to(ctx.values_dtype)
+        return None, dq, dv

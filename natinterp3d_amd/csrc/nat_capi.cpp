@@ -121,6 +121,7 @@ struct nat_mesh {
 
     int last_M = -1;
     long long last_nnz = 0;
+    long long last_units = 0;  // record units of the last compute
 
     // optional kernel timing (HIP events on the launch stream)
     bool timing = false;
@@ -437,6 +438,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
             continue;
         }
         m->units_per_row = std::max(8.0, 1.125 * static_cast<double>(units) / M);
+        m->last_units = units;
         nat::Scratch scg{};
         scg.nb_vol = m->geom_vol.as<double>();
         scg.nthr = m->nthr_geom;
@@ -489,6 +491,82 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
         want = static_cast<long long>(used) + 1024;
     }
     return fail(-5, "row pool did not converge");
+}
+
+// Saved topology: int64 header {magic, M, record units, N}, rec_off
+// int64[M], the query order int32[M], then the records (16-byte aligned).
+constexpr long long kTopoMagic = 0x4e41545450304c31LL;
+size_t topo_order_off(long long M) { return 32 + 8 * static_cast<size_t>(M); }
+size_t topo_rec_off(long long M) {
+    return (topo_order_off(M) + 4 * static_cast<size_t>(M) + 15) & ~static_cast<size_t>(15);
+}
+
+int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const double *d_gy,
+              const double *d_values, int D, double *d_dq, hipStream_t st) {
+    long long hdr[4];
+    HIP_OR_FAIL(hipMemcpyAsync(hdr, blob, sizeof(hdr), hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    if (hdr[0] != kTopoMagic) return fail(-1, "d_topo is not a saved topology");
+    if (hdr[1] != M)
+        return fail(-1, "saved topology has %lld queries, not %d", hdr[1], M);
+    if (hdr[3] != m->host.n_data) return fail(-1, "saved topology is of another mesh");
+    const size_t Mz = static_cast<size_t>(M);
+    HIP_OR_FAIL(m->ovf[0].ensure(sizeof(int) * Mz));
+    HIP_OR_FAIL(m->ovf[1].ensure(sizeof(int) * Mz));
+    HIP_OR_FAIL(m->counters.ensure(256));
+    HIP_OR_FAIL(m->geom_vol.ensure(sizeof(double) * static_cast<size_t>(nat::kLevel0Row) *
+                                   static_cast<size_t>(std::max(m->nthr_geom, 64))));
+    // counters: [0] record work, [1..3] fused-level work, [8..11] overflow counts
+    unsigned *ctr = m->counters.as<unsigned>();
+    HIP_OR_FAIL(hipMemsetAsync(m->counters.p, 0, 256, st));
+    nat::Topo topo;
+    topo.rec = reinterpret_cast<int4 *>(const_cast<char *>(blob) + topo_rec_off(M));
+    topo.rec_cap = hdr[2];
+    topo.rec_used = nullptr;
+    topo.rec_off = reinterpret_cast<long long *>(const_cast<char *>(blob) + 32);
+    const int *order = reinterpret_cast<const int *>(blob + topo_order_off(M));
+    nat::GradIn g{d_gy, d_values, D, d_dq};
+    nat::Scratch scg{};
+    scg.nb_vol = m->geom_vol.as<double>();
+    scg.nthr = m->nthr_geom;
+    for (int l = 0; l < kLevels; l++) {
+        m->times.grad_ms[l] = 0.0;
+        m->times.grad_items[l] = 0;
+    }
+    mark(m, 2, st);
+    HIP_OR_FAIL(nat::launch_grad_geom(m->dev, scg, m->nthr_geom / 256, d_q, order, M, ctr, topo,
+                                      g, m->ovf[0].as<int>(), ctr + 8, st));
+    mark(m, 3, st);
+    unsigned novf = 0;
+    HIP_OR_FAIL(hipMemcpyAsync(&novf, ctr + 8, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    m->times.grad_ms[0] = elapsed(m, 2, 3);
+    m->times.grad_items[0] = M;
+    const int *items = m->ovf[0].as<int>();
+    int n_items = static_cast<int>(novf);
+    for (int level = 1; level < kLevels && n_items > 0; level++) {
+        if (int rc = ensure_level_scratch(m, level)) return rc;
+        nat::Scratch sc;
+        nat::scratch_carve(m->scratch[level].p, m->caps[level], m->nthr[level], &sc);
+        const int nblocks = (m->nthr[level] + 255) / 256;
+        mark(m, 2, st);
+        HIP_OR_FAIL(nat::launch_grad_rows(m->dev, m->caps[level], sc, nblocks, d_q, items,
+                                          n_items, ctr + level, g, m->ovf[level & 1].as<int>(),
+                                          ctr + 8 + level, st));
+        mark(m, 3, st);
+        unsigned nnext = 0;
+        HIP_OR_FAIL(hipMemcpyAsync(&nnext, ctr + 8 + level, sizeof(unsigned),
+                                   hipMemcpyDeviceToHost, st));
+        HIP_OR_FAIL(hipStreamSynchronize(st));
+        m->times.grad_ms[level] = elapsed(m, 2, 3);
+        m->times.grad_items[level] = n_items;
+        n_items = static_cast<int>(nnext);
+        items = m->ovf[level & 1].as<int>();
+    }
+    if (n_items > 0)
+        return fail(-4, "%d queries need more than %d cavity tets", n_items,
+                    m->caps[kLevels - 1].cav);
+    return 0;
 }
 
 int emit_rows(nat_mesh *m, double *d_vals, int *d_cols, int *d_rowptr, hipStream_t st) {
@@ -696,6 +774,70 @@ int nat_spmm_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
                                     d_values, values_dtype, d_out, out_dtype,
                                     static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail(-10, "spmm launch failed: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int nat_topology_save(nat_mesh *m, void *d_dst, int64_t capacity, int64_t *bytes,
+                      void *stream) {
+    g_err.clear();
+    if (!m || !bytes) return fail(-1, "NULL argument");
+    if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    std::lock_guard<std::mutex> lk(m->mu);
+    if (m->last_M < 0) return fail(-6, "nat_topology_save without a preceding compute");
+    const long long M = m->last_M;
+    const long long units = M > 0 ? m->last_units : 0;
+    const size_t need = topo_rec_off(M) + sizeof(int4) * static_cast<size_t>(units);
+    *bytes = static_cast<int64_t>(need);
+    if (!d_dst) return 0;
+    if (capacity < static_cast<int64_t>(need))
+        return fail(-7, "topology needs %zu bytes, capacity %lld", need,
+                    static_cast<long long>(capacity));
+    DeviceGuard g(m->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const long long hdr[4] = {kTopoMagic, M, units, m->host.n_data};
+    char *dst = static_cast<char *>(d_dst);
+    HIP_OR_FAIL(hipMemcpyAsync(dst, hdr, sizeof(hdr), hipMemcpyHostToDevice, st));
+    if (M > 0) {
+        HIP_OR_FAIL(hipMemcpyAsync(dst + 32, m->rec_off.p, 8 * static_cast<size_t>(M),
+                                   hipMemcpyDeviceToDevice, st));
+        HIP_OR_FAIL(hipMemcpyAsync(dst + topo_order_off(M), m->order.p,
+                                   4 * static_cast<size_t>(M), hipMemcpyDeviceToDevice, st));
+        if (units > 0)
+            HIP_OR_FAIL(hipMemcpyAsync(dst + topo_rec_off(M), m->rec.p,
+                                       sizeof(int4) * static_cast<size_t>(units),
+                                       hipMemcpyDeviceToDevice, st));
+    }
+    // the header comes from the stack: finish before returning
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    return 0;
+}
+
+int nat_grad_queries(nat_mesh *m, const void *d_topo, const double *d_queries,
+                     int num_query_points, const double *d_gy, const double *d_values,
+                     int dim, double *d_dq, void *stream) {
+    g_err.clear();
+    if (!m || !d_topo) return fail(-1, "NULL argument");
+    if (num_query_points < 0 || dim <= 0) return fail(-1, "bad size (M %d, D %d)",
+                                                      num_query_points, dim);
+    if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    if (num_query_points > 0 && (!d_queries || !d_gy || !d_values || !d_dq))
+        return fail(-1, "NULL argument");
+    std::lock_guard<std::mutex> lk(m->mu);
+    DeviceGuard g(m->device);
+    return grad_rows(m, static_cast<const char *>(d_topo), d_queries, num_query_points, d_gy,
+                     d_values, dim, d_dq, static_cast<hipStream_t>(stream));
+}
+
+int nat_spmm_t_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
+                   const int32_t *d_cols, const double *d_vals, const double *d_gy,
+                   double *d_out, void *stream) {
+    g_err.clear();
+    if (num_rows < 0 || num_cols < 0 || dim < 0) return fail(-1, "negative size");
+    if (num_cols == 0 || dim == 0) return 0;
+    if (!d_out || (num_rows > 0 && (!d_rowptr || !d_gy))) return fail(-1, "NULL argument");
+    hipError_t e = nat::launch_spmm_t(num_rows, num_cols, dim, d_rowptr, d_cols, d_vals, d_gy,
+                                      d_out, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail(-10, "spmm_t launch failed: %s", hipGetErrorString(e));
     return 0;
 }
 

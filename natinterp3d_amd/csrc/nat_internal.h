@@ -85,6 +85,14 @@ struct Topo {
     long long *rec_off;          // [M] record offset, -1 = row made by another level
 };
 
+// Backward inputs: dL/dy rows and the interpolated values, both float64.
+struct GradIn {
+    const double *gy;      // [M, D]
+    const double *values;  // [N, D]
+    int D;
+    double *dq;            // [M, 3] out: dL/dq
+};
+
 size_t scratch_bytes_per_thread(const Caps &c);
 void scratch_carve(void *base, const Caps &c, int nthr, Scratch *s);
 
@@ -110,6 +118,18 @@ hipError_t launch_topo(const DevMesh &m, int nblocks, const double *d_q, const i
 hipError_t launch_geom(const DevMesh &m, const Scratch &sc, int nblocks, const double *d_q,
                        const int *d_items, int n_items, unsigned int *d_work,
                        const Topo &topo, const RowOut &out, hipStream_t st);
+// backward: dL/dq from the topology records, then the fused kernel for the
+// queries without one (and those that overflow it); W^T g
+hipError_t launch_grad_geom(const DevMesh &m, const Scratch &sc, int nblocks, const double *d_q,
+                            const int *d_items, int n_items, unsigned int *d_work,
+                            const Topo &topo, const GradIn &g, int *ovf_list,
+                            unsigned int *ovf_count, hipStream_t st);
+hipError_t launch_grad_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
+                            const double *d_q, const int *d_items, int n_items,
+                            unsigned int *d_work, const GradIn &g, int *ovf_list,
+                            unsigned int *ovf_count, hipStream_t st);
+hipError_t launch_spmm_t(int M, int N, int D, const int *rowptr, const int *cols,
+                         const double *vals, const double *gy, double *out, hipStream_t st);
 // exclusive scan of n int32 counts into int64 offsets; total in out[n]
 hipError_t launch_scan_counts(const int *d_in, long long *d_out, int n, long long *d_tmp,
                               hipStream_t st);

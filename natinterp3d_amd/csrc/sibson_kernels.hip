@@ -436,11 +436,40 @@ __device__ __forceinline__ void old_contributions(const DevMesh &m, int t, unsig
     }
 }
 
+// Derivative of a stolen volume: d sigma_k / dq = (1 / |pk - q|) times the
+// moment, integral of (x - q) dA, of the new Voronoi facet between q and pk.
+// Inside the virtual tet (pk, q, A, B) that facet is the quadrilateral
+// (m, ca, cv, cb): m the midpoint of q and pk, ca and cb the circumcentres of
+// faces (pk, q, A) and (pk, q, B), cv the tet's circumcentre.  Returns its
+// moment divided by |pk - q|^2, triangle areas signed along pk - q (the
+// boundary faces' consistent orientation makes the pieces add up).
+__device__ __forceinline__ d3 facet_moment(const d3 &pk, const d3 &q, const d3 &ca,
+                                           const d3 &cb, const d3 &cv) {
+    const d3 w = sub3(pk, q);
+    const d3 mq = {(pk.x + q.x) * 0.5, (pk.y + q.y) * 0.5, (pk.z + q.z) * 0.5};
+    const d3 u = sub3(ca, mq), v = sub3(cv, mq), x = sub3(cb, mq);
+    const double a1 = 0.5 * dot3(cross3(u, v), w);
+    const double a2 = 0.5 * dot3(cross3(v, x), w);
+    const double inv = 1.0 / dot3(w, w);
+    const d3 g1 = {(mq.x + ca.x + cv.x) / 3.0 - q.x, (mq.y + ca.y + cv.y) / 3.0 - q.y,
+                   (mq.z + ca.z + cv.z) / 3.0 - q.z};
+    const d3 g2 = {(mq.x + cv.x + cb.x) / 3.0 - q.x, (mq.y + cv.y + cb.y) / 3.0 - q.y,
+                   (mq.z + cv.z + cb.z) / 3.0 - q.z};
+    return {(a1 * g1.x + a2 * g2.x) * inv, (a1 * g1.y + a2 * g2.y) * inv,
+            (a1 * g1.z + a2 * g2.z) * inv};
+}
+
+struct NoMoment {
+    __device__ void operator()(int, const d3 &) const {}
+    static constexpr bool active = false;
+};
+
 // Step 4 for one boundary face, face fi of cavity tet t
-// (natural_insertionfree.c:681-719): calls sub(slot, vol) per face vertex.
-template <typename SlotOf, typename Sub>
+// (natural_insertionfree.c:681-719): calls sub(slot, vol) per face vertex,
+// and mom(slot, facet moment) when the gradient is wanted.
+template <typename SlotOf, typename Sub, typename Mom = NoMoment>
 __device__ __forceinline__ void new_contributions(const DevMesh &m, int t, int fi, const d3 &q,
-                                                  SlotOf slot_of, Sub sub) {
+                                                  SlotOf slot_of, Sub sub, Mom mom = Mom()) {
     bool neg;
     const int4 tv = ld_tet_v(m, t, neg);
     d3 P[4];
@@ -464,7 +493,61 @@ __device__ __forceinline__ void new_contributions(const DevMesh &m, int t, int f
         const d3 ca = cc_tri(F[k], q, A);
         const d3 cb = cc_tri(F[k], q, B);
         sub(slot, subcell_volume(F[k], q, A, B, cv, ca, cb, cf));
+        if constexpr (Mom::active) mom(slot, facet_moment(F[k], q, ca, cb, cv));
     }
+}
+
+template <typename F>
+struct MomentFn {
+    F f;
+    static constexpr bool active = true;
+    __device__ void operator()(int slot, const d3 &gm) const { f(slot, gm); }
+};
+template <typename F>
+__device__ __forceinline__ MomentFn<F> moment_fn(F f) {
+    return MomentFn<F>{f};
+}
+
+// a_j = <dL/dy[qi], values[j]>
+__device__ __forceinline__ double grad_dot(const GradIn &g, int qi, int j) {
+    const double *gr = g.gy + static_cast<size_t>(qi) * g.D;
+    const double *vr = g.values + static_cast<size_t>(j) * g.D;
+    double a = 0.0;
+    for (int d = 0; d < g.D; d++) a += gr[d] * vr[d];
+    return a;
+}
+
+// dL/dq of one row, accumulated while its volumes are computed.  With
+// sigma_j the stolen volume of neighbour j, S = sum sigma_j,
+// lambda_j = sigma_j / S, G_j = d sigma_j / dq and a_j = <dL/dy, values[j]>:
+//   dL/dq = sum_j a_j d lambda_j / dq = (sum_j a_j G_j - abar sum_j G_j) / S,
+//   abar = sum_j a_j sigma_j / S,
+// so four running sums replace per-neighbour storage.  Rows the reference
+// leaves empty (S <= 1e-30) get 0.
+struct GradAcc {
+    double s = 0.0, as = 0.0;
+    d3 ag = {0.0, 0.0, 0.0}, tg = {0.0, 0.0, 0.0};
+    __device__ void vol(double a, double v) {
+        s += v;
+        as += a * v;
+    }
+    __device__ void mom(double a, const d3 &gm) {
+        ag = {ag.x + a * gm.x, ag.y + a * gm.y, ag.z + a * gm.z};
+        tg = {tg.x + gm.x, tg.y + gm.y, tg.z + gm.z};
+    }
+    __device__ d3 result() const {
+        if (!(s > 1e-30)) return {0.0, 0.0, 0.0};
+        const double inv = 1.0 / s;
+        const double abar = as * inv;
+        return {(ag.x - abar * tg.x) * inv, (ag.y - abar * tg.y) * inv,
+                (ag.z - abar * tg.z) * inv};
+    }
+};
+
+__device__ __forceinline__ void store_dq(const GradIn &g, int qi, const d3 &v) {
+    g.dq[3 * static_cast<size_t>(qi)] = v.x;
+    g.dq[3 * static_cast<size_t>(qi) + 1] = v.y;
+    g.dq[3 * static_cast<size_t>(qi) + 2] = v.z;
 }
 
 __device__ __forceinline__ int wave_excl_scan(int v, int &total) {
@@ -504,8 +587,12 @@ __device__ __forceinline__ bool next_batch(unsigned int *work, int n_items, int 
 // One query with this level's capacities: the whole of
 // getWeightsSingleQueryIF (natural_insertionfree.c:591-738).  Returns the
 // row length (row in nb_idx / nb_vol) or -1 when a capacity was exceeded.
+// GRAD: the same cavity and volumes, but the row is reduced to dL/dq of
+// query qi (written to *dq; nb_vol then holds <dL/dy, values[j]> per slot).
+template <bool GRAD = false>
 __device__ int fused_row(const DevMesh &m, const Caps &caps, const Scratch &sc, int tid,
-                         int stamp, const d3 &q) {
+                         int stamp, const d3 &q, const GradIn *g = nullptr, int qi = 0,
+                         d3 *dq = nullptr) {
     const size_t S = static_cast<size_t>(sc.nthr);
     const int N = m.n_data;
     bool ovf = false;
@@ -515,7 +602,9 @@ __device__ int fused_row(const DevMesh &m, const Caps &caps, const Scratch &sc, 
     const HashTab vis(sc.vis, sc.vis_pos, tid, caps.vis_log2, stamp);
     const int ncav = cavity_search(m, caps, sc, tid, vis, start, q);
     if (ncav < 0) return -1;
+    if (GRAD) *dq = {0.0, 0.0, 0.0};  // empty and coincident rows
     if (ncav == 0) {
+        if (GRAD) return 0;
         int coin[4];
         const int nco = coincident(m, start, q, coin);
         if (nco > 0) {
@@ -557,10 +646,7 @@ __device__ int fused_row(const DevMesh &m, const Caps &caps, const Scratch &sc, 
     }
     if (nnb == 0) return 0;
     auto slot_of = [&](int, int v) { return v < N ? nbm.get(v) : -1; };
-    auto add = [&](int slot, double vol) { sc.nb_vol[il(slot, S, tid)] += vol; };
-    auto sub = [&](int slot, double vol) { sc.nb_vol[il(slot, S, tid)] -= vol; };
-    for (int ci = 0; ci < ncav; ci++) {
-        const int t = sc.cav[il(ci, S, tid)];
+    auto fc_mask = [&](int ci, int t) {
         const int4 tn = m.tet_n[t];
         unsigned mask = 0;
 #pragma unroll
@@ -570,7 +656,33 @@ __device__ int fused_row(const DevMesh &m, const Caps &caps, const Scratch &sc, 
             const int cj = vis.get(nb);
             if (cj >= 0 && cj < ci) mask |= 1u << k;
         }
-        old_contributions(m, t, mask, slot_of, add);
+        return mask;
+    };
+    if constexpr (GRAD) {
+        for (int j = 0; j < nnb; j++)
+            sc.nb_vol[il(j, S, tid)] = grad_dot(*g, qi, sc.nb_idx[il(j, S, tid)]);
+        GradAcc acc;
+        auto a_of = [&](int slot) { return sc.nb_vol[il(slot, S, tid)]; };
+        for (int ci = 0; ci < ncav; ci++) {
+            const int t = sc.cav[il(ci, S, tid)];
+            old_contributions(m, t, fc_mask(ci, t), slot_of,
+                              [&](int slot, double vol) { acc.vol(a_of(slot), vol); });
+        }
+        for (int bi = 0; bi < nbnd; bi++) {
+            const int code = sc.bnd[il(bi, S, tid)];
+            new_contributions(
+                m, sc.cav[il(code >> 2, S, tid)], code & 3, q, slot_of,
+                [&](int slot, double vol) { acc.vol(a_of(slot), -vol); },
+                moment_fn([&](int slot, const d3 &gm) { acc.mom(a_of(slot), gm); }));
+        }
+        *dq = acc.result();
+        return 0;
+    }
+    auto add = [&](int slot, double vol) { sc.nb_vol[il(slot, S, tid)] += vol; };
+    auto sub = [&](int slot, double vol) { sc.nb_vol[il(slot, S, tid)] -= vol; };
+    for (int ci = 0; ci < ncav; ci++) {
+        const int t = sc.cav[il(ci, S, tid)];
+        old_contributions(m, t, fc_mask(ci, t), slot_of, add);
     }
     for (int bi = 0; bi < nbnd; bi++) {
         const int code = sc.bnd[il(bi, S, tid)];
@@ -619,6 +731,32 @@ __global__ void __launch_bounds__(256) sibson_rows_kernel(DevMesh m, Caps caps, 
                 }
             }
         }
+    }
+    sc.epoch[tid] = epoch;
+}
+
+// Gradient of the rows of the given queries (those without a topology
+// record): the fused row with the gradient reduction.  Queries that exceed
+// the level's capacities go to ovf_list for the next level.
+__global__ void __launch_bounds__(256) sibson_grad_rows_kernel(
+    DevMesh m, Caps caps, Scratch sc, const double *__restrict__ q,
+    const int *__restrict__ items, int n_items, unsigned int *work, GradIn g, int *ovf_list,
+    unsigned int *ovf_count) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= sc.nthr) return;
+    int epoch = sc.epoch[tid];
+    int item;
+    while (next_batch(work, n_items, item)) {
+        if (item >= n_items) continue;
+        const int qi = items[item];
+        epoch = epoch == 0x7ffffffe ? 1 : epoch + 1;
+        const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
+                       q[3 * static_cast<size_t>(qi) + 2]};
+        d3 dq;
+        if (fused_row<true>(m, caps, sc, tid, epoch, qq, &g, qi, &dq) < 0)
+            ovf_list[atomicAdd(ovf_count, 1u)] = qi;
+        else
+            store_dq(g, qi, dq);
     }
     sc.epoch[tid] = epoch;
 }
@@ -1094,6 +1232,85 @@ __global__ void __launch_bounds__(256) geom_kernel(DevMesh m, double *nb_vol, in
     }
 }
 
+// Gradient of the rows that have a topology record (the cached cavity):
+// steps 3-4 recomputed from the record with the gradient reduction.  Queries
+// without a record (made by a larger level in the forward pass) are listed in
+// ovf_list.  nb_a holds <dL/dy, values[j]> per slot (interleaved per thread).
+__global__ void __launch_bounds__(256) grad_geom_kernel(DevMesh m, double *nb_a, int nthr,
+                                                        const double *__restrict__ q,
+                                                        const int *__restrict__ items,
+                                                        int n_items, unsigned int *work,
+                                                        Topo topo, GradIn g, int *ovf_list,
+                                                        unsigned int *ovf_count) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nthr) return;
+    const size_t S = static_cast<size_t>(nthr);
+    int item;
+    while (next_batch(work, n_items, item)) {
+        if (item >= n_items) continue;
+        const int qi = items ? items[item] : item;
+        const long long roff = topo.rec_off[qi];
+        if (roff < 0) {
+            ovf_list[atomicAdd(ovf_count, 1u)] = qi;
+            continue;
+        }
+        const int4 *r = topo.rec + roff;
+        const int4 hd = r[0];
+        const int ncav = hd.x, nbnd = hd.y, nnb = hd.z;
+        if (nnb > kLevel0Row) {  // not produced by the topology kernel; recompute
+            ovf_list[atomicAdd(ovf_count, 1u)] = qi;
+            continue;
+        }
+        if (ncav == 0) {  // coincident or empty row: dq = 0
+            store_dq(g, qi, {0.0, 0.0, 0.0});
+            continue;
+        }
+        const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
+                       q[3 * static_cast<size_t>(qi) + 2]};
+        const int *rn = reinterpret_cast<const int *>(r + 1 + ncav + (nbnd + 3) / 4);
+        for (int j = 0; j < nnb; j++) nb_a[il(j, S, tid)] = grad_dot(g, qi, rn[j]);
+        auto a_of = [&](int slot) { return nb_a[il(slot, S, tid)]; };
+        GradAcc acc;
+        for (int ci = 0; ci < ncav; ci++) {
+            const int4 cr = r[1 + ci];
+            old_contributions(
+                m, cr.x, static_cast<unsigned>(cr.w), [&](int vi, int) { return rec_slot(cr, vi); },
+                [&](int slot, double vol) { acc.vol(a_of(slot), vol); });
+        }
+        const int *rf = reinterpret_cast<const int *>(r + 1 + ncav);
+        for (int bi = 0; bi < nbnd; bi++) {
+            const int code = rf[bi];
+            const int4 cr = r[1 + (code >> 2)];
+            new_contributions(
+                m, cr.x, code & 3, qq, [&](int vi, int) { return rec_slot(cr, vi); },
+                [&](int slot, double vol) { acc.vol(a_of(slot), -vol); },
+                moment_fn([&](int slot, const d3 &gm) { acc.mom(a_of(slot), gm); }));
+        }
+        store_dq(g, qi, acc.result());
+    }
+}
+
+// W^T g: out[cols[e], d] += vals[e] * gy[row, d] (float64 atomics; the
+// summation order across rows is not fixed).  One thread per (row, d).
+__global__ void __launch_bounds__(256) spmm_t_kernel(int M, int D, const int *__restrict__ rowptr,
+                                                     const int *__restrict__ cols,
+                                                     const double *__restrict__ vals,
+                                                     const double *__restrict__ gy,
+                                                     double *__restrict__ out) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<long long>(M) * D) return;
+    const int row = static_cast<int>(t / D), d = static_cast<int>(t % D);
+    const double gv = gy[t];
+    if (gv == 0.0) return;
+    for (int e = rowptr[row]; e < rowptr[row + 1]; e++)
+        atomicAdd(out + static_cast<size_t>(cols[e]) * D + d, vals[e] * gv);
+}
+
+__global__ void __launch_bounds__(256) fill_zero_kernel(double *p, long long n) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0.0;
+}
+
 // ------------------------------------------------------------ host helpers
 
 size_t scratch_bytes_per_thread(const Caps &c) {
@@ -1177,6 +1394,37 @@ hipError_t launch_geom(const DevMesh &m, const Scratch &sc, int nblocks, const d
                        const Topo &topo, const RowOut &out, hipStream_t st) {
     hipLaunchKernelGGL(geom_kernel, dim3(nblocks), dim3(256), 0, st, m, sc.nb_vol, sc.nthr, d_q,
                        d_items, n_items, d_work, topo, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
+                            const double *d_q, const int *d_items, int n_items,
+                            unsigned int *d_work, const GradIn &g, int *ovf_list,
+                            unsigned int *ovf_count, hipStream_t st) {
+    hipLaunchKernelGGL(sibson_grad_rows_kernel, dim3(nblocks), dim3(256), 0, st, m, caps, sc, d_q,
+                       d_items, n_items, d_work, g, ovf_list, ovf_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_geom(const DevMesh &m, const Scratch &sc, int nblocks, const double *d_q,
+                            const int *d_items, int n_items, unsigned int *d_work,
+                            const Topo &topo, const GradIn &g, int *ovf_list,
+                            unsigned int *ovf_count, hipStream_t st) {
+    hipLaunchKernelGGL(grad_geom_kernel, dim3(nblocks), dim3(256), 0, st, m, sc.nb_vol, sc.nthr,
+                       d_q, d_items, n_items, d_work, topo, g, ovf_list, ovf_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_spmm_t(int M, int N, int D, const int *rowptr, const int *cols,
+                         const double *vals, const double *gy, double *out, hipStream_t st) {
+    const long long nz = static_cast<long long>(N) * D;
+    if (nz > 0)
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(static_cast<unsigned>((nz + 255) / 256)),
+                           dim3(256), 0, st, out, nz);
+    const long long nt = static_cast<long long>(M) * D;
+    if (nt > 0)
+        hipLaunchKernelGGL(spmm_t_kernel, dim3(static_cast<unsigned>((nt + 255) / 256)),
+                           dim3(256), 0, st, M, D, rowptr, cols, vals, gy, out);
     return hipGetLastError();
 }
 

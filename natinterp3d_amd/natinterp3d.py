@@ -97,7 +97,8 @@ class MeshAndVerticesHIP:
         return {"order_ms": t.order_ms, "locate_ms": t.locate_ms, "topo_ms": t.topo_ms,
                 "geom_ms": t.geom_ms,
                 "rows_ms": list(t.rows_ms)[:n],
-                "rows_items": list(t.rows_items)[:n], "emit_ms": t.emit_ms}
+                "rows_items": list(t.rows_items)[:n], "emit_ms": t.emit_ms,
+                "grad_ms": list(t.grad_ms), "grad_items": list(t.grad_items)}
 
     def export(self):
         """The packed host mesh: verts [(N+4),3], tet_v/tet_n [T,4], k-d arrays."""
@@ -301,3 +302,28 @@ class Interpolator:
         if out.dtype != res_dtype:
             out = out.astype(res_dtype)
         return np.squeeze(out, axis=1) if one_d else out
+
+    def interpolate_torch(self, query_points, values):
+        """Differentiable ``W(query_points) @ values`` for torch tensors
+        (query_points [M,3], values [N] or [N,D], floating point).  The
+        weights are the float64 Sibson coordinates of ``get_weights``; the
+        product accumulates in float64 and is returned in the promoted dtype
+        of the two inputs.  Gradients flow to both inputs (see
+        natinterp3d_amd.autograd)."""
+        import torch
+
+        from .autograd import SibsonInterpolation
+        dev = torch.device("cuda", self.mesh_and_vertices.device)
+        if not (query_points.is_floating_point() and values.is_floating_point()):
+            raise TypeError("query_points and values must be floating point tensors")
+        one_d = values.dim() == 1
+        v2 = values.unsqueeze(1) if one_d else values
+        if v2.dim() != 2 or v2.shape[0] != self.mesh_and_vertices.numDataPoints:
+            raise ValueError("values must have shape (N,) or (N, D)")
+        if query_points.dim() != 2 or query_points.shape[1] != 3:
+            raise ValueError("Query points must have shape (num_query_points, 3)")
+        centroid = torch.as_tensor(self._centroid, dtype=torch.float64, device=dev)
+        q = query_points.to(device=dev, dtype=torch.float64) - centroid
+        y = SibsonInterpolation.apply(self.mesh_and_vertices, q, v2.to(dev))
+        y = y.to(torch.promote_types(query_points.dtype, values.dtype))
+        return y.squeeze(1) if one_d else y
