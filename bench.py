@@ -153,6 +153,9 @@ def main():
     ap.add_argument("--queries", type=int, default=10_000_000)
     ap.add_argument("--cpu-baseline", choices=["reference", "none"], default="reference")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the steps, copy this many MiB device to device once: a "
+                         "kernel of known traffic that scales hardware byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the query path runs on a single MI355X "
@@ -238,6 +241,16 @@ def main():
     log(f"timed {args.steps} steps: fwd+bwd {ms_per_step:.2f} ms/step "
         f"({qps / 1e6:.1f} M queries/s), plain fwd {fwd_ms:.2f} ms/step, "
         f"ratio {ms_per_step / fwd_ms:.3f}; last timings {per_step[-1]}")
+
+    if args.calib_copy_mib > 0:
+        n = args.calib_copy_mib << 20
+        src = torch.ones(n, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {n:,} bytes read and written")
+        del src, dst
 
     nbytes = ctypes.c_int64()
     natinterp3d_amd._lib.lib().nat_topology_save(mesh.handle, None, 0, ctypes.byref(nbytes),
