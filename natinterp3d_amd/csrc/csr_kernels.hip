@@ -174,19 +174,23 @@ hipError_t launch_scan_counts(const int *d_in, long long *d_out, int n, long lon
 
 // ------------------------------------------------------------ CSR gather
 
-__global__ void gather_csr_kernel(const long long *__restrict__ rowptr64,
-                                  const long long *__restrict__ row_off,
-                                  const int *__restrict__ row_cnt,
-                                  const double *__restrict__ pv, const int *__restrict__ pc,
-                                  int M, double *__restrict__ vals, int *__restrict__ cols,
-                                  int *__restrict__ rowptr32) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// 16 lanes per row: each row's pool range and CSR range are contiguous, so
+// the lanes of a row read and write consecutive entries.
+constexpr int kGatherLanes = 16;
+
+__global__ void __launch_bounds__(256) gather_csr_kernel(
+    const long long *__restrict__ rowptr64, const long long *__restrict__ row_off,
+    const int *__restrict__ row_cnt, const double *__restrict__ pv, const int *__restrict__ pc,
+    int M, double *__restrict__ vals, int *__restrict__ cols, int *__restrict__ rowptr32) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long i = t / kGatherLanes;
+    const int lane = static_cast<int>(t % kGatherLanes);
     if (i > M) return;
-    rowptr32[i] = static_cast<int>(rowptr64[i]);
+    if (lane == 0) rowptr32[i] = static_cast<int>(rowptr64[i]);
     if (i == M) return;
     const long long dst = rowptr64[i], src = row_off[i];
     const int c = row_cnt[i];
-    for (int j = 0; j < c; j++) {
+    for (int j = lane; j < c; j += kGatherLanes) {
         vals[dst + j] = pv[src + j];
         cols[dst + j] = pc[src + j];
     }
@@ -196,9 +200,10 @@ hipError_t launch_gather_csr(const long long *d_rowptr64, const long long *d_row
                              const int *d_row_cnt, const double *pool_vals,
                              const int *pool_cols, int M, double *d_vals, int *d_cols,
                              int *d_rowptr32, hipStream_t st) {
-    hipLaunchKernelGGL(gather_csr_kernel, dim3((M + 1 + 255) / 256), dim3(256), 0, st,
-                       d_rowptr64, d_row_off, d_row_cnt, pool_vals, pool_cols, M, d_vals,
-                       d_cols, d_rowptr32);
+    const long long threads = (static_cast<long long>(M) + 1) * kGatherLanes;
+    hipLaunchKernelGGL(gather_csr_kernel, dim3(static_cast<unsigned>((threads + 255) / 256)),
+                       dim3(256), 0, st, d_rowptr64, d_row_off, d_row_cnt, pool_vals, pool_cols,
+                       M, d_vals, d_cols, d_rowptr32);
     return hipGetLastError();
 }
 

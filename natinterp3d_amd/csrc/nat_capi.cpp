@@ -104,7 +104,6 @@ struct nat_mesh {
     int topo_blocks = 0;
     DevBuf scratch[kLevels];
     DevBuf start;                      // start tet per query
-    DevBuf geom_vol;                   // geometry kernel: per-thread row accumulators
     DevBuf rec, rec_off;               // topology records
     long long rec_cap = 0;
     double units_per_row = 40.0;
@@ -136,7 +135,7 @@ struct nat_mesh {
         size_t s = q.bytes + order.bytes + keys.bytes + hist.bytes + scan.bytes +
                    row_off.bytes + row_cnt.bytes + rowptr64.bytes + ovf[0].bytes +
                    ovf[1].bytes + counters.bytes + pool_vals.bytes + pool_cols.bytes +
-                   csr_vals.bytes + csr_cols.bytes + csr_rowptr.bytes + geom_vol.bytes +
+                   csr_vals.bytes + csr_cols.bytes + csr_rowptr.bytes +
                    rec.bytes + rec_off.bytes + start.bytes;
         for (int l = 0; l < kLevels; l++) s += scratch[l].bytes;
         return static_cast<int64_t>(s);
@@ -145,7 +144,7 @@ struct nat_mesh {
         DevBuf *all[] = {&verts, &tet_v, &tet_n, &kd, &cell_start, &cell_pts, &q, &order,
                          &keys, &hist, &scan, &row_off, &row_cnt, &rowptr64, &ovf[0],
                          &ovf[1], &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,
-                         &csr_rowptr, &geom_vol, &rec, &rec_off, &start};
+                         &csr_rowptr, &rec, &rec_off, &start};
         for (DevBuf *b : all) b->release();
         for (int l = 0; l < kLevels; l++) scratch[l].release();
         for (hipEvent_t &e : ev) {
@@ -178,7 +177,7 @@ void set_levels(nat_mesh *m) {
     m->caps[0] = {kd, 1, 1, 1, 1, 1, 1};
     m->nthr[0] = cus * nat::locate_blocks_per_cu() * 256;
     m->topo_blocks = cus * nat::topo_blocks_per_cu();
-    m->nthr_geom = cus * nat::geom_blocks_per_cu() * 256;
+    m->nthr_geom = cus * nat::geom_blocks_per_cu() * nat::kGeomThreads;
     // larger levels (fused kernel) only see the queries that overflowed
     m->caps[1] = {kd, 2048, 12, 1024, 2560, 1536, 12};
     m->nthr[1] = 8192;
@@ -378,9 +377,6 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
 
     HIP_OR_FAIL(m->rec_off.ensure(sizeof(long long) * Mz));
     HIP_OR_FAIL(m->start.ensure(sizeof(int) * Mz));
-    // the geometry kernel accumulates up to kLevel0Row slots per thread
-    HIP_OR_FAIL(m->geom_vol.ensure(sizeof(double) * static_cast<size_t>(nat::kLevel0Row) *
-                                   static_cast<size_t>(std::max(m->nthr_geom, 64))));
     long long want = static_cast<long long>(std::ceil(m->nnz_per_row * M)) + 1024;
     long long want_rec = static_cast<long long>(std::ceil(m->units_per_row * M)) + 1024;
     for (int attempt = 0; attempt < 4; attempt++) {
@@ -439,12 +435,9 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
         }
         m->units_per_row = std::max(8.0, 1.125 * static_cast<double>(units) / M);
         m->last_units = units;
-        nat::Scratch scg{};
-        scg.nb_vol = m->geom_vol.as<double>();
-        scg.nthr = m->nthr_geom;
         mark(m, 2, st);
-        HIP_OR_FAIL(nat::launch_geom(m->dev, scg, m->nthr_geom / 256, d_q, m->order.as<int>(),
-                                     M, ctr + 5, topo, out, st));
+        HIP_OR_FAIL(nat::launch_geom(m->dev, m->nthr_geom / nat::kGeomThreads, d_q,
+                                     m->order.as<int>(), M, ctr + 5, topo, out, st));
         mark(m, 3, st);
         m->times.geom_ms = elapsed(m, 2, 3);
         m->times.rows_ms[0] = m->times.topo_ms + m->times.geom_ms;
@@ -514,8 +507,6 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const dou
     HIP_OR_FAIL(m->ovf[0].ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->ovf[1].ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->counters.ensure(256));
-    HIP_OR_FAIL(m->geom_vol.ensure(sizeof(double) * static_cast<size_t>(nat::kLevel0Row) *
-                                   static_cast<size_t>(std::max(m->nthr_geom, 64))));
     // counters: [0] record work, [1..3] fused-level work, [8..11] overflow counts
     unsigned *ctr = m->counters.as<unsigned>();
     HIP_OR_FAIL(hipMemsetAsync(m->counters.p, 0, 256, st));
@@ -526,16 +517,13 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const dou
     topo.rec_off = reinterpret_cast<long long *>(const_cast<char *>(blob) + 32);
     const int *order = reinterpret_cast<const int *>(blob + topo_order_off(M));
     nat::GradIn g{d_gy, d_values, D, d_dq};
-    nat::Scratch scg{};
-    scg.nb_vol = m->geom_vol.as<double>();
-    scg.nthr = m->nthr_geom;
     for (int l = 0; l < kLevels; l++) {
         m->times.grad_ms[l] = 0.0;
         m->times.grad_items[l] = 0;
     }
     mark(m, 2, st);
-    HIP_OR_FAIL(nat::launch_grad_geom(m->dev, scg, m->nthr_geom / 256, d_q, order, M, ctr, topo,
-                                      g, m->ovf[0].as<int>(), ctr + 8, st));
+    HIP_OR_FAIL(nat::launch_grad_geom(m->dev, m->nthr_geom / nat::kGeomThreads, d_q, order, M,
+                                      ctr, topo, g, m->ovf[0].as<int>(), ctr + 8, st));
     mark(m, 3, st);
     unsigned novf = 0;
     HIP_OR_FAIL(hipMemcpyAsync(&novf, ctr + 8, sizeof(unsigned), hipMemcpyDeviceToHost, st));

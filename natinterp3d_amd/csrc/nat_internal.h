@@ -30,8 +30,10 @@ struct DevMesh {
 };
 
 // Longest row the level-0 kernels handle (topology LDS neighbour list and
-// the geometry kernel's per-thread accumulators); longer rows go to level 1.
-constexpr int kLevel0Row = 96;
+// the geometry kernels' per-thread LDS accumulators); longer rows go to
+// level 1.  Threads per block of the geometry kernels.
+constexpr int kLevel0Row = 32;
+constexpr int kGeomThreads = 128;
 
 // Per-thread capacities.  A query that needs more marks itself for the next,
 // larger level.
@@ -101,7 +103,7 @@ hipError_t launch_morton_sort(const double *d_q, int M, const double bbox[6], in
                               long long *d_scan_tmp, hipStream_t st);
 int rows_blocks_per_cu();  // resident 256-thread blocks of the fused row kernel per CU
 int topo_blocks_per_cu();   // 128-thread blocks of the topology kernel
-int geom_blocks_per_cu();
+int geom_blocks_per_cu();  // kGeomThreads-thread blocks
 // fused row kernel: used for the larger capacity levels
 hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
                        const double *d_q, const int *d_items, int n_items,
@@ -115,15 +117,15 @@ hipError_t launch_locate(const DevMesh &m, const Caps &caps, const Scratch &sc, 
 hipError_t launch_topo(const DevMesh &m, int nblocks, const double *d_q, const int *d_items,
                        int n_items, const int *d_start, unsigned int *d_work, const Topo &topo,
                        unsigned int *ovf_count, int *ovf_list, hipStream_t st);
-hipError_t launch_geom(const DevMesh &m, const Scratch &sc, int nblocks, const double *d_q,
-                       const int *d_items, int n_items, unsigned int *d_work,
-                       const Topo &topo, const RowOut &out, hipStream_t st);
+hipError_t launch_geom(const DevMesh &m, int nblocks, const double *d_q, const int *d_items,
+                       int n_items, unsigned int *d_work, const Topo &topo, const RowOut &out,
+                       hipStream_t st);
 // backward: dL/dq from the topology records, then the fused kernel for the
 // queries without one (and those that overflow it); W^T g
-hipError_t launch_grad_geom(const DevMesh &m, const Scratch &sc, int nblocks, const double *d_q,
-                            const int *d_items, int n_items, unsigned int *d_work,
-                            const Topo &topo, const GradIn &g, int *ovf_list,
-                            unsigned int *ovf_count, hipStream_t st);
+hipError_t launch_grad_geom(const DevMesh &m, int nblocks, const double *d_q, const int *d_items,
+                            int n_items, unsigned int *d_work, const Topo &topo,
+                            const GradIn &g, int *ovf_list, unsigned int *ovf_count,
+                            hipStream_t st);
 hipError_t launch_grad_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
                             const double *d_q, const int *d_items, int n_items,
                             unsigned int *d_work, const GradIn &g, int *ovf_list,

@@ -472,18 +472,14 @@ __device__ __forceinline__ void new_contributions(const DevMesh &m, int t, int f
                                                   SlotOf slot_of, Sub sub, Mom mom = Mom()) {
     bool neg;
     const int4 tv = ld_tet_v(m, t, neg);
-    d3 P[4];
-    P[0] = ld_vert(m, tv.x);
-    P[1] = ld_vert(m, tv.y);
-    P[2] = ld_vert(m, tv.z);
-    P[3] = ld_vert(m, tv.w);
     const int s[3] = {face_slot(fi, 0), face_slot(fi, 1), face_slot(fi, 2)};
-    d3 F[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-        F[k] = s[k] == 0 ? P[0] : (s[k] == 1 ? P[1] : (s[k] == 2 ? P[2] : P[3]));
+    const d3 F[3] = {ld_vert(m, sel4(tv, s[0])), ld_vert(m, sel4(tv, s[1])),
+                     ld_vert(m, sel4(tv, s[2]))};
     const d3 cv = cc_tet(q, F[0], F[1], F[2]);
-    const d3 cf = face_cc_own(P, 3 - fi);
+    // the face's circumcentre in ascending slot order (as face_cc_own):
+    // faces 0 and 2 list their slots ascending, faces 1 and 3 descending
+    const bool desc = fi & 1;
+    const d3 cf = cc_tri(desc ? F[2] : F[0], F[1], desc ? F[0] : F[2]);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const int slot = slot_of(s[k], sel4(tv, s[k]));
@@ -801,7 +797,8 @@ constexpr int VT = 128;       // visited-table slots
 constexpr int VMAX = 96;      // visited tets
 constexpr int CMAX = 64;      // cavity tets
 constexpr int BMAX = 160;     // boundary faces
-constexpr int NT = 128;       // neighbour-table slots
+constexpr int NT_LOG2 = 6;
+constexpr int NT = 1 << NT_LOG2;  // neighbour-table slots
 constexpr int NMAX = kLevel0Row;  // neighbours
 constexpr int SMAX = 256;     // order-replay stack
 }  // namespace grp
@@ -849,7 +846,7 @@ __device__ __forceinline__ int vis_insert(GroupMem &g, int t, bool &isnew) {
 }
 
 __device__ __forceinline__ int nb_insert(GroupMem &g, int v) {
-    unsigned h = hash_key(v, 7);
+    unsigned h = hash_key(v, grp::NT_LOG2);
     for (int n = 0; n < grp::NT; n++) {
         const int old = atomicCAS(&g.nkey[h], -1, v);
         if (old == -1 || old == v) return static_cast<int>(h);
@@ -859,7 +856,7 @@ __device__ __forceinline__ int nb_insert(GroupMem &g, int v) {
 }
 
 __device__ __forceinline__ int nb_find(const GroupMem &g, int v) {
-    unsigned h = hash_key(v, 7);
+    unsigned h = hash_key(v, grp::NT_LOG2);
     for (int n = 0; n < grp::NT; n++) {
         const int k = g.nkey[h];
         if (k == v) return static_cast<int>(h);
@@ -1020,10 +1017,13 @@ __global__ void __launch_bounds__(128) topo_group_kernel(DevMesh m, const double
                     g.vpos[s] = static_cast<unsigned char>(nc);
                     nc++;
                     const unsigned nbs = g.vnb[s];
+                    // only in-sphere neighbours are pushed: a tet outside the
+                    // sphere is never appended or expanded, so leaving it out
+                    // of the stack does not change the order of the others
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
                         const unsigned ns = (nbs >> (8 * k)) & 0xffu;
-                        if (ns == 0xffu || (g.vflag[ns] & 4u)) continue;
+                        if (ns == 0xffu || (g.vflag[ns] & 6u) != 2u) continue;
                         if (top >= grp::SMAX) {
                             nc = -1;
                             top = 0;
@@ -1157,13 +1157,16 @@ __device__ __forceinline__ int rec_slot(const int4 &cr, int vi) {
     return s == 0xffff ? -1 : s;
 }
 
-__global__ void __launch_bounds__(256) geom_kernel(DevMesh m, double *nb_vol, int nthr,
-                                                   const double *__restrict__ q,
-                                                   const int *__restrict__ items, int n_items,
-                                                   unsigned int *work, Topo topo, RowOut out) {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= nthr) return;
-    const size_t S = static_cast<size_t>(nthr);
+// Steps 3-5 from the topology record, one thread per query; the row's
+// per-neighbour accumulators live in LDS (slot j of thread t at
+// [j * kGeomThreads + t]: the lanes of a wave always hit distinct banks).
+__global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
+                                                            const double *__restrict__ q,
+                                                            const int *__restrict__ items,
+                                                            int n_items, unsigned int *work,
+                                                            Topo topo, RowOut out) {
+    __shared__ double acc_lds[kLevel0Row * kGeomThreads];
+    double *acc = acc_lds + threadIdx.x;
     int item;
     while (next_batch(work, n_items, item)) {
         int qi = -1, cnt = 0, nnb = 0;
@@ -1182,13 +1185,13 @@ __global__ void __launch_bounds__(256) geom_kernel(DevMesh m, double *nb_vol, in
             } else {
                 const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                                q[3 * static_cast<size_t>(qi) + 2]};
-                for (int j = 0; j < nnb; j++) nb_vol[il(j, S, tid)] = 0.0;
+                for (int j = 0; j < nnb; j++) acc[j * kGeomThreads] = 0.0;
                 for (int ci = 0; ci < ncav; ci++) {
                     const int4 cr = r[1 + ci];
                     old_contributions(
                         m, cr.x, static_cast<unsigned>(cr.w),
                         [&](int vi, int) { return rec_slot(cr, vi); },
-                        [&](int slot, double vol) { nb_vol[il(slot, S, tid)] += vol; });
+                        [&](int slot, double vol) { acc[slot * kGeomThreads] += vol; });
                 }
                 const int *rf = reinterpret_cast<const int *>(r + 1 + ncav);
                 for (int bi = 0; bi < nbnd; bi++) {
@@ -1196,13 +1199,13 @@ __global__ void __launch_bounds__(256) geom_kernel(DevMesh m, double *nb_vol, in
                     const int4 cr = r[1 + (code >> 2)];
                     new_contributions(
                         m, cr.x, code & 3, qq, [&](int vi, int) { return rec_slot(cr, vi); },
-                        [&](int slot, double vol) { nb_vol[il(slot, S, tid)] -= vol; });
+                        [&](int slot, double vol) { acc[slot * kGeomThreads] -= vol; });
                 }
                 double total = 0.0;
-                for (int j = 0; j < nnb; j++) total += nb_vol[il(j, S, tid)];
+                for (int j = 0; j < nnb; j++) total += acc[j * kGeomThreads];
                 if (total > 1e-30) {
                     const double inv = 1.0 / total;
-                    for (int j = 0; j < nnb; j++) nb_vol[il(j, S, tid)] *= inv;
+                    for (int j = 0; j < nnb; j++) acc[j * kGeomThreads] *= inv;
                     cnt = nnb;
                 } else {
                     cnt = 0;
@@ -1226,7 +1229,7 @@ __global__ void __launch_bounds__(256) geom_kernel(DevMesh m, double *nb_vol, in
             const int *rn = reinterpret_cast<const int *>(r + 1 + hd.x + (hd.y + 3) / 4);
             for (int j = 0; j < cnt; j++) {
                 out.pool_cols[off + j] = rn[j];
-                out.pool_vals[off + j] = nb_vol[il(j, S, tid)];
+                out.pool_vals[off + j] = acc[j * kGeomThreads];
             }
         }
     }
@@ -1236,15 +1239,11 @@ __global__ void __launch_bounds__(256) geom_kernel(DevMesh m, double *nb_vol, in
 // steps 3-4 recomputed from the record with the gradient reduction.  Queries
 // without a record (made by a larger level in the forward pass) are listed in
 // ovf_list.  nb_a holds <dL/dy, values[j]> per slot (interleaved per thread).
-__global__ void __launch_bounds__(256) grad_geom_kernel(DevMesh m, double *nb_a, int nthr,
-                                                        const double *__restrict__ q,
-                                                        const int *__restrict__ items,
-                                                        int n_items, unsigned int *work,
-                                                        Topo topo, GradIn g, int *ovf_list,
-                                                        unsigned int *ovf_count) {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= nthr) return;
-    const size_t S = static_cast<size_t>(nthr);
+__global__ void __launch_bounds__(kGeomThreads) grad_geom_kernel(
+    DevMesh m, const double *__restrict__ q, const int *__restrict__ items, int n_items,
+    unsigned int *work, Topo topo, GradIn g, int *ovf_list, unsigned int *ovf_count) {
+    __shared__ double a_lds[kLevel0Row * kGeomThreads];
+    double *nb_a = a_lds + threadIdx.x;
     int item;
     while (next_batch(work, n_items, item)) {
         if (item >= n_items) continue;
@@ -1268,8 +1267,8 @@ __global__ void __launch_bounds__(256) grad_geom_kernel(DevMesh m, double *nb_a,
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
         const int *rn = reinterpret_cast<const int *>(r + 1 + ncav + (nbnd + 3) / 4);
-        for (int j = 0; j < nnb; j++) nb_a[il(j, S, tid)] = grad_dot(g, qi, rn[j]);
-        auto a_of = [&](int slot) { return nb_a[il(slot, S, tid)]; };
+        for (int j = 0; j < nnb; j++) nb_a[j * kGeomThreads] = grad_dot(g, qi, rn[j]);
+        auto a_of = [&](int slot) { return nb_a[slot * kGeomThreads]; };
         GradAcc acc;
         for (int ci = 0; ci < ncav; ci++) {
             const int4 cr = r[1 + ci];
@@ -1363,7 +1362,13 @@ int topo_blocks_per_cu() {
     return n > 0 ? n : 1;
 }
 int locate_blocks_per_cu() { return blocks_per_cu(locate_kernel); }
-int geom_blocks_per_cu() { return blocks_per_cu(geom_kernel); }
+int geom_blocks_per_cu() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, geom_kernel, kGeomThreads, 0) !=
+        hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
+}
 
 hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
                        const double *d_q, const int *d_items, int n_items,
@@ -1389,11 +1394,11 @@ hipError_t launch_topo(const DevMesh &m, int nblocks, const double *d_q, const i
     return hipGetLastError();
 }
 
-hipError_t launch_geom(const DevMesh &m, const Scratch &sc, int nblocks, const double *d_q,
-                       const int *d_items, int n_items, unsigned int *d_work,
-                       const Topo &topo, const RowOut &out, hipStream_t st) {
-    hipLaunchKernelGGL(geom_kernel, dim3(nblocks), dim3(256), 0, st, m, sc.nb_vol, sc.nthr, d_q,
-                       d_items, n_items, d_work, topo, out);
+hipError_t launch_geom(const DevMesh &m, int nblocks, const double *d_q, const int *d_items,
+                       int n_items, unsigned int *d_work, const Topo &topo, const RowOut &out,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(geom_kernel, dim3(nblocks), dim3(kGeomThreads), 0, st, m, d_q, d_items,
+                       n_items, d_work, topo, out);
     return hipGetLastError();
 }
 
@@ -1406,12 +1411,12 @@ hipError_t launch_grad_rows(const DevMesh &m, const Caps &caps, const Scratch &s
     return hipGetLastError();
 }
 
-hipError_t launch_grad_geom(const DevMesh &m, const Scratch &sc, int nblocks, const double *d_q,
-                            const int *d_items, int n_items, unsigned int *d_work,
-                            const Topo &topo, const GradIn &g, int *ovf_list,
-                            unsigned int *ovf_count, hipStream_t st) {
-    hipLaunchKernelGGL(grad_geom_kernel, dim3(nblocks), dim3(256), 0, st, m, sc.nb_vol, sc.nthr,
-                       d_q, d_items, n_items, d_work, topo, g, ovf_list, ovf_count);
+hipError_t launch_grad_geom(const DevMesh &m, int nblocks, const double *d_q, const int *d_items,
+                            int n_items, unsigned int *d_work, const Topo &topo,
+                            const GradIn &g, int *ovf_list, unsigned int *ovf_count,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(grad_geom_kernel, dim3(nblocks), dim3(kGeomThreads), 0, st, m, d_q,
+                       d_items, n_items, d_work, topo, g, ovf_list, ovf_count);
     return hipGetLastError();
 }
 
