@@ -124,18 +124,28 @@ int nat_topology_save(nat_mesh *m, void *d_dst, int64_t capacity, int64_t *bytes
  * dL/dq_i = sum_j <dL/dy_i, values_j> d W_ij / d q_i, with the Sibson
  * coordinate derivative from the Voronoi facets between q_i and its
  * neighbours (d sigma_j/dq = facet moment about q / |p_j - q|).  d_topo is a
- * nat_topology_save of the forward call on the same centred queries; rows it
- * lacks are recomputed.  Empty and coincident rows get 0. */
+ * nat_topology_save of the forward call on the same centred queries, and
+ * d_rowptr / d_vals that call's CSR (nat_weights_emit); rows without a saved
+ * record are recomputed.  Empty and coincident rows get 0. */
 int nat_grad_queries(nat_mesh *m, const void *d_topo, const double *d_queries,
-                     int num_query_points, const double *d_gy, const double *d_values,
-                     int dim, double *d_dq, void *stream);
+                     int num_query_points, const int32_t *d_rowptr, const double *d_vals,
+                     const double *d_gy, const double *d_values, int dim, double *d_dq,
+                     void *stream);
+
+/* Device pointer to the query order (int32[M], a permutation of 0..M-1 in
+ * which neighbouring queries are close in space) held inside a saved
+ * topology; reads and checks the topology's header. */
+int nat_topology_row_order(const void *d_topo, int num_query_points, const int32_t **out);
 
 /* dL/dvalues = W^T @ dL/dy: d_out [N,D] float64 (overwritten) from the
- * device CSR of nat_weights_emit and d_gy [M,D] float64.  Float64 atomic
- * accumulation: the order of the additions into one output is not fixed. */
+ * device CSR of nat_weights_emit and d_gy [M,D] float64.  Rows are visited
+ * in d_row_order when it is given (NULL: 0..M-1); a spatial order such as
+ * nat_topology_row_order's keeps the atomic additions of neighbouring rows
+ * on the same cache lines.  Float64 atomic accumulation: the order of the
+ * additions into one output is not fixed. */
 int nat_spmm_t_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
                    const int32_t *d_cols, const double *d_vals, const double *d_gy,
-                   double *d_out, void *stream);
+                   const int32_t *d_row_order, double *d_out, void *stream);
 
 /* Kernel timings of the last nat_weights_compute / nat_weights_emit on a
  * mesh, from HIP events recorded on the stream the kernels ran on (no

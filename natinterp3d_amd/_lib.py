@@ -14,7 +14,8 @@ _lib = None
 EXPORTS = (
     "nat_mesh_build", "nat_mesh_free", "nat_mesh_get_info", "nat_mesh_export",
     "nat_get_weights", "nat_weights_compute", "nat_weights_emit", "nat_spmm_csr",
-    "nat_csr_sort_rows", "nat_topology_save", "nat_grad_queries", "nat_spmm_t_csr",
+    "nat_csr_sort_rows", "nat_topology_save", "nat_topology_row_order", "nat_grad_queries",
+    "nat_spmm_t_csr",
     "nat_mesh_set_timing", "nat_mesh_get_timings", "nat_free_host", "nat_last_error",
 )
 
@@ -76,10 +77,13 @@ def lib():
     L.nat_topology_save.restype = ctypes.c_int
     L.nat_topology_save.argtypes = [vp, vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), vp]
     L.nat_grad_queries.restype = ctypes.c_int
-    L.nat_grad_queries.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, ctypes.c_int, vp, vp]
+    L.nat_grad_queries.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, vp,
+                                   vp]
     L.nat_spmm_t_csr.restype = ctypes.c_int
     L.nat_spmm_t_csr.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp,
-                                 vp]
+                                 vp, vp]
+    L.nat_topology_row_order.restype = ctypes.c_int
+    L.nat_topology_row_order.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
     L.nat_mesh_set_timing.restype = ctypes.c_int
     L.nat_mesh_set_timing.argtypes = [vp, ctypes.c_int]
     L.nat_mesh_get_timings.restype = ctypes.c_int

@@ -75,12 +75,17 @@ class SibsonInterpolation(torch.autograd.Function):
             dq = torch.empty((M, 3), dtype=torch.float64, device=gy.device)
             if M:
                 _lib.check(L.nat_grad_queries(mesh.handle, ctypes.c_void_p(topo.data_ptr()),
-                                              _vp(q), M, _vp(gy), _vp(v), D, _vp(dq), stream),
-                           "nat_grad_queries")
+                                              _vp(q), M, _vp(crow), _vp(val), _vp(gy), _vp(v),
+                                              D, _vp(dq), stream), "nat_grad_queries")
             dq = dq.to(ctx.queries_dtype)
         if ctx.needs_input_grad[2]:
             dv = torch.empty(v.shape, dtype=torch.float64, device=gy.device)
+            order = ctypes.c_void_p()
+            if topo is not None and M:
+                _lib.check(L.nat_topology_row_order(ctypes.c_void_p(topo.data_ptr()), M,
+                                                    ctypes.byref(order)),
+                           "nat_topology_row_order")
             _lib.check(L.nat_spmm_t_csr(M, v.shape[0], D, _vp(crow), _vp(col), _vp(val),
-                                        _vp(gy), _vp(dv), stream), "nat_spmm_t_csr")
+                                        _vp(gy), order, _vp(dv), stream), "nat_spmm_t_csr")
             dv = dv.to(ctx.values_dtype)
         return None, dq, dv

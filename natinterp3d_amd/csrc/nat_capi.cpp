@@ -110,7 +110,7 @@ struct nat_mesh {
     DevBuf q;                    // host-path query upload
     DevBuf order, keys, hist;    // Morton bucketing
     DevBuf scan;                 // int64 scan scratch
-    DevBuf row_off, row_cnt, rowptr64;
+    DevBuf row_off, row_cnt, rowptr64, row_total;
     DevBuf ovf[2];
     DevBuf counters;             // [0..7] unsigned: work counters, ovf counts; [8] u64 pool_used
     DevBuf pool_vals, pool_cols;
@@ -133,7 +133,8 @@ struct nat_mesh {
     }
     int64_t workspace_bytes() const {
         size_t s = q.bytes + order.bytes + keys.bytes + hist.bytes + scan.bytes +
-                   row_off.bytes + row_cnt.bytes + rowptr64.bytes + ovf[0].bytes +
+                   row_off.bytes + row_cnt.bytes + rowptr64.bytes + row_total.bytes +
+                   ovf[0].bytes +
                    ovf[1].bytes + counters.bytes + pool_vals.bytes + pool_cols.bytes +
                    csr_vals.bytes + csr_cols.bytes + csr_rowptr.bytes +
                    rec.bytes + rec_off.bytes + start.bytes;
@@ -142,7 +143,7 @@ struct nat_mesh {
     }
     void release_all() {
         DevBuf *all[] = {&verts, &tet_v, &tet_n, &kd, &cell_start, &cell_pts, &q, &order,
-                         &keys, &hist, &scan, &row_off, &row_cnt, &rowptr64, &ovf[0],
+                         &keys, &hist, &scan, &row_off, &row_cnt, &rowptr64, &row_total, &ovf[0],
                          &ovf[1], &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,
                          &csr_rowptr, &rec, &rec_off, &start};
         for (DevBuf *b : all) b->release();
@@ -361,6 +362,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
     HIP_OR_FAIL(m->scan.ensure(sizeof(long long) * scan_elems));
     HIP_OR_FAIL(m->row_off.ensure(sizeof(long long) * Mz));
     HIP_OR_FAIL(m->row_cnt.ensure(sizeof(int) * Mz));
+    HIP_OR_FAIL(m->row_total.ensure(sizeof(double) * Mz));
     HIP_OR_FAIL(m->rowptr64.ensure(sizeof(long long) * (Mz + 1)));
     HIP_OR_FAIL(m->ovf[0].ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->ovf[1].ensure(sizeof(int) * Mz));
@@ -403,6 +405,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
         out.pool_used = pool_used;
         out.row_off = m->row_off.as<long long>();
         out.row_cnt = m->row_cnt.as<int>();
+        out.row_total = m->row_total.as<double>();
 
         // level 0: start tets, topology records, then rows from them
         nat::Scratch sc0;
@@ -487,15 +490,20 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
 }
 
 // Saved topology: int64 header {magic, M, record units, N}, rec_off
-// int64[M], the query order int32[M], then the records (16-byte aligned).
-constexpr long long kTopoMagic = 0x4e41545450304c31LL;
+// int64[M], the query order int32[M], the row totals float64[M] (8-byte
+// aligned), then the records (16-byte aligned).
+constexpr long long kTopoMagic = 0x4e41545450304c32LL;
 size_t topo_order_off(long long M) { return 32 + 8 * static_cast<size_t>(M); }
+size_t topo_total_off(long long M) {
+    return (topo_order_off(M) + 4 * static_cast<size_t>(M) + 7) & ~static_cast<size_t>(7);
+}
 size_t topo_rec_off(long long M) {
-    return (topo_order_off(M) + 4 * static_cast<size_t>(M) + 15) & ~static_cast<size_t>(15);
+    return (topo_total_off(M) + 8 * static_cast<size_t>(M) + 15) & ~static_cast<size_t>(15);
 }
 
-int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const double *d_gy,
-              const double *d_values, int D, double *d_dq, hipStream_t st) {
+int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int *d_rowptr,
+              const double *d_w, const double *d_gy, const double *d_values, int D,
+              double *d_dq, hipStream_t st) {
     long long hdr[4];
     HIP_OR_FAIL(hipMemcpyAsync(hdr, blob, sizeof(hdr), hipMemcpyDeviceToHost, st));
     HIP_OR_FAIL(hipStreamSynchronize(st));
@@ -516,7 +524,8 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const dou
     topo.rec_used = nullptr;
     topo.rec_off = reinterpret_cast<long long *>(const_cast<char *>(blob) + 32);
     const int *order = reinterpret_cast<const int *>(blob + topo_order_off(M));
-    nat::GradIn g{d_gy, d_values, D, d_dq};
+    nat::GradIn g{d_gy, d_values, D, d_dq, d_rowptr, d_w,
+                  reinterpret_cast<const double *>(blob + topo_total_off(M))};
     for (int l = 0; l < kLevels; l++) {
         m->times.grad_ms[l] = 0.0;
         m->times.grad_items[l] = 0;
@@ -790,6 +799,8 @@ int nat_topology_save(nat_mesh *m, void *d_dst, int64_t capacity, int64_t *bytes
                                    hipMemcpyDeviceToDevice, st));
         HIP_OR_FAIL(hipMemcpyAsync(dst + topo_order_off(M), m->order.p,
                                    4 * static_cast<size_t>(M), hipMemcpyDeviceToDevice, st));
+        HIP_OR_FAIL(hipMemcpyAsync(dst + topo_total_off(M), m->row_total.p,
+                                   8 * static_cast<size_t>(M), hipMemcpyDeviceToDevice, st));
         if (units > 0)
             HIP_OR_FAIL(hipMemcpyAsync(dst + topo_rec_off(M), m->rec.p,
                                        sizeof(int4) * static_cast<size_t>(units),
@@ -801,30 +812,45 @@ int nat_topology_save(nat_mesh *m, void *d_dst, int64_t capacity, int64_t *bytes
 }
 
 int nat_grad_queries(nat_mesh *m, const void *d_topo, const double *d_queries,
-                     int num_query_points, const double *d_gy, const double *d_values,
-                     int dim, double *d_dq, void *stream) {
+                     int num_query_points, const int32_t *d_rowptr, const double *d_vals,
+                     const double *d_gy, const double *d_values, int dim, double *d_dq,
+                     void *stream) {
     g_err.clear();
     if (!m || !d_topo) return fail(-1, "NULL argument");
     if (num_query_points < 0 || dim <= 0) return fail(-1, "bad size (M %d, D %d)",
                                                       num_query_points, dim);
     if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
-    if (num_query_points > 0 && (!d_queries || !d_gy || !d_values || !d_dq))
+    if (num_query_points > 0 &&
+        (!d_queries || !d_rowptr || !d_gy || !d_values || !d_dq))
         return fail(-1, "NULL argument");
     std::lock_guard<std::mutex> lk(m->mu);
     DeviceGuard g(m->device);
-    return grad_rows(m, static_cast<const char *>(d_topo), d_queries, num_query_points, d_gy,
-                     d_values, dim, d_dq, static_cast<hipStream_t>(stream));
+    return grad_rows(m, static_cast<const char *>(d_topo), d_queries, num_query_points, d_rowptr,
+                     d_vals, d_gy, d_values, dim, d_dq, static_cast<hipStream_t>(stream));
+}
+
+int nat_topology_row_order(const void *d_topo, int num_query_points, const int32_t **out) {
+    g_err.clear();
+    if (!d_topo || !out) return fail(-1, "NULL argument");
+    long long hdr[4];
+    HIP_OR_FAIL(hipMemcpy(hdr, d_topo, sizeof(hdr), hipMemcpyDeviceToHost));
+    if (hdr[0] != kTopoMagic) return fail(-1, "d_topo is not a saved topology");
+    if (hdr[1] != num_query_points)
+        return fail(-1, "saved topology has %lld queries, not %d", hdr[1], num_query_points);
+    *out = reinterpret_cast<const int32_t *>(static_cast<const char *>(d_topo) +
+                                             topo_order_off(num_query_points));
+    return 0;
 }
 
 int nat_spmm_t_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
                    const int32_t *d_cols, const double *d_vals, const double *d_gy,
-                   double *d_out, void *stream) {
+                   const int32_t *d_row_order, double *d_out, void *stream) {
     g_err.clear();
     if (num_rows < 0 || num_cols < 0 || dim < 0) return fail(-1, "negative size");
     if (num_cols == 0 || dim == 0) return 0;
     if (!d_out || (num_rows > 0 && (!d_rowptr || !d_gy))) return fail(-1, "NULL argument");
     hipError_t e = nat::launch_spmm_t(num_rows, num_cols, dim, d_rowptr, d_cols, d_vals, d_gy,
-                                      d_out, static_cast<hipStream_t>(stream));
+                                      d_row_order, d_out, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail(-10, "spmm_t launch failed: %s", hipGetErrorString(e));
     return 0;
 }

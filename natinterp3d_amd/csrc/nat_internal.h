@@ -74,6 +74,7 @@ struct RowOut {
     unsigned long long *pool_used;
     long long *row_off;          // [M] offset of each row in the pool
     int *row_cnt;                // [M] row length
+    double *row_total;           // [M] total stolen volume of each row (level 0 rows)
     int *ovf_list;               // queries that exceeded the level's capacities
     unsigned int *ovf_count;
 };
@@ -87,12 +88,16 @@ struct Topo {
     long long *rec_off;          // [M] record offset, -1 = row made by another level
 };
 
-// Backward inputs: dL/dy rows and the interpolated values, both float64.
+// Backward inputs: dL/dy rows and the interpolated values, both float64;
+// the forward's CSR (rows in neighbour-slot order) and row totals.
 struct GradIn {
-    const double *gy;      // [M, D]
-    const double *values;  // [N, D]
+    const double *gy;         // [M, D]
+    const double *values;     // [N, D]
     int D;
-    double *dq;            // [M, 3] out: dL/dq
+    double *dq;               // [M, 3] out: dL/dq
+    const int *rowptr;        // [M + 1]
+    const double *w;          // [nnz] weights
+    const double *row_total;  // [M] total stolen volume (rows with a record)
 };
 
 size_t scratch_bytes_per_thread(const Caps &c);
@@ -131,7 +136,8 @@ hipError_t launch_grad_rows(const DevMesh &m, const Caps &caps, const Scratch &s
                             unsigned int *d_work, const GradIn &g, int *ovf_list,
                             unsigned int *ovf_count, hipStream_t st);
 hipError_t launch_spmm_t(int M, int N, int D, const int *rowptr, const int *cols,
-                         const double *vals, const double *gy, double *out, hipStream_t st);
+                         const double *vals, const double *gy, const int *order, double *out,
+                         hipStream_t st);
 // exclusive scan of n int32 counts into int64 offsets; total in out[n]
 hipError_t launch_scan_counts(const int *d_in, long long *d_out, int n, long long *d_tmp,
                               hipStream_t st);

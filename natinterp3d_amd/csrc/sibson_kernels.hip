@@ -466,8 +466,9 @@ struct NoMoment {
 
 // Step 4 for one boundary face, face fi of cavity tet t
 // (natural_insertionfree.c:681-719): calls sub(slot, vol) per face vertex,
-// and mom(slot, facet moment) when the gradient is wanted.
-template <typename SlotOf, typename Sub, typename Mom = NoMoment>
+// and mom(slot, facet moment) when the gradient is wanted.  WithVol = false
+// computes the moments only.
+template <bool WithVol = true, typename SlotOf, typename Sub, typename Mom = NoMoment>
 __device__ __forceinline__ void new_contributions(const DevMesh &m, int t, int fi, const d3 &q,
                                                   SlotOf slot_of, Sub sub, Mom mom = Mom()) {
     bool neg;
@@ -479,7 +480,8 @@ __device__ __forceinline__ void new_contributions(const DevMesh &m, int t, int f
     // the face's circumcentre in ascending slot order (as face_cc_own):
     // faces 0 and 2 list their slots ascending, faces 1 and 3 descending
     const bool desc = fi & 1;
-    const d3 cf = cc_tri(desc ? F[2] : F[0], F[1], desc ? F[0] : F[2]);
+    d3 cf = {0.0, 0.0, 0.0};
+    if constexpr (WithVol) cf = cc_tri(desc ? F[2] : F[0], F[1], desc ? F[0] : F[2]);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const int slot = slot_of(s[k], sel4(tv, s[k]));
@@ -488,7 +490,7 @@ __device__ __forceinline__ void new_contributions(const DevMesh &m, int t, int f
         const d3 &B = F[(k + 2) % 3];
         const d3 ca = cc_tri(F[k], q, A);
         const d3 cb = cc_tri(F[k], q, B);
-        sub(slot, subcell_volume(F[k], q, A, B, cv, ca, cb, cf));
+        if constexpr (WithVol) sub(slot, subcell_volume(F[k], q, A, B, cv, ca, cb, cf));
         if constexpr (Mom::active) mom(slot, facet_moment(F[k], q, ca, cb, cv));
     }
 }
@@ -796,11 +798,11 @@ constexpr int GPB = 8;        // queries per 128-thread block
 constexpr int VT = 128;       // visited-table slots
 constexpr int VMAX = 96;      // visited tets
 constexpr int CMAX = 64;      // cavity tets
-constexpr int BMAX = 160;     // boundary faces
+constexpr int BMAX = 136;     // boundary faces (a cavity of C tets has about 2C + 2)
 constexpr int NT_LOG2 = 6;
 constexpr int NT = 1 << NT_LOG2;  // neighbour-table slots
 constexpr int NMAX = kLevel0Row;  // neighbours
-constexpr int SMAX = 256;     // order-replay stack
+constexpr int SMAX = 2 * VMAX;  // order-replay stack (shares the fronts' storage)
 }  // namespace grp
 
 struct GroupMem {
@@ -808,13 +810,15 @@ struct GroupMem {
     unsigned vnb[grp::VT];         // visited slots of the 4 neighbours, 8 bits each (0xff none)
     unsigned char vflag[grp::VT];  // 1 tested, 2 inside the circumsphere, 4 popped by the replay
     unsigned char vpos[grp::VT];   // cavity position
-    unsigned char front[2][grp::VMAX];
-    unsigned char stk[grp::SMAX];
+    union {  // the fronts live during the search, the stack during the replay
+        unsigned char front[2][grp::VMAX];
+        unsigned char stk[grp::SMAX];
+    };
     unsigned char cav[grp::CMAX];  // visited slots in cavity order
     unsigned short bnd[grp::BMAX]; // boundary faces, cavity position << 2 | face
     int nkey[grp::NT];             // vertex index, -1 = empty slot
     int nmin[grp::NT];             // first appearance (face-vertex sequence number)
-    int nslot[grp::NT];            // neighbour slot
+    unsigned char nslot[grp::NT];  // neighbour slot
     int nbv[grp::NMAX];            // neighbours in slot order
     int cnt[4];                    // frontier sizes, visited count, overflow flag
 };
@@ -897,13 +901,10 @@ __device__ __forceinline__ int face_vertex(const DevMesh &m, const GroupMem &g, 
 //      neighbour slots by least first appearance, ranked in sequence order
 //      (natural_insertionfree.c:559-584)
 //   E  the record, written by all 16 lanes
-__global__ void __launch_bounds__(128) topo_group_kernel(DevMesh m, const double *__restrict__ q,
-                                                         const int *__restrict__ items,
-                                                         int n_items,
-                                                         const int *__restrict__ start_of,
-                                                         unsigned int *work, Topo topo,
-                                                         unsigned int *ovf_count,
-                                                         int *ovf_list) {
+__global__ void __launch_bounds__(128)
+topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict__ items,
+                  int n_items, const int *__restrict__ start_of, unsigned int *work, Topo topo,
+                  unsigned int *ovf_count, int *ovf_list) {
     __shared__ GroupMem gmem[grp::GPB];
     const int lane = __lane_id();
     const int gl = lane & (grp::G - 1);
@@ -959,13 +960,13 @@ __global__ void __launch_bounds__(128) topo_group_kernel(DevMesh m, const double
                     const int t = g.vkey[slot];
                     bool neg;
                     const int4 tv = ld_tet_v(m, t, neg);
+                    const int4 tn = m.tet_n[t];  // issued with tv, used if inside
                     const d3 p0 = ld_vert(m, tv.x), p1 = ld_vert(m, tv.y);
                     const d3 p2 = ld_vert(m, tv.z), p3 = ld_vert(m, tv.w);
                     const double ins =
                         neg ? insphere(p1, p0, p2, p3, qq) : insphere(p0, p1, p2, p3, qq);
                     unsigned packed = 0xffffffffu;
                     if (ins > 0) {
-                        const int4 tn = m.tet_n[t];
                         packed = 0;
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
@@ -1091,7 +1092,7 @@ __global__ void __launch_bounds__(128) topo_group_kernel(DevMesh m, const double
                         static_cast<unsigned>(__ballot(first) >> (16 * gw)) & 0xffffu;
                     if (first) {
                         const int slot = nnb + __popc(mine & ((1u << gl) - 1u));
-                        g.nslot[h] = slot;
+                        g.nslot[h] = static_cast<unsigned char>(slot);
                         if (slot < grp::NMAX) g.nbv[slot] = v;
                     }
                     nnb += __popc(mine);
@@ -1203,6 +1204,7 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
                 }
                 double total = 0.0;
                 for (int j = 0; j < nnb; j++) total += acc[j * kGeomThreads];
+                out.row_total[qi] = total;
                 if (total > 1e-30) {
                     const double inv = 1.0 / total;
                     for (int j = 0; j < nnb; j++) acc[j * kGeomThreads] *= inv;
@@ -1235,10 +1237,13 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
     }
 }
 
-// Gradient of the rows that have a topology record (the cached cavity):
-// steps 3-4 recomputed from the record with the gradient reduction.  Queries
-// without a record (made by a larger level in the forward pass) are listed in
-// ovf_list.  nb_a holds <dL/dy, values[j]> per slot (interleaved per thread).
+// Gradient of the rows that have a topology record (the cached cavity).
+// With lambda_j the forward's weights and S its total stolen volume,
+// dL/dq = (sum_j a_j G_j - abar sum_j G_j) / S, abar = sum_j a_j lambda_j,
+// so only the step-4 circumcentres and facet moments G_j are recomputed: no
+// old subcells, no volumes.  Queries without a record (made by a larger
+// level in the forward pass) are listed in ovf_list.  a_j =
+// <dL/dy, values[j]> per neighbour slot lives in LDS.
 __global__ void __launch_bounds__(kGeomThreads) grad_geom_kernel(
     DevMesh m, const double *__restrict__ q, const int *__restrict__ items, int n_items,
     unsigned int *work, Topo topo, GradIn g, int *ovf_list, unsigned int *ovf_count) {
@@ -1260,46 +1265,55 @@ __global__ void __launch_bounds__(kGeomThreads) grad_geom_kernel(
             ovf_list[atomicAdd(ovf_count, 1u)] = qi;
             continue;
         }
-        if (ncav == 0) {  // coincident or empty row: dq = 0
+        const int e0 = g.rowptr[qi];
+        // coincident rows (constant weights nearby) and empty rows: dq = 0
+        if (ncav == 0 || g.rowptr[qi + 1] - e0 != nnb) {
             store_dq(g, qi, {0.0, 0.0, 0.0});
             continue;
         }
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
         const int *rn = reinterpret_cast<const int *>(r + 1 + ncav + (nbnd + 3) / 4);
-        for (int j = 0; j < nnb; j++) nb_a[j * kGeomThreads] = grad_dot(g, qi, rn[j]);
-        auto a_of = [&](int slot) { return nb_a[slot * kGeomThreads]; };
-        GradAcc acc;
-        for (int ci = 0; ci < ncav; ci++) {
-            const int4 cr = r[1 + ci];
-            old_contributions(
-                m, cr.x, static_cast<unsigned>(cr.w), [&](int vi, int) { return rec_slot(cr, vi); },
-                [&](int slot, double vol) { acc.vol(a_of(slot), vol); });
+        double abar = 0.0;
+        for (int j = 0; j < nnb; j++) {
+            const double a = grad_dot(g, qi, rn[j]);
+            nb_a[j * kGeomThreads] = a;
+            abar += a * g.w[e0 + j];
         }
+        d3 ag = {0.0, 0.0, 0.0}, tg = {0.0, 0.0, 0.0};
         const int *rf = reinterpret_cast<const int *>(r + 1 + ncav);
         for (int bi = 0; bi < nbnd; bi++) {
             const int code = rf[bi];
             const int4 cr = r[1 + (code >> 2)];
-            new_contributions(
+            new_contributions<false>(
                 m, cr.x, code & 3, qq, [&](int vi, int) { return rec_slot(cr, vi); },
-                [&](int slot, double vol) { acc.vol(a_of(slot), -vol); },
-                moment_fn([&](int slot, const d3 &gm) { acc.mom(a_of(slot), gm); }));
+                [](int, double) {},
+                moment_fn([&](int slot, const d3 &gm) {
+                    const double a = nb_a[slot * kGeomThreads];
+                    ag = {ag.x + a * gm.x, ag.y + a * gm.y, ag.z + a * gm.z};
+                    tg = {tg.x + gm.x, tg.y + gm.y, tg.z + gm.z};
+                }));
         }
-        store_dq(g, qi, acc.result());
+        const double inv = 1.0 / g.row_total[qi];
+        store_dq(g, qi, {(ag.x - abar * tg.x) * inv, (ag.y - abar * tg.y) * inv,
+                         (ag.z - abar * tg.z) * inv});
     }
 }
 
 // W^T g: out[cols[e], d] += vals[e] * gy[row, d] (float64 atomics; the
-// summation order across rows is not fixed).  One thread per (row, d).
+// summation order across rows is not fixed).  One thread per (row, d), rows
+// taken in `order` when given.
 __global__ void __launch_bounds__(256) spmm_t_kernel(int M, int D, const int *__restrict__ rowptr,
                                                      const int *__restrict__ cols,
                                                      const double *__restrict__ vals,
                                                      const double *__restrict__ gy,
+                                                     const int *__restrict__ order,
                                                      double *__restrict__ out) {
     const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= static_cast<long long>(M) * D) return;
-    const int row = static_cast<int>(t / D), d = static_cast<int>(t % D);
-    const double gv = gy[t];
+    const int i = static_cast<int>(t / D), d = static_cast<int>(t % D);
+    const int row = order ? order[i] : i;
+    const double gv = gy[static_cast<size_t>(row) * D + d];
     if (gv == 0.0) return;
     for (int e = rowptr[row]; e < rowptr[row + 1]; e++)
         atomicAdd(out + static_cast<size_t>(cols[e]) * D + d, vals[e] * gv);
@@ -1421,7 +1435,8 @@ hipError_t launch_grad_geom(const DevMesh &m, int nblocks, const double *d_q, co
 }
 
 hipError_t launch_spmm_t(int M, int N, int D, const int *rowptr, const int *cols,
-                         const double *vals, const double *gy, double *out, hipStream_t st) {
+                         const double *vals, const double *gy, const int *order, double *out,
+                         hipStream_t st) {
     const long long nz = static_cast<long long>(N) * D;
     if (nz > 0)
         hipLaunchKernelGGL(fill_zero_kernel, dim3(static_cast<unsigned>((nz + 255) / 256)),
@@ -1429,7 +1444,7 @@ hipError_t launch_spmm_t(int M, int N, int D, const int *rowptr, const int *cols
     const long long nt = static_cast<long long>(M) * D;
     if (nt > 0)
         hipLaunchKernelGGL(spmm_t_kernel, dim3(static_cast<unsigned>((nt + 255) / 256)),
-                           dim3(256), 0, st, M, D, rowptr, cols, vals, gy, out);
+                           dim3(256), 0, st, M, D, rowptr, cols, vals, gy, order, out);
     return hipGetLastError();
 }
 

@@ -220,8 +220,9 @@ def test_grad_queries_rejects_mismatched_topology():
     gy = torch.zeros((99, 1), dtype=torch.float64, device=dev)
     vals = torch.zeros((500, 1), dtype=torch.float64, device=dev)
     dq = torch.zeros((99, 3), dtype=torch.float64, device=dev)
+    rowptr = torch.zeros(100, dtype=torch.int32, device=dev)
     rc = L.nat_grad_queries(m.handle, ctypes.c_void_p(topo.data_ptr()),
-                            ctypes.c_void_p(q.data_ptr()), 99, ctypes.c_void_p(gy.data_ptr()),
-                            ctypes.c_void_p(vals.data_ptr()), 1, ctypes.c_void_p(dq.data_ptr()),
-                            None)
+                            ctypes.c_void_p(q.data_ptr()), 99, ctypes.c_void_p(rowptr.data_ptr()),
+                            None, ctypes.c_void_p(gy.data_ptr()), ctypes.c_void_p(vals.data_ptr()),
+                            1, ctypes.c_void_p(dq.data_ptr()), None)
     assert rc == -1 and "100 queries" in _lib.last_error()
