@@ -109,6 +109,7 @@ struct nat_mesh {
     double units_per_row = 40.0;
     DevBuf q;                    // host-path query upload
     DevBuf order, keys, hist;    // Morton bucketing
+    DevBuf geom_order;           // work order of the forward geometry kernel
     DevBuf scan;                 // int64 scan scratch
     DevBuf row_off, row_cnt, rowptr64, row_total;
     DevBuf ovf[2];
@@ -132,7 +133,7 @@ struct nat_mesh {
                                     cell_start.bytes + cell_pts.bytes);
     }
     int64_t workspace_bytes() const {
-        size_t s = q.bytes + order.bytes + keys.bytes + hist.bytes + scan.bytes +
+        size_t s = q.bytes + order.bytes + geom_order.bytes + keys.bytes + hist.bytes + scan.bytes +
                    row_off.bytes + row_cnt.bytes + rowptr64.bytes + row_total.bytes +
                    ovf[0].bytes +
                    ovf[1].bytes + counters.bytes + pool_vals.bytes + pool_cols.bytes +
@@ -142,10 +143,11 @@ struct nat_mesh {
         return static_cast<int64_t>(s);
     }
     void release_all() {
-        DevBuf *all[] = {&verts, &tet_v, &tet_n, &kd, &cell_start, &cell_pts, &q, &order,
-                         &keys, &hist, &scan, &row_off, &row_cnt, &rowptr64, &row_total, &ovf[0],
-                         &ovf[1], &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,
-                         &csr_rowptr, &rec, &rec_off, &start};
+        DevBuf *all[] = {&verts,    &tet_v,     &tet_n,     &kd,       &cell_start, &cell_pts,
+                         &q,        &order,     &geom_order, &keys,    &hist,       &scan,
+                         &row_off,  &row_cnt,   &rowptr64,  &row_total, &ovf[0],    &ovf[1],
+                         &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,   &csr_rowptr,
+                         &rec,      &rec_off,   &start};
         for (DevBuf *b : all) b->release();
         for (int l = 0; l < kLevels; l++) scratch[l].release();
         for (hipEvent_t &e : ev) {
@@ -357,6 +359,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
         std::max(nat::scan_tmp_elems(M) + Mz + 1,
                  static_cast<size_t>(nbuckets) + 1 + nat::scan_tmp_elems(nbuckets));
     HIP_OR_FAIL(m->order.ensure(sizeof(int) * Mz));
+    HIP_OR_FAIL(m->geom_order.ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->keys.ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->hist.ensure(sizeof(int) * nbuckets));
     HIP_OR_FAIL(m->scan.ensure(sizeof(long long) * scan_elems));
@@ -439,8 +442,10 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
         m->units_per_row = std::max(8.0, 1.125 * static_cast<double>(units) / M);
         m->last_units = units;
         mark(m, 2, st);
+        HIP_OR_FAIL(nat::launch_work_sort(m->order.as<int>(), M, topo, m->geom_order.as<int>(),
+                                          st));
         HIP_OR_FAIL(nat::launch_geom(m->dev, m->nthr_geom / nat::kGeomThreads, d_q,
-                                     m->order.as<int>(), M, ctr + 5, topo, out, st));
+                                     m->geom_order.as<int>(), M, ctr + 5, topo, out, st));
         mark(m, 3, st);
         m->times.geom_ms = elapsed(m, 2, 3);
         m->times.rows_ms[0] = m->times.topo_ms + m->times.geom_ms;
@@ -797,6 +802,8 @@ int nat_topology_save(nat_mesh *m, void *d_dst, int64_t capacity, int64_t *bytes
     if (M > 0) {
         HIP_OR_FAIL(hipMemcpyAsync(dst + 32, m->rec_off.p, 8 * static_cast<size_t>(M),
                                    hipMemcpyDeviceToDevice, st));
+        // the backward keeps the Morton order: its gathers (values, dL/dy)
+        // favour spatial locality over grouping by cavity size
         HIP_OR_FAIL(hipMemcpyAsync(dst + topo_order_off(M), m->order.p,
                                    4 * static_cast<size_t>(M), hipMemcpyDeviceToDevice, st));
         HIP_OR_FAIL(hipMemcpyAsync(dst + topo_total_off(M), m->row_total.p,

@@ -122,6 +122,9 @@ hipError_t launch_locate(const DevMesh &m, const Caps &caps, const Scratch &sc, 
 hipError_t launch_topo(const DevMesh &m, int nblocks, const double *d_q, const int *d_items,
                        int n_items, const int *d_start, unsigned int *d_work, const Topo &topo,
                        unsigned int *ovf_count, int *ovf_list, hipStream_t st);
+// the items regrouped by record size within spatial windows (geometry order)
+hipError_t launch_work_sort(const int *d_items, int n, const Topo &topo, int *d_out,
+                            hipStream_t st);
 hipError_t launch_geom(const DevMesh &m, int nblocks, const double *d_q, const int *d_items,
                        int n_items, unsigned int *d_work, const Topo &topo, const RowOut &out,
                        hipStream_t st);

@@ -372,32 +372,86 @@ __device__ int coincident(const DevMesh &m, int start, const d3 &q, int out[4]) 
     return n;
 }
 
-// Circumcentre of the face of a tet opposite its vertex k, vertices in
-// ascending slot order -- the order faceCCLookup is first called with
-// (natural_insertionfree.c:649-652).
-__device__ __forceinline__ d3 face_cc_own(const d3 P[4], int k) {
-    const int a = k == 0 ? 1 : 0;
-    const int b = k <= 1 ? 2 : 1;
-    const int c = k <= 2 ? 3 : 2;
-    return cc_tri(P[a], P[b], P[c]);
+// Slot of vertex v in a tet's vertex record (v is one of the four).
+__device__ __forceinline__ int slot_in(const int4 &tv, int v) {
+    return tv.x == v ? 0 : (tv.y == v ? 1 : (tv.z == v ? 2 : 3));
 }
 
-// Circumcentre of face k of tet t (opposite its vertex k) evaluated in the
-// ascending-slot order of neighbour `nb`, which holds the same face.
-__device__ d3 face_cc_from_neighbour(const DevMesh &m, const int4 &tv, int k, int nb) {
-    bool nneg;
-    const int4 nv = ld_tet_v(m, nb, nneg);
-    const int fa = sel4(tv, k == 0 ? 1 : 0), fb = sel4(tv, k <= 1 ? 2 : 1),
-              fc = sel4(tv, k <= 2 ? 3 : 2);
-    int kk = 0;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int x = sel4(nv, j);
-        if (x != fa && x != fb && x != fc) kk = j;
+// Component-wise: a conditional on whole structs would select addresses and
+// send the points through private memory.
+__device__ __forceinline__ void cswap(d3 &x, d3 &y, int &rx, int &ry) {
+    const bool s = rx > ry;
+    const double x0 = x.x, x1 = x.y, x2 = x.z;
+    x.x = s ? y.x : x0;
+    x.y = s ? y.y : x1;
+    x.z = s ? y.z : x2;
+    y.x = s ? x0 : y.x;
+    y.y = s ? x1 : y.y;
+    y.z = s ? x2 : y.z;
+    const int r = rx;
+    rx = s ? ry : rx;
+    ry = s ? r : ry;
+}
+
+// Orders three points by ascending rank (ranks distinct), with selects only.
+__device__ __forceinline__ void sort3_by_rank(d3 &x0, d3 &x1, d3 &x2, int r0, int r1, int r2) {
+    cswap(x0, x1, r0, r1);
+    cswap(x1, x2, r1, r2);
+    cswap(x0, x1, r0, r1);
+}
+
+// Vertex slots of the face of a tet opposite its vertex K, ascending.
+template <int K>
+struct FaceOf {
+    static constexpr int a = K == 0 ? 1 : 0, b = K <= 1 ? 2 : 1, c = K <= 2 ? 3 : 2;
+};
+
+// The four points of a tet as separate values (not an array: selects among
+// them must stay selects rather than become indexed private memory).
+struct Quad {
+    d3 p0, p1, p2, p3;
+    template <int I>
+    __device__ __forceinline__ const d3 &get() const {
+        if constexpr (I == 0) return p0;
+        else if constexpr (I == 1) return p1;
+        else if constexpr (I == 2) return p2;
+        else return p3;
     }
-    const int ia = sel4(nv, kk == 0 ? 1 : 0), ib = sel4(nv, kk <= 1 ? 2 : 1),
-              ic = sel4(nv, kk <= 2 ? 3 : 2);
-    return cc_tri(ld_vert(m, ia), ld_vert(m, ib), ld_vert(m, ic));
+};
+
+// Circumcentre of face K of a cavity tet (natural_insertionfree.c:649-652):
+// its three vertices in ascending slot order of this tet, or, when fcmask
+// bit K says an earlier cavity tet holds the face (the value was computed
+// first there), in that tet's ascending slot order.  Both orders are of the
+// same three points, so only the order is looked up.
+template <int K>
+__device__ __forceinline__ d3 step3_face_cc(const DevMesh &m, const Quad &P, const int4 &tv,
+                                            const int4 &tn, unsigned fcmask) {
+    constexpr int a = FaceOf<K>::a, b = FaceOf<K>::b, c = FaceOf<K>::c;
+    d3 x0 = P.get<a>(), x1 = P.get<b>(), x2 = P.get<c>();
+    int r0 = 0, r1 = 1, r2 = 2;
+    if ((fcmask >> K) & 1u) {
+        bool nneg;
+        const int4 nv = ld_tet_v(m, sel4(tn, 3 - K), nneg);
+        r0 = slot_in(nv, sel4(tv, a));
+        r1 = slot_in(nv, sel4(tv, b));
+        r2 = slot_in(nv, sel4(tv, c));
+    }
+    sort3_by_rank(x0, x1, x2, r0, r1, r2);
+    return cc_tri(x0, x1, x2);
+}
+
+// Old subcell of vertex VI: the other three vertices ascending, the face
+// circumcentres opposite them in descending order (natural_insertionfree.c:
+// 660-672).
+template <int VI, typename SlotOf, typename Add>
+__device__ __forceinline__ void step3_vertex(const Quad &P, const Quad &F, const d3 &ct,
+                                             const int4 &tv, SlotOf slot_of, Add add) {
+    constexpr int a = FaceOf<VI>::a, b = FaceOf<VI>::b, c = FaceOf<VI>::c;
+    const int slot = slot_of(VI, sel4(tv, VI));
+    if (slot < 0) return;
+    add(slot, subcell_volume(P.get<VI>(), P.get<a>(), P.get<b>(), P.get<c>(), ct, F.get<c>(),
+                             F.get<b>(), F.get<a>()));
 }
 
 // Step 3 for one cavity tet (natural_insertionfree.c:642-676): calls
@@ -408,32 +462,16 @@ __device__ __forceinline__ void old_contributions(const DevMesh &m, int t, unsig
                                                   SlotOf slot_of, Add add) {
     bool neg;
     const int4 tv = ld_tet_v(m, t, neg);
-    d3 P[4];
-    P[0] = ld_vert(m, tv.x);
-    P[1] = ld_vert(m, tv.y);
-    P[2] = ld_vert(m, tv.z);
-    P[3] = ld_vert(m, tv.w);
-    const d3 ct = cc_tet(P[0], P[1], P[2], P[3]);
-    d3 fcc[4];
+    const Quad P = {ld_vert(m, tv.x), ld_vert(m, tv.y), ld_vert(m, tv.z), ld_vert(m, tv.w)};
+    const d3 ct = cc_tet(P.p0, P.p1, P.p2, P.p3);
     int4 tn = make_int4(-1, -1, -1, -1);
     if (fcmask) tn = m.tet_n[t];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        if ((fcmask >> k) & 1u)
-            fcc[k] = face_cc_from_neighbour(m, tv, k, sel4(tn, 3 - k));
-        else
-            fcc[k] = face_cc_own(P, k);
-    }
-#pragma unroll
-    for (int vi = 0; vi < 4; vi++) {
-        const int slot = slot_of(vi, sel4(tv, vi));
-        if (slot < 0) continue;
-        int o0 = (vi + 1) & 3, o1 = (vi + 2) & 3, o2 = (vi + 3) & 3, tmp;
-        if (o2 < o0) { tmp = o0; o0 = o2; o2 = tmp; }
-        if (o1 > o2) { tmp = o1; o1 = o2; o2 = tmp; }
-        if (o0 > o1) { tmp = o0; o0 = o1; o1 = tmp; }
-        add(slot, subcell_volume(P[vi], P[o0], P[o1], P[o2], ct, fcc[o2], fcc[o1], fcc[o0]));
-    }
+    const Quad F = {step3_face_cc<0>(m, P, tv, tn, fcmask), step3_face_cc<1>(m, P, tv, tn, fcmask),
+                    step3_face_cc<2>(m, P, tv, tn, fcmask), step3_face_cc<3>(m, P, tv, tn, fcmask)};
+    step3_vertex<0>(P, F, ct, tv, slot_of, add);
+    step3_vertex<1>(P, F, ct, tv, slot_of, add);
+    step3_vertex<2>(P, F, ct, tv, slot_of, add);
+    step3_vertex<3>(P, F, ct, tv, slot_of, add);
 }
 
 // Derivative of a stolen volume: d sigma_k / dq = (1 / |pk - q|) times the
@@ -477,8 +515,9 @@ __device__ __forceinline__ void new_contributions(const DevMesh &m, int t, int f
     const d3 F[3] = {ld_vert(m, sel4(tv, s[0])), ld_vert(m, sel4(tv, s[1])),
                      ld_vert(m, sel4(tv, s[2]))};
     const d3 cv = cc_tet(q, F[0], F[1], F[2]);
-    // the face's circumcentre in ascending slot order (as face_cc_own):
-    // faces 0 and 2 list their slots ascending, faces 1 and 3 descending
+    // the face's circumcentre in ascending slot order (the order of
+    // faceCCLookup's first call, natural_insertionfree.c:649-652): faces 0
+    // and 2 list their slots ascending, faces 1 and 3 descending
     const bool desc = fi & 1;
     d3 cf = {0.0, 0.0, 0.0};
     if constexpr (WithVol) cf = cc_tri(desc ? F[2] : F[0], F[1], desc ? F[0] : F[2]);
@@ -1152,6 +1191,49 @@ topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict
 
 // ----------------------------------------------------------- geometry kernel
 
+// Work order of the geometry kernels: the Morton-ordered items regrouped,
+// within windows of kSortWindow consecutive items (a compact region of
+// space), by the size of their cavity + boundary, so that the 64 rows of a
+// wave take about the same number of loop trips.  Rows without a record sort
+// first.
+constexpr int kSortWindow = 4096;
+constexpr int kSortKeys = 128;
+
+__global__ void __launch_bounds__(256) work_sort_kernel(const int *__restrict__ items, int n,
+                                                        Topo topo, int *__restrict__ out) {
+    __shared__ unsigned char key[kSortWindow];
+    __shared__ int cnt[kSortKeys];
+    const int base = blockIdx.x * kSortWindow;
+    const int len = min(kSortWindow, n - base);
+    for (int i = threadIdx.x; i < kSortKeys; i += blockDim.x) cnt[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        const int qi = items[base + i];
+        const long long roff = topo.rec_off[qi];
+        int k = 0;
+        if (roff >= 0) {
+            const int4 hd = topo.rec[roff];
+            k = min(hd.x + hd.y, kSortKeys - 1);
+        }
+        key[i] = static_cast<unsigned char>(k);
+        atomicAdd(&cnt[k], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {  // exclusive scan of the counts, one wave
+        const int lane = threadIdx.x;
+        int a = cnt[2 * lane], b = cnt[2 * lane + 1];
+        int total;
+        const int pre = wave_excl_scan(a + b, total);
+        cnt[2 * lane] = pre;
+        cnt[2 * lane + 1] = pre + a;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        const int pos = atomicAdd(&cnt[key[i]], 1);
+        out[base + pos] = items[base + i];
+    }
+}
+
 __device__ __forceinline__ int rec_slot(const int4 &cr, int vi) {
     const int w = vi < 2 ? cr.y : cr.z;
     const int s = (vi & 1) ? ((w >> 16) & 0xffff) : (w & 0xffff);
@@ -1405,6 +1487,14 @@ hipError_t launch_topo(const DevMesh &m, int nblocks, const double *d_q, const i
                        unsigned int *ovf_count, int *ovf_list, hipStream_t st) {
     hipLaunchKernelGGL(topo_group_kernel, dim3(nblocks), dim3(128), 0, st, m, d_q, d_items,
                        n_items, d_start, d_work, topo, ovf_count, ovf_list);
+    return hipGetLastError();
+}
+
+hipError_t launch_work_sort(const int *d_items, int n, const Topo &topo, int *d_out,
+                            hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(work_sort_kernel, dim3((n + kSortWindow - 1) / kSortWindow), dim3(256), 0,
+                       st, d_items, n, topo, d_out);
     return hipGetLastError();
 }
 
