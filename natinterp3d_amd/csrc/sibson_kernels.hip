@@ -868,18 +868,13 @@ __device__ __forceinline__ void group_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// visited-table insert: slot of t, -1 when the table is full
-__device__ __forceinline__ int vis_insert(GroupMem &g, int t, bool &isnew) {
-    unsigned h = hash_key(t, 7);
+// visited-table insert of tet t, probing from slot h: its slot (isnew when
+// this call put it there), -1 when the table is full
+__device__ __forceinline__ int vis_probe(GroupMem &g, int t, unsigned h, bool &isnew) {
     for (int n = 0; n < grp::VT; n++) {
         const int old = atomicCAS(&g.vkey[h], -1, t);
-        if (old == -1) {
-            isnew = true;
-            if (atomicAdd(&g.cnt[2], 1) >= grp::VMAX) g.cnt[3] = 1;
-            return static_cast<int>(h);
-        }
-        if (old == t) {
-            isnew = false;
+        if (old == -1 || old == t) {
+            isnew = old == -1;
             return static_cast<int>(h);
         }
         h = (h + 1) & (grp::VT - 1);
@@ -940,7 +935,9 @@ __device__ __forceinline__ int face_vertex(const DevMesh &m, const GroupMem &g, 
 //      neighbour slots by least first appearance, ranked in sequence order
 //      (natural_insertionfree.c:559-584)
 //   E  the record, written by all 16 lanes
-__global__ void __launch_bounds__(128)
+// 4 waves per SIMD (the 20 KB of LDS per block allow it) matter more here than
+// the few registers the compiler has to give up for them
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4)))
 topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict__ items,
                   int n_items, const int *__restrict__ start_of, unsigned int *work, Topo topo,
                   unsigned int *ovf_count, int *ovf_list) {
@@ -980,64 +977,105 @@ topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict
             }
             if (gl == 0) g.cnt[0] = g.cnt[1] = g.cnt[2] = g.cnt[3] = 0;
             group_sync();
-            // ---- phase B: test every tet the search pops, front by front
+            // ---- phase B: test every tet the search pops, front by front.
+            // The start tet contains the query, so it is in the cavity and its
+            // neighbours are tested with it in the first front.  (Should the
+            // rounded test still put it outside, the replay below never leaves
+            // it, and the extra tests only cost capacity.)
             int s0 = 0;
             if (gl == 0) {
                 bool isnew;
-                s0 = vis_insert(g, st, isnew);
+                s0 = vis_probe(g, st, hash_key(st, 7), isnew);
                 g.front[0][0] = static_cast<unsigned char>(s0);
-                g.cnt[0] = 1;
+                int n = 1;
+                const int4 tn0 = m.tet_n[st];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int nb = sel4(tn0, k);
+                    if (nb < 0) continue;
+                    const int s2 = vis_probe(g, nb, hash_key(nb, 7), isnew);
+                    if (s2 >= 0 && isnew) g.front[0][n++] = static_cast<unsigned char>(s2);
+                }
+                g.cnt[0] = n;
             }
             group_sync();
             int cur = 0;
             int nf = g.cnt[0];
+            int visited = nf;
             while (nf > 0) {
-                if (gl == 0) g.cnt[cur ^ 1] = 0;
-                group_sync();
-                for (int i = gl; i < nf; i += grp::G) {
-                    const int slot = g.front[cur][i];
-                    const int t = g.vkey[slot];
-                    bool neg;
-                    const int4 tv = ld_tet_v(m, t, neg);
-                    const int4 tn = m.tet_n[t];  // issued with tv, used if inside
-                    const d3 p0 = ld_vert(m, tv.x), p1 = ld_vert(m, tv.y);
-                    const d3 p2 = ld_vert(m, tv.z), p3 = ld_vert(m, tv.w);
-                    const double ins =
-                        neg ? insphere(p1, p0, p2, p3, qq) : insphere(p0, p1, p2, p3, qq);
-                    unsigned packed = 0xffffffffu;
-                    if (ins > 0) {
-                        packed = 0;
+                int nnext = 0;  // size of the next front (uniform over the 16 lanes)
+                for (int i0 = 0; i0 < nf; i0 += grp::G) {
+                    const int i = i0 + gl;
+                    unsigned newslots = 0;
+                    int nnew = 0;
+                    if (i < nf) {
+                        const int slot = g.front[cur][i];
+                        const int t = g.vkey[slot];
+                        bool neg;
+                        const int4 tv = ld_tet_v(m, t, neg);
+                        const int4 tn = m.tet_n[t];  // issued with tv, used if inside
+                        const d3 p0 = ld_vert(m, tv.x), p1 = ld_vert(m, tv.y);
+                        const d3 p2 = ld_vert(m, tv.z), p3 = ld_vert(m, tv.w);
+                        const double ins =
+                            neg ? insphere(p1, p0, p2, p3, qq) : insphere(p0, p1, p2, p3, qq);
+                        unsigned packed = 0xffffffffu;
+                        if (ins > 0) {
+                            packed = 0;
+                            // first probes of the four neighbours issued together
+                            int nb[4], old[4];
+                            unsigned hh[4];
 #pragma unroll
-                        for (int k = 0; k < 4; k++) {
-                            const int nb = sel4(tn, k);
-                            unsigned ns = 0xff;
-                            if (nb >= 0) {
-                                bool isnew;
-                                const int s2 = vis_insert(g, nb, isnew);
-                                if (s2 < 0) {
-                                    g.cnt[3] = 1;
-                                } else {
-                                    ns = static_cast<unsigned>(s2);
-                                    if (isnew) {
-                                        const int p = atomicAdd(&g.cnt[cur ^ 1], 1);
-                                        if (p < grp::VMAX)
-                                            g.front[cur ^ 1][p] = static_cast<unsigned char>(s2);
-                                        else
-                                            g.cnt[3] = 1;
+                            for (int k = 0; k < 4; k++) {
+                                nb[k] = sel4(tn, k);
+                                hh[k] = hash_key(nb[k], 7);
+                                old[k] = nb[k] >= 0 ? atomicCAS(&g.vkey[hh[k]], -1, nb[k]) : -2;
+                            }
+#pragma unroll
+                            for (int k = 0; k < 4; k++) {
+                                unsigned ns = 0xff;
+                                if (nb[k] >= 0) {
+                                    bool isnew = old[k] == -1;
+                                    int s2 = static_cast<int>(hh[k]);
+                                    if (old[k] != -1 && old[k] != nb[k])
+                                        s2 = vis_probe(g, nb[k], (hh[k] + 1) & (grp::VT - 1),
+                                                       isnew);
+                                    if (s2 < 0) {
+                                        g.cnt[3] = 1;
+                                    } else {
+                                        ns = static_cast<unsigned>(s2);
+                                        if (isnew) newslots |= ns << (8 * nnew++);
                                     }
                                 }
+                                packed |= ns << (8 * k);
                             }
-                            packed |= ns << (8 * k);
                         }
+                        g.vnb[slot] = packed;
+                        g.vflag[slot] = ins > 0 ? 3 : 1;
                     }
-                    g.vnb[slot] = packed;
-                    g.vflag[slot] = ins > 0 ? 3 : 1;
+                    // next-front positions: exclusive scan over the 16 lanes
+                    int incl = nnew;
+#pragma unroll
+                    for (int d = 1; d < grp::G; d <<= 1) {
+                        const int y = __shfl_up(incl, d, grp::G);
+                        if (gl >= d) incl += y;
+                    }
+                    const int p = nnext + incl - nnew;
+                    for (int j = 0; j < nnew; j++) {
+                        if (p + j < grp::VMAX)
+                            g.front[cur ^ 1][p + j] =
+                                static_cast<unsigned char>((newslots >> (8 * j)) & 0xffu);
+                        else
+                            g.cnt[3] = 1;
+                    }
+                    nnext += __shfl(incl, grp::G - 1, grp::G);
                 }
                 group_sync();
-                nf = g.cnt[3] ? 0 : g.cnt[cur ^ 1];
+                visited += nnext;
+                if (visited > grp::VMAX) nnext = 0;
+                nf = g.cnt[3] ? 0 : nnext;
                 cur ^= 1;
             }
-            ok = g.cnt[3] == 0;
+            ok = g.cnt[3] == 0 && visited <= grp::VMAX;
             // ---- phase C: the reference's cavity order
             int nc = 0;
             if (ok && gl == 0) {
