@@ -45,6 +45,7 @@ typedef struct {
     int32_t num_cus;         /* compute units of that device */
     int64_t device_bytes;    /* device memory held by the mesh (not workspace) */
     int64_t workspace_bytes; /* device workspace currently held */
+    int32_t n_problems;      /* 1, or the number of problems of a batch mesh */
 } nat_mesh_info;
 
 /* Replaces buildNewMeshAndVertices (natural.c:114-118) plus the one-time
@@ -107,6 +108,26 @@ int nat_spmm_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
  * columns, so sorting is the whole canonicalisation. */
 int nat_csr_sort_rows(int num_rows, const int32_t *d_rowptr, int32_t *d_cols, void *d_vals,
                       int vals_dtype, void *stream);
+
+/* ---- batches (no reference counterpart: the reference builds and queries
+ * one key set at a time).  Many independent problems share one mesh object
+ * and every call runs all of their queries in the same kernel launches.
+ *
+ * nat_batch_build builds problem p's mesh from data_points[p]
+ * [num_data_points[p], 3] (centred float64, as nat_mesh_build) on
+ * num_threads host threads (<= 0: all cores) and uploads them together.
+ * Columns are global: problem p's data point i is column
+ * vertex_offsets[p] + i (nat_batch_offsets), so a batch's W is block
+ * diagonal and W @ values works on the concatenated values. */
+int nat_batch_build(const double *const *data_points, const int32_t *num_data_points,
+                    int num_problems, int device, int num_threads, nat_mesh **out);
+int nat_batch_offsets(const nat_mesh *m, int64_t *vertex_offsets, int64_t *tet_offsets);
+
+/* nat_weights_compute for a batch: d_queries holds every problem's queries
+ * back to back, problem p's at rows [query_offsets[p], query_offsets[p+1])
+ * (host array of num_problems + 1).  Follow with nat_weights_emit. */
+int nat_batch_weights_compute(nat_mesh *m, const double *d_queries,
+                              const int64_t *query_offsets, void *stream, int64_t *nnz_out);
 
 /* ---- backward (no reference counterpart: the reference has no autograd;
  * these serve the torch.autograd.Function of natinterp3d_amd).

@@ -5,8 +5,10 @@ Drop-in for natinterp3d's API (natinterp3d/__init__.py:7): ``Interpolator``,
 ``interpolate`` and ``get_weights`` return the same results (bit-identical
 CSR matrices) with the weights computed by hand-written HIP kernels.
 """
+from .batch import BatchInterpolator
 from .natinterp3d import (Interpolator, MeshAndVerticesHIP, get_weights, interpolate,
                           spmm_device)
 
-__all__ = ["Interpolator", "MeshAndVerticesHIP", "get_weights", "interpolate", "spmm_device"]
+__all__ = ["BatchInterpolator", "Interpolator", "MeshAndVerticesHIP", "get_weights",
+           "interpolate", "spmm_device"]
 __version__ = "0.1.0"

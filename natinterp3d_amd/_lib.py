@@ -17,6 +17,7 @@ EXPORTS = (
     "nat_csr_sort_rows", "nat_topology_save", "nat_topology_row_order", "nat_grad_queries",
     "nat_spmm_t_csr",
     "nat_mesh_set_timing", "nat_mesh_get_timings", "nat_free_host", "nat_last_error",
+    "nat_batch_build", "nat_batch_offsets", "nat_batch_weights_compute",
 )
 
 
@@ -37,7 +38,7 @@ class MeshInfo(ctypes.Structure):
         ("coplanar", ctypes.c_int32), ("cospherical", ctypes.c_int32),
         ("edge_anomalies", ctypes.c_int32), ("device", ctypes.c_int32),
         ("num_cus", ctypes.c_int32), ("device_bytes", ctypes.c_int64),
-        ("workspace_bytes", ctypes.c_int64),
+        ("workspace_bytes", ctypes.c_int64), ("n_problems", ctypes.c_int32),
     ]
 
 
@@ -84,6 +85,15 @@ def lib():
                                  vp, vp]
     L.nat_topology_row_order.restype = ctypes.c_int
     L.nat_topology_row_order.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    L.nat_batch_build.restype = ctypes.c_int
+    L.nat_batch_build.argtypes = [ctypes.POINTER(dp), ip, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.POINTER(vp)]
+    L.nat_batch_offsets.restype = ctypes.c_int
+    L.nat_batch_offsets.argtypes = [vp, ctypes.POINTER(ctypes.c_int64),
+                                    ctypes.POINTER(ctypes.c_int64)]
+    L.nat_batch_weights_compute.restype = ctypes.c_int
+    L.nat_batch_weights_compute.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int64), vp,
+                                            ctypes.POINTER(ctypes.c_int64)]
     L.nat_mesh_set_timing.restype = ctypes.c_int
     L.nat_mesh_set_timing.argtypes = [vp, ctypes.c_int]
     L.nat_mesh_get_timings.restype = ctypes.c_int

@@ -21,22 +21,27 @@ __device__ __forceinline__ unsigned spread3(unsigned v) {
     return v;
 }
 
-__global__ void morton_key_kernel(const double *__restrict__ q, int M, double x0, double y0,
-                                  double z0, double sx, double sy, double sz, int bits,
-                                  int *__restrict__ keys, int *__restrict__ hist) {
+__global__ void morton_key_kernel(const double *__restrict__ q, int M,
+                                  const LocParams *__restrict__ lp, const int *__restrict__ prob,
+                                  int bits, int *__restrict__ keys, int *__restrict__ hist) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
+    const int p = prob ? prob[i] : 0;
+    const LocParams &L = lp[p];
     const int lim = (1 << bits) - 1;
-    auto cell = [&](double v, double o, double s) {
-        double c = (v - o) * s;
+    auto cell = [&](double v, int d) {
+        const double r = L.bbox[3 + d] - L.bbox[d];
+        const double s = r > 0 ? static_cast<double>(1 << bits) / r : 1.0;
+        double c = (v - L.bbox[d]) * s;
         int k = (c >= 0.0 && c < 2147483647.0) ? static_cast<int>(c) : 0;  // NaN -> 0
         if (c >= 2147483647.0) k = lim;
         return k > lim ? lim : k;
     };
-    const unsigned ix = cell(q[3 * static_cast<size_t>(i)], x0, sx);
-    const unsigned iy = cell(q[3 * static_cast<size_t>(i) + 1], y0, sy);
-    const unsigned iz = cell(q[3 * static_cast<size_t>(i) + 2], z0, sz);
-    const int key = static_cast<int>(spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2));
+    const unsigned ix = cell(q[3 * static_cast<size_t>(i)], 0);
+    const unsigned iy = cell(q[3 * static_cast<size_t>(i) + 1], 1);
+    const unsigned iz = cell(q[3 * static_cast<size_t>(i) + 2], 2);
+    const int key = (p << (3 * bits)) |
+                    static_cast<int>(spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2));
     keys[i] = key;
     atomicAdd(&hist[key], 1);
 }
@@ -51,19 +56,14 @@ __global__ void morton_scatter_kernel(const int *__restrict__ keys, int M,
     order[start[key] + k] = i;
 }
 
-hipError_t launch_morton_sort(const double *d_q, int M, const double bbox[6], int bits,
-                              int *d_order, int *d_hist, int *d_keys, long long *d_scan_tmp,
-                              hipStream_t st) {
-    const int nb = 1 << (3 * bits);
+hipError_t launch_morton_sort(const double *d_q, int M, const LocParams *d_lp,
+                              const int *d_prob, int P, int bits, int *d_order, int *d_hist,
+                              int *d_keys, long long *d_scan_tmp, hipStream_t st) {
+    const int nb = P << (3 * bits);
     hipError_t e = hipMemsetAsync(d_hist, 0, sizeof(int) * nb, st);
     if (e != hipSuccess) return e;
-    auto scale = [&](int d) {
-        const double r = bbox[3 + d] - bbox[d];
-        return r > 0 ? static_cast<double>(1 << bits) / r : 1.0;
-    };
-    hipLaunchKernelGGL(morton_key_kernel, dim3((M + 255) / 256), dim3(256), 0, st, d_q, M,
-                       bbox[0], bbox[1], bbox[2], scale(0), scale(1), scale(2), bits, d_keys,
-                       d_hist);
+    hipLaunchKernelGGL(morton_key_kernel, dim3((M + 255) / 256), dim3(256), 0, st, d_q, M, d_lp,
+                       d_prob, bits, d_keys, d_hist);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     long long *starts = d_scan_tmp;  // [nb + 1]
@@ -74,6 +74,26 @@ hipError_t launch_morton_sort(const double *d_q, int M, const double bbox[6], in
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(morton_scatter_kernel, dim3((M + 255) / 256), dim3(256), 0, st, d_keys,
                        M, starts, d_hist, d_order);
+    return hipGetLastError();
+}
+
+__global__ void problem_ids_kernel(const long long *__restrict__ off, int P, int M,
+                                   int *__restrict__ prob) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    int lo = 0, hi = P - 1;  // last p with off[p] <= i
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    prob[i] = lo;
+}
+
+hipError_t launch_problem_ids(const long long *d_offsets, int P, int M, int *d_prob,
+                              hipStream_t st) {
+    if (M <= 0) return hipSuccess;
+    hipLaunchKernelGGL(problem_ids_kernel, dim3((M + 255) / 256), dim3(256), 0, st, d_offsets, P,
+                       M, d_prob);
     return hipGetLastError();
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -11,6 +12,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "geom.h"
@@ -91,10 +93,14 @@ struct DeviceGuard {
 struct nat_mesh {
     int device = 0;
     int num_cus = 0;
-    nat::PackedMesh host;
+    // one problem per part (a plain mesh has one); global numbering in dev
+    std::vector<nat::PackedMesh> parts;
+    std::vector<long long> vbase, tbase;  // [P + 1] data-vertex and tet offsets
+    int kd_depth_max = 0;
+    std::vector<nat::LocParams> lph;      // [P]
     nat::DevMesh dev{};
-    DevBuf verts, tet_v, tet_n, kd, cell_start, cell_pts;
-    double bbox[6] = {0, 0, 0, 1, 1, 1};
+    DevBuf verts, tet_v, tet_n, kd, cell_start, cell_pts, lp;
+    DevBuf prob, qoff;                    // batch calls: problem of each query
     std::mutex mu;
 
     // workspace
@@ -130,10 +136,12 @@ struct nat_mesh {
 
     int64_t device_bytes() const {
         return static_cast<int64_t>(verts.bytes + tet_v.bytes + tet_n.bytes + kd.bytes +
-                                    cell_start.bytes + cell_pts.bytes);
+                                    cell_start.bytes + cell_pts.bytes + lp.bytes);
     }
+    int problems() const { return static_cast<int>(parts.size()); }
     int64_t workspace_bytes() const {
         size_t s = q.bytes + order.bytes + geom_order.bytes + keys.bytes + hist.bytes + scan.bytes +
+                   prob.bytes + qoff.bytes +
                    row_off.bytes + row_cnt.bytes + rowptr64.bytes + row_total.bytes +
                    ovf[0].bytes +
                    ovf[1].bytes + counters.bytes + pool_vals.bytes + pool_cols.bytes +
@@ -147,7 +155,7 @@ struct nat_mesh {
                          &q,        &order,     &geom_order, &keys,    &hist,       &scan,
                          &row_off,  &row_cnt,   &rowptr64,  &row_total, &ovf[0],    &ovf[1],
                          &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,   &csr_rowptr,
-                         &rec,      &rec_off,   &start};
+                         &rec,      &rec_off,   &start,     &lp,       &prob,       &qoff};
         for (DevBuf *b : all) b->release();
         for (int l = 0; l < kLevels; l++) scratch[l].release();
         for (hipEvent_t &e : ev) {
@@ -171,7 +179,7 @@ double elapsed(nat_mesh *m, int a, int b) {
 }
 
 void set_levels(nat_mesh *m) {
-    const int kd = m->host.kd_depth + 2;
+    const int kd = m->kd_depth_max + 2;
     const int cus = std::max(1, m->num_cus);
     // level 0 (locate, topology and geometry kernels) sees every query; the
     // topology kernel's LDS capacities cover all but a handful of rows of
@@ -253,83 +261,157 @@ void build_grid(const nat::PackedMesh &h, double g0[3], double &gh, int gn[3],
     }
 }
 
+// Offsets of the parts in the global numbering.
+void index_parts(nat_mesh *m) {
+    const int P = m->problems();
+    m->vbase.assign(P + 1, 0);
+    m->tbase.assign(P + 1, 0);
+    m->kd_depth_max = 0;
+    for (int p = 0; p < P; p++) {
+        m->vbase[p + 1] = m->vbase[p] + m->parts[p].n_data;
+        m->tbase[p + 1] = m->tbase[p] + m->parts[p].n_tets;
+        m->kd_depth_max = std::max(m->kd_depth_max, m->parts[p].kd_depth);
+    }
+}
+
+int upload_mesh(nat_mesh *m);
+
+// Device set-up of a built mesh (any number of parts).
+int finish_build(nat_mesh *m) {
+    index_parts(m);
+    if (m->device == -1) return 0;  // host-only mesh: export and inspection, no queries
+    DeviceGuard g(m->device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, m->device) != hipSuccess)
+        return fail(-10, "hipGetDeviceProperties failed");
+    m->num_cus = prop.multiProcessorCount;
+    set_levels(m);
+    const int urc = upload_mesh(m);
+    if (urc) m->release_all();
+    return urc;
+}
+
+int check_device(int device) {
+    if (device >= 0) {
+        int ndev = 0;
+        HIP_OR_FAIL(hipGetDeviceCount(&ndev));
+        if (device >= ndev) return fail(-1, "device %d out of range (%d)", device, ndev);
+    } else if (device != -1) {
+        return fail(-1, "device %d invalid", device);
+    }
+    return 0;
+}
+
+int builder_error(int rc) {
+    if (rc == -2) return fail(-2, "mesh builder met a tet of non-positive orientation");
+    if (rc == -3) return fail(-3, "mesh builder could not locate a point");
+    return fail(rc, "mesh builder failed (%d)", rc);
+}
+
+// Uploads the parts as one mesh with global numbering: data vertices of all
+// problems first (problem p's from vbase[p]), then the four super vertices of
+// each problem; tets and k-d nodes from tbase[p]; per-problem grid blocks.
 int upload_mesh(nat_mesh *m) {
-    const nat::PackedMesh &h = m->host;
-    const int nv = h.n_data + 4, T = h.n_tets;
+    const int P = m->problems();
+    const long long NT = m->vbase[P], TT = m->tbase[P];
+    if (NT + 4LL * P > 2147483647LL || TT > 2147483647LL)
+        return fail(-1, "batch too large for 32-bit vertex and tet indices");
+    const size_t nv = static_cast<size_t>(NT + 4LL * P), T = static_cast<size_t>(TT);
     std::vector<double4> verts(nv);
-    for (int i = 0; i < nv; i++)
-        verts[i] = make_double4(h.verts[3 * i], h.verts[3 * i + 1], h.verts[3 * i + 2], 0.0);
-    std::vector<int4> tv(T), tn(T);
-    std::vector<double4> kd(T);
-    for (int t = 0; t < T; t++) {
-        const int *p = &h.tet_v[4 * t];
-        const nat::d3 a = {h.verts[3 * p[0]], h.verts[3 * p[0] + 1], h.verts[3 * p[0] + 2]};
-        const nat::d3 b = {h.verts[3 * p[1]], h.verts[3 * p[1] + 1], h.verts[3 * p[1] + 2]};
-        const nat::d3 c = {h.verts[3 * p[2]], h.verts[3 * p[2] + 1], h.verts[3 * p[2] + 2]};
-        const nat::d3 d = {h.verts[3 * p[3]], h.verts[3 * p[3] + 1], h.verts[3 * p[3] + 2]};
-        // orientation bit (natural_insertionfree.c:457-476), stored inverted in
-        // bit 31 of p3 so that the common positive case reads unchanged
-        const bool pos = nat::orient3d(a, b, c, d) > 0;
-        tv[t] = make_int4(p[0], p[1], p[2], pos ? p[3] : (p[3] | static_cast<int>(0x80000000u)));
-        tn[t] = make_int4(h.tet_n[4 * t], h.tet_n[4 * t + 1], h.tet_n[4 * t + 2],
-                          h.tet_n[4 * t + 3]);
-        int2 lr = make_int2(h.kd_left[t], h.kd_right[t]);
-        double w;
-        std::memcpy(&w, &lr, sizeof(w));
-        kd[t] = make_double4(h.kd_pos[3 * t], h.kd_pos[3 * t + 1], h.kd_pos[3 * t + 2], w);
+    std::vector<int4> tv(std::max<size_t>(T, 1)), tn(std::max<size_t>(T, 1));
+    std::vector<double4> kd(std::max<size_t>(T, 1));
+    std::vector<int> start_all;
+    std::vector<double4> pts_all(std::max<size_t>(T, 1));
+    m->lph.assign(P, nat::LocParams{});
+    for (int p = 0; p < P; p++) {
+        const nat::PackedMesh &h = m->parts[p];
+        const int N = h.n_data;
+        const int vb = static_cast<int>(m->vbase[p]), tb = static_cast<int>(m->tbase[p]);
+        const int sb = static_cast<int>(NT) + 4 * p;
+        auto gv = [&](int v) { return v < N ? vb + v : sb + (v - N); };
+        for (int i = 0; i < N + 4; i++)
+            verts[gv(i)] =
+                make_double4(h.verts[3 * i], h.verts[3 * i + 1], h.verts[3 * i + 2], 0.0);
+        for (int t = 0; t < h.n_tets; t++) {
+            const int *pv = &h.tet_v[4 * t];
+            auto pt = [&](int v) {
+                return nat::d3{h.verts[3 * v], h.verts[3 * v + 1], h.verts[3 * v + 2]};
+            };
+            // orientation bit (natural_insertionfree.c:457-476), stored inverted
+            // in bit 31 of p3 so that the common positive case reads unchanged
+            const bool pos = nat::orient3d(pt(pv[0]), pt(pv[1]), pt(pv[2]), pt(pv[3])) > 0;
+            const int w = gv(pv[3]);
+            tv[tb + t] = make_int4(gv(pv[0]), gv(pv[1]), gv(pv[2]),
+                                   pos ? w : (w | static_cast<int>(0x80000000u)));
+            int nb[4];
+            for (int k = 0; k < 4; k++) {
+                const int x = h.tet_n[4 * t + k];
+                nb[k] = x < 0 ? -1 : tb + x;
+            }
+            tn[tb + t] = make_int4(nb[0], nb[1], nb[2], nb[3]);
+            int2 lr = make_int2(h.kd_left[t] < 0 ? -1 : tb + h.kd_left[t],
+                                h.kd_right[t] < 0 ? -1 : tb + h.kd_right[t]);
+            double wd;
+            std::memcpy(&wd, &lr, sizeof(wd));
+            kd[tb + t] = make_double4(h.kd_pos[3 * t], h.kd_pos[3 * t + 1], h.kd_pos[3 * t + 2], wd);
+        }
+        nat::LocParams &L = m->lph[p];
+        std::vector<int> start;
+        std::vector<double4> pts;
+        build_grid(h, L.g0, L.gh, L.gn, start, pts);
+        L.ginv = 1.0 / L.gh;
+        L.cell_base = static_cast<int>(start_all.size());
+        for (int v : start) start_all.push_back(v + tb);
+        for (int t = 0; t < h.n_tets; t++) {
+            long long id;
+            std::memcpy(&id, &pts[t].w, sizeof(id));
+            id += tb;
+            double wd;
+            std::memcpy(&wd, &id, sizeof(wd));
+            pts_all[tb + t] = make_double4(pts[t].x, pts[t].y, pts[t].z, wd);
+        }
+        L.kd_root = h.n_tets > 0 ? tb : -1;
+        for (int k = 0; k < 3; k++) {
+            L.rmin[k] = h.kd_rect[k];
+            L.rmax[k] = h.kd_rect[3 + k];
+        }
+        // bounding box of the data points: the Morton grid of the queries
+        for (int k = 0; k < 3; k++) L.bbox[k] = L.bbox[3 + k] = h.verts[k];
+        for (int i = 0; i < N; i++)
+            for (int k = 0; k < 3; k++) {
+                L.bbox[k] = std::min(L.bbox[k], h.verts[3 * i + k]);
+                L.bbox[3 + k] = std::max(L.bbox[3 + k], h.verts[3 * i + k]);
+            }
+        if (start_all.size() > 2147483647ULL) return fail(-1, "batch grid too large");
     }
     HIP_OR_FAIL(m->verts.ensure(sizeof(double4) * nv));
-    HIP_OR_FAIL(m->tet_v.ensure(sizeof(int4) * std::max(T, 1)));
-    HIP_OR_FAIL(m->tet_n.ensure(sizeof(int4) * std::max(T, 1)));
-    HIP_OR_FAIL(m->kd.ensure(sizeof(double4) * std::max(T, 1)));
+    HIP_OR_FAIL(m->tet_v.ensure(sizeof(int4) * tv.size()));
+    HIP_OR_FAIL(m->tet_n.ensure(sizeof(int4) * tn.size()));
+    HIP_OR_FAIL(m->kd.ensure(sizeof(double4) * kd.size()));
+    HIP_OR_FAIL(m->cell_start.ensure(sizeof(int) * start_all.size()));
+    HIP_OR_FAIL(m->cell_pts.ensure(sizeof(double4) * pts_all.size()));
+    HIP_OR_FAIL(m->lp.ensure(sizeof(nat::LocParams) * P));
     HIP_OR_FAIL(hipMemcpy(m->verts.p, verts.data(), sizeof(double4) * nv, hipMemcpyHostToDevice));
-    if (T > 0) {
-        HIP_OR_FAIL(hipMemcpy(m->tet_v.p, tv.data(), sizeof(int4) * T, hipMemcpyHostToDevice));
-        HIP_OR_FAIL(hipMemcpy(m->tet_n.p, tn.data(), sizeof(int4) * T, hipMemcpyHostToDevice));
-        HIP_OR_FAIL(hipMemcpy(m->kd.p, kd.data(), sizeof(double4) * T, hipMemcpyHostToDevice));
-    }
+    HIP_OR_FAIL(hipMemcpy(m->tet_v.p, tv.data(), sizeof(int4) * tv.size(), hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMemcpy(m->tet_n.p, tn.data(), sizeof(int4) * tn.size(), hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMemcpy(m->kd.p, kd.data(), sizeof(double4) * kd.size(), hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMemcpy(m->cell_start.p, start_all.data(), sizeof(int) * start_all.size(),
+                          hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMemcpy(m->cell_pts.p, pts_all.data(), sizeof(double4) * pts_all.size(),
+                          hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMemcpy(m->lp.p, m->lph.data(), sizeof(nat::LocParams) * P,
+                          hipMemcpyHostToDevice));
     nat::DevMesh &d = m->dev;
     d.verts = m->verts.as<double4>();
     d.tet_v = m->tet_v.as<int4>();
     d.tet_n = m->tet_n.as<int4>();
     d.kd = m->kd.as<double4>();
-    for (int k = 0; k < 3; k++) {
-        d.rmin[k] = h.kd_rect[k];
-        d.rmax[k] = h.kd_rect[3 + k];
-    }
-    d.n_data = h.n_data;
-    d.n_tets = T;
-    {
-        std::vector<int> start;
-        std::vector<double4> pts;
-        double g0[3], gh;
-        int gn[3];
-        build_grid(h, g0, gh, gn, start, pts);
-        HIP_OR_FAIL(m->cell_start.ensure(sizeof(int) * start.size()));
-        HIP_OR_FAIL(m->cell_pts.ensure(sizeof(double4) * pts.size()));
-        HIP_OR_FAIL(hipMemcpy(m->cell_start.p, start.data(), sizeof(int) * start.size(),
-                              hipMemcpyHostToDevice));
-        HIP_OR_FAIL(hipMemcpy(m->cell_pts.p, pts.data(), sizeof(double4) * pts.size(),
-                              hipMemcpyHostToDevice));
-        d.cell_start = m->cell_start.as<int>();
-        d.cell_pts = m->cell_pts.as<double4>();
-        for (int k = 0; k < 3; k++) {
-            d.g0[k] = g0[k];
-            d.gn[k] = gn[k];
-        }
-        d.gh = gh;
-        d.ginv = 1.0 / gh;
-    }
-    // bounding box of the data points: the Morton grid of the queries
-    for (int k = 0; k < 3; k++) {
-        m->bbox[k] = h.verts[k];
-        m->bbox[3 + k] = h.verts[k];
-    }
-    for (int i = 0; i < h.n_data; i++)
-        for (int k = 0; k < 3; k++) {
-            m->bbox[k] = std::min(m->bbox[k], h.verts[3 * i + k]);
-            m->bbox[3 + k] = std::max(m->bbox[3 + k], h.verts[3 * i + k]);
-        }
+    d.n_data = static_cast<int>(NT);
+    d.n_tets = static_cast<int>(TT);
+    d.cell_start = m->cell_start.as<int>();
+    d.cell_pts = m->cell_pts.as<double4>();
+    d.lp = m->lp.as<nat::LocParams>();
+    d.prob = nullptr;
     return 0;
 }
 
@@ -342,8 +424,11 @@ int ensure_level_scratch(nat_mesh *m, int level) {
     return 0;
 }
 
-int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long long *nnz_out) {
+// d_prob: problem of each query (batch meshes), NULL for a single problem
+int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipStream_t st,
+                 long long *nnz_out) {
     m->last_M = -1;
+    m->dev.prob = d_prob;
     if (M == 0) {
         m->last_M = 0;
         m->last_nnz = 0;
@@ -351,10 +436,15 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
         return 0;
     }
     const size_t Mz = static_cast<size_t>(M);
-    // Morton bucketing: 3*bits-bit cells over the data bounding box
+    // Morton bucketing: per problem, 3*bits-bit cells over its data bounding box
+    const int P = m->problems();
     int bits = 1;
-    while (bits < 7 && (1ull << (3 * (bits + 1))) <= Mz * 2) bits++;
-    const int nbuckets = 1 << (3 * bits);
+    while (bits < 7 && (static_cast<unsigned long long>(P) << (3 * (bits + 1))) <= Mz * 2 &&
+           (static_cast<unsigned long long>(P) << (3 * (bits + 1))) <= (1ull << 24))
+        bits++;
+    if ((static_cast<unsigned long long>(P) << (3 * bits)) > (1ull << 26))
+        return fail(-1, "too many problems in one batch (%d)", P);
+    const int nbuckets = P << (3 * bits);
     const size_t scan_elems =
         std::max(nat::scan_tmp_elems(M) + Mz + 1,
                  static_cast<size_t>(nbuckets) + 1 + nat::scan_tmp_elems(nbuckets));
@@ -374,9 +464,9 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
 
     m->times = nat_timings{};
     mark(m, 0, st);
-    HIP_OR_FAIL(nat::launch_morton_sort(d_q, M, m->bbox, bits, m->order.as<int>(),
-                                        m->hist.as<int>(), m->keys.as<int>(),
-                                        m->scan.as<long long>(), st));
+    HIP_OR_FAIL(nat::launch_morton_sort(d_q, M, m->lp.as<nat::LocParams>(), d_prob, P, bits,
+                                        m->order.as<int>(), m->hist.as<int>(),
+                                        m->keys.as<int>(), m->scan.as<long long>(), st));
     mark(m, 1, st);
     m->times.order_ms = elapsed(m, 0, 1);
 
@@ -494,13 +584,16 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, hipStream_t st, long lon
     return fail(-5, "row pool did not converge");
 }
 
-// Saved topology: int64 header {magic, M, record units, N}, rec_off
-// int64[M], the query order int32[M], the row totals float64[M] (8-byte
-// aligned), then the records (16-byte aligned).
-constexpr long long kTopoMagic = 0x4e41545450304c32LL;
-size_t topo_order_off(long long M) { return 32 + 8 * static_cast<size_t>(M); }
+// Saved topology: int64 header {magic, M, record units, N, P, 0, 0, 0},
+// rec_off int64[M], the query order int32[M], the problem of each query
+// int32[M] (filled for batches, P > 1), the row totals float64[M], then the
+// records (16-byte aligned).
+constexpr long long kTopoMagic = 0x4e41545450304c33LL;
+constexpr size_t kTopoHeader = 64;
+size_t topo_order_off(long long M) { return kTopoHeader + 8 * static_cast<size_t>(M); }
+size_t topo_prob_off(long long M) { return topo_order_off(M) + 4 * static_cast<size_t>(M); }
 size_t topo_total_off(long long M) {
-    return (topo_order_off(M) + 4 * static_cast<size_t>(M) + 7) & ~static_cast<size_t>(7);
+    return (topo_prob_off(M) + 4 * static_cast<size_t>(M) + 7) & ~static_cast<size_t>(7);
 }
 size_t topo_rec_off(long long M) {
     return (topo_total_off(M) + 8 * static_cast<size_t>(M) + 15) & ~static_cast<size_t>(15);
@@ -509,13 +602,16 @@ size_t topo_rec_off(long long M) {
 int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int *d_rowptr,
               const double *d_w, const double *d_gy, const double *d_values, int D,
               double *d_dq, hipStream_t st) {
-    long long hdr[4];
+    long long hdr[8];
     HIP_OR_FAIL(hipMemcpyAsync(hdr, blob, sizeof(hdr), hipMemcpyDeviceToHost, st));
     HIP_OR_FAIL(hipStreamSynchronize(st));
     if (hdr[0] != kTopoMagic) return fail(-1, "d_topo is not a saved topology");
     if (hdr[1] != M)
         return fail(-1, "saved topology has %lld queries, not %d", hdr[1], M);
-    if (hdr[3] != m->host.n_data) return fail(-1, "saved topology is of another mesh");
+    if (hdr[3] != m->dev.n_data || hdr[4] != m->problems())
+        return fail(-1, "saved topology is of another mesh");
+    // the larger levels locate again: they need each query's problem
+    m->dev.prob = hdr[4] > 1 ? reinterpret_cast<const int *>(blob + topo_prob_off(M)) : nullptr;
     const size_t Mz = static_cast<size_t>(M);
     HIP_OR_FAIL(m->ovf[0].ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->ovf[1].ensure(sizeof(int) * Mz));
@@ -527,7 +623,7 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
     topo.rec = reinterpret_cast<int4 *>(const_cast<char *>(blob) + topo_rec_off(M));
     topo.rec_cap = hdr[2];
     topo.rec_used = nullptr;
-    topo.rec_off = reinterpret_cast<long long *>(const_cast<char *>(blob) + 32);
+    topo.rec_off = reinterpret_cast<long long *>(const_cast<char *>(blob) + kTopoHeader);
     const int *order = reinterpret_cast<const int *>(blob + topo_order_off(M));
     nat::GradIn g{d_gy, d_values, D, d_dq, d_rowptr, d_w,
                   reinterpret_cast<const double *>(blob + topo_total_off(M))};
@@ -601,41 +697,80 @@ int nat_mesh_build(const double *data_points, int num_data_points, int device, n
     *out = nullptr;
     if (!data_points || num_data_points <= 0)
         return fail(-1, "need at least one data point (got %d)", num_data_points);
-    if (device >= 0) {
-        int ndev = 0;
-        HIP_OR_FAIL(hipGetDeviceCount(&ndev));
-        if (device >= ndev) return fail(-1, "device %d out of range (%d)", device, ndev);
-    } else if (device != -1) {
-        return fail(-1, "device %d invalid", device);
-    }
+    if (int rc = check_device(device)) return rc;
     nat_mesh *m = new nat_mesh();
     m->device = device;
-    const int rc = nat::build_packed_mesh(data_points, num_data_points, &m->host);
+    m->parts.resize(1);
+    const int rc = nat::build_packed_mesh(data_points, num_data_points, &m->parts[0]);
     if (rc) {
         delete m;
-        if (rc == -2) return fail(-2, "mesh builder met a tet of non-positive orientation");
-        if (rc == -3) return fail(-3, "mesh builder could not locate a point");
-        return fail(rc, "mesh builder failed (%d)", rc);
+        return builder_error(rc);
     }
-    if (device == -1) {  // host-only mesh: export and inspection, no queries
-        *out = m;
-        return 0;
-    }
-    DeviceGuard g(device);
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
-        delete m;
-        return fail(-10, "hipGetDeviceProperties failed");
-    }
-    m->num_cus = prop.multiProcessorCount;
-    set_levels(m);
-    const int urc = upload_mesh(m);
-    if (urc) {
-        m->release_all();
+    if (int urc = finish_build(m)) {
         delete m;
         return urc;
     }
     *out = m;
+    return 0;
+}
+
+int nat_batch_build(const double *const *data_points, const int32_t *num_data_points,
+                    int num_problems, int device, int num_threads, nat_mesh **out) {
+    g_err.clear();
+    if (!out) return fail(-1, "out is NULL");
+    *out = nullptr;
+    if (num_problems <= 0 || !data_points || !num_data_points)
+        return fail(-1, "need at least one problem (got %d)", num_problems);
+    for (int p = 0; p < num_problems; p++)
+        if (!data_points[p] || num_data_points[p] <= 0)
+            return fail(-1, "problem %d needs at least one data point (got %d)", p,
+                        num_data_points[p]);
+    if (int rc = check_device(device)) return rc;
+    nat_mesh *m = new nat_mesh();
+    m->device = device;
+    m->parts.resize(num_problems);
+    // independent host builds, largest first over a small thread pool
+    std::vector<int> order(num_problems), rcs(num_problems, 0);
+    for (int p = 0; p < num_problems; p++) order[p] = p;
+    std::sort(order.begin(), order.end(),
+              [&](int a, int b) { return num_data_points[a] > num_data_points[b]; });
+    int nt = num_threads > 0 ? num_threads : static_cast<int>(std::thread::hardware_concurrency());
+    nt = std::max(1, std::min(nt, num_problems));
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= num_problems) return;
+            const int p = order[i];
+            rcs[p] = nat::build_packed_mesh(data_points[p], num_data_points[p], &m->parts[p]);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+    for (int p = 0; p < num_problems; p++)
+        if (rcs[p]) {
+            delete m;
+            const int rc = builder_error(rcs[p]);
+            g_err = "problem " + std::to_string(p) + ": " + g_err;
+            return rc;
+        }
+    if (int urc = finish_build(m)) {
+        delete m;
+        return urc;
+    }
+    *out = m;
+    return 0;
+}
+
+int nat_batch_offsets(const nat_mesh *m, int64_t *vertex_offsets, int64_t *tet_offsets) {
+    if (!m) return fail(-1, "NULL mesh");
+    const int P = m->problems();
+    for (int p = 0; p <= P; p++) {
+        if (vertex_offsets) vertex_offsets[p] = m->vbase[p];
+        if (tet_offsets) tet_offsets[p] = m->tbase[p];
+    }
     return 0;
 }
 
@@ -650,12 +785,17 @@ void nat_mesh_free(nat_mesh *m) {
 
 int nat_mesh_get_info(const nat_mesh *m, nat_mesh_info *info) {
     if (!m || !info) return fail(-1, "NULL argument");
-    info->n_data = m->host.n_data;
-    info->n_tets = m->host.n_tets;
-    info->kd_depth = m->host.kd_depth;
-    info->coplanar = m->host.coplanar;
-    info->cospherical = m->host.cospherical;
-    info->edge_anomalies = m->host.edge_anomalies;
+    const int P = m->problems();
+    info->n_data = static_cast<int32_t>(m->vbase[P]);
+    info->n_tets = static_cast<int32_t>(m->tbase[P]);
+    info->kd_depth = m->kd_depth_max;
+    info->coplanar = info->cospherical = info->edge_anomalies = 0;
+    for (const nat::PackedMesh &h : m->parts) {
+        info->coplanar += h.coplanar;
+        info->cospherical += h.cospherical;
+        info->edge_anomalies += h.edge_anomalies;
+    }
+    info->n_problems = P;
     info->device = m->device;
     info->num_cus = m->num_cus;
     info->device_bytes = m->device_bytes();
@@ -666,7 +806,8 @@ int nat_mesh_get_info(const nat_mesh *m, nat_mesh_info *info) {
 int nat_mesh_export(const nat_mesh *m, double *verts, int32_t *tet_v, int32_t *tet_n,
                     int32_t *kd_left, int32_t *kd_right, double *kd_pos, double *kd_rect) {
     if (!m) return fail(-1, "NULL mesh");
-    const nat::PackedMesh &h = m->host;
+    if (m->problems() != 1) return fail(-1, "nat_mesh_export needs a single-problem mesh");
+    const nat::PackedMesh &h = m->parts[0];
     if (verts) std::memcpy(verts, h.verts.data(), sizeof(double) * h.verts.size());
     if (tet_v) std::memcpy(tet_v, h.tet_v.data(), sizeof(int32_t) * h.tet_v.size());
     if (tet_n) std::memcpy(tet_n, h.tet_n.data(), sizeof(int32_t) * h.tet_n.size());
@@ -684,11 +825,46 @@ int nat_weights_compute(nat_mesh *m, const double *d_queries, int num_query_poin
     if (num_query_points < 0) return fail(-1, "negative query count");
     if (num_query_points > 0 && !d_queries) return fail(-1, "NULL queries");
     if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    if (m->problems() != 1)
+        return fail(-1, "a batch mesh needs nat_batch_weights_compute (query offsets)");
     std::lock_guard<std::mutex> lk(m->mu);
     DeviceGuard g(m->device);
     long long nnz = 0;
-    const int rc = compute_rows(m, d_queries, num_query_points,
+    const int rc = compute_rows(m, d_queries, num_query_points, nullptr,
                                 static_cast<hipStream_t>(stream), &nnz);
+    if (rc) return rc;
+    *nnz_out = nnz;
+    return 0;
+}
+
+int nat_batch_weights_compute(nat_mesh *m, const double *d_queries,
+                              const int64_t *query_offsets, void *stream, int64_t *nnz_out) {
+    g_err.clear();
+    if (!m || !nnz_out || !query_offsets) return fail(-1, "NULL argument");
+    if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    const int P = m->problems();
+    if (query_offsets[0] != 0) return fail(-1, "query_offsets[0] must be 0");
+    for (int p = 0; p < P; p++)
+        if (query_offsets[p + 1] < query_offsets[p])
+            return fail(-1, "query_offsets must not decrease (problem %d)", p);
+    const long long M = query_offsets[P];
+    if (M > 2147483647LL) return fail(-1, "too many queries (%lld)", M);
+    if (M > 0 && !d_queries) return fail(-1, "NULL queries");
+    std::lock_guard<std::mutex> lk(m->mu);
+    DeviceGuard g(m->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int *d_prob = nullptr;
+    if (P > 1 && M > 0) {
+        HIP_OR_FAIL(m->qoff.ensure(sizeof(long long) * (P + 1)));
+        HIP_OR_FAIL(m->prob.ensure(sizeof(int) * static_cast<size_t>(M)));
+        HIP_OR_FAIL(hipMemcpyAsync(m->qoff.p, query_offsets, sizeof(long long) * (P + 1),
+                                   hipMemcpyHostToDevice, st));
+        HIP_OR_FAIL(nat::launch_problem_ids(m->qoff.as<long long>(), P, static_cast<int>(M),
+                                            m->prob.as<int>(), st));
+        d_prob = m->prob.as<int>();
+    }
+    long long nnz = 0;
+    const int rc = compute_rows(m, d_queries, static_cast<int>(M), d_prob, st, &nnz);
     if (rc) return rc;
     *nnz_out = nnz;
     return 0;
@@ -711,9 +887,11 @@ int nat_get_weights(const double *query_points, int num_query_points, nat_mesh *
     if (!m || !weight_values || !weight_col_inds || !weight_row_ptrs)
         return fail(-2, "NULL argument");
     if (num_query_points < 0) return fail(-2, "negative query count");
-    if (num_data_points != m->host.n_data)
+    if (m->problems() != 1)
+        return fail(-2, "nat_get_weights needs a single-problem mesh (use the batch calls)");
+    if (num_data_points != m->parts[0].n_data)
         return fail(-2, "num_data_points %d does not match the mesh (%d)", num_data_points,
-                    m->host.n_data);
+                    m->parts[0].n_data);
     if (m->device < 0) return fail(-2, "host-only mesh has no device copy");
     std::lock_guard<std::mutex> lk(m->mu);
     DeviceGuard g(m->device);
@@ -726,7 +904,7 @@ int nat_get_weights(const double *query_points, int num_query_points, nat_mesh *
                               hipMemcpyHostToDevice));
     }
     long long nnz = 0;
-    int rc = compute_rows(m, m->q.as<double>(), M, st, &nnz);
+    int rc = compute_rows(m, m->q.as<double>(), M, nullptr, st, &nnz);
     if (rc) return rc;
     if (nnz > 2147483647LL) return -1;
     const size_t nz = static_cast<size_t>(nnz);
@@ -796,12 +974,16 @@ int nat_topology_save(nat_mesh *m, void *d_dst, int64_t capacity, int64_t *bytes
                     static_cast<long long>(capacity));
     DeviceGuard g(m->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const long long hdr[4] = {kTopoMagic, M, units, m->host.n_data};
+    const int P = m->problems();
+    const long long hdr[8] = {kTopoMagic, M, units, m->dev.n_data, P, 0, 0, 0};
     char *dst = static_cast<char *>(d_dst);
     HIP_OR_FAIL(hipMemcpyAsync(dst, hdr, sizeof(hdr), hipMemcpyHostToDevice, st));
     if (M > 0) {
-        HIP_OR_FAIL(hipMemcpyAsync(dst + 32, m->rec_off.p, 8 * static_cast<size_t>(M),
+        HIP_OR_FAIL(hipMemcpyAsync(dst + kTopoHeader, m->rec_off.p, 8 * static_cast<size_t>(M),
                                    hipMemcpyDeviceToDevice, st));
+        if (P > 1)
+            HIP_OR_FAIL(hipMemcpyAsync(dst + topo_prob_off(M), m->prob.p,
+                                       4 * static_cast<size_t>(M), hipMemcpyDeviceToDevice, st));
         // the backward keeps the Morton order: its gathers (values, dL/dy)
         // favour spatial locality over grouping by cavity size
         HIP_OR_FAIL(hipMemcpyAsync(dst + topo_order_off(M), m->order.p,
@@ -839,7 +1021,7 @@ int nat_grad_queries(nat_mesh *m, const void *d_topo, const double *d_queries,
 int nat_topology_row_order(const void *d_topo, int num_query_points, const int32_t **out) {
     g_err.clear();
     if (!d_topo || !out) return fail(-1, "NULL argument");
-    long long hdr[4];
+    long long hdr[8];
     HIP_OR_FAIL(hipMemcpy(hdr, d_topo, sizeof(hdr), hipMemcpyDeviceToHost));
     if (hdr[0] != kTopoMagic) return fail(-1, "d_topo is not a saved topology");
     if (hdr[1] != num_query_points)

@@ -7,27 +7,47 @@
 
 namespace nat {
 
+// Per-problem parameters of the start-tet search (and of the Morton
+// bucketing of its queries).  A mesh holds one problem, a batch many.
+struct LocParams {
+    double g0[3];        // grid origin
+    double gh, ginv;     // cell edge and its inverse
+    int gn[3];           // cells per axis
+    int cell_base;       // first entry of this problem's cells in cell_start
+    int kd_root;         // tet index of the k-d root (the problem's first tet)
+    int pad_;
+    double rmin[3], rmax[3];  // k-d root hyper-rectangle
+    double bbox[6];           // data bounding box min[3], max[3]
+};
+
 // Immutable device mesh.  All arrays are packed SoA in tet order (the
 // reference's linked-list order), 16- or 32-byte records so that one tet or
-// one vertex is one aligned vector load.
+// one vertex is one aligned vector load.  A batch of independent problems is
+// one mesh with global numbering: the data vertices of every problem first
+// (problem p's at its vertex offset, so CSR columns are global), then the
+// four super vertices of each problem; tets and k-d nodes at the problem's
+// tet offset.  The super-vertex test stays "slot >= n_data".
 struct DevMesh {
-    const double4 *verts;  // [N+4] x,y,z,0; slots N..N+3 are the super vertices
+    const double4 *verts;  // [N+4P] x,y,z,0; slots N.. are the super vertices
     const int4 *tet_v;     // [T] vertex slots p0..p3; bit 31 of .w set when
                            //     orient3d(p0,p1,p2,p3) <= 0
     const int4 *tet_n;     // [T] neighbour across face 0..3, -1 = none
     const double4 *kd;     // [T] k-d node i = tet i: centroid x,y,z and
                            //     .w = bit pattern of int2(left, right)
-    double rmin[3], rmax[3];  // k-d root hyper-rectangle
-    int n_data;               // N
-    int n_tets;               // T
+    int n_data;            // N (all problems)
+    int n_tets;            // T (all problems)
     // Uniform grid over the same tet centroids, for the exact nearest-centroid
     // search that replaces the k-d descent whenever its answer is certified.
-    const int *cell_start;    // [ncell + 1] records of cell c: [cell_start[c], cell_start[c+1])
+    const int *cell_start;    // per problem [ncell + 1] from LocParams::cell_base:
+                              // records of cell c at [cell_start[c], cell_start[c+1])
     const double4 *cell_pts;  // [T] centroid x,y,z; .w = bit pattern of the tet index (int64)
-    double g0[3];             // grid origin
-    double gh, ginv;          // cell edge and its inverse
-    int gn[3];                // cells per axis
+    const LocParams *lp;      // [P]
+    const int *prob;          // per query: its problem (set per call; NULL = problem 0)
 };
+
+__device__ __forceinline__ const LocParams &loc_of(const DevMesh &m, int qi) {
+    return m.lp[m.prob ? m.prob[qi] : 0];
+}
 
 // Longest row the level-0 kernels handle (topology LDS neighbour list and
 // the geometry kernels' per-thread LDS accumulators); longer rows go to
@@ -103,9 +123,13 @@ struct GradIn {
 size_t scratch_bytes_per_thread(const Caps &c);
 void scratch_carve(void *base, const Caps &c, int nthr, Scratch *s);
 
-hipError_t launch_morton_sort(const double *d_q, int M, const double bbox[6], int bits,
-                              int *d_order, int *d_hist, int *d_keys,
-                              long long *d_scan_tmp, hipStream_t st);
+// buckets: problem-major, then 3*bits-bit Morton cells of the problem's box
+hipError_t launch_morton_sort(const double *d_q, int M, const LocParams *d_lp,
+                              const int *d_prob, int P, int bits, int *d_order, int *d_hist,
+                              int *d_keys, long long *d_scan_tmp, hipStream_t st);
+// prob[i] = p with offsets[p] <= i < offsets[p+1]
+hipError_t launch_problem_ids(const long long *d_offsets, int P, int M, int *d_prob,
+                              hipStream_t st);
 int rows_blocks_per_cu();  // resident 256-thread blocks of the fused row kernel per CU
 int topo_blocks_per_cu();   // 128-thread blocks of the topology kernel
 int geom_blocks_per_cu();  // kGeomThreads-thread blocks

@@ -54,21 +54,22 @@ __device__ __forceinline__ size_t il(int k, size_t S, int tid) {  // interleaved
 // explicit stack: frame d holds the node at depth d (split axis d mod 3),
 // the phase reached (0 = enter, 1 = nearer subtree done, 2 = farther subtree
 // done) and the rectangle bound it saved before slicing.
-__device__ int kd_nearest(const DevMesh &m, const d3 &q, const Scratch &sc, int tid, int cap,
-                          bool &ovf) {
-    if (m.n_tets <= 0) return -1;
-    double rmin[3] = {m.rmin[0], m.rmin[1], m.rmin[2]};
-    double rmax[3] = {m.rmax[0], m.rmax[1], m.rmax[2]};
+__device__ int kd_nearest(const DevMesh &m, const LocParams &L, const d3 &q, const Scratch &sc,
+                          int tid, int cap, bool &ovf) {
+    const int root = L.kd_root;
+    if (root < 0) return -1;
+    double rmin[3] = {L.rmin[0], L.rmin[1], L.rmin[2]};
+    double rmax[3] = {L.rmax[0], L.rmax[1], L.rmax[2]};
     const size_t S = static_cast<size_t>(sc.nthr);
-    const double4 r0 = m.kd[0];
+    const double4 r0 = m.kd[root];
     double best_d;
     {
         const double dx = r0.x - q.x, dy = r0.y - q.y, dz = r0.z - q.z;
         best_d = dx * dx + dy * dy + dz * dz;
     }
-    int best = 0;
+    int best = root;
     int sp = 1;
-    sc.kd_node[tid] = 0;
+    sc.kd_node[tid] = root;
     while (sp > 0) {
         const size_t fi = il(sp - 1, S, tid);
         const int fr = sc.kd_node[fi];
@@ -155,41 +156,41 @@ __device__ int kd_nearest(const DevMesh &m, const d3 &q, const Scratch &sc, int 
 // and returns its tet when the answer is certain: no unexamined centroid can
 // be as close (distance certificate) and no examined one ties.  Otherwise it
 // returns -1 and the caller runs the k-d replica.
-__device__ int grid_nearest(const DevMesh &m, const d3 &q) {
-    const double fx = (q.x - m.g0[0]) * m.ginv;
-    const double fy = (q.y - m.g0[1]) * m.ginv;
-    const double fz = (q.z - m.g0[2]) * m.ginv;
-    if (!(fx >= 0.0 && fx < m.gn[0] && fy >= 0.0 && fy < m.gn[1] && fz >= 0.0 &&
-          fz < m.gn[2]))
+__device__ int grid_nearest(const DevMesh &m, const LocParams &L, const d3 &q) {
+    const double fx = (q.x - L.g0[0]) * L.ginv;
+    const double fy = (q.y - L.g0[1]) * L.ginv;
+    const double fz = (q.z - L.g0[2]) * L.ginv;
+    if (!(fx >= 0.0 && fx < L.gn[0] && fy >= 0.0 && fy < L.gn[1] && fz >= 0.0 &&
+          fz < L.gn[2]))
         return -1;
     const int ix = static_cast<int>(fx), iy = static_cast<int>(fy), iz = static_cast<int>(fz);
     const int ii[3] = {ix, iy, iz};
     double best_d = __builtin_huge_val();
     int best = -1;
     bool tie = false;
-    const double h = m.gh;
+    const double h = L.gh;
     constexpr double kSafe = 1.0 - 1e-9;
     for (int r = 0; r <= 2; r++) {
         for (int dz = -r; dz <= r; dz++) {
             const int z = iz + dz;
-            if (z < 0 || z >= m.gn[2]) continue;
+            if (z < 0 || z >= L.gn[2]) continue;
             for (int dy = -r; dy <= r; dy++) {
                 const int y = iy + dy;
-                if (y < 0 || y >= m.gn[1]) continue;
+                if (y < 0 || y >= L.gn[1]) continue;
                 const bool shell = (dz == -r || dz == r || dy == -r || dy == r);
                 const int step = shell ? 1 : (r > 0 ? 2 * r : 1);
                 for (int dx = -r; dx <= r; dx += step) {
                     const int x = ix + dx;
-                    if (x < 0 || x >= m.gn[0]) continue;
+                    if (x < 0 || x >= L.gn[0]) continue;
                     // squared distance from q to the cell box, rounded down
-                    const double lx = m.g0[0] + x * h, ly = m.g0[1] + y * h,
-                                 lz = m.g0[2] + z * h;
+                    const double lx = L.g0[0] + x * h, ly = L.g0[1] + y * h,
+                                 lz = L.g0[2] + z * h;
                     const double ex = fmax(fmax(lx - q.x, q.x - (lx + h)), 0.0);
                     const double ey = fmax(fmax(ly - q.y, q.y - (ly + h)), 0.0);
                     const double ez = fmax(fmax(lz - q.z, q.z - (lz + h)), 0.0);
                     const double cd = (ex * ex + ey * ey + ez * ez) * kSafe;
                     if (cd > best_d) continue;
-                    const int c = (z * m.gn[1] + y) * m.gn[0] + x;
+                    const int c = L.cell_base + (z * L.gn[1] + y) * L.gn[0] + x;
                     const int b = m.cell_start[c], e = m.cell_start[c + 1];
                     for (int j = b; j < e; j++) {
                         const double4 p = m.cell_pts[j];
@@ -213,9 +214,9 @@ __device__ int grid_nearest(const DevMesh &m, const d3 &q) {
         const double qq[3] = {q.x, q.y, q.z};
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            if (ii[a] - r > 0) rho = fmin(rho, qq[a] - (m.g0[a] + (ii[a] - r) * h));
-            if (ii[a] + r < m.gn[a] - 1)
-                rho = fmin(rho, (m.g0[a] + (ii[a] + r + 1) * h) - qq[a]);
+            if (ii[a] - r > 0) rho = fmin(rho, qq[a] - (L.g0[a] + (ii[a] - r) * h));
+            if (ii[a] + r < L.gn[a] - 1)
+                rho = fmin(rho, (L.g0[a] + (ii[a] + r + 1) * h) - qq[a]);
         }
         if (best >= 0 && best_d < rho * rho * kSafe) return tie ? -1 : best;
     }
@@ -251,10 +252,10 @@ __device__ int walk(const DevMesh &m, int s, const d3 &q, bool &ovf) {
 }
 
 // Start tet of the cavity search (findContainingSimplex with the k-d seed).
-__device__ int locate(const DevMesh &m, const d3 &q, const Scratch &sc, int tid,
-                      const Caps &caps, bool &ovf) {
-    int s = grid_nearest(m, q);
-    if (s < 0) s = kd_nearest(m, q, sc, tid, caps.kd, ovf);
+__device__ int locate(const DevMesh &m, const LocParams &L, const d3 &q, const Scratch &sc,
+                      int tid, const Caps &caps, bool &ovf) {
+    int s = grid_nearest(m, L, q);
+    if (s < 0) s = kd_nearest(m, L, q, sc, tid, caps.kd, ovf);
     if (ovf || s < 0) return -1;
     return walk(m, s, q, ovf);
 }
@@ -628,12 +629,12 @@ __device__ __forceinline__ bool next_batch(unsigned int *work, int n_items, int 
 // query qi (written to *dq; nb_vol then holds <dL/dy, values[j]> per slot).
 template <bool GRAD = false>
 __device__ int fused_row(const DevMesh &m, const Caps &caps, const Scratch &sc, int tid,
-                         int stamp, const d3 &q, const GradIn *g = nullptr, int qi = 0,
+                         int stamp, const d3 &q, int qi, const GradIn *g = nullptr,
                          d3 *dq = nullptr) {
     const size_t S = static_cast<size_t>(sc.nthr);
     const int N = m.n_data;
     bool ovf = false;
-    const int start = locate(m, q, sc, tid, caps, ovf);
+    const int start = locate(m, loc_of(m, qi), q, sc, tid, caps, ovf);
     if (ovf) return -1;
     if (start < 0) return 0;  // outside the super simplex: empty row
     const HashTab vis(sc.vis, sc.vis_pos, tid, caps.vis_log2, stamp);
@@ -751,7 +752,7 @@ __global__ void __launch_bounds__(256) sibson_rows_kernel(DevMesh m, Caps caps, 
             epoch = epoch == 0x7ffffffe ? 1 : epoch + 1;
             const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                            q[3 * static_cast<size_t>(qi) + 2]};
-            cnt = fused_row(m, caps, sc, tid, epoch, qq);
+            cnt = fused_row(m, caps, sc, tid, epoch, qq, qi);
             if (cnt < 0) {
                 out.ovf_list[atomicAdd(out.ovf_count, 1u)] = qi;
                 cnt = 0;
@@ -790,7 +791,7 @@ __global__ void __launch_bounds__(256) sibson_grad_rows_kernel(
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
         d3 dq;
-        if (fused_row<true>(m, caps, sc, tid, epoch, qq, &g, qi, &dq) < 0)
+        if (fused_row<true>(m, caps, sc, tid, epoch, qq, qi, &g, &dq) < 0)
             ovf_list[atomicAdd(ovf_count, 1u)] = qi;
         else
             store_dq(g, qi, dq);
@@ -824,7 +825,7 @@ __global__ void __launch_bounds__(256) locate_kernel(DevMesh m, Caps caps, Scrat
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
         bool ovf = false;
-        const int s = locate(m, qq, sc, tid, caps, ovf);
+        const int s = locate(m, loc_of(m, qi), qq, sc, tid, caps, ovf);
         start[qi] = ovf ? -2 : s;
     }
 }
