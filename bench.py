@@ -144,6 +144,107 @@ def cpu_baseline(refbuild, qc, target_s):
                       f"host {platform.processor() or platform.machine()}"}
 
 
+def run_batch(args):
+    """Batching workload: independent problems with log-uniform key counts in
+    [500, 50,000] and query counts in [1,000, 200,000], points uniform in
+    per-problem boxes of sides U[0.5, 2], values float32 [N, 8].  One step =
+    the forward pass (all weight rows + W @ values) of every problem; batched:
+    one BatchInterpolator call (one launch per kernel for all problems);
+    sequential: one single-problem call per problem."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import torch
+
+    import natinterp3d_amd
+    from natinterp3d_amd import workloads
+    from natinterp3d_amd.natinterp3d import spmm_device
+
+    probs = workloads.batch_problems(n_problems=args.problems)
+    nk = sum(len(k) for k, _, _ in probs)
+    nq = sum(len(q) for _, q, _ in probs)
+    log(f"batch workload: {len(probs)} problems, {nk:,} keys, {nq:,} queries")
+    th = cpu_threads()
+    t = time.time()
+    batch = natinterp3d_amd.BatchInterpolator([k for k, _, _ in probs], device=0, num_threads=th)
+    build_s = time.time() - t
+    log(f"batched mesh built in {build_s:.1f} s on {th} threads")
+    cents = [np.ascontiguousarray(k).mean(axis=0) for k, _, _ in probs]
+
+    def single(i):
+        k = np.ascontiguousarray(probs[i][0])
+        return natinterp3d_amd.MeshAndVerticesHIP(np.ascontiguousarray(k - cents[i]), device=0)
+
+    t = time.time()
+    with ThreadPoolExecutor(th) as ex:
+        singles = list(ex.map(single, range(len(probs))))
+    log(f"{len(singles)} single meshes built in {time.time() - t:.1f} s")
+
+    dev = torch.device("cuda", 0)
+    qs = [np.ascontiguousarray(q - c) for (_, q, _), c in zip(probs, cents)]
+    off = np.zeros(len(probs) + 1, np.int64)
+    off[1:] = np.cumsum([len(q) for q in qs])
+    qcat = torch.from_numpy(np.concatenate(qs)).to(dev)
+    vcat = torch.from_numpy(np.concatenate([v for _, _, v in probs])).to(dev)
+    qd = [torch.from_numpy(q).to(dev) for q in qs]
+    vd = [torch.from_numpy(v).to(dev) for _, _, v in probs]
+
+    def batched():
+        crow, col, val = batch.weights_device(qcat, off)
+        return spmm_device(crow, col, val, vcat, int(off[-1])), col.numel()
+
+    def sequential():
+        n = 0
+        outs = []
+        for m, q, v in zip(singles, qd, vd):
+            crow, col, val = m.weights_device(q)
+            outs.append(spmm_device(crow, col, val, v, q.shape[0]))
+            n += col.numel()
+        return outs, n
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps, res
+
+    b_ms, (yb, nnz_b) = timed(batched)
+    s_ms, (ys, nnz_s) = timed(sequential)
+    same = all(torch.equal(yb[off[i]:off[i + 1]], ys[i]) for i in range(len(probs)))
+    log(f"batched {b_ms:.2f} ms/step, sequential {s_ms:.2f} ms/step, "
+        f"speed-up {s_ms / b_ms:.2f}x, nnz {nnz_b:,} / {nnz_s:,}, products equal: {same}")
+    out = {
+        "metric": "batched Sibson queries/s, forward, 256 independent problems in one set of "
+                  "launches vs 256 sequential single-problem calls",
+        "value": nq / (b_ms / 1e3),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": b_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batch: {len(probs)} problems, keys log-uniform [500, 50000], "
+                        "queries log-uniform [1000, 200000], uniform in boxes of sides "
+                        "U[0.5, 2], values float32 [N, 8]",
+            "keys": nk, "queries": nq, "nnz": int(nnz_b),
+            "sequential_ms_per_step": s_ms,
+            "sequential_queries_per_s": nq / (s_ms / 1e3),
+            "speedup_vs_sequential": s_ms / b_ms,
+            "products_equal_to_sequential": bool(same),
+            "batched_mesh_build_s": round(build_s, 1),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +254,10 @@ def main():
     ap.add_argument("--queries", type=int, default=10_000_000)
     ap.add_argument("--cpu-baseline", choices=["reference", "none"], default="reference")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--workload", choices=["plain", "batch"], default="plain",
+                    help="plain: the fwd+bwd headline; batch: 256 independent problems in "
+                         "one set of launches against 256 sequential single-problem calls")
+    ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the steps, copy this many MiB device to device once: a "
                          "kernel of known traffic that scales hardware byte counters")
@@ -161,6 +266,8 @@ def main():
         print("bench.py measures one GPU: the query path runs on a single MI355X "
               "(multi-GPU sharding is a design only); run it with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        return run_batch(args)
 
     import torch
 

@@ -105,7 +105,8 @@ struct nat_mesh {
 
     // workspace
     nat::Caps caps[kLevels];
-    int nthr[kLevels] = {0, 0, 0, 0};  // [0]: locate kernel threads
+    int nthr[kLevels] = {0, 0, 0, 0};  // [0]: locate kernel threads; the most per level
+    int scap[kLevels] = {0, 0, 0, 0};  // threads the level's scratch is laid out for
     int nthr_geom = 0;
     int topo_blocks = 0;
     DevBuf scratch[kLevels];
@@ -415,12 +416,26 @@ int upload_mesh(nat_mesh *m) {
     return 0;
 }
 
-int ensure_level_scratch(nat_mesh *m, int level) {
-    // each of the (at most 16) arrays is rounded up to 256 bytes when carved
-    const size_t per = nat::scratch_bytes_per_thread(m->caps[level]);
-    const size_t need = per * static_cast<size_t>(m->nthr[level]) + 16 * 256;
-    if (m->scratch[level].bytes >= need && m->scratch[level].p) return 0;
-    HIP_OR_FAIL(m->scratch[level].ensure(need, /*zero=*/true));
+// Per-thread scratch of a level for a launch of `want` threads: the level's
+// thread count is capped by the work present, so small calls hold little
+// memory.  The layout (stride = capacity in threads) only changes when the
+// capacity grows, and then the scratch is cleared: the per-thread stamps of
+// the hash tables stay consistent.  Returns the threads to launch; the carve
+// stride is m->scap[level].
+int ensure_level_scratch(nat_mesh *m, int level, long long want, int *launch) {
+    const int cap_max = m->nthr[level];
+    const int n = static_cast<int>(std::min<long long>(cap_max, std::max<long long>(want, 1)));
+    if (n > m->scap[level]) {
+        int cap = std::max(n, 2 * m->scap[level]);
+        cap = std::min(cap_max, (cap + 255) / 256 * 256);
+        // each of the (at most 16) arrays is rounded up to 256 bytes when carved
+        const size_t per = nat::scratch_bytes_per_thread(m->caps[level]);
+        const size_t need = per * static_cast<size_t>(cap) + 16 * 256;
+        m->scratch[level].release();
+        HIP_OR_FAIL(m->scratch[level].ensure(need, /*zero=*/true));
+        m->scap[level] = cap;
+    }
+    *launch = n;
     return 0;
 }
 
@@ -460,7 +475,8 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
     HIP_OR_FAIL(m->ovf[0].ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->ovf[1].ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->counters.ensure(256));
-    if (int rc = ensure_level_scratch(m, 0)) return rc;
+    int nthr0 = 0;
+    if (int rc = ensure_level_scratch(m, 0, M, &nthr0)) return rc;
 
     m->times = nat_timings{};
     mark(m, 0, st);
@@ -502,14 +518,14 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
 
         // level 0: start tets, topology records, then rows from them
         nat::Scratch sc0;
-        nat::scratch_carve(m->scratch[0].p, m->caps[0], m->nthr[0], &sc0);
+        nat::scratch_carve(m->scratch[0].p, m->caps[0], m->scap[0], &sc0);
         nat::Topo topo;
         topo.rec = m->rec.as<int4>();
         topo.rec_cap = m->rec_cap;
         topo.rec_used = rec_used;
         topo.rec_off = m->rec_off.as<long long>();
         mark(m, 2, st);
-        HIP_OR_FAIL(nat::launch_locate(m->dev, m->caps[0], sc0, m->nthr[0] / 256, d_q,
+        HIP_OR_FAIL(nat::launch_locate(m->dev, m->caps[0], sc0, (nthr0 + 255) / 256, d_q,
                                        m->order.as<int>(), M, m->start.as<int>(), st));
         mark(m, 3, st);
         m->times.locate_ms = elapsed(m, 2, 3);
@@ -546,12 +562,13 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         const int *items = m->ovf[0].as<int>();
         int n_items = static_cast<int>(novf);
         for (int level = 1; level < kLevels && n_items > 0; level++) {
-            if (int rc = ensure_level_scratch(m, level)) return rc;
+            int nthr = 0;
+            if (int rc = ensure_level_scratch(m, level, n_items, &nthr)) return rc;
             nat::Scratch sc;
-            nat::scratch_carve(m->scratch[level].p, m->caps[level], m->nthr[level], &sc);
+            nat::scratch_carve(m->scratch[level].p, m->caps[level], m->scap[level], &sc);
             out.ovf_list = m->ovf[level & 1].as<int>();
             out.ovf_count = ctr + 8 + level;
-            const int nblocks = (m->nthr[level] + 255) / 256;
+            const int nblocks = (nthr + 255) / 256;
             mark(m, 2, st);
             HIP_OR_FAIL(nat::launch_rows(m->dev, m->caps[level], sc, nblocks, d_q, items,
                                          n_items, ctr + level, out, st));
@@ -643,10 +660,11 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
     const int *items = m->ovf[0].as<int>();
     int n_items = static_cast<int>(novf);
     for (int level = 1; level < kLevels && n_items > 0; level++) {
-        if (int rc = ensure_level_scratch(m, level)) return rc;
+        int nthr = 0;
+        if (int rc = ensure_level_scratch(m, level, n_items, &nthr)) return rc;
         nat::Scratch sc;
-        nat::scratch_carve(m->scratch[level].p, m->caps[level], m->nthr[level], &sc);
-        const int nblocks = (m->nthr[level] + 255) / 256;
+        nat::scratch_carve(m->scratch[level].p, m->caps[level], m->scap[level], &sc);
+        const int nblocks = (nthr + 255) / 256;
         mark(m, 2, st);
         HIP_OR_FAIL(nat::launch_grad_rows(m->dev, m->caps[level], sc, nblocks, d_q, items,
                                           n_items, ctr + level, g, m->ovf[level & 1].as<int>(),

@@ -819,8 +819,9 @@ __global__ void __launch_bounds__(256) locate_kernel(DevMesh m, Caps caps, Scrat
                                                      const int *__restrict__ items, int n_items,
                                                      int *__restrict__ start) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= sc.nthr) return;
-    for (int i = tid; i < n_items; i += sc.nthr) {
+    if (tid >= sc.nthr) return;  // sc.nthr: the scratch stride, at least the threads used
+    const int stride = min(static_cast<int>(gridDim.x * blockDim.x), sc.nthr);
+    for (int i = tid; i < n_items; i += stride) {
         const int qi = items ? items[i] : i;
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
