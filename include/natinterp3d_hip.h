@@ -183,9 +183,28 @@ typedef struct {
     double grad_ms[4];        /* nat_grad_queries: [0] from the saved records,
                                  [1..3] recomputed rows per capacity level */
     int32_t grad_items[4];
+    int64_t mixed_fp32;       /* mixed precision: tets and boundary faces done in
+                                 FP32 by the last compute (always counted) */
+    int64_t mixed_fp64;       /* ... and recomputed in FP64 (ill-conditioned) */
+    int64_t mixed_rows_fp64;  /* rows redone in FP64 (cancelling volumes) */
 } nat_timings;
 
 int nat_mesh_set_timing(nat_mesh *m, int enable);
+
+/* Row arithmetic of a mesh (no reference counterpart).  mode 0 (default):
+ * FP64 with the reference's operation order, weights bit-identical.  mode 1:
+ * mixed -- the cavity, boundary and neighbour order still come from the FP64
+ * predicates (neighbour sets and CSR columns unchanged), the circumcentres
+ * and subcell volumes of the rows made by the first capacity level run in
+ * FP32 relative to the query; every tet or boundary face whose condition
+ * estimate |det| / (product of edge lengths) is below cond_threshold, every
+ * row whose subcell volumes cancel (sum |vol| > cancel_threshold * |sum vol|,
+ * the factor by which FP32 rounding of the terms reaches the weights) and
+ * every row of the larger levels is computed in FP64.  nat_timings reports
+ * how many tets, faces and rows took each path.  The backward pass is FP64
+ * in both modes (from the forward's weights). */
+int nat_mesh_set_precision(nat_mesh *m, int mode, double cond_threshold,
+                           double cancel_threshold);
 int nat_mesh_get_timings(const nat_mesh *m, nat_timings *out);
 
 /* Release memory returned by nat_get_weights. */

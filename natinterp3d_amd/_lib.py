@@ -18,6 +18,7 @@ EXPORTS = (
     "nat_spmm_t_csr",
     "nat_mesh_set_timing", "nat_mesh_get_timings", "nat_free_host", "nat_last_error",
     "nat_batch_build", "nat_batch_offsets", "nat_batch_weights_compute",
+    "nat_mesh_set_precision",
 )
 
 
@@ -28,7 +29,8 @@ class Timings(ctypes.Structure):
         ("geom_ms", ctypes.c_double), ("rows_ms", ctypes.c_double * 4),
         ("rows_items", ctypes.c_int32 * 4), ("levels", ctypes.c_int32),
         ("emit_ms", ctypes.c_double), ("grad_ms", ctypes.c_double * 4),
-        ("grad_items", ctypes.c_int32 * 4),
+        ("grad_items", ctypes.c_int32 * 4), ("mixed_fp32", ctypes.c_int64),
+        ("mixed_fp64", ctypes.c_int64), ("mixed_rows_fp64", ctypes.c_int64),
     ]
 
 
@@ -94,6 +96,8 @@ def lib():
     L.nat_batch_weights_compute.restype = ctypes.c_int
     L.nat_batch_weights_compute.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int64), vp,
                                             ctypes.POINTER(ctypes.c_int64)]
+    L.nat_mesh_set_precision.restype = ctypes.c_int
+    L.nat_mesh_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.nat_mesh_set_timing.restype = ctypes.c_int
     L.nat_mesh_set_timing.argtypes = [vp, ctypes.c_int]
     L.nat_mesh_get_timings.restype = ctypes.c_int

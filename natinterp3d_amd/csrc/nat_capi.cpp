@@ -130,6 +130,11 @@ struct nat_mesh {
     long long last_nnz = 0;
     long long last_units = 0;  // record units of the last compute
 
+    // row arithmetic: 0 = FP64, bit-identical to the reference; 1 = mixed
+    int precision = 0;
+    float mixed_tau = 0.02f;
+    float mixed_kappa = 20.0f;
+
     // optional kernel timing (HIP events on the launch stream)
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -502,7 +507,8 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
             m->rec_cap = want_rec;
         }
         // counters: [0..3] fused-level work, [4] topology work, [5] geometry
-        // work, [8..11] overflow counts, u64 pool use at [16], records at [18]
+        // work, [8..11] overflow counts, u64 pool use at [16], records at [18],
+        // mixed-path FP32 / FP64 counts at [20], [22], rows redone at [24]
         unsigned *ctr = m->counters.as<unsigned>();
         unsigned long long *pool_used = reinterpret_cast<unsigned long long *>(ctr + 16);
         unsigned long long *rec_used = reinterpret_cast<unsigned long long *>(ctr + 18);
@@ -550,8 +556,10 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         mark(m, 2, st);
         HIP_OR_FAIL(nat::launch_work_sort(m->order.as<int>(), M, topo, m->geom_order.as<int>(),
                                           st));
-        HIP_OR_FAIL(nat::launch_geom(m->dev, m->nthr_geom / nat::kGeomThreads, d_q,
-                                     m->geom_order.as<int>(), M, ctr + 5, topo, out, st));
+        HIP_OR_FAIL(nat::launch_geom(
+            m->dev, m->nthr_geom / nat::kGeomThreads, d_q, m->geom_order.as<int>(), M, ctr + 5,
+            topo, out, m->mixed_tau, m->mixed_kappa,
+            m->precision == 1 ? reinterpret_cast<unsigned long long *>(ctr + 20) : nullptr, st));
         mark(m, 3, st);
         m->times.geom_ms = elapsed(m, 2, 3);
         m->times.rows_ms[0] = m->times.topo_ms + m->times.geom_ms;
@@ -586,9 +594,15 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         if (n_items > 0)
             return fail(-4, "%d queries need more than %d cavity tets", n_items,
                         m->caps[kLevels - 1].cav);
-        unsigned long long used = 0;
-        HIP_OR_FAIL(hipMemcpyAsync(&used, pool_used, sizeof(used), hipMemcpyDeviceToHost, st));
+        // pool use, record use, FP32 / FP64 tets and faces and FP64 rows of
+        // the mixed path
+        unsigned long long tail[5] = {0, 0, 0, 0, 0};
+        HIP_OR_FAIL(hipMemcpyAsync(tail, pool_used, sizeof(tail), hipMemcpyDeviceToHost, st));
         HIP_OR_FAIL(hipStreamSynchronize(st));
+        const unsigned long long used = tail[0];
+        m->times.mixed_fp32 = static_cast<int64_t>(tail[2]);
+        m->times.mixed_fp64 = static_cast<int64_t>(tail[3]);
+        m->times.mixed_rows_fp64 = static_cast<int64_t>(tail[4]);
         if (static_cast<long long>(used) <= m->pool_cap) {
             m->last_M = M;
             m->last_nnz = static_cast<long long>(used);
@@ -1071,6 +1085,24 @@ int nat_csr_sort_rows(int num_rows, const int32_t *d_rowptr, int32_t *d_cols, vo
     hipError_t e = nat::launch_sort_rows(num_rows, d_rowptr, d_cols, d_vals, vals_dtype,
                                          static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail(-10, "sort launch failed: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int nat_mesh_set_precision(nat_mesh *m, int mode, double cond_threshold,
+                           double cancel_threshold) {
+    g_err.clear();
+    if (!m) return fail(-1, "NULL mesh");
+    if (mode != 0 && mode != 1) return fail(-1, "precision mode %d (0 = fp64, 1 = mixed)", mode);
+    if (mode == 1 && !(cond_threshold > 0.0 && cond_threshold <= 1.0))
+        return fail(-1, "condition threshold %g outside (0, 1]", cond_threshold);
+    if (mode == 1 && !(cancel_threshold >= 1.0))
+        return fail(-1, "cancellation threshold %g below 1", cancel_threshold);
+    std::lock_guard<std::mutex> lk(m->mu);
+    m->precision = mode;
+    if (mode == 1) {
+        m->mixed_tau = static_cast<float>(cond_threshold);
+        m->mixed_kappa = static_cast<float>(cancel_threshold);
+    }
     return 0;
 }
 

@@ -149,8 +149,12 @@ hipError_t launch_topo(const DevMesh &m, int nblocks, const double *d_q, const i
 // the items regrouped by record size within spatial windows (geometry order)
 hipError_t launch_work_sort(const int *d_items, int n, const Topo &topo, int *d_out,
                             hipStream_t st);
+// mix != NULL: mixed precision (FP32 where the condition estimate is at least
+// mixed_tau; rows with sum |vol| > mixed_kappa |sum vol| redone in FP64),
+// counts of FP32 / FP64 tets and faces and of redone rows added to mix[0..2]
 hipError_t launch_geom(const DevMesh &m, int nblocks, const double *d_q, const int *d_items,
                        int n_items, unsigned int *d_work, const Topo &topo, const RowOut &out,
+                       float mixed_tau, float mixed_kappa, unsigned long long *mix,
                        hipStream_t st);
 // backward: dL/dq from the topology records, then the fused kernel for the
 // queries without one (and those that overflow it); W^T g

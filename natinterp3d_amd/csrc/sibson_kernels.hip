@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include "geom.h"
+#include "geom32.h"
 #include "nat_internal.h"
 
 namespace nat {
@@ -533,6 +534,72 @@ __device__ __forceinline__ void new_contributions(const DevMesh &m, int t, int f
         if constexpr (WithVol) sub(slot, subcell_volume(F[k], q, A, B, cv, ca, cb, cf));
         if constexpr (Mom::active) mom(slot, facet_moment(F[k], q, ca, cb, cv));
     }
+}
+
+// ---- mixed precision: steps 3 and 4 in FP32 relative to the query, with
+// FP64 (the functions above, bit-identical to the plain path) for tets whose
+// condition estimate is below tau and for tets with a super vertex (points
+// far from the query, whose FP32 offsets would lose the local detail).
+// Return false when FP64 is needed.
+
+template <typename SlotOf, typename Add>
+__device__ __forceinline__ bool old_contributions32(const DevMesh &m, int t, const d3 &q,
+                                                    float tau, SlotOf slot_of, Add add) {
+    bool neg;
+    const int4 tv = ld_tet_v(m, t, neg);
+    const int N = m.n_data;
+    if (tv.x >= N || tv.y >= N || tv.z >= N || tv.w >= N) return false;
+    const f3 r0 = rel32(ld_vert(m, tv.x), q), r1 = rel32(ld_vert(m, tv.y), q);
+    const f3 r2 = rel32(ld_vert(m, tv.z), q), r3 = rel32(ld_vert(m, tv.w), q);
+    if (!(tet_cond(r0, r1, r2, r3) >= tau)) return false;
+    const f3 ct = cc_tet32(r0, r1, r2, r3);
+    // face circumcentres opposite each vertex, vertices ascending
+    const f3 c0 = cc_tri32(r1, r2, r3), c1 = cc_tri32(r0, r2, r3);
+    const f3 c2 = cc_tri32(r0, r1, r3), c3 = cc_tri32(r0, r1, r2);
+    auto one = [&](int vi, const f3 &pk, const f3 &pa, const f3 &pb, const f3 &pc, const f3 &fa,
+                   const f3 &fb, const f3 &fc) {
+        const int slot = slot_of(vi, sel4(tv, vi));
+        if (slot >= 0)
+            add(slot, static_cast<double>(subcell_volume32(pk, pa, pb, pc, ct, fc, fb, fa)));
+    };
+    one(0, r0, r1, r2, r3, c1, c2, c3);
+    one(1, r1, r0, r2, r3, c0, c2, c3);
+    one(2, r2, r0, r1, r3, c0, c1, c3);
+    one(3, r3, r0, r1, r2, c0, c1, c2);
+    return true;
+}
+
+template <typename SlotOf, typename Sub>
+__device__ __forceinline__ bool new_contributions32(const DevMesh &m, int t, int fi, const d3 &q,
+                                                    float tau, SlotOf slot_of, Sub sub) {
+    bool neg;
+    const int4 tv = ld_tet_v(m, t, neg);
+    const int s[3] = {face_slot(fi, 0), face_slot(fi, 1), face_slot(fi, 2)};
+    const int N = m.n_data;
+    if (sel4(tv, s[0]) >= N || sel4(tv, s[1]) >= N || sel4(tv, s[2]) >= N) return false;
+    const f3 F[3] = {rel32(ld_vert(m, sel4(tv, s[0])), q), rel32(ld_vert(m, sel4(tv, s[1])), q),
+                     rel32(ld_vert(m, sel4(tv, s[2])), q)};
+    const f3 o = {0.0f, 0.0f, 0.0f};  // the query
+    const float tau2 = tau * tau;
+    bool good = tet_cond(o, F[0], F[1], F[2]) >= tau;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        good = good && tri_cond(F[k], o, F[(k + 1) % 3]) >= tau2 &&
+               tri_cond(F[k], o, F[(k + 2) % 3]) >= tau2;
+    if (!good) return false;
+    const f3 cv = cc_tet32(o, F[0], F[1], F[2]);
+    const f3 cf = cc_tri32(F[0], F[1], F[2]);  // the order is immaterial in FP32
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int slot = slot_of(s[k], sel4(tv, s[k]));
+        if (slot < 0) continue;
+        const f3 &A = F[(k + 1) % 3];
+        const f3 &B = F[(k + 2) % 3];
+        const f3 ca = cc_tri32(F[k], o, A);
+        const f3 cb = cc_tri32(F[k], o, B);
+        sub(slot, static_cast<double>(subcell_volume32(F[k], o, A, B, cv, ca, cb, cf)));
+    }
+    return true;
 }
 
 template <typename F>
@@ -1283,13 +1350,22 @@ __device__ __forceinline__ int rec_slot(const int4 &cr, int vi) {
 // Steps 3-5 from the topology record, one thread per query; the row's
 // per-neighbour accumulators live in LDS (slot j of thread t at
 // [j * kGeomThreads + t]: the lanes of a wave always hit distinct banks).
+// MIXED: steps 3-4 in FP32 where the condition estimate (>= tau) allows.
+// A row whose subcell volumes cancel, sum |vol| > kappa |sum vol|, would
+// carry the FP32 rounding of its largest terms into its weights: it is
+// recomputed in FP64.  Counts of FP32 and FP64 tets and faces and of
+// recomputed rows are added to mix[0..2].
+template <bool MIXED>
 __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
                                                             const double *__restrict__ q,
                                                             const int *__restrict__ items,
                                                             int n_items, unsigned int *work,
-                                                            Topo topo, RowOut out) {
+                                                            Topo topo, RowOut out, float tau,
+                                                            float kappa,
+                                                            unsigned long long *mix) {
     __shared__ double acc_lds[kLevel0Row * kGeomThreads];
     double *acc = acc_lds + threadIdx.x;
+    unsigned n32 = 0, n64 = 0, nredo = 0;
     int item;
     while (next_batch(work, n_items, item)) {
         int qi = -1, cnt = 0, nnb = 0;
@@ -1308,24 +1384,60 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
             } else {
                 const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                                q[3 * static_cast<size_t>(qi) + 2]};
-                for (int j = 0; j < nnb; j++) acc[j * kGeomThreads] = 0.0;
-                for (int ci = 0; ci < ncav; ci++) {
-                    const int4 cr = r[1 + ci];
-                    old_contributions(
-                        m, cr.x, static_cast<unsigned>(cr.w),
-                        [&](int vi, int) { return rec_slot(cr, vi); },
-                        [&](int slot, double vol) { acc[slot * kGeomThreads] += vol; });
-                }
                 const int *rf = reinterpret_cast<const int *>(r + 1 + ncav);
-                for (int bi = 0; bi < nbnd; bi++) {
-                    const int code = rf[bi];
-                    const int4 cr = r[1 + (code >> 2)];
-                    new_contributions(
-                        m, cr.x, code & 3, qq, [&](int vi, int) { return rec_slot(cr, vi); },
-                        [&](int slot, double vol) { acc[slot * kGeomThreads] -= vol; });
-                }
+                double absum = 0.0;
+                auto run = [&](bool fp32) {
+                    for (int j = 0; j < nnb; j++) acc[j * kGeomThreads] = 0.0;
+                    for (int ci = 0; ci < ncav; ci++) {
+                        const int4 cr = r[1 + ci];
+                        auto slot_of = [&](int vi, int) { return rec_slot(cr, vi); };
+                        auto add = [&](int slot, double vol) {
+                            acc[slot * kGeomThreads] += vol;
+                            if constexpr (MIXED) absum += fabs(vol);
+                        };
+                        if constexpr (MIXED) {
+                            if (fp32) {
+                                if (old_contributions32(m, cr.x, qq, tau, slot_of, add)) {
+                                    n32++;
+                                    continue;
+                                }
+                                n64++;
+                            }
+                        }
+                        old_contributions(m, cr.x, static_cast<unsigned>(cr.w), slot_of, add);
+                    }
+                    for (int bi = 0; bi < nbnd; bi++) {
+                        const int code = rf[bi];
+                        const int4 cr = r[1 + (code >> 2)];
+                        auto slot_of = [&](int vi, int) { return rec_slot(cr, vi); };
+                        auto sub = [&](int slot, double vol) {
+                            acc[slot * kGeomThreads] -= vol;
+                            if constexpr (MIXED) absum += fabs(vol);
+                        };
+                        if constexpr (MIXED) {
+                            if (fp32) {
+                                if (new_contributions32(m, cr.x, code & 3, qq, tau, slot_of,
+                                                        sub)) {
+                                    n32++;
+                                    continue;
+                                }
+                                n64++;
+                            }
+                        }
+                        new_contributions(m, cr.x, code & 3, qq, slot_of, sub);
+                    }
+                };
+                run(MIXED);
                 double total = 0.0;
                 for (int j = 0; j < nnb; j++) total += acc[j * kGeomThreads];
+                if constexpr (MIXED) {
+                    if (!(absum <= kappa * fabs(total))) {  // cancelling row: FP64
+                        run(false);
+                        total = 0.0;
+                        for (int j = 0; j < nnb; j++) total += acc[j * kGeomThreads];
+                        nredo++;
+                    }
+                }
                 out.row_total[qi] = total;
                 if (total > 1e-30) {
                     const double inv = 1.0 / total;
@@ -1355,6 +1467,17 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
                 out.pool_cols[off + j] = rn[j];
                 out.pool_vals[off + j] = acc[j * kGeomThreads];
             }
+        }
+    }
+    if constexpr (MIXED) {
+        int t32, t64, tre;
+        wave_excl_scan(static_cast<int>(n32), t32);
+        wave_excl_scan(static_cast<int>(n64), t64);
+        wave_excl_scan(static_cast<int>(nredo), tre);
+        if (__lane_id() == 0) {
+            atomicAdd(mix, static_cast<unsigned long long>(t32));
+            atomicAdd(mix + 1, static_cast<unsigned long long>(t64));
+            atomicAdd(mix + 2, static_cast<unsigned long long>(tre));
         }
     }
 }
@@ -1500,7 +1623,7 @@ int topo_blocks_per_cu() {
 int locate_blocks_per_cu() { return blocks_per_cu(locate_kernel); }
 int geom_blocks_per_cu() {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, geom_kernel, kGeomThreads, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, geom_kernel<false>, kGeomThreads, 0) !=
         hipSuccess)
         return 1;
     return n > 0 ? n : 1;
@@ -1540,9 +1663,14 @@ hipError_t launch_work_sort(const int *d_items, int n, const Topo &topo, int *d_
 
 hipError_t launch_geom(const DevMesh &m, int nblocks, const double *d_q, const int *d_items,
                        int n_items, unsigned int *d_work, const Topo &topo, const RowOut &out,
+                       float mixed_tau, float mixed_kappa, unsigned long long *mix,
                        hipStream_t st) {
-    hipLaunchKernelGGL(geom_kernel, dim3(nblocks), dim3(kGeomThreads), 0, st, m, d_q, d_items,
-                       n_items, d_work, topo, out);
+    if (mix)
+        hipLaunchKernelGGL(geom_kernel<true>, dim3(nblocks), dim3(kGeomThreads), 0, st, m, d_q,
+                           d_items, n_items, d_work, topo, out, mixed_tau, mixed_kappa, mix);
+    else
+        hipLaunchKernelGGL(geom_kernel<false>, dim3(nblocks), dim3(kGeomThreads), 0, st, m, d_q,
+                           d_items, n_items, d_work, topo, out, 0.0f, 0.0f, mix);
     return hipGetLastError();
 }
 

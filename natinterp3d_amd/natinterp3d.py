@@ -98,7 +98,22 @@ class MeshAndVerticesHIP:
                 "geom_ms": t.geom_ms,
                 "rows_ms": list(t.rows_ms)[:n],
                 "rows_items": list(t.rows_items)[:n], "emit_ms": t.emit_ms,
-                "grad_ms": list(t.grad_ms), "grad_items": list(t.grad_items)}
+                "grad_ms": list(t.grad_ms), "grad_items": list(t.grad_items),
+                "mixed_fp32": t.mixed_fp32, "mixed_fp64": t.mixed_fp64,
+                "mixed_rows_fp64": t.mixed_rows_fp64}
+
+    def set_precision(self, mode="fp64", cond_threshold=0.02, cancel_threshold=20.0):
+        """Row arithmetic: "fp64" (default; weights bit-identical to the
+        reference) or "mixed" (FP32 circumcentres and volumes relative to the
+        query; FP64 for tets whose condition estimate is below cond_threshold
+        and for rows whose subcell volumes cancel by more than
+        cancel_threshold; neighbour sets unchanged)."""
+        modes = {"fp64": 0, "mixed": 1}
+        if mode not in modes:
+            raise ValueError(f"precision must be one of {sorted(modes)}")
+        _lib.check(_lib.lib().nat_mesh_set_precision(self._m, modes[mode], cond_threshold,
+                                                     cancel_threshold),
+                   "nat_mesh_set_precision")
 
     def export(self):
         """The packed host mesh: verts [(N+4),3], tet_v/tet_n [T,4], k-d arrays."""
