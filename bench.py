@@ -144,6 +144,100 @@ def cpu_baseline(refbuild, qc, target_s):
                       f"host {platform.processor() or platform.machine()}"}
 
 
+def run_precision(args):
+    """Precision / degeneracy stress: keys on a 100^3 lattice scaled to
+    [0,1]^3 with per-coordinate jitter U[-1e-9, 1e-9] (near-cospherical
+    tets), queries uniform in [0.01,0.99]^3, scalar float64 values.  One step
+    = the forward pass (all weight rows + W @ values), FP64 (bit-identical to
+    the reference) and mixed precision; reports the mixed weights' error
+    against the FP64 weights, the share recomputed in FP64 and the insphere
+    decisions the FP64 static error bound does not certify."""
+    import torch
+
+    import natinterp3d_amd
+    from natinterp3d_amd import workloads
+    from natinterp3d_amd.natinterp3d import spmm_device
+
+    keys, queries, values = workloads.lattice_jitter(n_queries=args.queries)
+    keys = np.ascontiguousarray(keys, np.float64)
+    c = keys.mean(axis=0)
+    log(f"precision workload: {len(keys):,} lattice keys, {len(queries):,} queries")
+    t = time.time()
+    mesh = natinterp3d_amd.MeshAndVerticesHIP(np.ascontiguousarray(keys - c), device=0)
+    build_s = time.time() - t
+    log(f"mesh: {mesh.info()['n_tets']:,} tets, built in {build_s:.1f} s "
+        f"({mesh.info()['coplanar']} coplanar, {mesh.info()['cospherical']} cospherical fixes)")
+    dev = torch.device("cuda", 0)
+    qd = torch.from_numpy(np.ascontiguousarray(queries - c)).to(dev)
+    xd = torch.from_numpy(np.ascontiguousarray(values[:, None])).to(dev)
+    M = qd.shape[0]
+    mesh.set_timing(True)
+
+    def fwd():
+        crow, col, val = mesh.weights_device(qd)
+        return spmm_device(crow, col, val, xd, M), col, val
+
+    def timed():
+        for _ in range(args.warmup):
+            fwd()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res = fwd()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps, res
+
+    ms64, (y64, col64, w64) = timed()
+    geom64 = mesh.timings()["geom_ms"]
+    mesh.set_precision("mixed")
+    msmx, (ymx, colmx, wmx) = timed()
+    tm = mesh.timings()
+    mesh.set_precision("fp64")
+    same_cols = bool(torch.equal(col64, colmx))
+    werr = float((wmx - w64).abs().max()) if w64.numel() else 0.0
+    yerr = float((ymx - y64).abs().max()) if M else 0.0
+    decisions, uncertain = mesh.predicate_audit(qd)
+    share = tm["mixed_fp64"] / max(1, tm["mixed_fp32"] + tm["mixed_fp64"])
+    log(f"fp64 {ms64:.2f} ms/step (geometry {geom64:.2f} ms), mixed {msmx:.2f} ms/step "
+        f"(geometry {tm['geom_ms']:.2f} ms); neighbour sets equal: {same_cols}; max weight "
+        f"error {werr:.3g}, max value error {yerr:.3g}; FP64 tets/faces {share:.4f}, rows "
+        f"redone {tm['mixed_rows_fp64']:,}; insphere decisions {decisions:,}, uncertified "
+        f"{uncertain:,}")
+    out = {
+        "metric": "Sibson queries/s, forward, jittered-lattice stress: FP64 path and mixed "
+                  "path, mixed error vs FP64",
+        "value": M / (ms64 / 1e3),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms64,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "precision: 1,000,000 keys on a 100^3 lattice in [0,1]^3 with jitter "
+                        "U[-1e-9,1e-9], 10,000,000 queries uniform [0.01,0.99]^3, float64 "
+                        "scalar values",
+            "queries": M, "nnz": int(col64.numel()),
+            "mixed_ms_per_step": msmx,
+            "mixed_queries_per_s": M / (msmx / 1e3),
+            "geometry_ms_fp64": geom64, "geometry_ms_mixed": tm["geom_ms"],
+            "neighbour_sets_equal": same_cols,
+            "mixed_max_abs_weight_error": werr,
+            "mixed_max_abs_value_error": yerr,
+            "mixed_fp64_share_tets_faces": share,
+            "mixed_rows_redone_fp64": tm["mixed_rows_fp64"],
+            "insphere_decisions": decisions,
+            "insphere_uncertified": uncertain,
+            "mesh_build_s": round(build_s, 1),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_batch(args):
     """Batching workload: independent problems with log-uniform key counts in
     [500, 50,000] and query counts in [1,000, 200,000], points uniform in
@@ -254,9 +348,10 @@ def main():
     ap.add_argument("--queries", type=int, default=10_000_000)
     ap.add_argument("--cpu-baseline", choices=["reference", "none"], default="reference")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", choices=["plain", "batch"], default="plain",
+    ap.add_argument("--workload", choices=["plain", "batch", "precision"], default="plain",
                     help="plain: the fwd+bwd headline; batch: 256 independent problems in "
-                         "one set of launches against 256 sequential single-problem calls")
+                         "one set of launches against 256 sequential single-problem calls; "
+                         "precision: jittered-lattice stress, FP64 against mixed precision")
     ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the steps, copy this many MiB device to device once: a "
@@ -268,6 +363,8 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         return run_batch(args)
+    if args.workload == "precision":
+        return run_precision(args)
 
     import torch
 

@@ -191,6 +191,22 @@ typedef struct {
 
 int nat_mesh_set_timing(nat_mesh *m, int enable);
 
+/* Robustness audit (no reference counterpart): re-tests the insphere
+ * decisions behind the rows of a saved topology (nat_topology_save) -- the
+ * cavity tets and the tets across its boundary faces -- under Shewchuk's
+ * static error bound for the FP64 determinant.  *decisions counts them,
+ * *uncertain those whose sign FP64 evaluation does not certify (where an
+ * exact predicate could decide differently from the reference). */
+int nat_predicate_audit(nat_mesh *m, const void *d_topo, const double *d_queries,
+                        int num_query_points, int64_t *decisions, int64_t *uncertain,
+                        void *stream);
+
+/* Host evaluation of the same test for n point tuples (a, b, c, d, e), 15
+ * doubles each: det[i] = the FP64 insphere determinant (the reference's
+ * inspherefast expression), uncertain[i] = 1 when its sign is not certified
+ * by the static error bound.  No GPU needed. */
+int nat_insphere_classify(const double *points, int n, double *det, int32_t *uncertain);
+
 /* Row arithmetic of a mesh (no reference counterpart).  mode 0 (default):
  * FP64 with the reference's operation order, weights bit-identical.  mode 1:
  * mixed -- the cavity, boundary and neighbour order still come from the FP64

@@ -18,7 +18,7 @@ EXPORTS = (
     "nat_spmm_t_csr",
     "nat_mesh_set_timing", "nat_mesh_get_timings", "nat_free_host", "nat_last_error",
     "nat_batch_build", "nat_batch_offsets", "nat_batch_weights_compute",
-    "nat_mesh_set_precision",
+    "nat_mesh_set_precision", "nat_predicate_audit", "nat_insphere_classify",
 )
 
 
@@ -96,6 +96,11 @@ def lib():
     L.nat_batch_weights_compute.restype = ctypes.c_int
     L.nat_batch_weights_compute.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int64), vp,
                                             ctypes.POINTER(ctypes.c_int64)]
+    L.nat_predicate_audit.restype = ctypes.c_int
+    L.nat_predicate_audit.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                      ctypes.POINTER(ctypes.c_int64), vp]
+    L.nat_insphere_classify.restype = ctypes.c_int
+    L.nat_insphere_classify.argtypes = [dp, ctypes.c_int, dp, ip]
     L.nat_mesh_set_precision.restype = ctypes.c_int
     L.nat_mesh_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.nat_mesh_set_timing.restype = ctypes.c_int

@@ -60,6 +60,46 @@ __host__ __device__ __forceinline__ double insphere(const d3 &a, const d3 &b, co
     return (dlift * abc - clift * dab) + (blift * cda - alift * bcd);
 }
 
+// Static error bound of the insphere determinant (Shewchuk, "Adaptive
+// precision floating-point arithmetic and fast robust geometric predicates",
+// 1997): with eps = 2^-53, the computed sign is certain when
+// |det| > (16 + 224 eps) eps * permanent, the permanent being the same
+// expression with every product and difference replaced by the sum of the
+// magnitudes of its terms.  Returns true when the sign is not certified.
+__host__ __device__ __forceinline__ bool insphere_uncertain(const d3 &a, const d3 &b, const d3 &c,
+                                                            const d3 &d, const d3 &e) {
+    const double aex = a.x - e.x, bex = b.x - e.x, cex = c.x - e.x, dex = d.x - e.x;
+    const double aey = a.y - e.y, bey = b.y - e.y, cey = c.y - e.y, dey = d.y - e.y;
+    const double aez = a.z - e.z, bez = b.z - e.z, cez = c.z - e.z, dez = d.z - e.z;
+    auto minor = [](double p, double q, double r, double s, double &perm) {
+        perm = fabs(p * q) + fabs(r * s);
+        return p * q - r * s;
+    };
+    double pab, pbc, pcd, pda, pac, pbd;
+    const double ab = minor(aex, bey, bex, aey, pab);
+    const double bc = minor(bex, cey, cex, bey, pbc);
+    const double cd = minor(cex, dey, dex, cey, pcd);
+    const double da = minor(dex, aey, aex, dey, pda);
+    const double ac = minor(aex, cey, cex, aey, pac);
+    const double bd = minor(bex, dey, dex, bey, pbd);
+    const double abc = aez * bc - bez * ac + cez * ab;
+    const double bcd = bez * cd - cez * bd + dez * bc;
+    const double cda = cez * da + dez * ac + aez * cd;
+    const double dab = dez * ab + aez * bd + bez * da;
+    const double pabc = fabs(aez) * pbc + fabs(bez) * pac + fabs(cez) * pab;
+    const double pbcd = fabs(bez) * pcd + fabs(cez) * pbd + fabs(dez) * pbc;
+    const double pcda = fabs(cez) * pda + fabs(dez) * pac + fabs(aez) * pcd;
+    const double pdab = fabs(dez) * pab + fabs(aez) * pbd + fabs(bez) * pda;
+    const double alift = aex * aex + aey * aey + aez * aez;
+    const double blift = bex * bex + bey * bey + bez * bez;
+    const double clift = cex * cex + cey * cey + cez * cez;
+    const double dlift = dex * dex + dey * dey + dez * dez;
+    const double det = (dlift * abc - clift * dab) + (blift * cda - alift * bcd);
+    const double perm = (dlift * pabc + clift * pdab) + (blift * pcda + alift * pbcd);
+    const double eps = 1.1102230246251565e-16;  // 2^-53
+    return fabs(det) <= (16.0 + 224.0 * eps) * eps * perm;
+}
+
 // Circumcentre of triangle (a,b,c); the centroid when |ab x ac|^2 < 1e-30.
 __host__ __device__ __forceinline__ d3 cc_tri(const d3 &a, const d3 &b, const d3 &c) {
     const d3 ab = sub3(b, a), ac = sub3(c, a);

@@ -1050,6 +1050,53 @@ int nat_grad_queries(nat_mesh *m, const void *d_topo, const double *d_queries,
                      d_vals, d_gy, d_values, dim, d_dq, static_cast<hipStream_t>(stream));
 }
 
+int nat_insphere_classify(const double *points, int n, double *det, int32_t *uncertain) {
+    g_err.clear();
+    if (n < 0 || (n > 0 && (!points || !det || !uncertain))) return fail(-1, "bad arguments");
+    for (int i = 0; i < n; i++) {
+        const double *p = points + 15 * i;
+        const nat::d3 a = {p[0], p[1], p[2]}, b = {p[3], p[4], p[5]}, c = {p[6], p[7], p[8]};
+        const nat::d3 d = {p[9], p[10], p[11]}, e = {p[12], p[13], p[14]};
+        det[i] = nat::insphere(a, b, c, d, e);
+        uncertain[i] = nat::insphere_uncertain(a, b, c, d, e) ? 1 : 0;
+    }
+    return 0;
+}
+
+int nat_predicate_audit(nat_mesh *m, const void *d_topo, const double *d_queries,
+                        int num_query_points, int64_t *decisions, int64_t *uncertain,
+                        void *stream) {
+    g_err.clear();
+    if (!m || !d_topo || !decisions || !uncertain) return fail(-1, "NULL argument");
+    if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    if (num_query_points > 0 && !d_queries) return fail(-1, "NULL queries");
+    std::lock_guard<std::mutex> lk(m->mu);
+    DeviceGuard g(m->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const char *blob = static_cast<const char *>(d_topo);
+    long long hdr[8];
+    HIP_OR_FAIL(hipMemcpyAsync(hdr, blob, sizeof(hdr), hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    const int M = num_query_points;
+    if (hdr[0] != kTopoMagic || hdr[1] != M || hdr[3] != m->dev.n_data)
+        return fail(-1, "d_topo is not a saved topology of these queries on this mesh");
+    HIP_OR_FAIL(m->counters.ensure(256));
+    unsigned long long *cnt = reinterpret_cast<unsigned long long *>(m->counters.p) + 16;
+    HIP_OR_FAIL(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), st));
+    nat::Topo topo;
+    topo.rec = reinterpret_cast<int4 *>(const_cast<char *>(blob) + topo_rec_off(M));
+    topo.rec_cap = hdr[2];
+    topo.rec_used = nullptr;
+    topo.rec_off = reinterpret_cast<long long *>(const_cast<char *>(blob) + kTopoHeader);
+    if (M > 0) HIP_OR_FAIL(nat::launch_predicate_audit(m->dev, d_queries, M, topo, cnt, st));
+    unsigned long long h[2] = {0, 0};
+    HIP_OR_FAIL(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    *decisions = static_cast<int64_t>(h[0]);
+    *uncertain = static_cast<int64_t>(h[1]);
+    return 0;
+}
+
 int nat_topology_row_order(const void *d_topo, int num_query_points, const int32_t **out) {
     g_err.clear();
     if (!d_topo || !out) return fail(-1, "NULL argument");

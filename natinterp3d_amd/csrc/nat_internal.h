@@ -166,6 +166,9 @@ hipError_t launch_grad_rows(const DevMesh &m, const Caps &caps, const Scratch &s
                             const double *d_q, const int *d_items, int n_items,
                             unsigned int *d_work, const GradIn &g, int *ovf_list,
                             unsigned int *ovf_count, hipStream_t st);
+// insphere decisions behind saved rows re-tested under the static error bound
+hipError_t launch_predicate_audit(const DevMesh &m, const double *d_q, int n, const Topo &topo,
+                                  unsigned long long *d_counts, hipStream_t st);
 hipError_t launch_spmm_t(int M, int N, int D, const int *rowptr, const int *cols,
                          const double *vals, const double *gy, const int *order, double *out,
                          hipStream_t st);

@@ -1545,6 +1545,45 @@ __global__ void __launch_bounds__(kGeomThreads) grad_geom_kernel(
     }
 }
 
+// Audit of the insphere decisions behind each saved row: the cavity tets
+// (decided inside) and the tets across its boundary faces (decided outside)
+// are tested again under the static error bound.  counts[0] += decisions,
+// counts[1] += decisions whose sign the bound does not certify.
+__global__ void __launch_bounds__(256) predicate_audit_kernel(DevMesh m,
+                                                              const double *__restrict__ q,
+                                                              int n, Topo topo,
+                                                              unsigned long long *counts) {
+    unsigned tested = 0, uncertain = 0;
+    for (int qi = blockIdx.x * blockDim.x + threadIdx.x; qi < n; qi += gridDim.x * blockDim.x) {
+        const long long roff = topo.rec_off[qi];
+        if (roff < 0) continue;
+        const int4 *r = topo.rec + roff;
+        const int4 hd = r[0];
+        const int ncav = hd.x, nbnd = hd.y;
+        const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
+                       q[3 * static_cast<size_t>(qi) + 2]};
+        auto check = [&](int t) {
+            bool neg;
+            const int4 tv = ld_tet_v(m, t, neg);
+            const d3 p0 = ld_vert(m, tv.x), p1 = ld_vert(m, tv.y);
+            const d3 p2 = ld_vert(m, tv.z), p3 = ld_vert(m, tv.w);
+            tested++;
+            if (neg ? insphere_uncertain(p1, p0, p2, p3, qq)
+                    : insphere_uncertain(p0, p1, p2, p3, qq))
+                uncertain++;
+        };
+        for (int ci = 0; ci < ncav; ci++) check(r[1 + ci].x);
+        const int *rf = reinterpret_cast<const int *>(r + 1 + ncav);
+        for (int bi = 0; bi < nbnd; bi++) {
+            const int code = rf[bi];
+            const int nb = sel4(m.tet_n[r[1 + (code >> 2)].x], code & 3);
+            if (nb >= 0) check(nb);
+        }
+    }
+    atomicAdd(counts, static_cast<unsigned long long>(tested));
+    atomicAdd(counts + 1, static_cast<unsigned long long>(uncertain));
+}
+
 // W^T g: out[cols[e], d] += vals[e] * gy[row, d] (float64 atomics; the
 // summation order across rows is not fixed).  One thread per (row, d), rows
 // taken in `order` when given.
@@ -1689,6 +1728,14 @@ hipError_t launch_grad_geom(const DevMesh &m, int nblocks, const double *d_q, co
                             hipStream_t st) {
     hipLaunchKernelGGL(grad_geom_kernel, dim3(nblocks), dim3(kGeomThreads), 0, st, m, d_q,
                        d_items, n_items, d_work, topo, g, ovf_list, ovf_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_predicate_audit(const DevMesh &m, const double *d_q, int n, const Topo &topo,
+                                  unsigned long long *d_counts, hipStream_t st) {
+    const int blocks = std::max(1, std::min((n + 255) / 256, 4096));
+    hipLaunchKernelGGL(predicate_audit_kernel, dim3(blocks), dim3(256), 0, st, m, d_q, n, topo,
+                       d_counts);
     return hipGetLastError();
 }
 

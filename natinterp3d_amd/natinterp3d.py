@@ -115,6 +115,29 @@ class MeshAndVerticesHIP:
                                                      cancel_threshold),
                    "nat_mesh_set_precision")
 
+    def predicate_audit(self, queries):
+        """Re-test the insphere decisions behind the rows of centred float64
+        CUDA queries under the FP64 static error bound: returns (decisions,
+        uncertain) -- how many cavity / boundary-neighbour tests the rows
+        rest on and how many of their signs FP64 does not certify."""
+        import torch
+        q = queries.detach().to(dtype=torch.float64).contiguous()
+        self.weights_device(q)
+        L = _lib.lib()
+        nbytes = ctypes.c_int64()
+        _lib.check(L.nat_topology_save(self._m, None, 0, ctypes.byref(nbytes), None),
+                   "nat_topology_save")
+        topo = torch.empty(int(nbytes.value), dtype=torch.uint8, device=q.device)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(q.device).cuda_stream)
+        _lib.check(L.nat_topology_save(self._m, ctypes.c_void_p(topo.data_ptr()), topo.numel(),
+                                       ctypes.byref(nbytes), stream), "nat_topology_save")
+        dec, unc = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(L.nat_predicate_audit(self._m, ctypes.c_void_p(topo.data_ptr()),
+                                         ctypes.c_void_p(q.data_ptr() if q.numel() else 0),
+                                         q.shape[0], ctypes.byref(dec), ctypes.byref(unc),
+                                         stream), "nat_predicate_audit")
+        return int(dec.value), int(unc.value)
+
     def export(self):
         """The packed host mesh: verts [(N+4),3], tet_v/tet_n [T,4], k-d arrays."""
         inf = self.info()

@@ -59,6 +59,35 @@ def test_mixed_lattice_near_cospherical():
     assert n64 > 0  # near-degenerate tets are recomputed in FP64
 
 
+def test_predicate_audit():
+    import torch
+    # well-separated random data: every decision certified
+    k, q, _ = workloads.uniform_small(n_keys=3000, n_queries=5000)
+    c = k.mean(axis=0)
+    m = MeshAndVerticesHIP(np.ascontiguousarray(k - c), device=0)
+    dec, unc = m.predicate_audit(torch.tensor(q - c, device="cuda"))
+    say(f"[audit uniform] {dec:,} decisions, {unc} uncertain")
+    assert dec > 5000 * 40 and unc == 0
+    # queries on the circumsphere of a tet, on the ray from its circumcentre
+    # through its centroid (away from its vertices, where the determinant's
+    # terms would all shrink with it): that tet's test is decided by rounding
+    e = m.export()
+    V, tv = e["verts"], e["tet_v"]
+    qs = []
+    for t in range(400):
+        if (tv[t] >= 3000).any():
+            continue
+        P = V[tv[t]]
+        A = 2.0 * (P[1:] - P[0])
+        b = (P[1:] ** 2).sum(1) - (P[0] ** 2).sum()
+        cc = np.linalg.solve(A, b)
+        u = P.mean(axis=0) - cc
+        qs.append(cc + np.linalg.norm(P[0] - cc) * u / np.linalg.norm(u))
+    dec, unc = m.predicate_audit(torch.tensor(np.array(qs), device="cuda"))
+    say(f"[audit on circumspheres] {dec:,} decisions, {unc} uncertain")
+    assert unc > 0
+
+
 def test_precision_argument_checks():
     rng = np.random.default_rng(0)
     m = MeshAndVerticesHIP(np.ascontiguousarray(rng.random((50, 3)) - 0.5), device=0)
