@@ -195,11 +195,13 @@ void set_levels(nat_mesh *m) {
     m->nthr[0] = cus * nat::locate_blocks_per_cu() * 256;
     m->topo_blocks = cus * nat::topo_blocks_per_cu();
     m->nthr_geom = cus * nat::geom_blocks_per_cu() * nat::kGeomThreads;
-    // larger levels (fused kernel) only see the queries that overflowed
-    m->caps[1] = {kd, 2048, 12, 1024, 2560, 1536, 12};
-    m->nthr[1] = 8192;
-    m->caps[2] = {kd, 32768, 16, 16384, 40960, 24576, 16};
-    m->nthr[2] = 512;
+    // larger levels (fused kernel) only see the queries that overflowed; a
+    // level's scratch holds as many threads as it has queries, up to these
+    // counts (level 1 at most ~14 GB, level 2 ~7 GB, level 3 ~2 GB)
+    m->caps[1] = {kd, 8192, 14, 4096, 10240, 4096, 13};
+    m->nthr[1] = 32768;
+    m->caps[2] = {kd, 65536, 17, 32768, 81920, 32768, 16};
+    m->nthr[2] = 2048;
     m->caps[3] = {kd, 524288, 20, 262144, 655360, 393216, 20};
     m->nthr[3] = 64;
 }
