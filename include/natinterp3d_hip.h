@@ -129,6 +129,25 @@ int nat_batch_offsets(const nat_mesh *m, int64_t *vertex_offsets, int64_t *tet_o
 int nat_batch_weights_compute(nat_mesh *m, const double *d_queries,
                               const int64_t *query_offsets, void *stream, int64_t *nnz_out);
 
+/* ---- graphs (no reference counterpart).  nat_graph_create runs one
+ * weights computation of d_queries [M,3] (with query_offsets for a batch
+ * mesh, else NULL) on `stream` to size every capacity (+25%), then captures
+ * the whole launch sequence -- bucketing, start tets, topology, geometry, the
+ * larger levels and the CSR assembly -- in a hipGraph on a private stream.
+ * nat_graph_launch replays it on `stream` for whatever d_queries holds now
+ * (same M and offsets) and returns nnz; it returns -8 when a capacity was
+ * exceeded (compute eagerly instead, or make a new graph) and -9 when an
+ * eager call on the mesh has reallocated its workspace since.
+ * nat_graph_copy_out copies the last replay's CSR to caller buffers.  The
+ * graph uses the mesh's workspace: calls on the mesh are serialised with it. */
+typedef struct nat_graph nat_graph;
+int nat_graph_create(nat_mesh *m, const double *d_queries, int num_query_points,
+                     const int64_t *query_offsets, void *stream, nat_graph **out);
+int nat_graph_launch(nat_graph *g, void *stream, int64_t *nnz_out);
+int nat_graph_copy_out(nat_graph *g, double *d_vals, int32_t *d_cols, int32_t *d_rowptr,
+                       void *stream);
+void nat_graph_free(nat_graph *g);
+
 /* ---- backward (no reference counterpart: the reference has no autograd;
  * these serve the torch.autograd.Function of natinterp3d_amd).
  *

@@ -6,9 +6,9 @@ Drop-in for natinterp3d's API (natinterp3d/__init__.py:7): ``Interpolator``,
 CSR matrices) with the weights computed by hand-written HIP kernels.
 """
 from .batch import BatchInterpolator
-from .natinterp3d import (Interpolator, MeshAndVerticesHIP, get_weights, interpolate,
-                          spmm_device)
+from .natinterp3d import (GraphCapacityError, Interpolator, MeshAndVerticesHIP, WeightsGraph,
+                          get_weights, interpolate, spmm_device)
 
-__all__ = ["BatchInterpolator", "Interpolator", "MeshAndVerticesHIP", "get_weights",
-           "interpolate", "spmm_device"]
+__all__ = ["BatchInterpolator", "GraphCapacityError", "Interpolator", "MeshAndVerticesHIP",
+           "WeightsGraph", "get_weights", "interpolate", "spmm_device"]
 __version__ = "0.1.0"

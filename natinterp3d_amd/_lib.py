@@ -19,6 +19,7 @@ EXPORTS = (
     "nat_mesh_set_timing", "nat_mesh_get_timings", "nat_free_host", "nat_last_error",
     "nat_batch_build", "nat_batch_offsets", "nat_batch_weights_compute",
     "nat_mesh_set_precision", "nat_predicate_audit", "nat_insphere_classify",
+    "nat_graph_create", "nat_graph_launch", "nat_graph_copy_out", "nat_graph_free",
 )
 
 
@@ -99,6 +100,15 @@ def lib():
     L.nat_predicate_audit.restype = ctypes.c_int
     L.nat_predicate_audit.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
                                       ctypes.POINTER(ctypes.c_int64), vp]
+    L.nat_graph_create.restype = ctypes.c_int
+    L.nat_graph_create.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), vp,
+                                   ctypes.POINTER(vp)]
+    L.nat_graph_launch.restype = ctypes.c_int
+    L.nat_graph_launch.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int64)]
+    L.nat_graph_copy_out.restype = ctypes.c_int
+    L.nat_graph_copy_out.argtypes = [vp, vp, vp, vp, vp]
+    L.nat_graph_free.restype = None
+    L.nat_graph_free.argtypes = [vp]
     L.nat_insphere_classify.restype = ctypes.c_int
     L.nat_insphere_classify.argtypes = [dp, ctypes.c_int, dp, ip]
     L.nat_mesh_set_precision.restype = ctypes.c_int

@@ -56,11 +56,23 @@ __global__ void morton_scatter_kernel(const int *__restrict__ keys, int M,
     order[start[key] + k] = i;
 }
 
+__global__ void zero_u32_kernel(unsigned *__restrict__ p, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+hipError_t launch_zero_u32(void *d_p, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                       st, static_cast<unsigned *>(d_p), n);
+    return hipGetLastError();
+}
+
 hipError_t launch_morton_sort(const double *d_q, int M, const LocParams *d_lp,
                               const int *d_prob, int P, int bits, int *d_order, int *d_hist,
                               int *d_keys, long long *d_scan_tmp, hipStream_t st) {
     const int nb = P << (3 * bits);
-    hipError_t e = hipMemsetAsync(d_hist, 0, sizeof(int) * nb, st);
+    hipError_t e = launch_zero_u32(d_hist, nb, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(morton_key_kernel, dim3((M + 255) / 256), dim3(256), 0, st, d_q, M, d_lp,
                        d_prob, bits, d_keys, d_hist);
@@ -70,7 +82,7 @@ hipError_t launch_morton_sort(const double *d_q, int M, const LocParams *d_lp,
     long long *tmp = d_scan_tmp + nb + 1;
     e = launch_scan_counts(d_hist, starts, nb, tmp, st);
     if (e != hipSuccess) return e;
-    e = hipMemsetAsync(d_hist, 0, sizeof(int) * nb, st);
+    e = launch_zero_u32(d_hist, nb, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(morton_scatter_kernel, dim3((M + 255) / 256), dim3(256), 0, st, d_keys,
                        M, starts, d_hist, d_order);
@@ -195,21 +207,24 @@ hipError_t launch_scan_counts(const int *d_in, long long *d_out, int n, long lon
 // ------------------------------------------------------------ CSR gather
 
 // 16 lanes per row: each row's pool range and CSR range are contiguous, so
-// the lanes of a row read and write consecutive entries.
+// the lanes of a row read and write consecutive entries.  Rows reaching past
+// cap (a replay whose pool overflowed, reported separately) are skipped.
 constexpr int kGatherLanes = 16;
 
 __global__ void __launch_bounds__(256) gather_csr_kernel(
     const long long *__restrict__ rowptr64, const long long *__restrict__ row_off,
     const int *__restrict__ row_cnt, const double *__restrict__ pv, const int *__restrict__ pc,
-    int M, double *__restrict__ vals, int *__restrict__ cols, int *__restrict__ rowptr32) {
+    int M, long long cap, double *__restrict__ vals, int *__restrict__ cols,
+    int *__restrict__ rowptr32) {
     const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const long long i = t / kGatherLanes;
     const int lane = static_cast<int>(t % kGatherLanes);
     if (i > M) return;
-    if (lane == 0) rowptr32[i] = static_cast<int>(rowptr64[i]);
+    if (lane == 0) rowptr32[i] = static_cast<int>(min(rowptr64[i], cap));
     if (i == M) return;
     const long long dst = rowptr64[i], src = row_off[i];
     const int c = row_cnt[i];
+    if (src < 0 || src + c > cap || dst + c > cap) return;
     for (int j = lane; j < c; j += kGatherLanes) {
         vals[dst + j] = pv[src + j];
         cols[dst + j] = pc[src + j];
@@ -218,12 +233,12 @@ __global__ void __launch_bounds__(256) gather_csr_kernel(
 
 hipError_t launch_gather_csr(const long long *d_rowptr64, const long long *d_row_off,
                              const int *d_row_cnt, const double *pool_vals,
-                             const int *pool_cols, int M, double *d_vals, int *d_cols,
-                             int *d_rowptr32, hipStream_t st) {
+                             const int *pool_cols, int M, long long cap, double *d_vals,
+                             int *d_cols, int *d_rowptr32, hipStream_t st) {
     const long long threads = (static_cast<long long>(M) + 1) * kGatherLanes;
     hipLaunchKernelGGL(gather_csr_kernel, dim3(static_cast<unsigned>((threads + 255) / 256)),
                        dim3(256), 0, st, d_rowptr64, d_row_off, d_row_cnt, pool_vals, pool_cols,
-                       M, d_vals, d_cols, d_rowptr32);
+                       M, cap, d_vals, d_cols, d_rowptr32);
     return hipGetLastError();
 }
 

@@ -514,7 +514,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         unsigned *ctr = m->counters.as<unsigned>();
         unsigned long long *pool_used = reinterpret_cast<unsigned long long *>(ctr + 16);
         unsigned long long *rec_used = reinterpret_cast<unsigned long long *>(ctr + 18);
-        HIP_OR_FAIL(hipMemsetAsync(m->counters.p, 0, 256, st));
+        HIP_OR_FAIL(nat::launch_zero_u32(m->counters.p, 64, st));
         nat::RowOut out;
         out.pool_vals = m->pool_vals.as<double>();
         out.pool_cols = m->pool_cols.as<int>();
@@ -581,7 +581,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
             const int nblocks = (nthr + 255) / 256;
             mark(m, 2, st);
             HIP_OR_FAIL(nat::launch_rows(m->dev, m->caps[level], sc, nblocks, d_q, items,
-                                         n_items, ctr + level, out, st));
+                                         n_items, nullptr, ctr + level, out, st));
             mark(m, 3, st);
             unsigned nnext = 0;
             HIP_OR_FAIL(hipMemcpyAsync(&nnext, ctr + 8 + level, sizeof(unsigned),
@@ -715,9 +715,106 @@ int emit_rows(nat_mesh *m, double *d_vals, int *d_cols, int *d_rowptr, hipStream
                                         st));
     HIP_OR_FAIL(nat::launch_gather_csr(rp64, m->row_off.as<long long>(), m->row_cnt.as<int>(),
                                        m->pool_vals.as<double>(), m->pool_cols.as<int>(), M,
-                                       d_vals, d_cols, d_rowptr, st));
+                                       m->last_nnz, d_vals, d_cols, d_rowptr, st));
     mark(m, 1, st);
     m->times.emit_ms = elapsed(m, 0, 1);
+    return 0;
+}
+
+// ---------------------------------------------------------------- graphs
+//
+// The launch sequence of one weights computation without host round trips:
+// capacities fixed by a warm-up call, level kernels reading their item counts
+// on the device, counters cleared by kernels.  Overflows land in counters
+// that nat_graph_launch checks after the replay.
+
+// Device buffers of the mesh workspace a graph's nodes point to.
+std::vector<std::pair<const void *, size_t>> workspace_fingerprint(const nat_mesh *m) {
+    std::vector<std::pair<const void *, size_t>> f;
+    const DevBuf *bufs[] = {&m->order, &m->geom_order, &m->keys, &m->hist, &m->scan,
+                            &m->row_off, &m->row_cnt, &m->row_total, &m->rowptr64, &m->ovf[0],
+                            &m->ovf[1], &m->counters, &m->pool_vals, &m->pool_cols, &m->rec,
+                            &m->rec_off, &m->start};
+    for (const DevBuf *b : bufs) f.emplace_back(b->p, b->bytes);
+    for (int l = 0; l < kLevels; l++) f.emplace_back(m->scratch[l].p, m->scratch[l].bytes);
+    return f;
+}
+
+}  // namespace
+
+struct nat_graph {
+    nat_mesh *m = nullptr;
+    int M = 0;
+    hipStream_t st = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    DevBuf prob, qoff;                  // problem of each query (batch)
+    DevBuf vals, cols, rowptr;          // the CSR of the last replay
+    long long cap = 0;                  // CSR capacity (entries)
+    std::vector<std::pair<const void *, size_t>> fingerprint;
+    long long last_nnz = -1;
+};
+
+namespace {
+
+int issue_graph_rows(nat_mesh *m, nat_graph *g, const double *d_q, hipStream_t st) {
+    const int M = g->M;
+    const int P = m->problems();
+    const size_t Mz = static_cast<size_t>(M);
+    int bits = 1;
+    while (bits < 7 && (static_cast<unsigned long long>(P) << (3 * (bits + 1))) <= Mz * 2 &&
+           (static_cast<unsigned long long>(P) << (3 * (bits + 1))) <= (1ull << 24))
+        bits++;
+    unsigned *ctr = m->counters.as<unsigned>();
+    HIP_OR_FAIL(nat::launch_zero_u32(m->counters.p, 64, st));
+    const int *d_prob = P > 1 ? g->prob.as<int>() : nullptr;
+    if (d_prob)
+        HIP_OR_FAIL(nat::launch_problem_ids(g->qoff.as<long long>(), P, M, g->prob.as<int>(), st));
+    m->dev.prob = d_prob;
+    HIP_OR_FAIL(nat::launch_morton_sort(d_q, M, m->lp.as<nat::LocParams>(), d_prob, P, bits,
+                                        m->order.as<int>(), m->hist.as<int>(),
+                                        m->keys.as<int>(), m->scan.as<long long>(), st));
+    nat::RowOut out;
+    out.pool_vals = m->pool_vals.as<double>();
+    out.pool_cols = m->pool_cols.as<int>();
+    out.pool_cap = m->pool_cap;
+    out.pool_used = reinterpret_cast<unsigned long long *>(ctr + 16);
+    out.row_off = m->row_off.as<long long>();
+    out.row_cnt = m->row_cnt.as<int>();
+    out.row_total = m->row_total.as<double>();
+    nat::Scratch sc0;
+    nat::scratch_carve(m->scratch[0].p, m->caps[0], m->scap[0], &sc0);
+    nat::Topo topo;
+    topo.rec = m->rec.as<int4>();
+    topo.rec_cap = m->rec_cap;
+    topo.rec_used = reinterpret_cast<unsigned long long *>(ctr + 18);
+    topo.rec_off = m->rec_off.as<long long>();
+    HIP_OR_FAIL(nat::launch_locate(m->dev, m->caps[0], sc0, m->scap[0] / 256, d_q,
+                                   m->order.as<int>(), M, m->start.as<int>(), st));
+    HIP_OR_FAIL(nat::launch_topo(m->dev, m->topo_blocks, d_q, m->order.as<int>(), M,
+                                 m->start.as<int>(), ctr + 4, topo, ctr + 8,
+                                 m->ovf[0].as<int>(), st));
+    HIP_OR_FAIL(nat::launch_work_sort(m->order.as<int>(), M, topo, m->geom_order.as<int>(), st));
+    HIP_OR_FAIL(nat::launch_geom(
+        m->dev, m->nthr_geom / nat::kGeomThreads, d_q, m->geom_order.as<int>(), M, ctr + 5, topo,
+        out, m->mixed_tau, m->mixed_kappa,
+        m->precision == 1 ? reinterpret_cast<unsigned long long *>(ctr + 20) : nullptr, st));
+    for (int level = 1; level < kLevels; level++) {
+        nat::Scratch sc;
+        nat::scratch_carve(m->scratch[level].p, m->caps[level], m->scap[level], &sc);
+        out.ovf_list = m->ovf[level & 1].as<int>();
+        out.ovf_count = ctr + 8 + level;
+        HIP_OR_FAIL(nat::launch_rows(m->dev, m->caps[level], sc, (m->scap[level] + 255) / 256,
+                                     d_q, m->ovf[(level - 1) & 1].as<int>(), 0,
+                                     ctr + 8 + level - 1, ctr + level, out, st));
+    }
+    long long *rp64 = m->rowptr64.as<long long>();
+    HIP_OR_FAIL(nat::launch_scan_counts(m->row_cnt.as<int>(), rp64, M, m->scan.as<long long>(),
+                                        st));
+    HIP_OR_FAIL(nat::launch_gather_csr(rp64, m->row_off.as<long long>(), m->row_cnt.as<int>(),
+                                       m->pool_vals.as<double>(), m->pool_cols.as<int>(), M,
+                                       g->cap, g->vals.as<double>(), g->cols.as<int>(),
+                                       g->rowptr.as<int>(), st));
     return 0;
 }
 
@@ -1050,6 +1147,166 @@ int nat_grad_queries(nat_mesh *m, const void *d_topo, const double *d_queries,
     DeviceGuard g(m->device);
     return grad_rows(m, static_cast<const char *>(d_topo), d_queries, num_query_points, d_rowptr,
                      d_vals, d_gy, d_values, dim, d_dq, static_cast<hipStream_t>(stream));
+}
+
+int nat_graph_create(nat_mesh *m, const double *d_queries, int num_query_points,
+                     const int64_t *query_offsets, void *stream, nat_graph **out) {
+    g_err.clear();
+    if (!m || !out || !d_queries) return fail(-1, "NULL argument");
+    *out = nullptr;
+    if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    const int P = m->problems();
+    if (num_query_points <= 0) return fail(-1, "a graph needs queries (got %d)", num_query_points);
+    if (P > 1) {
+        if (!query_offsets) return fail(-1, "a batch mesh needs query offsets");
+        if (query_offsets[0] != 0 || query_offsets[P] != num_query_points)
+            return fail(-1, "query_offsets must run from 0 to num_query_points");
+        for (int p = 0; p < P; p++)
+            if (query_offsets[p + 1] < query_offsets[p])
+                return fail(-1, "query_offsets must not decrease (problem %d)", p);
+    }
+    std::lock_guard<std::mutex> lk(m->mu);
+    DeviceGuard dg(m->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    nat_graph *g = new nat_graph();
+    g->m = m;
+    g->M = num_query_points;
+    const size_t Mz = static_cast<size_t>(g->M);
+    auto bail = [&](int rc) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        if (g->st) (void)hipStreamDestroy(g->st);
+        g->prob.release();
+        g->qoff.release();
+        g->vals.release();
+        g->cols.release();
+        g->rowptr.release();
+        delete g;
+        return rc;
+    };
+    int rc = 0;
+    if (P > 1) {
+        if (g->qoff.ensure(sizeof(long long) * (P + 1)) != hipSuccess ||
+            g->prob.ensure(sizeof(int) * Mz) != hipSuccess)
+            return bail(fail(-10, "graph buffer allocation failed"));
+        if (hipMemcpy(g->qoff.p, query_offsets, sizeof(long long) * (P + 1),
+                      hipMemcpyHostToDevice) != hipSuccess ||
+            nat::launch_problem_ids(g->qoff.as<long long>(), P, g->M, g->prob.as<int>(), st) !=
+                hipSuccess)
+            return bail(fail(-10, "graph problem ids failed"));
+    }
+    // warm-up: sizes every capacity, then 25% headroom for the replays
+    long long nnz = 0;
+    const bool timing = m->timing;
+    m->timing = false;
+    rc = compute_rows(m, d_queries, g->M, P > 1 ? g->prob.as<int>() : nullptr, st, &nnz);
+    m->timing = timing;
+    if (rc) return bail(rc);
+    int items[kLevels];
+    for (int l = 0; l < kLevels; l++) items[l] = l < m->times.levels ? m->times.rows_items[l] : 0;
+    const long long pool = nnz + nnz / 4 + 4096, rec = m->last_units + m->last_units / 4 + 4096;
+    if (pool > 2147483647LL) return bail(fail(-1, "graph output above INT32_MAX entries"));
+    if (m->pool_cap < pool) {
+        if (m->pool_vals.ensure(sizeof(double) * pool) != hipSuccess ||
+            m->pool_cols.ensure(sizeof(int) * pool) != hipSuccess)
+            return bail(fail(-10, "graph pool allocation failed"));
+        m->pool_cap = pool;
+    }
+    if (m->rec_cap < rec) {
+        if (m->rec.ensure(sizeof(int4) * rec) != hipSuccess)
+            return bail(fail(-10, "graph record allocation failed"));
+        m->rec_cap = rec;
+    }
+    for (int l = 1; l < kLevels; l++) {
+        int n = 0;
+        if ((rc = ensure_level_scratch(m, l, std::max(256, 2 * items[l]), &n))) return bail(rc);
+    }
+    g->cap = m->pool_cap;
+    if (g->vals.ensure(sizeof(double) * g->cap) != hipSuccess ||
+        g->cols.ensure(sizeof(int) * g->cap) != hipSuccess ||
+        g->rowptr.ensure(sizeof(int) * (Mz + 1)) != hipSuccess)
+        return bail(fail(-10, "graph output allocation failed"));
+    if (hipStreamSynchronize(st) != hipSuccess) return bail(fail(-10, "warm-up failed"));
+    g->fingerprint = workspace_fingerprint(m);
+    if (hipStreamCreate(&g->st) != hipSuccess) return bail(fail(-10, "stream creation failed"));
+    if (hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return bail(fail(-10, "hipStreamBeginCapture failed"));
+    const nat::DevMesh saved = m->dev;
+    rc = issue_graph_rows(m, g, d_queries, g->st);
+    m->dev = saved;
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(g->st, &graph);
+    g->graph = graph;
+    if (rc) return bail(rc);
+    if (ec != hipSuccess) return bail(fail(-10, "hipStreamEndCapture: %s", hipGetErrorString(ec)));
+    if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+        return bail(fail(-10, "hipGraphInstantiate failed"));
+    m->last_M = -1;  // the workspace now serves the graph
+    *out = g;
+    return 0;
+}
+
+int nat_graph_launch(nat_graph *g, void *stream, int64_t *nnz_out) {
+    g_err.clear();
+    if (!g || !nnz_out) return fail(-1, "NULL argument");
+    nat_mesh *m = g->m;
+    std::lock_guard<std::mutex> lk(m->mu);
+    DeviceGuard dg(m->device);
+    if (workspace_fingerprint(m) != g->fingerprint)
+        return fail(-9, "the mesh workspace was reallocated since the graph was made");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HIP_OR_FAIL(hipGraphLaunch(g->exec, st));
+    unsigned h[20];
+    HIP_OR_FAIL(hipMemcpyAsync(h, m->counters.p, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    unsigned long long pool_used, rec_used;
+    std::memcpy(&pool_used, h + 16, sizeof(pool_used));
+    std::memcpy(&rec_used, h + 18, sizeof(rec_used));
+    m->last_M = -1;
+    g->last_nnz = -1;
+    if (static_cast<long long>(pool_used) > g->cap || static_cast<long long>(rec_used) > m->rec_cap)
+        return fail(-8, "graph capacity exceeded (entries %llu of %lld, records %llu of %lld)",
+                    pool_used, g->cap, rec_used, m->rec_cap);
+    if (h[11] > 0)
+        return fail(-4, "%u queries need more than %d cavity tets", h[11],
+                    m->caps[kLevels - 1].cav);
+    g->last_nnz = static_cast<long long>(pool_used);
+    *nnz_out = g->last_nnz;
+    return 0;
+}
+
+int nat_graph_copy_out(nat_graph *g, double *d_vals, int32_t *d_cols, int32_t *d_rowptr,
+                       void *stream) {
+    g_err.clear();
+    if (!g || !d_rowptr) return fail(-1, "NULL argument");
+    if (g->last_nnz < 0) return fail(-6, "no successful replay to copy");
+    if (g->last_nnz > 0 && (!d_vals || !d_cols)) return fail(-1, "NULL argument");
+    DeviceGuard dg(g->m->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t n = static_cast<size_t>(g->last_nnz);
+    if (n) {
+        HIP_OR_FAIL(hipMemcpyAsync(d_vals, g->vals.p, sizeof(double) * n,
+                                   hipMemcpyDeviceToDevice, st));
+        HIP_OR_FAIL(hipMemcpyAsync(d_cols, g->cols.p, sizeof(int) * n, hipMemcpyDeviceToDevice,
+                                   st));
+    }
+    HIP_OR_FAIL(hipMemcpyAsync(d_rowptr, g->rowptr.p, sizeof(int) * (static_cast<size_t>(g->M) + 1),
+                               hipMemcpyDeviceToDevice, st));
+    return 0;
+}
+
+void nat_graph_free(nat_graph *g) {
+    if (!g) return;
+    DeviceGuard dg(g->m->device);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->st) (void)hipStreamDestroy(g->st);
+    g->prob.release();
+    g->qoff.release();
+    g->vals.release();
+    g->cols.release();
+    g->rowptr.release();
+    delete g;
 }
 
 int nat_insphere_classify(const double *points, int n, double *det, int32_t *uncertain) {

@@ -134,9 +134,13 @@ int rows_blocks_per_cu();  // resident 256-thread blocks of the fused row kernel
 int topo_blocks_per_cu();   // 128-thread blocks of the topology kernel
 int geom_blocks_per_cu();  // kGeomThreads-thread blocks
 // fused row kernel: used for the larger capacity levels
+// d_n_items != NULL: the item count is read on the device (graph form)
 hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
                        const double *d_q, const int *d_items, int n_items,
-                       unsigned int *d_work, const RowOut &out, hipStream_t st);
+                       const unsigned *d_n_items, unsigned int *d_work, const RowOut &out,
+                       hipStream_t st);
+// zero-fill of n 32-bit words (a kernel, so that captured graphs hold no memset)
+hipError_t launch_zero_u32(void *d_p, size_t n, hipStream_t st);
 // level 0: start tets, topology records, then rows from them
 int locate_blocks_per_cu();  // 256-thread blocks
 hipError_t launch_locate(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
@@ -176,10 +180,11 @@ hipError_t launch_spmm_t(int M, int N, int D, const int *rowptr, const int *cols
 hipError_t launch_scan_counts(const int *d_in, long long *d_out, int n, long long *d_tmp,
                               hipStream_t st);
 size_t scan_tmp_elems(int n);
+// rows reaching past cap entries (pool or output) are skipped
 hipError_t launch_gather_csr(const long long *d_rowptr64, const long long *d_row_off,
                              const int *d_row_cnt, const double *pool_vals,
-                             const int *pool_cols, int M, double *d_vals, int *d_cols,
-                             int *d_rowptr32, hipStream_t st);
+                             const int *pool_cols, int M, long long cap, double *d_vals,
+                             int *d_cols, int *d_rowptr32, hipStream_t st);
 hipError_t launch_sort_rows(int M, const int *rowptr, int *cols, void *vals, int vdt,
                             hipStream_t st);
 hipError_t launch_spmm(int M, int D, const int *rowptr, const int *cols, const void *vals,

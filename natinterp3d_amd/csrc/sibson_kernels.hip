@@ -802,13 +802,17 @@ __device__ int fused_row(const DevMesh &m, const Caps &caps, const Scratch &sc, 
     return nnb;
 }
 
+// n_items_dev, when given, holds the item count (written by an earlier kernel
+// of the same stream: the graph form of the launch sequence).
 __global__ void __launch_bounds__(256) sibson_rows_kernel(DevMesh m, Caps caps, Scratch sc,
                                                           const double *__restrict__ q,
                                                           const int *__restrict__ items,
-                                                          int n_items, unsigned int *work,
-                                                          RowOut out) {
+                                                          int n_items,
+                                                          const unsigned *n_items_dev,
+                                                          unsigned int *work, RowOut out) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= sc.nthr) return;  // nthr is a multiple of 64: whole waves leave
+    if (n_items_dev) n_items = static_cast<int>(*n_items_dev);
     const size_t S = static_cast<size_t>(sc.nthr);
     int epoch = sc.epoch[tid];
     int item;
@@ -1670,9 +1674,10 @@ int geom_blocks_per_cu() {
 
 hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
                        const double *d_q, const int *d_items, int n_items,
-                       unsigned int *d_work, const RowOut &out, hipStream_t st) {
+                       const unsigned *d_n_items, unsigned int *d_work, const RowOut &out,
+                       hipStream_t st) {
     hipLaunchKernelGGL(sibson_rows_kernel, dim3(nblocks), dim3(256), 0, st, m, caps, sc, d_q,
-                       d_items, n_items, d_work, out);
+                       d_items, n_items, d_n_items, d_work, out);
     return hipGetLastError();
 }
 

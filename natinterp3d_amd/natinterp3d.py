@@ -365,3 +365,60 @@ class Interpolator:
         y = SibsonInterpolation.apply(self.mesh_and_vertices, q, v2.to(dev))
         y = y.to(torch.promote_types(query_points.dtype, values.dtype))
         return y.squeeze(1) if one_d else y
+
+
+class WeightsGraph:
+    """A weights computation captured once in a hipGraph and replayed.
+
+    ``WeightsGraph(mesh, queries[, query_offsets])`` runs one computation of
+    the centred float64 CUDA ``queries`` [M,3] (a batch mesh also takes its
+    query offsets) to size the workspace, then captures the launch sequence.
+    ``run()`` replays it for whatever ``queries`` holds at that moment (update
+    it in place) and returns the CSR (crow, col, val) as new tensors, or
+    raises ``GraphCapacityError`` when the new queries need more workspace
+    than the graph was made with (then compute them with ``weights_device``
+    or make a new graph)."""
+
+    def __init__(self, mesh, queries, query_offsets=None):
+        import torch
+        if queries.dtype != torch.float64 or not queries.is_cuda or not queries.is_contiguous():
+            raise ValueError("queries must be a contiguous float64 CUDA tensor")
+        if queries.dim() != 2 or queries.shape[1] != 3 or queries.shape[0] == 0:
+            raise ValueError("queries must have shape (num_query_points, 3), M > 0")
+        self._mesh = mesh
+        self.queries = queries
+        self.M = queries.shape[0]
+        L = _lib.lib()
+        off = None
+        if query_offsets is not None:
+            self._off = np.ascontiguousarray(query_offsets, np.int64)
+            off = self._off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        stream = ctypes.c_void_p(torch.cuda.current_stream(queries.device).cuda_stream)
+        self._g = ctypes.c_void_p()
+        _lib.check(L.nat_graph_create(mesh.handle, ctypes.c_void_p(queries.data_ptr()), self.M,
+                                      off, stream, ctypes.byref(self._g)), "nat_graph_create")
+        self._finalizer = weakref.finalize(self, L.nat_graph_free, self._g)
+
+    def run(self):
+        import torch
+        dev = self.queries.device
+        L = _lib.lib()
+        stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        nnz = ctypes.c_int64()
+        rc = L.nat_graph_launch(self._g, stream, ctypes.byref(nnz))
+        if rc == -8:
+            raise GraphCapacityError(_lib.last_error())
+        _lib.check(rc, "nat_graph_launch")
+        n = int(nnz.value)
+        val = torch.empty(n, dtype=torch.float64, device=dev)
+        col = torch.empty(n, dtype=torch.int32, device=dev)
+        crow = torch.empty(self.M + 1, dtype=torch.int32, device=dev)
+        _lib.check(L.nat_graph_copy_out(self._g, ctypes.c_void_p(val.data_ptr() if n else 0),
+                                        ctypes.c_void_p(col.data_ptr() if n else 0),
+                                        ctypes.c_void_p(crow.data_ptr()), stream),
+                   "nat_graph_copy_out")
+        return crow, col, val
+
+
+class GraphCapacityError(RuntimeError):
+    """A graph replay needed more workspace than the graph was made with."""
