@@ -241,10 +241,13 @@ def run_precision(args):
 def run_batch(args):
     """Batching workload: independent problems with log-uniform key counts in
     [500, 50,000] and query counts in [1,000, 200,000], points uniform in
-    per-problem boxes of sides U[0.5, 2], values float32 [N, 8].  One step =
-    the forward pass (all weight rows + W @ values) of every problem; batched:
-    one BatchInterpolator call (one launch per kernel for all problems);
-    sequential: one single-problem call per problem."""
+    per-problem boxes of sides U[0.5, 2], values float32 [N, 8]; every
+    --shell-every-th problem is a hollow shell instead (keys [300, 3,000],
+    queries [1,000, 10,000]).  One step = the forward pass (all weight rows +
+    W @ values) of every problem.  Timed: one BatchInterpolator call per step
+    (one launch per kernel for all problems) under the default row policy,
+    then under the autotuner's policy, then the tuned launch sequence replayed
+    as a hipGraph; and one single-problem call per problem (sequential)."""
     from concurrent.futures import ThreadPoolExecutor
 
     import torch
@@ -253,7 +256,7 @@ def run_batch(args):
     from natinterp3d_amd import workloads
     from natinterp3d_amd.natinterp3d import spmm_device
 
-    probs = workloads.batch_problems(n_problems=args.problems)
+    probs = workloads.batch_problems(n_problems=args.problems, shell_every=args.shell_every)
     nk = sum(len(k) for k, _, _ in probs)
     nq = sum(len(q) for _, q, _ in probs)
     log(f"batch workload: {len(probs)} problems, {nk:,} keys, {nq:,} queries")
@@ -305,14 +308,34 @@ def run_batch(args):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) * 1e3 / args.steps, res
 
-    b_ms, (yb, nnz_b) = timed(batched)
+    d_ms, (yd, _) = timed(batched)
     s_ms, (ys, nnz_s) = timed(sequential)
+    t = time.time()
+    report = batch.autotune_device(qcat, off, repeats=2)
+    tune_s = time.time() - t
+    policy = batch.policy()
+    routed = [p for p, b in enumerate(policy["budgets"]) if b == 0]
+    log(f"autotuner: {tune_s:.1f} s, {report['candidates']} candidates, compute "
+        f"{report['default_ms']:.2f} -> {report['tuned_ms']:.2f} ms; {len(routed)} problems "
+        f"routed to thread-per-query rows, first level {policy['first_level']}, work sort "
+        f"{policy['work_sort']}, grids x{policy['topo_grid']} / x{policy['geom_grid']}")
+    b_ms, (yb, nnz_b) = timed(batched)
+    graph = natinterp3d_amd.WeightsGraph(batch, qcat, off)
+
+    def replayed():
+        crow, col, val = graph.run()
+        return spmm_device(crow, col, val, vcat, int(off[-1])), col.numel()
+
+    g_ms, (yg, _) = timed(replayed)
     same = all(torch.equal(yb[off[i]:off[i + 1]], ys[i]) for i in range(len(probs)))
-    log(f"batched {b_ms:.2f} ms/step, sequential {s_ms:.2f} ms/step, "
-        f"speed-up {s_ms / b_ms:.2f}x, nnz {nnz_b:,} / {nnz_s:,}, products equal: {same}")
+    same = same and torch.equal(yb, yd) and torch.equal(yb, yg)
+    log(f"batched: default policy {d_ms:.2f} ms/step, tuned {b_ms:.2f}, tuned graph "
+        f"{g_ms:.2f}; sequential {s_ms:.2f} ms/step, speed-up {s_ms / b_ms:.2f}x, "
+        f"nnz {nnz_b:,} / {nnz_s:,}, products equal: {same}")
     out = {
-        "metric": "batched Sibson queries/s, forward, 256 independent problems in one set of "
-                  "launches vs 256 sequential single-problem calls",
+        "metric": f"batched Sibson queries/s, forward, {len(probs)} independent problems in one "
+                  f"set of launches (autotuned row policy) vs {len(probs)} sequential "
+                  "single-problem calls",
         "value": nq / (b_ms / 1e3),
         "unit": "queries/s",
         "n_gpus": 1,
@@ -327,8 +350,21 @@ def run_batch(args):
         "config": {
             "workload": f"batch: {len(probs)} problems, keys log-uniform [500, 50000], "
                         "queries log-uniform [1000, 200000], uniform in boxes of sides "
-                        "U[0.5, 2], values float32 [N, 8]",
+                        "U[0.5, 2], values float32 [N, 8]"
+                        + (f"; every {args.shell_every}th problem a hollow shell (keys "
+                           "[300, 3000], queries [1000, 10000])" if args.shell_every else ""),
             "keys": nk, "queries": nq, "nnz": int(nnz_b),
+            "default_policy_ms_per_step": d_ms,
+            "graph_ms_per_step": g_ms,
+            "graph_queries_per_s": nq / (g_ms / 1e3),
+            "tuner": {"seconds": round(tune_s, 1), "candidates": report["candidates"],
+                      "compute_default_ms": report["default_ms"],
+                      "compute_tuned_ms": report["tuned_ms"],
+                      "left_group_kernel": report["left_group"],
+                      "routed_problems": len(routed),
+                      "routed_queries": report["routed_queries"],
+                      "first_level": policy["first_level"], "work_sort": policy["work_sort"],
+                      "topo_grid": policy["topo_grid"], "geom_grid": policy["geom_grid"]},
             "sequential_ms_per_step": s_ms,
             "sequential_queries_per_s": nq / (s_ms / 1e3),
             "speedup_vs_sequential": s_ms / b_ms,
@@ -352,7 +388,11 @@ def main():
                     help="plain: the fwd+bwd headline; batch: 256 independent problems in "
                          "one set of launches against 256 sequential single-problem calls; "
                          "precision: jittered-lattice stress, FP64 against mixed precision")
+    ap.add_argument("--autotune", choices=["on", "off"], default="on",
+                    help="plain workload: tune the row policy on the queries before timing")
     ap.add_argument("--problems", type=int, default=256)
+    ap.add_argument("--shell-every", type=int, default=0,
+                    help="batch workload: every k-th problem a hollow shell (0: none)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the steps, copy this many MiB device to device once: a "
                          "kernel of known traffic that scales hardware byte counters")
@@ -401,6 +441,12 @@ def main():
     xd = torch.from_numpy(values).to(dev).requires_grad_(True)
     gy = torch.randn((M, D), dtype=torch.float64, device=dev,
                      generator=torch.Generator(device=dev).manual_seed(5))
+    tuned = None
+    if args.autotune == "on":
+        t = time.time()
+        rep_t = mesh.autotune(qd.detach(), repeats=2)
+        tuned = {"seconds": round(time.time() - t, 1), **rep_t, **mesh.policy()}
+        log(f"autotuner: {tuned}")
     mesh.set_timing(True)
 
     def step():
@@ -507,6 +553,7 @@ def main():
                           "larger_levels_bwd": float(sum(per_step[-1]["grad_ms"][1:])),
                           "emit": per_step[-1]["emit_ms"]},
             "mesh_build_s": round(build_s, 1),
+            "row_policy": tuned if tuned is not None else "default (autotuner off)",
         },
         "roofline": {
             "kernel": dom,

@@ -242,6 +242,57 @@ int nat_mesh_set_precision(nat_mesh *m, int mode, double cond_threshold,
                            double cancel_threshold);
 int nat_mesh_get_timings(const nat_mesh *m, nat_timings *out);
 
+/* ---- row-path policy and its autotuner (no reference counterpart).
+ * Which kernel makes a row changes speed, never the row: every policy gives
+ * the same CSR, bit for bit.
+ *
+ * A query first goes to the group kernel (16 lanes of a wave per query,
+ * cavity tables in LDS) unless its problem's budget is 0; the group kernel
+ * gives up on a query once it has visited more tets than the budget (at most
+ * 96, the LDS tables' size), or when the row has more than 32 neighbours.
+ * Those queries go to the thread-per-query levels (one thread per query,
+ * tables in global memory: level 1 up to 4096 cavity tets on up to 32768
+ * threads, level 2 up to 32768 on 2048, level 3 up to 262144 on 64), starting
+ * at first_level. */
+typedef struct {
+    int32_t first_level;  /* 1 or 2 */
+    int32_t work_sort;    /* 1: the geometry kernel takes rows regrouped by
+                             cavity size within windows of 4096 queries */
+    int32_t topo_grid;    /* group-kernel grid = resident blocks x this (1..4) */
+    int32_t geom_grid;    /* geometry-kernel grid = resident blocks x this (1..4) */
+    int32_t reserved[4];
+} nat_policy;
+
+/* Sets the policy and, when budgets is given, the group kernel's visited-tet
+ * budget per problem (budgets[P], 0..96; 0 = thread-per-query rows only).
+ * A graph keeps the grid sizes and work sort of its capture and reads the
+ * budgets and first level at every replay. */
+int nat_mesh_set_policy(nat_mesh *m, const nat_policy *policy, const uint8_t *budgets);
+int nat_mesh_get_policy(const nat_mesh *m, nat_policy *policy, uint8_t *budgets);
+
+typedef struct {
+    double default_ms;       /* compute with the default policy (best of repeats) */
+    double tuned_ms;         /* compute with the chosen policy */
+    int32_t candidates;      /* policies timed */
+    int32_t routed_problems; /* problems given budget 0 */
+    int64_t routed_queries;  /* their queries */
+    double left_group;       /* fraction of queries that left the group kernel
+                                under the default policy */
+} nat_tune_report;
+
+/* Times row-path policies on the caller's queries (centred float64 [M,3] in
+ * device memory; query_offsets[P+1] on the host for a batch mesh, NULL for a
+ * single problem) and keeps the fastest on the mesh.  Candidates, tried one
+ * dimension at a time from the default: problems whose queries mostly leave
+ * the group kernel routed to thread-per-query rows (several thresholds),
+ * first_level 2, smaller budgets for the other problems, the work sort off,
+ * larger grids.  Each candidate is timed as the best of `repeats` computes
+ * (host clock, synchronised).  The last compute uses the chosen policy, so
+ * nat_weights_emit may follow.  Earlier topologies and graphs stay valid. */
+int nat_mesh_autotune(nat_mesh *m, const double *d_queries, int64_t num_query_points,
+                      const int64_t *query_offsets, int repeats, void *stream,
+                      nat_tune_report *report);
+
 /* Release memory returned by nat_get_weights. */
 void nat_free_host(void *p);
 

@@ -20,6 +20,7 @@ EXPORTS = (
     "nat_batch_build", "nat_batch_offsets", "nat_batch_weights_compute",
     "nat_mesh_set_precision", "nat_predicate_audit", "nat_insphere_classify",
     "nat_graph_create", "nat_graph_launch", "nat_graph_copy_out", "nat_graph_free",
+    "nat_mesh_set_policy", "nat_mesh_get_policy", "nat_mesh_autotune",
 )
 
 
@@ -32,6 +33,22 @@ class Timings(ctypes.Structure):
         ("emit_ms", ctypes.c_double), ("grad_ms", ctypes.c_double * 4),
         ("grad_items", ctypes.c_int32 * 4), ("mixed_fp32", ctypes.c_int64),
         ("mixed_fp64", ctypes.c_int64), ("mixed_rows_fp64", ctypes.c_int64),
+    ]
+
+
+class Policy(ctypes.Structure):
+    _fields_ = [
+        ("first_level", ctypes.c_int32), ("work_sort", ctypes.c_int32),
+        ("topo_grid", ctypes.c_int32), ("geom_grid", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
+    ]
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [
+        ("default_ms", ctypes.c_double), ("tuned_ms", ctypes.c_double),
+        ("candidates", ctypes.c_int32), ("routed_problems", ctypes.c_int32),
+        ("routed_queries", ctypes.c_int64), ("left_group", ctypes.c_double),
     ]
 
 
@@ -113,6 +130,13 @@ def lib():
     L.nat_insphere_classify.argtypes = [dp, ctypes.c_int, dp, ip]
     L.nat_mesh_set_precision.restype = ctypes.c_int
     L.nat_mesh_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    L.nat_mesh_set_policy.restype = ctypes.c_int
+    L.nat_mesh_set_policy.argtypes = [vp, ctypes.POINTER(Policy), vp]
+    L.nat_mesh_get_policy.restype = ctypes.c_int
+    L.nat_mesh_get_policy.argtypes = [vp, ctypes.POINTER(Policy), vp]
+    L.nat_mesh_autotune.restype = ctypes.c_int
+    L.nat_mesh_autotune.argtypes = [vp, vp, ctypes.c_int64, vp, ctypes.c_int, vp,
+                                    ctypes.POINTER(TuneReport)]
     L.nat_mesh_set_timing.restype = ctypes.c_int
     L.nat_mesh_set_timing.argtypes = [vp, ctypes.c_int]
     L.nat_mesh_get_timings.restype = ctypes.c_int

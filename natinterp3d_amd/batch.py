@@ -16,7 +16,8 @@ import numpy as np
 import scipy.sparse
 
 from . import _lib
-from .natinterp3d import _DT_CODE, _default_device, spmm_device
+from .natinterp3d import (_DT_CODE, _default_device, autotune_rows, get_row_policy,
+                          set_row_policy, spmm_device)
 
 
 class BatchInterpolator:
@@ -60,6 +61,31 @@ class BatchInterpolator:
     def set_timing(self, enable=True):
         _lib.check(_lib.lib().nat_mesh_set_timing(self._m, 1 if enable else 0),
                    "nat_mesh_set_timing")
+
+    def policy(self):
+        """The row-path policy, budgets per problem (see set_policy)."""
+        return get_row_policy(self._m, self.num_problems)
+
+    def set_policy(self, **kw):
+        """Change the row-path policy: budgets (per problem, 0..96; 0 =
+        thread-per-query rows), first_level (1 or 2), work_sort (bool),
+        topo_grid, geom_grid (1..4).  Results do not depend on it."""
+        set_row_policy(self._m, self.num_problems, **kw)
+
+    def autotune(self, query_sets, repeats=3):
+        """Time row-path policies on these per-problem query sets and keep the
+        fastest (problems whose queries mostly overflow the group kernel's
+        LDS tables go straight to thread-per-query rows, and so on); returns
+        the tuner's report (dict)."""
+        import torch
+        q, off = self._centre(query_sets)
+        dev = torch.device("cuda", self.device)
+        with torch.cuda.device(dev):
+            return autotune_rows(self._m, torch.from_numpy(q).to(dev), off, repeats)
+
+    def autotune_device(self, queries, query_offsets, repeats=3):
+        """autotune on centred float64 CUDA queries with their offsets."""
+        return autotune_rows(self._m, queries, query_offsets, repeats)
 
     def _centre(self, query_sets):
         if len(query_sets) != self.num_problems:

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -103,6 +104,15 @@ struct nat_mesh {
     DevBuf prob, qoff;                    // batch calls: problem of each query
     std::mutex mu;
 
+    // row-path policy (nat_policy).  route[p] = visited-tet budget of the
+    // group kernel | first thread-per-query level << 8, read by the kernels at
+    // launch time; grids are resident blocks times the multipliers.
+    std::vector<unsigned char> budget;  // [P]
+    DevBuf route;
+    int first_level = 1;
+    int work_sort = 1;
+    int topo_grid = 1, geom_grid = 1;
+
     // workspace
     nat::Caps caps[kLevels];
     int nthr[kLevels] = {0, 0, 0, 0};  // [0]: locate kernel threads; the most per level
@@ -161,7 +171,8 @@ struct nat_mesh {
                          &q,        &order,     &geom_order, &keys,    &hist,       &scan,
                          &row_off,  &row_cnt,   &rowptr64,  &row_total, &ovf[0],    &ovf[1],
                          &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,   &csr_rowptr,
-                         &rec,      &rec_off,   &start,     &lp,       &prob,       &qoff};
+                         &rec,      &rec_off,   &start,     &lp,       &prob,       &qoff,
+                         &route};
         for (DevBuf *b : all) b->release();
         for (int l = 0; l < kLevels; l++) scratch[l].release();
         for (hipEvent_t &e : ev) {
@@ -283,6 +294,18 @@ void index_parts(nat_mesh *m) {
 }
 
 int upload_mesh(nat_mesh *m);
+
+// Most tets the group kernel's LDS tables hold (its default budget).
+constexpr int kGroupBudget = 96;
+
+int upload_route(nat_mesh *m) {
+    std::vector<unsigned short> r(m->budget.size());
+    for (size_t p = 0; p < r.size(); p++)
+        r[p] = static_cast<unsigned short>(m->budget[p] | (m->first_level << 8));
+    HIP_OR_FAIL(hipMemcpy(m->route.p, r.data(), sizeof(unsigned short) * r.size(),
+                          hipMemcpyHostToDevice));
+    return 0;
+}
 
 // Device set-up of a built mesh (any number of parts).
 int finish_build(nat_mesh *m) {
@@ -420,7 +443,10 @@ int upload_mesh(nat_mesh *m) {
     d.cell_pts = m->cell_pts.as<double4>();
     d.lp = m->lp.as<nat::LocParams>();
     d.prob = nullptr;
-    return 0;
+    HIP_OR_FAIL(m->route.ensure(sizeof(unsigned short) * P));
+    d.route = m->route.as<unsigned short>();
+    m->budget.assign(P, static_cast<unsigned char>(kGroupBudget));
+    return upload_route(m);
 }
 
 // Per-thread scratch of a level for a launch of `want` threads: the level's
@@ -538,9 +564,9 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         mark(m, 3, st);
         m->times.locate_ms = elapsed(m, 2, 3);
         mark(m, 2, st);
-        HIP_OR_FAIL(nat::launch_topo(m->dev, m->topo_blocks, d_q, m->order.as<int>(), M,
-                                     m->start.as<int>(), ctr + 4, topo, ctr + 8,
-                                     m->ovf[0].as<int>(), st));
+        HIP_OR_FAIL(nat::launch_topo(m->dev, m->topo_blocks * m->topo_grid, d_q,
+                                     m->order.as<int>(), M, m->start.as<int>(), ctr + 4, topo,
+                                     ctr + 8, m->ovf[0].as<int>(), st));
         mark(m, 3, st);
         unsigned long long meta[2] = {0, 0};
         HIP_OR_FAIL(hipMemcpyAsync(meta, ctr + 16, sizeof(meta), hipMemcpyDeviceToHost, st));
@@ -556,10 +582,12 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         m->units_per_row = std::max(8.0, 1.125 * static_cast<double>(units) / M);
         m->last_units = units;
         mark(m, 2, st);
-        HIP_OR_FAIL(nat::launch_work_sort(m->order.as<int>(), M, topo, m->geom_order.as<int>(),
-                                          st));
+        if (m->work_sort)
+            HIP_OR_FAIL(nat::launch_work_sort(m->order.as<int>(), M, topo,
+                                              m->geom_order.as<int>(), st));
         HIP_OR_FAIL(nat::launch_geom(
-            m->dev, m->nthr_geom / nat::kGeomThreads, d_q, m->geom_order.as<int>(), M, ctr + 5,
+            m->dev, m->nthr_geom / nat::kGeomThreads * m->geom_grid, d_q,
+            (m->work_sort ? m->geom_order : m->order).as<int>(), M, ctr + 5,
             topo, out, m->mixed_tau, m->mixed_kappa,
             m->precision == 1 ? reinterpret_cast<unsigned long long *>(ctr + 20) : nullptr, st));
         mark(m, 3, st);
@@ -580,7 +608,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
             out.ovf_count = ctr + 8 + level;
             const int nblocks = (nthr + 255) / 256;
             mark(m, 2, st);
-            HIP_OR_FAIL(nat::launch_rows(m->dev, m->caps[level], sc, nblocks, d_q, items,
+            HIP_OR_FAIL(nat::launch_rows(m->dev, m->caps[level], sc, level, nblocks, d_q, items,
                                          n_items, nullptr, ctr + level, out, st));
             mark(m, 3, st);
             unsigned nnext = 0;
@@ -682,7 +710,7 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
         nat::scratch_carve(m->scratch[level].p, m->caps[level], m->scap[level], &sc);
         const int nblocks = (nthr + 255) / 256;
         mark(m, 2, st);
-        HIP_OR_FAIL(nat::launch_grad_rows(m->dev, m->caps[level], sc, nblocks, d_q, items,
+        HIP_OR_FAIL(nat::launch_grad_rows(m->dev, m->caps[level], sc, level, nblocks, d_q, items,
                                           n_items, ctr + level, g, m->ovf[level & 1].as<int>(),
                                           ctr + 8 + level, st));
         mark(m, 3, st);
@@ -791,12 +819,15 @@ int issue_graph_rows(nat_mesh *m, nat_graph *g, const double *d_q, hipStream_t s
     topo.rec_off = m->rec_off.as<long long>();
     HIP_OR_FAIL(nat::launch_locate(m->dev, m->caps[0], sc0, m->scap[0] / 256, d_q,
                                    m->order.as<int>(), M, m->start.as<int>(), st));
-    HIP_OR_FAIL(nat::launch_topo(m->dev, m->topo_blocks, d_q, m->order.as<int>(), M,
-                                 m->start.as<int>(), ctr + 4, topo, ctr + 8,
+    HIP_OR_FAIL(nat::launch_topo(m->dev, m->topo_blocks * m->topo_grid, d_q, m->order.as<int>(),
+                                 M, m->start.as<int>(), ctr + 4, topo, ctr + 8,
                                  m->ovf[0].as<int>(), st));
-    HIP_OR_FAIL(nat::launch_work_sort(m->order.as<int>(), M, topo, m->geom_order.as<int>(), st));
+    if (m->work_sort)
+        HIP_OR_FAIL(
+            nat::launch_work_sort(m->order.as<int>(), M, topo, m->geom_order.as<int>(), st));
     HIP_OR_FAIL(nat::launch_geom(
-        m->dev, m->nthr_geom / nat::kGeomThreads, d_q, m->geom_order.as<int>(), M, ctr + 5, topo,
+        m->dev, m->nthr_geom / nat::kGeomThreads * m->geom_grid, d_q,
+        (m->work_sort ? m->geom_order : m->order).as<int>(), M, ctr + 5, topo,
         out, m->mixed_tau, m->mixed_kappa,
         m->precision == 1 ? reinterpret_cast<unsigned long long *>(ctr + 20) : nullptr, st));
     for (int level = 1; level < kLevels; level++) {
@@ -804,8 +835,8 @@ int issue_graph_rows(nat_mesh *m, nat_graph *g, const double *d_q, hipStream_t s
         nat::scratch_carve(m->scratch[level].p, m->caps[level], m->scap[level], &sc);
         out.ovf_list = m->ovf[level & 1].as<int>();
         out.ovf_count = ctr + 8 + level;
-        HIP_OR_FAIL(nat::launch_rows(m->dev, m->caps[level], sc, (m->scap[level] + 255) / 256,
-                                     d_q, m->ovf[(level - 1) & 1].as<int>(), 0,
+        HIP_OR_FAIL(nat::launch_rows(m->dev, m->caps[level], sc, level,
+                                     (m->scap[level] + 255) / 256, d_q, m->ovf[(level - 1) & 1].as<int>(), 0,
                                      ctr + 8 + level - 1, ctr + level, out, st));
     }
     long long *rp64 = m->rowptr64.as<long long>();
@@ -815,6 +846,81 @@ int issue_graph_rows(nat_mesh *m, nat_graph *g, const double *d_q, hipStream_t s
                                        m->pool_vals.as<double>(), m->pool_cols.as<int>(), M,
                                        g->cap, g->vals.as<double>(), g->cols.as<int>(),
                                        g->rowptr.as<int>(), st));
+    return 0;
+}
+
+int check_offsets(const nat_mesh *m, const int64_t *query_offsets) {
+    const int P = m->problems();
+    if (query_offsets[0] != 0) return fail(-1, "query_offsets[0] must be 0");
+    for (int p = 0; p < P; p++)
+        if (query_offsets[p + 1] < query_offsets[p])
+            return fail(-1, "query_offsets must not decrease (problem %d)", p);
+    if (query_offsets[P] > 2147483647LL) return fail(-1, "too many queries (%lld)",
+                                                     static_cast<long long>(query_offsets[P]));
+    return 0;
+}
+
+// Problem of each query of a batch call, in m->prob (*d_prob stays NULL for
+// a single problem or no queries).
+int problem_ids(nat_mesh *m, const int64_t *query_offsets, long long M, hipStream_t st,
+                const int **d_prob) {
+    const int P = m->problems();
+    *d_prob = nullptr;
+    if (P > 1 && M > 0) {
+        HIP_OR_FAIL(m->qoff.ensure(sizeof(long long) * (P + 1)));
+        HIP_OR_FAIL(m->prob.ensure(sizeof(int) * static_cast<size_t>(M)));
+        HIP_OR_FAIL(hipMemcpyAsync(m->qoff.p, query_offsets, sizeof(long long) * (P + 1),
+                                   hipMemcpyHostToDevice, st));
+        HIP_OR_FAIL(nat::launch_problem_ids(m->qoff.as<long long>(), P, static_cast<int>(M),
+                                            m->prob.as<int>(), st));
+        *d_prob = m->prob.as<int>();
+    }
+    return 0;
+}
+
+// ---- row-path policy and autotuner
+
+struct Policy {
+    std::vector<unsigned char> budget;
+    int first_level = 1, work_sort = 1, topo_grid = 1, geom_grid = 1;
+};
+
+Policy policy_of(const nat_mesh *m) {
+    Policy p;
+    p.budget = m->budget;
+    p.first_level = m->first_level;
+    p.work_sort = m->work_sort;
+    p.topo_grid = m->topo_grid;
+    p.geom_grid = m->geom_grid;
+    return p;
+}
+
+int apply_policy(nat_mesh *m, const Policy &p) {
+    m->budget = p.budget;
+    m->first_level = p.first_level;
+    m->work_sort = p.work_sort;
+    m->topo_grid = p.topo_grid;
+    m->geom_grid = p.geom_grid;
+    return upload_route(m);
+}
+
+// Best wall time of `repeats` synchronised computes under policy p.
+int time_policy(nat_mesh *m, const Policy &p, const double *d_q, int M, const int *d_prob,
+                hipStream_t st, int repeats, double *best_ms) {
+    if (int rc = apply_policy(m, p)) return rc;
+    double best = 1e300;
+    for (int r = 0; r < repeats; r++) {
+        HIP_OR_FAIL(hipStreamSynchronize(st));
+        const auto t0 = std::chrono::steady_clock::now();
+        long long nnz = 0;
+        if (int rc = compute_rows(m, d_q, M, d_prob, st, &nnz)) return rc;
+        HIP_OR_FAIL(hipStreamSynchronize(st));
+        const double ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                .count();
+        best = std::min(best, ms);
+    }
+    *best_ms = best;
     return 0;
 }
 
@@ -973,27 +1079,14 @@ int nat_batch_weights_compute(nat_mesh *m, const double *d_queries,
     g_err.clear();
     if (!m || !nnz_out || !query_offsets) return fail(-1, "NULL argument");
     if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
-    const int P = m->problems();
-    if (query_offsets[0] != 0) return fail(-1, "query_offsets[0] must be 0");
-    for (int p = 0; p < P; p++)
-        if (query_offsets[p + 1] < query_offsets[p])
-            return fail(-1, "query_offsets must not decrease (problem %d)", p);
-    const long long M = query_offsets[P];
-    if (M > 2147483647LL) return fail(-1, "too many queries (%lld)", M);
+    if (int rc = check_offsets(m, query_offsets)) return rc;
+    const long long M = query_offsets[m->problems()];
     if (M > 0 && !d_queries) return fail(-1, "NULL queries");
     std::lock_guard<std::mutex> lk(m->mu);
     DeviceGuard g(m->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int *d_prob = nullptr;
-    if (P > 1 && M > 0) {
-        HIP_OR_FAIL(m->qoff.ensure(sizeof(long long) * (P + 1)));
-        HIP_OR_FAIL(m->prob.ensure(sizeof(int) * static_cast<size_t>(M)));
-        HIP_OR_FAIL(hipMemcpyAsync(m->qoff.p, query_offsets, sizeof(long long) * (P + 1),
-                                   hipMemcpyHostToDevice, st));
-        HIP_OR_FAIL(nat::launch_problem_ids(m->qoff.as<long long>(), P, static_cast<int>(M),
-                                            m->prob.as<int>(), st));
-        d_prob = m->prob.as<int>();
-    }
+    if (int rc = problem_ids(m, query_offsets, M, st, &d_prob)) return rc;
     long long nnz = 0;
     const int rc = compute_rows(m, d_queries, static_cast<int>(M), d_prob, st, &nnz);
     if (rc) return rc;
@@ -1409,6 +1502,176 @@ int nat_mesh_set_precision(nat_mesh *m, int mode, double cond_threshold,
         m->mixed_tau = static_cast<float>(cond_threshold);
         m->mixed_kappa = static_cast<float>(cancel_threshold);
     }
+    return 0;
+}
+
+int nat_mesh_set_policy(nat_mesh *m, const nat_policy *policy, const uint8_t *budgets) {
+    g_err.clear();
+    if (!m || !policy) return fail(-1, "NULL argument");
+    if (policy->first_level != 1 && policy->first_level != 2)
+        return fail(-1, "first_level %d (1 or 2)", policy->first_level);
+    if (policy->work_sort != 0 && policy->work_sort != 1)
+        return fail(-1, "work_sort %d (0 or 1)", policy->work_sort);
+    if (policy->topo_grid < 1 || policy->topo_grid > 4 || policy->geom_grid < 1 ||
+        policy->geom_grid > 4)
+        return fail(-1, "grid multipliers %d, %d outside 1..4", policy->topo_grid,
+                    policy->geom_grid);
+    if (budgets)
+        for (int i = 0; i < m->problems(); i++)
+            if (budgets[i] > kGroupBudget)
+                return fail(-1, "problem %d: budget %d above %d", i, budgets[i], kGroupBudget);
+    if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    std::lock_guard<std::mutex> lk(m->mu);
+    DeviceGuard g(m->device);
+    Policy p = policy_of(m);
+    if (budgets) p.budget.assign(budgets, budgets + m->problems());
+    p.first_level = policy->first_level;
+    p.work_sort = policy->work_sort;
+    p.topo_grid = policy->topo_grid;
+    p.geom_grid = policy->geom_grid;
+    return apply_policy(m, p);
+}
+
+int nat_mesh_get_policy(const nat_mesh *m, nat_policy *policy, uint8_t *budgets) {
+    g_err.clear();
+    if (!m || !policy) return fail(-1, "NULL argument");
+    *policy = nat_policy{};
+    policy->first_level = m->first_level;
+    policy->work_sort = m->work_sort;
+    policy->topo_grid = m->topo_grid;
+    policy->geom_grid = m->geom_grid;
+    if (budgets)
+        for (int p = 0; p < m->problems(); p++)
+            budgets[p] = p < static_cast<int>(m->budget.size()) ? m->budget[p] : kGroupBudget;
+    return 0;
+}
+
+int nat_mesh_autotune(nat_mesh *m, const double *d_queries, int64_t num_query_points,
+                      const int64_t *query_offsets, int repeats, void *stream,
+                      nat_tune_report *report) {
+    g_err.clear();
+    if (!m || !report) return fail(-1, "NULL argument");
+    if (repeats < 1 || repeats > 100) return fail(-1, "repeats %d outside 1..100", repeats);
+    if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    const int P = m->problems();
+    if (P > 1 && !query_offsets)
+        return fail(-1, "a batch mesh needs query offsets");
+    long long M = num_query_points;
+    if (query_offsets) {
+        if (int rc = check_offsets(m, query_offsets)) return rc;
+        if (query_offsets[P] != M)
+            return fail(-1, "query_offsets end at %lld, not at the %lld queries",
+                        static_cast<long long>(query_offsets[P]), M);
+    }
+    if (M < 0 || M > 2147483647LL) return fail(-1, "query count %lld", M);
+    if (M > 0 && !d_queries) return fail(-1, "NULL queries");
+    std::lock_guard<std::mutex> lk(m->mu);
+    DeviceGuard g(m->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    *report = nat_tune_report{};
+    if (M == 0) return 0;
+    const int *d_prob = nullptr;
+    if (int rc = problem_ids(m, query_offsets, M, st, &d_prob)) return rc;
+    const bool timing = m->timing;
+    m->timing = false;
+    struct Restore {  // timing flag back on every exit
+        nat_mesh *m;
+        bool t;
+        ~Restore() { m->timing = t; }
+    } restore{m, timing};
+    const int Mi = static_cast<int>(M);
+
+    Policy def;
+    def.budget.assign(P, static_cast<unsigned char>(kGroupBudget));
+    // pilot under the default policy: also sizes the workspace
+    double ms = 0;
+    if (int rc = time_policy(m, def, d_queries, Mi, d_prob, st, 1, &ms)) return rc;
+    // queries that left the group kernel, per problem (no record)
+    std::vector<long long> roff(static_cast<size_t>(M));
+    HIP_OR_FAIL(hipMemcpy(roff.data(), m->rec_off.p, sizeof(long long) * roff.size(),
+                          hipMemcpyDeviceToHost));
+    std::vector<long long> cnt(P, 0), left(P, 0);
+    long long left_all = 0;
+    for (int p = 0; p < P; p++) {
+        const long long a = query_offsets ? query_offsets[p] : 0;
+        const long long b = query_offsets ? query_offsets[p + 1] : M;
+        cnt[p] = b - a;
+        for (long long i = a; i < b; i++) left[p] += roff[i] < 0;
+        left_all += left[p];
+    }
+    report->left_group = static_cast<double>(left_all) / static_cast<double>(M);
+
+    double best_ms = 0;
+    if (int rc = time_policy(m, def, d_queries, Mi, d_prob, st, repeats, &best_ms)) return rc;
+    report->default_ms = best_ms;
+    Policy best = def;
+    int tried = 1;
+    auto consider = [&](const Policy &c) -> int {
+        double t = 0;
+        if (int rc = time_policy(m, c, d_queries, Mi, d_prob, st, repeats, &t)) return rc;
+        tried++;
+        if (t < 0.99 * best_ms) {  // 1% margin against timing noise
+            best_ms = t;
+            best = c;
+        }
+        return 0;
+    };
+    // 1. route problems whose queries mostly leave the group kernel
+    std::vector<std::vector<unsigned char>> seen;
+    for (double thr : {0.5, 0.25, 0.1}) {
+        Policy c = best;
+        bool any = false;
+        for (int p = 0; p < P; p++)
+            if (cnt[p] > 0 && static_cast<double>(left[p]) > thr * static_cast<double>(cnt[p])) {
+                c.budget[p] = 0;
+                any = true;
+            }
+        if (!any || std::find(seen.begin(), seen.end(), c.budget) != seen.end()) continue;
+        seen.push_back(c.budget);
+        if (int rc = consider(c)) return rc;
+    }
+    // 2. where thread-per-query rows start
+    if (left_all > 0) {
+        Policy c = best;
+        c.first_level = 2;
+        if (int rc = consider(c)) return rc;
+    }
+    // 3. smaller group budgets for the problems still in the group kernel:
+    //    big cavities give up sooner
+    if (left_all > 0) {
+        const Policy base = best;
+        for (int b : {64, 48}) {
+            Policy c = base;
+            for (int p = 0; p < P; p++)
+                if (c.budget[p] > 0) c.budget[p] = static_cast<unsigned char>(b);
+            if (int rc = consider(c)) return rc;
+        }
+    }
+    // 4. geometry work order, 5. grids
+    {
+        Policy c = best;
+        c.work_sort = 0;
+        if (int rc = consider(c)) return rc;
+    }
+    {
+        Policy c = best;
+        c.topo_grid = 2;
+        if (int rc = consider(c)) return rc;
+    }
+    {
+        Policy c = best;
+        c.geom_grid = 2;
+        if (int rc = consider(c)) return rc;
+    }
+    // final compute under the chosen policy (nat_weights_emit may follow)
+    if (int rc = time_policy(m, best, d_queries, Mi, d_prob, st, 1, &ms)) return rc;
+    report->tuned_ms = best_ms;
+    report->candidates = tried;
+    for (int p = 0; p < P; p++)
+        if (best.budget[p] == 0) {
+            report->routed_problems++;
+            report->routed_queries += cnt[p];
+        }
     return 0;
 }
 

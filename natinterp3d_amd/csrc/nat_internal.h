@@ -43,7 +43,16 @@ struct DevMesh {
     const double4 *cell_pts;  // [T] centroid x,y,z; .w = bit pattern of the tet index (int64)
     const LocParams *lp;      // [P]
     const int *prob;          // per query: its problem (set per call; NULL = problem 0)
+    // Per problem, the row-path policy: low byte = budget of tets the 16-lane
+    // group kernel may visit (0 sends the problem's queries straight to the
+    // thread-per-query levels), high byte = first thread-per-query level for
+    // the queries that leave the group kernel.  Read at launch time.
+    const unsigned short *route;
 };
+
+__device__ __forceinline__ unsigned route_of(const DevMesh &m, int qi) {
+    return m.route[m.prob ? m.prob[qi] : 0];
+}
 
 __device__ __forceinline__ const LocParams &loc_of(const DevMesh &m, int qi) {
     return m.lp[m.prob ? m.prob[qi] : 0];
@@ -135,8 +144,8 @@ int topo_blocks_per_cu();   // 128-thread blocks of the topology kernel
 int geom_blocks_per_cu();  // kGeomThreads-thread blocks
 // fused row kernel: used for the larger capacity levels
 // d_n_items != NULL: the item count is read on the device (graph form)
-hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
-                       const double *d_q, const int *d_items, int n_items,
+hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int level,
+                       int nblocks, const double *d_q, const int *d_items, int n_items,
                        const unsigned *d_n_items, unsigned int *d_work, const RowOut &out,
                        hipStream_t st);
 // zero-fill of n 32-bit words (a kernel, so that captured graphs hold no memset)
@@ -166,8 +175,8 @@ hipError_t launch_grad_geom(const DevMesh &m, int nblocks, const double *d_q, co
                             int n_items, unsigned int *d_work, const Topo &topo,
                             const GradIn &g, int *ovf_list, unsigned int *ovf_count,
                             hipStream_t st);
-hipError_t launch_grad_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
-                            const double *d_q, const int *d_items, int n_items,
+hipError_t launch_grad_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int level,
+                            int nblocks, const double *d_q, const int *d_items, int n_items,
                             unsigned int *d_work, const GradIn &g, int *ovf_list,
                             unsigned int *ovf_count, hipStream_t st);
 // insphere decisions behind saved rows re-tested under the static error bound

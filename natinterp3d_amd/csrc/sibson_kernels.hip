@@ -803,8 +803,10 @@ __device__ int fused_row(const DevMesh &m, const Caps &caps, const Scratch &sc, 
 }
 
 // n_items_dev, when given, holds the item count (written by an earlier kernel
-// of the same stream: the graph form of the launch sequence).
+// of the same stream: the graph form of the launch sequence).  Queries whose
+// problem starts its thread-per-query rows at a later level pass straight on.
 __global__ void __launch_bounds__(256) sibson_rows_kernel(DevMesh m, Caps caps, Scratch sc,
+                                                          int level,
                                                           const double *__restrict__ q,
                                                           const int *__restrict__ items,
                                                           int n_items,
@@ -820,10 +822,14 @@ __global__ void __launch_bounds__(256) sibson_rows_kernel(DevMesh m, Caps caps, 
         int qi = -1, cnt = 0;
         if (item < n_items) {
             qi = items ? items[item] : item;
-            epoch = epoch == 0x7ffffffe ? 1 : epoch + 1;
-            const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
-                           q[3 * static_cast<size_t>(qi) + 2]};
-            cnt = fused_row(m, caps, sc, tid, epoch, qq, qi);
+            if (static_cast<int>(route_of(m, qi) >> 8) > level) {
+                cnt = -1;
+            } else {
+                epoch = epoch == 0x7ffffffe ? 1 : epoch + 1;
+                const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
+                               q[3 * static_cast<size_t>(qi) + 2]};
+                cnt = fused_row(m, caps, sc, tid, epoch, qq, qi);
+            }
             if (cnt < 0) {
                 out.ovf_list[atomicAdd(out.ovf_count, 1u)] = qi;
                 cnt = 0;
@@ -848,7 +854,7 @@ __global__ void __launch_bounds__(256) sibson_rows_kernel(DevMesh m, Caps caps, 
 // record): the fused row with the gradient reduction.  Queries that exceed
 // the level's capacities go to ovf_list for the next level.
 __global__ void __launch_bounds__(256) sibson_grad_rows_kernel(
-    DevMesh m, Caps caps, Scratch sc, const double *__restrict__ q,
+    DevMesh m, Caps caps, Scratch sc, int level, const double *__restrict__ q,
     const int *__restrict__ items, int n_items, unsigned int *work, GradIn g, int *ovf_list,
     unsigned int *ovf_count) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -858,6 +864,10 @@ __global__ void __launch_bounds__(256) sibson_grad_rows_kernel(
     while (next_batch(work, n_items, item)) {
         if (item >= n_items) continue;
         const int qi = items[item];
+        if (static_cast<int>(route_of(m, qi) >> 8) > level) {
+            ovf_list[atomicAdd(ovf_count, 1u)] = qi;
+            continue;
+        }
         epoch = epoch == 0x7ffffffe ? 1 : epoch + 1;
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
@@ -1035,10 +1045,12 @@ topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict
             qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                   q[3 * static_cast<size_t>(qi) + 2]};
         }
-        bool ok = st != -2;
+        // the problem's visited-tet budget; 0 = thread-per-query rows only
+        const int vcap = active ? min(grp::VMAX, static_cast<int>(route_of(m, qi) & 0xffu)) : 0;
+        bool ok = st != -2 && vcap > 0;
         int ncav = 0, nbnd = 0, nnb = 0;
         int coin[4] = {0, 0, 0, 0};
-        if (active && st >= 0) {
+        if (active && st >= 0 && ok) {
             // ---- phase 0: empty tables
             for (int i = gl; i < grp::VT; i += grp::G) {
                 g.vkey[i] = -1;
@@ -1144,11 +1156,11 @@ topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict
                 }
                 group_sync();
                 visited += nnext;
-                if (visited > grp::VMAX) nnext = 0;
+                if (visited > vcap) nnext = 0;
                 nf = g.cnt[3] ? 0 : nnext;
                 cur ^= 1;
             }
-            ok = g.cnt[3] == 0 && visited <= grp::VMAX;
+            ok = g.cnt[3] == 0 && visited <= vcap;
             // ---- phase C: the reference's cavity order
             int nc = 0;
             if (ok && gl == 0) {
@@ -1672,12 +1684,12 @@ int geom_blocks_per_cu() {
     return n > 0 ? n : 1;
 }
 
-hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
-                       const double *d_q, const int *d_items, int n_items,
+hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int level,
+                       int nblocks, const double *d_q, const int *d_items, int n_items,
                        const unsigned *d_n_items, unsigned int *d_work, const RowOut &out,
                        hipStream_t st) {
-    hipLaunchKernelGGL(sibson_rows_kernel, dim3(nblocks), dim3(256), 0, st, m, caps, sc, d_q,
-                       d_items, n_items, d_n_items, d_work, out);
+    hipLaunchKernelGGL(sibson_rows_kernel, dim3(nblocks), dim3(256), 0, st, m, caps, sc, level,
+                       d_q, d_items, n_items, d_n_items, d_work, out);
     return hipGetLastError();
 }
 
@@ -1718,12 +1730,12 @@ hipError_t launch_geom(const DevMesh &m, int nblocks, const double *d_q, const i
     return hipGetLastError();
 }
 
-hipError_t launch_grad_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
-                            const double *d_q, const int *d_items, int n_items,
+hipError_t launch_grad_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int level,
+                            int nblocks, const double *d_q, const int *d_items, int n_items,
                             unsigned int *d_work, const GradIn &g, int *ovf_list,
                             unsigned int *ovf_count, hipStream_t st) {
-    hipLaunchKernelGGL(sibson_grad_rows_kernel, dim3(nblocks), dim3(256), 0, st, m, caps, sc, d_q,
-                       d_items, n_items, d_work, g, ovf_list, ovf_count);
+    hipLaunchKernelGGL(sibson_grad_rows_kernel, dim3(nblocks), dim3(256), 0, st, m, caps, sc,
+                       level, d_q, d_items, n_items, d_work, g, ovf_list, ovf_count);
     return hipGetLastError();
 }
 

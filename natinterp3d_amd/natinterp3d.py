@@ -115,6 +115,21 @@ class MeshAndVerticesHIP:
                                                      cancel_threshold),
                    "nat_mesh_set_precision")
 
+    def policy(self):
+        """The row-path policy (see set_policy)."""
+        return get_row_policy(self._m, 1)
+
+    def set_policy(self, **kw):
+        """Change the row-path policy: budgets (list of one visited-tet budget,
+        0..96; 0 = thread-per-query rows), first_level (1 or 2), work_sort
+        (bool), topo_grid, geom_grid (1..4).  Results do not depend on it."""
+        set_row_policy(self._m, 1, **kw)
+
+    def autotune(self, queries, repeats=3):
+        """Time row-path policies on centred float64 CUDA queries and keep the
+        fastest on this mesh; returns the tuner's report (dict)."""
+        return autotune_rows(self._m, queries, None, repeats)
+
     def predicate_audit(self, queries):
         """Re-test the insphere decisions behind the rows of centred float64
         CUDA queries under the FP64 static error bound: returns (decisions,
@@ -289,6 +304,15 @@ class Interpolator:
         self.num_threads = num_threads
         self.mesh_and_vertices = MeshAndVerticesHIP(centered, device=device)
 
+    def autotune(self, query_points: np.ndarray, repeats: int = 3) -> dict:
+        """Time the row-path policies on representative ``query_points`` and
+        keep the fastest for later calls (results never depend on it)."""
+        import torch
+        q = np.ascontiguousarray(query_points, np.float64) - self._centroid
+        dev = torch.device("cuda", self.mesh_and_vertices.device)
+        with torch.cuda.device(dev):
+            return self.mesh_and_vertices.autotune(torch.from_numpy(q).to(dev), repeats)
+
     def get_weights(self, query_points: np.ndarray) -> scipy.sparse.csr_matrix:
         """Sibson coordinates of ``query_points`` as a CSR matrix whose data has
         the dtype of ``query_points`` (natinterp3d.py:105-117)."""
@@ -422,3 +446,52 @@ class WeightsGraph:
 
 class GraphCapacityError(RuntimeError):
     """A graph replay needed more workspace than the graph was made with."""
+
+
+def get_row_policy(handle, num_problems):
+    """Row-path policy of a mesh handle as a dict (budgets per problem)."""
+    pol = _lib.Policy()
+    budgets = np.zeros(num_problems, np.uint8)
+    _lib.check(_lib.lib().nat_mesh_get_policy(handle, ctypes.byref(pol),
+                                              budgets.ctypes.data_as(ctypes.c_void_p)),
+               "nat_mesh_get_policy")
+    return {"budgets": budgets.tolist(), "first_level": pol.first_level,
+            "work_sort": bool(pol.work_sort), "topo_grid": pol.topo_grid,
+            "geom_grid": pol.geom_grid}
+
+
+def set_row_policy(handle, num_problems, budgets=None, first_level=None, work_sort=None,
+                   topo_grid=None, geom_grid=None):
+    """Set the given fields of a mesh handle's row-path policy."""
+    cur = get_row_policy(handle, num_problems)
+    pol = _lib.Policy()
+    pol.first_level = int(cur["first_level"] if first_level is None else first_level)
+    pol.work_sort = int(bool(cur["work_sort"] if work_sort is None else work_sort))
+    pol.topo_grid = int(cur["topo_grid"] if topo_grid is None else topo_grid)
+    pol.geom_grid = int(cur["geom_grid"] if geom_grid is None else geom_grid)
+    b = None
+    if budgets is not None:
+        arr = np.asarray(budgets)
+        if arr.shape != (num_problems,) or (arr < 0).any() or (arr > 255).any():
+            raise ValueError(f"budgets must be {num_problems} values in 0..96")
+        b = np.ascontiguousarray(arr, np.uint8)
+    _lib.check(_lib.lib().nat_mesh_set_policy(
+        handle, ctypes.byref(pol), None if b is None else b.ctypes.data_as(ctypes.c_void_p)),
+        "nat_mesh_set_policy")
+
+
+def autotune_rows(handle, queries, query_offsets, repeats):
+    """nat_mesh_autotune on centred float64 CUDA queries (query_offsets for a
+    batch mesh); returns the report as a dict."""
+    import torch
+    q = queries.detach().to(dtype=torch.float64).contiguous()
+    off = None
+    if query_offsets is not None:
+        off = np.ascontiguousarray(query_offsets, np.int64)
+    rep = _lib.TuneReport()
+    stream = ctypes.c_void_p(torch.cuda.current_stream(q.device).cuda_stream)
+    _lib.check(_lib.lib().nat_mesh_autotune(
+        handle, ctypes.c_void_p(q.data_ptr() if q.numel() else 0), q.shape[0],
+        None if off is None else off.ctypes.data_as(ctypes.c_void_p), int(repeats), stream,
+        ctypes.byref(rep)), "nat_mesh_autotune")
+    return {f: getattr(rep, f) for f, _ in rep._fields_}

@@ -77,13 +77,24 @@ def shell(seed=2, n_keys=20_000, n_queries=2_000_000, n_harmonics=8, max_degree=
 
 
 def batch_problems(seed=3, n_problems=256, keys_range=(500, 50_000),
-                   queries_range=(1_000, 200_000), dim=8):
+                   queries_range=(1_000, 200_000), dim=8, shell_every=0,
+                   shell_keys=(300, 3_000), shell_queries=(1_000, 10_000)):
     """Independent problems of uneven size: per problem, key and query counts
     log-uniform in the given ranges, points uniform in an axis-aligned box at
-    the origin with sides U[0.5, 2], float32 values [n_keys, dim]."""
+    the origin with sides U[0.5, 2], float32 values [n_keys, dim].  With
+    shell_every = k > 0, every k-th problem is instead a hollow shell (see
+    shell(); key and query counts log-uniform in shell_keys, shell_queries)."""
     rng = np.random.default_rng(seed)
     out = []
-    for _ in range(n_problems):
+    for i in range(n_problems):
+        if shell_every > 0 and i % shell_every == shell_every - 1:
+            nk = int(np.floor(np.exp(rng.uniform(*np.log(shell_keys)))))
+            nq = int(np.floor(np.exp(rng.uniform(*np.log(shell_queries)))))
+            keys, queries, _, _ = shell(seed=int(rng.integers(2**31)), n_keys=nk,
+                                        n_queries=nq)
+            values = rng.standard_normal((nk, dim)).astype(np.float32)
+            out.append((keys, queries, values))
+            continue
         nk = int(np.floor(np.exp(rng.uniform(np.log(keys_range[0]), np.log(keys_range[1])))))
         nq = int(np.floor(np.exp(rng.uniform(np.log(queries_range[0]),
                                              np.log(queries_range[1])))))

@@ -81,3 +81,23 @@ def test_query_shape_checks():
         m.get_weights(np.zeros((3, 2)))
     with pytest.raises(ValueError):
         m.get_weights(np.zeros((3, 3), np.float32))
+
+
+def test_policy_argument_checks():
+    from natinterp3d_amd.natinterp3d import get_row_policy, set_row_policy
+    rng = np.random.default_rng(3)
+    k = rng.random((60, 3))
+    m = MeshAndVerticesHIP(np.ascontiguousarray(k - k.mean(0)), device=-1)
+    assert get_row_policy(m.handle, 1) == {"budgets": [96], "first_level": 1,
+                                           "work_sort": True, "topo_grid": 1, "geom_grid": 1}
+    for kw, msg in (({"first_level": 3}, "first_level"), ({"topo_grid": 0}, "grid"),
+                    ({"geom_grid": 5}, "grid"), ({"budgets": [97]}, "budget"),
+                    ({"first_level": 2}, "host-only")):
+        with pytest.raises(RuntimeError, match=msg):
+            set_row_policy(m.handle, 1, **kw)
+    with pytest.raises(ValueError):
+        set_row_policy(m.handle, 1, budgets=[1, 2])
+    rep = _lib.TuneReport()
+    assert _lib.lib().nat_mesh_autotune(m.handle, None, 0, None, 0, None,
+                                        ctypes.byref(rep)) < 0
+    assert "repeats" in _lib.last_error()
