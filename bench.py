@@ -238,6 +238,126 @@ def run_precision(args):
     print(json.dumps(out), flush=True)
 
 
+def run_shell(args):
+    """Differentiation workload: 20,000 keys on a closed noisy shell (unit
+    sphere radially perturbed by 0.1 x 8 random real spherical harmonics of
+    degree <= 4, N(0, 0.01) thickness), 2,000,000 queries uniform inside 0.9x
+    the shell radius, values float64 [N,3].  One step = forward (all weight
+    rows + W @ values) + autograd backward to values and queries.  Hollow
+    shells give every query a cavity of hundreds to thousands of tets, so
+    these rows come from the thread-per-query levels.  After the timed steps:
+    a central finite-difference check (h = 1e-6) of dL/dq on 1,000 random
+    queries."""
+    import torch
+
+    import natinterp3d_amd
+    from natinterp3d_amd import workloads
+    from natinterp3d_amd.autograd import SibsonInterpolation
+
+    keys, queries, values, _ = workloads.shell(n_keys=args.shell_keys,
+                                               n_queries=args.shell_queries)
+    log(f"shell workload: {len(keys):,} keys, {len(queries):,} queries")
+    centroid = np.ascontiguousarray(keys, np.float64).mean(axis=0)
+    kc = np.ascontiguousarray(keys - centroid)
+    qc = np.ascontiguousarray(queries - centroid)
+    refbuild = None
+    if args.cpu_baseline == "reference":
+        from oracle import ref
+        if ref.available():
+            refbuild = RefBuild(kc)
+            refbuild.start()
+    mesh = natinterp3d_amd.MeshAndVerticesHIP(kc, device=0)
+    dev = torch.device("cuda", 0)
+    M = qc.shape[0]
+    qd = torch.from_numpy(qc).to(dev).requires_grad_(True)
+    xd = torch.from_numpy(np.ascontiguousarray(values)).to(dev).requires_grad_(True)
+    gy = torch.randn((M, 3), dtype=torch.float64, device=dev,
+                     generator=torch.Generator(device=dev).manual_seed(6))
+    mesh.set_timing(True)
+
+    def step():
+        qd.grad = None
+        xd.grad = None
+        y = SibsonInterpolation.apply(mesh, qd, xd)
+        y.backward(gy)
+
+    for i in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"warmup {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        torch.cuda.synchronize()
+        log(f"step {i + 1}/{args.steps} done")
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    tm = mesh.timings()
+    fwd_ms = float(sum(tm["rows_ms"]))
+    bwd_ms = float(sum(tm["grad_ms"]))
+    log(f"fwd+bwd {ms:.1f} ms/step ({M / ms * 1e3 / 1e6:.2f} M queries/s); "
+        f"timings {tm}")
+
+    # finite differences on 1,000 random queries
+    rng = np.random.default_rng(7)
+    sel = torch.as_tensor(rng.choice(M, 1000, replace=False), device=dev)
+    g_sel = gy[sel]
+    q0 = qd.detach()[sel].clone()
+    vals = xd.detach()
+
+    def loss(qq):
+        with torch.no_grad():
+            crow, col, val = mesh.weights_device(qq)
+            from natinterp3d_amd.natinterp3d import spmm_device
+            return (spmm_device(crow, col, val, vals, qq.shape[0]) * g_sel).sum(dim=1)
+
+    qa = q0.clone().requires_grad_(True)
+    (SibsonInterpolation.apply(mesh, qa, vals) * g_sel).sum().backward()
+    ana = qa.grad.detach()
+    fd = torch.empty_like(ana)
+    h = 1e-6
+    for k in range(3):
+        e = torch.zeros_like(q0)
+        e[:, k] = h
+        fd[:, k] = (loss(q0 + e) - loss(q0 - e)) / (2 * h)
+    scale = torch.maximum(ana.norm(dim=1), ana.norm(dim=1).median())
+    rel = ((fd - ana).norm(dim=1) / scale).cpu().numpy()
+    log(f"finite differences (h=1e-6, 1000 queries): relative error median "
+        f"{np.median(rel):.2e}, 99th percentile {np.percentile(rel, 99):.2e}, max {rel.max():.2e}")
+
+    base = None
+    if refbuild is not None:
+        log("cpu baseline: reference on the host cores (forward only)")
+        base = cpu_baseline(refbuild, qc, args.cpu_seconds)
+    out = {
+        "metric": "Sibson queries/s fwd+bwd, 20k-key hollow shell, 2M queries (differentiation "
+                  "workload)",
+        "value": M / (ms / 1e3),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"shell: {len(keys):,} keys on a noisy harmonic shell, {M:,} queries "
+                        "inside 0.9x its radius, values float64 [N,3]; forward + backward",
+            "fwd_kernels_ms": fwd_ms, "bwd_kernels_ms": bwd_ms,
+            "rows_items": tm["rows_items"], "grad_items": tm["grad_items"],
+            "fd_check": {"h": h, "queries": 1000, "rel_err_median": float(np.median(rel)),
+                         "rel_err_p99": float(np.percentile(rel, 99)),
+                         "rel_err_max": float(rel.max()),
+                         "scale": "max(|grad|, median |grad|) per query"},
+        },
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_batch(args):
     """Batching workload: independent problems with log-uniform key counts in
     [500, 50,000] and query counts in [1,000, 200,000], points uniform in
@@ -384,10 +504,14 @@ def main():
     ap.add_argument("--queries", type=int, default=10_000_000)
     ap.add_argument("--cpu-baseline", choices=["reference", "none"], default="reference")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", choices=["plain", "batch", "precision"], default="plain",
+    ap.add_argument("--workload", choices=["plain", "batch", "precision", "shell"],
+                    default="plain",
                     help="plain: the fwd+bwd headline; batch: 256 independent problems in "
                          "one set of launches against 256 sequential single-problem calls; "
-                         "precision: jittered-lattice stress, FP64 against mixed precision")
+                         "precision: jittered-lattice stress, FP64 against mixed precision; "
+                         "shell: fwd+bwd on a hollow shell (differentiation workload)")
+    ap.add_argument("--shell-keys", type=int, default=20_000)
+    ap.add_argument("--shell-queries", type=int, default=2_000_000)
     ap.add_argument("--autotune", choices=["on", "off"], default="on",
                     help="plain workload: tune the row policy on the queries before timing")
     ap.add_argument("--problems", type=int, default=256)
@@ -405,6 +529,8 @@ def main():
         return run_batch(args)
     if args.workload == "precision":
         return run_precision(args)
+    if args.workload == "shell":
+        return run_shell(args)
 
     import torch
 

@@ -251,8 +251,9 @@ int nat_mesh_get_timings(const nat_mesh *m, nat_timings *out);
  * gives up on a query once it has visited more tets than the budget (at most
  * 96, the LDS tables' size), or when the row has more than 32 neighbours.
  * Those queries go to the thread-per-query levels (one thread per query,
- * tables in global memory: level 1 up to 4096 cavity tets on up to 32768
- * threads, level 2 up to 32768 on 2048, level 3 up to 262144 on 64), starting
+ * tables in global memory: level 1 up to 4096 cavity tets and 2048
+ * neighbours on up to 131072 threads, level 2 up to 32768 on 2048, level 3
+ * up to 262144 on 64), starting
  * at first_level. */
 typedef struct {
     int32_t first_level;  /* 1 or 2 */

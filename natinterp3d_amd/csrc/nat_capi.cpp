@@ -208,9 +208,12 @@ void set_levels(nat_mesh *m) {
     m->nthr_geom = cus * nat::geom_blocks_per_cu() * nat::kGeomThreads;
     // larger levels (fused kernel) only see the queries that overflowed; a
     // level's scratch holds as many threads as it has queries, up to these
-    // counts (level 1 at most ~14 GB, level 2 ~7 GB, level 3 ~2 GB)
-    m->caps[1] = {kd, 8192, 14, 4096, 10240, 4096, 13};
-    m->nthr[1] = 32768;
+    // counts (level 1 at most ~31 GB, level 2 ~7 GB, level 3 ~2 GB).  Level 1
+    // takes rows of up to 4096 cavity tets and 2048 neighbours (hollow shells
+    // of 20k keys: up to 1951 and 821) on 2 waves per SIMD: these rows wait on
+    // memory, so its speed follows its thread count
+    m->caps[1] = {kd, 4096, 13, 4096, 8192, 2048, 12};
+    m->nthr[1] = 131072;
     m->caps[2] = {kd, 65536, 17, 32768, 81920, 32768, 16};
     m->nthr[2] = 2048;
     m->caps[3] = {kd, 524288, 20, 262144, 655360, 393216, 20};
