@@ -206,6 +206,10 @@ typedef struct {
                                  FP32 by the last compute (always counted) */
     int64_t mixed_fp64;       /* ... and recomputed in FP64 (ill-conditioned) */
     int64_t mixed_rows_fp64;  /* rows redone in FP64 (cancelling volumes) */
+    double wave_ms;           /* wave-per-query level: forward */
+    double grad_wave_ms;      /* ... and in nat_grad_queries (with its start tets) */
+    int32_t wave_items;
+    int32_t grad_wave_items;
 } nat_timings;
 
 int nat_mesh_set_timing(nat_mesh *m, int enable);
@@ -250,10 +254,12 @@ int nat_mesh_get_timings(const nat_mesh *m, nat_timings *out);
  * cavity tables in LDS) unless its problem's budget is 0; the group kernel
  * gives up on a query once it has visited more tets than the budget (at most
  * 96, the LDS tables' size), or when the row has more than 32 neighbours.
- * Those queries go to the thread-per-query levels (one thread per query,
- * tables in global memory: level 1 up to 4096 cavity tets and 2048
- * neighbours on up to 131072 threads, level 2 up to 32768 on 2048, level 3
- * up to 262144 on 64), starting
+ * Those queries go next, when wave_rows is set, to the wave-per-query level
+ * (a wave of 64 lanes per query, tables in global memory, up to 2048 cavity
+ * tets, 8192 boundary faces and 2048 neighbours), and then to the
+ * thread-per-query levels (one thread per query, tables in global memory:
+ * level 1 up to 4096 cavity tets and 2048 neighbours on up to 131072
+ * threads, level 2 up to 32768 on 2048, level 3 up to 262144 on 64), starting
  * at first_level. */
 typedef struct {
     int32_t first_level;  /* 1 or 2 */
@@ -261,7 +267,8 @@ typedef struct {
                              cavity size within windows of 4096 queries */
     int32_t topo_grid;    /* group-kernel grid = resident blocks x this (1..4) */
     int32_t geom_grid;    /* geometry-kernel grid = resident blocks x this (1..4) */
-    int32_t reserved[4];
+    int32_t wave_rows;    /* 1: the wave-per-query level runs */
+    int32_t reserved[3];
 } nat_policy;
 
 /* Sets the policy and, when budgets is given, the group kernel's visited-tet
@@ -285,9 +292,9 @@ typedef struct {
  * device memory; query_offsets[P+1] on the host for a batch mesh, NULL for a
  * single problem) and keeps the fastest on the mesh.  Candidates, tried one
  * dimension at a time from the default: problems whose queries mostly leave
- * the group kernel routed to thread-per-query rows (several thresholds),
- * first_level 2, smaller budgets for the other problems, the work sort off,
- * larger grids.  Each candidate is timed as the best of `repeats` computes
+ * the group kernel routed to the larger levels (several thresholds), the
+ * wave-per-query level off, first_level 2, smaller budgets for the other
+ * problems, the work sort off, larger grids.  Each candidate is timed as the best of `repeats` computes
  * (host clock, synchronised).  The last compute uses the chosen policy, so
  * nat_weights_emit may follow.  Earlier topologies and graphs stay valid. */
 int nat_mesh_autotune(nat_mesh *m, const double *d_queries, int64_t num_query_points,

@@ -33,6 +33,8 @@ class Timings(ctypes.Structure):
         ("emit_ms", ctypes.c_double), ("grad_ms", ctypes.c_double * 4),
         ("grad_items", ctypes.c_int32 * 4), ("mixed_fp32", ctypes.c_int64),
         ("mixed_fp64", ctypes.c_int64), ("mixed_rows_fp64", ctypes.c_int64),
+        ("wave_ms", ctypes.c_double), ("grad_wave_ms", ctypes.c_double),
+        ("wave_items", ctypes.c_int32), ("grad_wave_items", ctypes.c_int32),
     ]
 
 
@@ -40,7 +42,7 @@ class Policy(ctypes.Structure):
     _fields_ = [
         ("first_level", ctypes.c_int32), ("work_sort", ctypes.c_int32),
         ("topo_grid", ctypes.c_int32), ("geom_grid", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("wave_rows", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
     ]
 
 

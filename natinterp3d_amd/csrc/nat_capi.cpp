@@ -112,6 +112,9 @@ struct nat_mesh {
     int first_level = 1;
     int work_sort = 1;
     int topo_grid = 1, geom_grid = 1;
+    int wave_rows = 1;                  // wave-per-query level before the thread-per-query ones
+    DevBuf wscr;                        // its per-wave scratch
+    int wave_cap = 0;                   // waves the scratch holds
 
     // workspace
     nat::Caps caps[kLevels];
@@ -162,7 +165,7 @@ struct nat_mesh {
                    ovf[0].bytes +
                    ovf[1].bytes + counters.bytes + pool_vals.bytes + pool_cols.bytes +
                    csr_vals.bytes + csr_cols.bytes + csr_rowptr.bytes +
-                   rec.bytes + rec_off.bytes + start.bytes;
+                   rec.bytes + rec_off.bytes + start.bytes + wscr.bytes;
         for (int l = 0; l < kLevels; l++) s += scratch[l].bytes;
         return static_cast<int64_t>(s);
     }
@@ -172,7 +175,7 @@ struct nat_mesh {
                          &row_off,  &row_cnt,   &rowptr64,  &row_total, &ovf[0],    &ovf[1],
                          &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,   &csr_rowptr,
                          &rec,      &rec_off,   &start,     &lp,       &prob,       &qoff,
-                         &route};
+                         &route,    &wscr};
         for (DevBuf *b : all) b->release();
         for (int l = 0; l < kLevels; l++) scratch[l].release();
         for (hipEvent_t &e : ev) {
@@ -475,6 +478,21 @@ int ensure_level_scratch(nat_mesh *m, int level, long long want, int *launch) {
     return 0;
 }
 
+// Scratch of the wave-per-query level for `want` queries: one wave per
+// query up to the resident waves of the device.  Returns the waves to launch.
+int ensure_wave_scratch(nat_mesh *m, long long want, int *launch) {
+    const int cap_max = std::max(1, m->num_cus) * nat::wave_blocks_per_cu();
+    const int n = static_cast<int>(std::min<long long>(cap_max, std::max<long long>(want, 1)));
+    if (n > m->wave_cap) {
+        const int cap = std::min(cap_max, std::max(n, 2 * m->wave_cap));
+        m->wscr.release();
+        HIP_OR_FAIL(m->wscr.ensure(nat::wave_scratch_size() * static_cast<size_t>(cap)));
+        m->wave_cap = cap;
+    }
+    *launch = n;
+    return 0;
+}
+
 // d_prob: problem of each query (batch meshes), NULL for a single problem
 int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipStream_t st,
                  long long *nnz_out) {
@@ -538,8 +556,9 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
             m->rec_cap = want_rec;
         }
         // counters: [0..3] fused-level work, [4] topology work, [5] geometry
-        // work, [8..11] overflow counts, u64 pool use at [16], records at [18],
-        // mixed-path FP32 / FP64 counts at [20], [22], rows redone at [24]
+        // work, [6] wave-level work, [8..11] overflow counts, [12] wave-level
+        // overflow, u64 pool use at [16], records at [18], mixed-path FP32 /
+        // FP64 counts at [20], [22], rows redone at [24]
         unsigned *ctr = m->counters.as<unsigned>();
         unsigned long long *pool_used = reinterpret_cast<unsigned long long *>(ctr + 16);
         unsigned long long *rec_used = reinterpret_cast<unsigned long long *>(ctr + 18);
@@ -599,15 +618,37 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         m->times.rows_items[0] = M;
         m->times.levels = 1;
 
-        // larger levels: the queries that overflowed the previous one
+        // larger rows: the queries that overflowed the previous level, first
+        // one wave per query, then one thread per query
+        int cur = 0;  // ovf buffer holding the items
         const int *items = m->ovf[0].as<int>();
         int n_items = static_cast<int>(novf);
+        m->times.wave_ms = 0.0;
+        m->times.wave_items = 0;
+        if (n_items > 0 && m->wave_rows) {
+            int nw = 0;
+            if (int rc = ensure_wave_scratch(m, n_items, &nw)) return rc;
+            mark(m, 2, st);
+            HIP_OR_FAIL(nat::launch_wave_rows(m->dev, m->wscr.as<char>(), nw, d_q, items, n_items,
+                                              nullptr, m->start.as<int>(), ctr + 6, out,
+                                              m->ovf[1].as<int>(), ctr + 12, st));
+            mark(m, 3, st);
+            unsigned nnext = 0;
+            HIP_OR_FAIL(
+                hipMemcpyAsync(&nnext, ctr + 12, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+            HIP_OR_FAIL(hipStreamSynchronize(st));
+            m->times.wave_ms = elapsed(m, 2, 3);
+            m->times.wave_items = n_items;
+            n_items = static_cast<int>(nnext);
+            cur = 1;
+            items = m->ovf[1].as<int>();
+        }
         for (int level = 1; level < kLevels && n_items > 0; level++) {
             int nthr = 0;
             if (int rc = ensure_level_scratch(m, level, n_items, &nthr)) return rc;
             nat::Scratch sc;
             nat::scratch_carve(m->scratch[level].p, m->caps[level], m->scap[level], &sc);
-            out.ovf_list = m->ovf[level & 1].as<int>();
+            out.ovf_list = m->ovf[cur ^ 1].as<int>();
             out.ovf_count = ctr + 8 + level;
             const int nblocks = (nthr + 255) / 256;
             mark(m, 2, st);
@@ -622,7 +663,8 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
             m->times.rows_items[level] = n_items;
             m->times.levels = level + 1;
             n_items = static_cast<int>(nnext);
-            items = m->ovf[level & 1].as<int>();
+            cur ^= 1;
+            items = m->ovf[cur].as<int>();
         }
         if (n_items > 0)
             return fail(-4, "%d queries need more than %d cavity tets", n_items,
@@ -680,7 +722,8 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
     HIP_OR_FAIL(m->ovf[0].ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->ovf[1].ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->counters.ensure(256));
-    // counters: [0] record work, [1..3] fused-level work, [8..11] overflow counts
+    // counters: [0] record work, [1..3] fused-level work, [6] wave-level work,
+    // [8..11] overflow counts, [12] wave-level overflow
     unsigned *ctr = m->counters.as<unsigned>();
     HIP_OR_FAIL(hipMemsetAsync(m->counters.p, 0, 256, st));
     nat::Topo topo;
@@ -704,8 +747,35 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
     HIP_OR_FAIL(hipStreamSynchronize(st));
     m->times.grad_ms[0] = elapsed(m, 2, 3);
     m->times.grad_items[0] = M;
+    int cur = 0;
     const int *items = m->ovf[0].as<int>();
     int n_items = static_cast<int>(novf);
+    m->times.grad_wave_ms = 0.0;
+    m->times.grad_wave_items = 0;
+    if (n_items > 0 && m->wave_rows) {
+        // start tets of the rows without a record, then one wave per row
+        HIP_OR_FAIL(m->start.ensure(sizeof(int) * Mz));
+        int nthr0 = 0, nw = 0;
+        if (int rc = ensure_level_scratch(m, 0, n_items, &nthr0)) return rc;
+        if (int rc = ensure_wave_scratch(m, n_items, &nw)) return rc;
+        nat::Scratch sc0;
+        nat::scratch_carve(m->scratch[0].p, m->caps[0], m->scap[0], &sc0);
+        mark(m, 2, st);
+        HIP_OR_FAIL(nat::launch_locate(m->dev, m->caps[0], sc0, (nthr0 + 255) / 256, d_q, items,
+                                       n_items, m->start.as<int>(), st));
+        HIP_OR_FAIL(nat::launch_wave_grad(m->dev, m->wscr.as<char>(), nw, d_q, items, n_items,
+                                          m->start.as<int>(), ctr + 6, g, m->ovf[1].as<int>(),
+                                          ctr + 12, st));
+        mark(m, 3, st);
+        unsigned nnext = 0;
+        HIP_OR_FAIL(hipMemcpyAsync(&nnext, ctr + 12, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        HIP_OR_FAIL(hipStreamSynchronize(st));
+        m->times.grad_wave_ms = elapsed(m, 2, 3);
+        m->times.grad_wave_items = n_items;
+        n_items = static_cast<int>(nnext);
+        cur = 1;
+        items = m->ovf[1].as<int>();
+    }
     for (int level = 1; level < kLevels && n_items > 0; level++) {
         int nthr = 0;
         if (int rc = ensure_level_scratch(m, level, n_items, &nthr)) return rc;
@@ -714,7 +784,7 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
         const int nblocks = (nthr + 255) / 256;
         mark(m, 2, st);
         HIP_OR_FAIL(nat::launch_grad_rows(m->dev, m->caps[level], sc, level, nblocks, d_q, items,
-                                          n_items, ctr + level, g, m->ovf[level & 1].as<int>(),
+                                          n_items, ctr + level, g, m->ovf[cur ^ 1].as<int>(),
                                           ctr + 8 + level, st));
         mark(m, 3, st);
         unsigned nnext = 0;
@@ -724,7 +794,8 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
         m->times.grad_ms[level] = elapsed(m, 2, 3);
         m->times.grad_items[level] = n_items;
         n_items = static_cast<int>(nnext);
-        items = m->ovf[level & 1].as<int>();
+        cur ^= 1;
+        items = m->ovf[cur].as<int>();
     }
     if (n_items > 0)
         return fail(-4, "%d queries need more than %d cavity tets", n_items,
@@ -768,6 +839,7 @@ std::vector<std::pair<const void *, size_t>> workspace_fingerprint(const nat_mes
                             &m->rec_off, &m->start};
     for (const DevBuf *b : bufs) f.emplace_back(b->p, b->bytes);
     for (int l = 0; l < kLevels; l++) f.emplace_back(m->scratch[l].p, m->scratch[l].bytes);
+    f.emplace_back(m->wscr.p, m->wscr.bytes);
     return f;
 }
 
@@ -833,14 +905,26 @@ int issue_graph_rows(nat_mesh *m, nat_graph *g, const double *d_q, hipStream_t s
         (m->work_sort ? m->geom_order : m->order).as<int>(), M, ctr + 5, topo,
         out, m->mixed_tau, m->mixed_kappa,
         m->precision == 1 ? reinterpret_cast<unsigned long long *>(ctr + 20) : nullptr, st));
+    // the larger levels read their item counts from the device
+    int cur = 0;
+    const unsigned *n_dev = ctr + 8;
+    if (m->wave_rows) {
+        HIP_OR_FAIL(nat::launch_wave_rows(m->dev, m->wscr.as<char>(), m->wave_cap, d_q,
+                                          m->ovf[0].as<int>(), 0, n_dev, m->start.as<int>(),
+                                          ctr + 6, out, m->ovf[1].as<int>(), ctr + 12, st));
+        cur = 1;
+        n_dev = ctr + 12;
+    }
     for (int level = 1; level < kLevels; level++) {
         nat::Scratch sc;
         nat::scratch_carve(m->scratch[level].p, m->caps[level], m->scap[level], &sc);
-        out.ovf_list = m->ovf[level & 1].as<int>();
+        out.ovf_list = m->ovf[cur ^ 1].as<int>();
         out.ovf_count = ctr + 8 + level;
         HIP_OR_FAIL(nat::launch_rows(m->dev, m->caps[level], sc, level,
-                                     (m->scap[level] + 255) / 256, d_q, m->ovf[(level - 1) & 1].as<int>(), 0,
-                                     ctr + 8 + level - 1, ctr + level, out, st));
+                                     (m->scap[level] + 255) / 256, d_q, m->ovf[cur].as<int>(), 0,
+                                     n_dev, ctr + level, out, st));
+        cur ^= 1;
+        n_dev = ctr + 8 + level;
     }
     long long *rp64 = m->rowptr64.as<long long>();
     HIP_OR_FAIL(nat::launch_scan_counts(m->row_cnt.as<int>(), rp64, M, m->scan.as<long long>(),
@@ -885,7 +969,7 @@ int problem_ids(nat_mesh *m, const int64_t *query_offsets, long long M, hipStrea
 
 struct Policy {
     std::vector<unsigned char> budget;
-    int first_level = 1, work_sort = 1, topo_grid = 1, geom_grid = 1;
+    int first_level = 1, work_sort = 1, topo_grid = 1, geom_grid = 1, wave_rows = 1;
 };
 
 Policy policy_of(const nat_mesh *m) {
@@ -895,6 +979,7 @@ Policy policy_of(const nat_mesh *m) {
     p.work_sort = m->work_sort;
     p.topo_grid = m->topo_grid;
     p.geom_grid = m->geom_grid;
+    p.wave_rows = m->wave_rows;
     return p;
 }
 
@@ -904,6 +989,7 @@ int apply_policy(nat_mesh *m, const Policy &p) {
     m->work_sort = p.work_sort;
     m->topo_grid = p.topo_grid;
     m->geom_grid = p.geom_grid;
+    m->wave_rows = p.wave_rows;
     return upload_route(m);
 }
 
@@ -1317,6 +1403,11 @@ int nat_graph_create(nat_mesh *m, const double *d_queries, int num_query_points,
         int n = 0;
         if ((rc = ensure_level_scratch(m, l, std::max(256, 2 * items[l]), &n))) return bail(rc);
     }
+    if (m->wave_rows) {
+        int n = 0;
+        if ((rc = ensure_wave_scratch(m, std::max(64, 2 * m->times.wave_items), &n)))
+            return bail(rc);
+    }
     g->cap = m->pool_cap;
     if (g->vals.ensure(sizeof(double) * g->cap) != hipSuccess ||
         g->cols.ensure(sizeof(int) * g->cap) != hipSuccess ||
@@ -1515,6 +1606,8 @@ int nat_mesh_set_policy(nat_mesh *m, const nat_policy *policy, const uint8_t *bu
         return fail(-1, "first_level %d (1 or 2)", policy->first_level);
     if (policy->work_sort != 0 && policy->work_sort != 1)
         return fail(-1, "work_sort %d (0 or 1)", policy->work_sort);
+    if (policy->wave_rows != 0 && policy->wave_rows != 1)
+        return fail(-1, "wave_rows %d (0 or 1)", policy->wave_rows);
     if (policy->topo_grid < 1 || policy->topo_grid > 4 || policy->geom_grid < 1 ||
         policy->geom_grid > 4)
         return fail(-1, "grid multipliers %d, %d outside 1..4", policy->topo_grid,
@@ -1532,6 +1625,7 @@ int nat_mesh_set_policy(nat_mesh *m, const nat_policy *policy, const uint8_t *bu
     p.work_sort = policy->work_sort;
     p.topo_grid = policy->topo_grid;
     p.geom_grid = policy->geom_grid;
+    p.wave_rows = policy->wave_rows;
     return apply_policy(m, p);
 }
 
@@ -1543,6 +1637,7 @@ int nat_mesh_get_policy(const nat_mesh *m, nat_policy *policy, uint8_t *budgets)
     policy->work_sort = m->work_sort;
     policy->topo_grid = m->topo_grid;
     policy->geom_grid = m->geom_grid;
+    policy->wave_rows = m->wave_rows;
     if (budgets)
         for (int p = 0; p < m->problems(); p++)
             budgets[p] = p < static_cast<int>(m->budget.size()) ? m->budget[p] : kGroupBudget;
@@ -1633,9 +1728,12 @@ int nat_mesh_autotune(nat_mesh *m, const double *d_queries, int64_t num_query_po
         seen.push_back(c.budget);
         if (int rc = consider(c)) return rc;
     }
-    // 2. where thread-per-query rows start
-    if (left_all > 0) {
+    // 2. the wave-per-query level, where thread-per-query rows start
+    if (left_all > 0 || best.budget != def.budget) {
         Policy c = best;
+        c.wave_rows = 0;
+        if (int rc = consider(c)) return rc;
+        c = best;
         c.first_level = 2;
         if (int rc = consider(c)) return rc;
     }

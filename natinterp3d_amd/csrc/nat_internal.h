@@ -175,6 +175,18 @@ hipError_t launch_grad_geom(const DevMesh &m, int nblocks, const double *d_q, co
                             int n_items, unsigned int *d_work, const Topo &topo,
                             const GradIn &g, int *ovf_list, unsigned int *ovf_count,
                             hipStream_t st);
+// wave-per-query rows (cavities beyond the group kernel): per-wave scratch
+// of wave_scratch_size() bytes for each of nblocks one-wave blocks
+size_t wave_scratch_size();
+int wave_blocks_per_cu();
+hipError_t launch_wave_rows(const DevMesh &m, char *wscr, int nblocks, const double *d_q,
+                            const int *d_items, int n_items, const unsigned *d_n_items,
+                            const int *d_start, unsigned int *d_work, const RowOut &out,
+                            int *ovf_list, unsigned int *ovf_count, hipStream_t st);
+hipError_t launch_wave_grad(const DevMesh &m, char *wscr, int nblocks, const double *d_q,
+                            const int *d_items, int n_items, const int *d_start,
+                            unsigned int *d_work, const GradIn &g, int *ovf_list,
+                            unsigned int *ovf_count, hipStream_t st);
 hipError_t launch_grad_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int level,
                             int nblocks, const double *d_q, const int *d_items, int n_items,
                             unsigned int *d_work, const GradIn &g, int *ovf_list,

@@ -1312,6 +1312,662 @@ topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict
     }
 }
 
+// ------------------------------------------------------- wave-per-query rows
+
+// Rows too big for the group kernel (hundreds to thousands of cavity tets,
+// e.g. queries inside hollow shells): one wave per query.  The search is the
+// group kernel's -- every tet the reference pops is tested once, a front at
+// a time, across the 64 lanes -- with its tables in per-wave global scratch.
+// The reference's LIFO order (natural_insertionfree.c:498-530) is replayed by
+// one lane over a compact LDS graph of the in-sphere tets only (out-of-sphere
+// tets are leaves of that search, so leaving them out keeps the order).
+// Boundary faces and neighbour slots come in the reference's order by ballot
+// compaction (natural_insertionfree.c:559-584).  The forward pass computes
+// every subcell volume in parallel, groups the terms by neighbour with a
+// stable counting sort, and sums each neighbour's terms in the reference's
+// order (cavity tets, then boundary faces), so the weights are bit-identical.
+// The gradient pass reduces over the wave instead (order-free sums).  A row
+// beyond these capacities goes on to the thread-per-query levels.
+namespace wv {
+constexpr int VT_LOG2 = 13;
+constexpr int VT = 1 << VT_LOG2;  // visited-table slots
+constexpr int VMAX = 6144;        // visited tets
+constexpr int CMAX = 2048;        // cavity tets
+constexpr int SMAX = 1984;        // replay stack
+constexpr int BMAX = 8192;        // boundary faces
+constexpr int NT_LOG2 = 12;
+constexpr int NT = 1 << NT_LOG2;  // neighbour-table slots
+constexpr int NMAX = 2048;        // neighbours
+constexpr int IMAX = 4 * CMAX + 3 * BMAX;  // subcell-volume terms
+constexpr unsigned NONE = 0xffffu;
+}  // namespace wv
+
+// Per-wave global scratch.  Tables written with atomics are read with
+// device-coherent loads; the rest are written and read by plain accesses
+// separated by barriers (one wave per block).
+struct WaveScr {
+    int *vkey;             // [VT] tet, -1 empty
+    uint2 *vnb;            // [VT] visited slots of the 4 neighbours, 16 bits each
+    unsigned char *vflag;  // [VT] 1 tested, 3 tested and in the circumsphere
+    int *vlist;            // [VMAX] visited slots, in test order
+    int *vtet;             // [VMAX] their tets
+    int *dense;            // [VT] in-sphere index of a visited slot
+    int *dslot;            // [CMAX] visited slot of an in-sphere index
+    int *dtet;             // [CMAX] tet of an in-sphere index
+    int *cav;              // [CMAX] in-sphere index at each cavity position
+    int *pos;              // [CMAX] cavity position of each in-sphere index
+    uint2 *cslot;          // [CMAX] neighbour slots of the cavity tet's 4 vertices
+    int *cmask;            // [CMAX] its face-circumcentre mask
+    int *bnd;              // [BMAX] cavity position << 2 | face
+    int *nkey;             // [NT] vertex, -1 empty
+    int *nmin;             // [NT] first appearance (face-vertex sequence number)
+    int *nslot;            // [NT] neighbour slot
+    int *nbv;              // [NMAX] neighbours in slot order
+    int *off;              // [NMAX] first term of each neighbour in val
+    int *cnt;              // [NMAX] terms of each neighbour
+    int *tpos;             // [IMAX] position of each term in val
+    double *val;           // [IMAX] term volumes (step 4 negated) grouped by neighbour,
+                           //        each group in sequence order
+    double *acc;           // [NMAX] stolen volumes; <dL/dy, values[j]> in the gradient pass
+};
+
+__host__ __device__ constexpr size_t wv_round(size_t b) { return (b + 255) & ~size_t(255); }
+
+__host__ __device__ inline size_t wave_scratch_bytes() {
+    using namespace wv;
+    return wv_round(4 * VT) + wv_round(8 * VT) + wv_round(VT) + 2 * wv_round(4 * VMAX) +
+           wv_round(4 * VT) + 4 * wv_round(4 * CMAX) + wv_round(8 * CMAX) + wv_round(4 * CMAX) +
+           wv_round(4 * BMAX) + 3 * wv_round(4 * NT) + 3 * wv_round(4 * NMAX) +
+           wv_round(4 * IMAX) + wv_round(8 * IMAX) + wv_round(8 * NMAX);
+}
+
+__device__ inline WaveScr wave_carve(char *p) {
+    using namespace wv;
+    WaveScr w;
+    auto take = [&](size_t b) {
+        char *r = p;
+        p += wv_round(b);
+        return r;
+    };
+    w.vkey = reinterpret_cast<int *>(take(4 * VT));
+    w.vnb = reinterpret_cast<uint2 *>(take(8 * VT));
+    w.vflag = reinterpret_cast<unsigned char *>(take(VT));
+    w.vlist = reinterpret_cast<int *>(take(4 * VMAX));
+    w.vtet = reinterpret_cast<int *>(take(4 * VMAX));
+    w.dense = reinterpret_cast<int *>(take(4 * VT));
+    w.dslot = reinterpret_cast<int *>(take(4 * CMAX));
+    w.dtet = reinterpret_cast<int *>(take(4 * CMAX));
+    w.cav = reinterpret_cast<int *>(take(4 * CMAX));
+    w.pos = reinterpret_cast<int *>(take(4 * CMAX));
+    w.cslot = reinterpret_cast<uint2 *>(take(8 * CMAX));
+    w.cmask = reinterpret_cast<int *>(take(4 * CMAX));
+    w.bnd = reinterpret_cast<int *>(take(4 * BMAX));
+    w.nkey = reinterpret_cast<int *>(take(4 * NT));
+    w.nmin = reinterpret_cast<int *>(take(4 * NT));
+    w.nslot = reinterpret_cast<int *>(take(4 * NT));
+    w.nbv = reinterpret_cast<int *>(take(4 * NMAX));
+    w.off = reinterpret_cast<int *>(take(4 * NMAX));
+    w.cnt = reinterpret_cast<int *>(take(4 * NMAX));
+    w.tpos = reinterpret_cast<int *>(take(4 * IMAX));
+    w.val = reinterpret_cast<double *>(take(8 * IMAX));
+    w.acc = reinterpret_cast<double *>(take(8 * NMAX));
+    return w;
+}
+
+__device__ __forceinline__ int ld_coh(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// all of the wave's memory accesses so far are complete and visible to it
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+// 16-bit field k of v (a shift of the 64-bit value: a select between the two
+// halves with a run-time k would go through private memory)
+__device__ __forceinline__ unsigned u16_of(const uint2 &v, int k) {
+    const unsigned long long x = (static_cast<unsigned long long>(v.y) << 32) | v.x;
+    return static_cast<unsigned>(x >> (16 * k)) & 0xffffu;
+}
+__device__ __forceinline__ uint2 pack_u16(unsigned a, unsigned b, unsigned c, unsigned d) {
+    return make_uint2(a | (b << 16), c | (d << 16));
+}
+
+// visited-table insert of tet t: its slot (isnew when this call put it
+// there), -1 when the table is full
+__device__ __forceinline__ int wv_probe(int *vkey, int t, bool &isnew) {
+    unsigned h = hash_key(t, wv::VT_LOG2);
+    for (int n = 0; n < wv::VT; n++) {
+        const int old = atomicCAS(&vkey[h], -1, t);
+        if (old == -1 || old == t) {
+            isnew = old == -1;
+            return static_cast<int>(h);
+        }
+        h = (h + 1) & (wv::VT - 1);
+    }
+    isnew = false;
+    return -1;
+}
+
+__device__ __forceinline__ int wv_nb_insert(int *nkey, int v) {
+    unsigned h = hash_key(v, wv::NT_LOG2);
+    for (int n = 0; n < wv::NT; n++) {
+        const int old = atomicCAS(&nkey[h], -1, v);
+        if (old == -1 || old == v) return static_cast<int>(h);
+        h = (h + 1) & (wv::NT - 1);
+    }
+    return -1;
+}
+
+__device__ __forceinline__ int wv_nb_find(const int *nkey, int v) {
+    unsigned h = hash_key(v, wv::NT_LOG2);
+    for (int n = 0; n < wv::NT; n++) {
+        const int k = ld_coh(&nkey[h]);
+        if (k == v) return static_cast<int>(h);
+        if (k == -1) return -1;
+        h = (h + 1) & (wv::NT - 1);
+    }
+    return -1;
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                                static_cast<unsigned int>(lo));
+}
+
+__device__ __forceinline__ double wave_sum_f64(double x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+    return x;
+}
+
+// 20 KB: two waves per SIMD
+struct WaveLds {
+    uint2 adj[wv::CMAX];           // in-sphere index of the 4 neighbours (NONE: outside);
+                                   // later the neighbours' fill positions
+    unsigned short stk[wv::SMAX];  // replay stack
+};
+static_assert(sizeof(uint2) * wv::CMAX >= sizeof(int) * wv::NMAX,
+              "the fill positions reuse the adjacency");
+
+// Topology of one query (phases B-D): on success returns true with C cavity
+// tets (W.dtet[L.cav[ci]], W.cslot, W.cmask), B boundary faces (W.bnd) and NB
+// neighbours (W.nbv) in the reference's order.
+__device__ bool wave_topology(const DevMesh &m, const WaveScr &W, WaveLds &L, int st,
+                              const d3 &qq, int &C, int &B, int &NB) {
+    using namespace wv;
+    const int lane = __lane_id();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const int N = m.n_data;
+    for (int i = lane; i < VT; i += 64) st_coh(&W.vkey[i], -1);
+    for (int i = lane; i < NT; i += 64) {
+        st_coh(&W.nkey[i], -1);
+        st_coh(&W.nmin[i], 0x7fffffff);
+    }
+    wave_sync();
+    // ---- phase B: fronts of tested tets; the start tet and its neighbours first
+    int n0 = 0, s0 = -1;
+    if (lane == 0) {
+        bool isnew;
+        s0 = wv_probe(W.vkey, st, isnew);
+        W.vlist[0] = s0;
+        W.vtet[0] = st;
+        n0 = s0 < 0 ? -1 : 1;
+        const int4 tn0 = m.tet_n[st];
+        for (int k = 0; k < 4 && n0 > 0; k++) {
+            const int nb = sel4(tn0, k);
+            if (nb < 0) continue;
+            const int s2 = wv_probe(W.vkey, nb, isnew);
+            if (s2 < 0) {
+                n0 = -1;
+                break;
+            }
+            if (isnew) {
+                W.vlist[n0] = s2;
+                W.vtet[n0] = nb;
+                n0++;
+            }
+        }
+    }
+    n0 = __shfl(n0, 0, 64);
+    s0 = __shfl(s0, 0, 64);
+    if (n0 < 0) return false;
+    wave_sync();
+    int f0 = 0, f1 = n0;
+    bool fail = false;
+    while (f0 < f1) {
+        int nnext = 0;
+        for (int i0 = f0; i0 < f1; i0 += 64) {
+            const int i = i0 + lane;
+            int nnew = 0;
+            int nbt[4] = {-1, -1, -1, -1};                 // neighbour tets
+            unsigned ns[4] = {NONE, NONE, NONE, NONE};     // their visited slots
+            bool isn[4] = {false, false, false, false};    // inserted by this lane
+            bool full = false;
+            if (i < f1) {
+                const int slot = W.vlist[i];
+                const int t = W.vtet[i];
+                bool neg;
+                const int4 tv = ld_tet_v(m, t, neg);
+                const int4 tn = m.tet_n[t];
+                const d3 p0 = ld_vert(m, tv.x), p1 = ld_vert(m, tv.y);
+                const d3 p2 = ld_vert(m, tv.z), p3 = ld_vert(m, tv.w);
+                const double ins =
+                    neg ? insphere(p1, p0, p2, p3, qq) : insphere(p0, p1, p2, p3, qq);
+                if (ins > 0) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int nb = sel4(tn, k);
+                        if (nb < 0) continue;
+                        bool isnew;
+                        const int s2 = wv_probe(W.vkey, nb, isnew);
+                        if (s2 < 0) {
+                            full = true;
+                            continue;
+                        }
+                        ns[k] = static_cast<unsigned>(s2);
+                        nbt[k] = nb;
+                        isn[k] = isnew;
+                        nnew += isnew ? 1 : 0;
+                    }
+                }
+                W.vnb[slot] = pack_u16(ns[0], ns[1], ns[2], ns[3]);
+                W.vflag[slot] = ins > 0 ? 3 : 1;
+            }
+            int total;
+            const int pre = wave_excl_scan(nnew, total);
+            int p = f1 + nnext + pre;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (!isn[k]) continue;
+                if (p < VMAX) {
+                    W.vlist[p] = static_cast<int>(ns[k]);
+                    W.vtet[p] = nbt[k];
+                } else {
+                    full = true;
+                }
+                p++;
+            }
+            if (__ballot(full)) fail = true;
+            nnext += total;
+        }
+        wave_sync();
+        if (fail || f1 + nnext > VMAX) return false;
+        f0 = f1;
+        f1 += nnext;
+    }
+    const int V = f1;
+    // ---- in-sphere tets: dense indices and their adjacency
+    if (W.vflag[s0] != 3) return false;  // the start tet tested outside
+    int D = 0;
+    for (int i0 = 0; i0 < V; i0 += 64) {
+        const int i = i0 + lane;
+        int slot = -1;
+        bool in = false;
+        if (i < V) {
+            slot = W.vlist[i];
+            in = W.vflag[slot] == 3;
+        }
+        const unsigned long long mk = __ballot(in);
+        if (in) {
+            const int d = D + __popcll(mk & lt);
+            if (d < CMAX) {
+                W.dense[slot] = d;
+                W.dslot[d] = slot;
+                W.dtet[d] = W.vtet[i];
+            }
+        }
+        D += __popcll(mk);
+    }
+    if (D > CMAX) return false;
+    wave_sync();
+    for (int d = lane; d < D; d += 64) {
+        const uint2 nb = W.vnb[W.dslot[d]];
+        unsigned a[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const unsigned ns = u16_of(nb, k);
+            a[k] = (ns != NONE && W.vflag[ns] == 3) ? static_cast<unsigned>(W.dense[ns]) : NONE;
+        }
+        L.adj[d] = pack_u16(a[0], a[1], a[2], a[3]);
+    }
+    const int d0 = __builtin_amdgcn_readfirstlane(W.dense[s0]);
+    __syncthreads();
+    // ---- phase C: the reference's cavity order, run by the whole wave on
+    // uniform values.  Popped bits live in registers (lane l holds indices
+    // 32 l .. 32 l + 31).  Of the children a tet pushes, the last is popped
+    // next, so it is taken directly instead of through the stack.
+    unsigned popped = 0u;
+    int nc = 0, top = 0, d = d0;
+    bool ok = true;
+    for (;;) {
+        const unsigned w = __builtin_amdgcn_readlane(popped, d >> 5);
+        if (!((w >> (d & 31)) & 1u)) {
+            if (lane == (d >> 5)) popped |= 1u << (d & 31);
+            if (lane == 0) {
+                W.cav[nc] = d;
+                W.pos[d] = nc;
+            }
+            nc++;
+            const uint2 a = L.adj[d];
+            const unsigned ax = __builtin_amdgcn_readfirstlane(a.x);
+            const unsigned ay = __builtin_amdgcn_readfirstlane(a.y);
+            int last = -1;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const unsigned n = ((k < 2 ? ax : ay) >> (16 * (k & 1))) & 0xffffu;
+                if (n == NONE) continue;
+                const unsigned wn = __builtin_amdgcn_readlane(popped, static_cast<int>(n >> 5));
+                if ((wn >> (n & 31)) & 1u) continue;
+                if (last >= 0) {
+                    if (top >= SMAX) {
+                        ok = false;
+                        break;
+                    }
+                    if (lane == 0) L.stk[top] = static_cast<unsigned short>(last);
+                    top++;
+                }
+                last = static_cast<int>(n);
+            }
+            if (!ok) break;
+            if (last >= 0) {
+                d = last;
+                continue;
+            }
+        }
+        if (top == 0) break;
+        top--;
+        d = __builtin_amdgcn_readfirstlane(static_cast<int>(L.stk[top]));
+    }
+    __syncthreads();
+    // every in-sphere tet is reachable through in-sphere tets: nc < D only
+    // after a stack overflow
+    if (!ok || nc != D) return false;
+    C = D;
+    // ---- phase D: boundary faces in (cavity position, face) order
+    int nb_ = 0;
+    for (int j0 = 0; j0 < 4 * C; j0 += 64) {
+        const int j = j0 + lane;
+        bool isb = false;
+        if (j < 4 * C) isb = u16_of(L.adj[W.cav[j >> 2]], j & 3) == NONE;
+        const unsigned long long mk = __ballot(isb);
+        if (isb) {
+            const int p = nb_ + __popcll(mk & lt);
+            if (p < BMAX) W.bnd[p] = j;
+        }
+        nb_ += __popcll(mk);
+    }
+    if (nb_ > BMAX) return false;
+    B = nb_;
+    wave_sync();
+    // neighbours: least face-vertex sequence number of each data vertex
+    const int nent = 3 * B;
+    auto face_v = [&](int e) {
+        const int code = W.bnd[e / 3];
+        bool neg;
+        const int4 tv = ld_tet_v(m, W.dtet[W.cav[code >> 2]], neg);
+        return sel4(tv, face_slot(code & 3, e % 3));
+    };
+    bool full = false;
+    for (int e = lane; e < nent; e += 64) {
+        const int v = face_v(e);
+        if (v >= N) continue;
+        const int h = wv_nb_insert(W.nkey, v);
+        if (h < 0) {
+            full = true;
+            continue;
+        }
+        atomicMin(&W.nmin[h], e);
+    }
+    if (__ballot(full)) return false;
+    wave_sync();
+    // slots in order of first appearance
+    int nn = 0;
+    for (int e0 = 0; e0 < nent; e0 += 64) {
+        const int e = e0 + lane;
+        bool first = false;
+        int v = -1, h = -1;
+        if (e < nent) {
+            v = face_v(e);
+            if (v < N) {
+                h = wv_nb_find(W.nkey, v);
+                first = h >= 0 && ld_coh(&W.nmin[h]) == e;
+            }
+        }
+        const unsigned long long mk = __ballot(first);
+        if (first) {
+            const int slot = nn + __popcll(mk & lt);
+            W.nslot[h] = slot;
+            if (slot < NMAX) W.nbv[slot] = v;
+        }
+        nn += __popcll(mk);
+    }
+    if (nn > NMAX || nn == 0) return false;
+    NB = nn;
+    wave_sync();
+    // per cavity tet: neighbour slots of its vertices and the circumcentre mask
+    // (face opposite p[k] held by an earlier cavity tet)
+    for (int ci = lane; ci < C; ci += 64) {
+        const int d = W.cav[ci];
+        bool neg;
+        const int4 tv = ld_tet_v(m, W.dtet[d], neg);
+        unsigned sl[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int v = sel4(tv, k);
+            const int h = v < N ? wv_nb_find(W.nkey, v) : -1;
+            sl[k] = h < 0 ? NONE : static_cast<unsigned>(W.nslot[h]);
+        }
+        W.cslot[ci] = pack_u16(sl[0], sl[1], sl[2], sl[3]);
+        const uint2 a = L.adj[d];
+        int mask = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const unsigned n = u16_of(a, 3 - k);
+            if (n != NONE && W.pos[n] < ci) mask |= 1 << k;
+        }
+        W.cmask[ci] = mask;
+    }
+    wave_sync();
+    return true;
+}
+
+// Forward (GRAD = false): the row into the pool, bit-identical to the
+// reference.  Gradient: dL/dq from the same cavity (rows left empty by the
+// reference get 0).  Queries that exceed a capacity go to ovf_list.
+template <bool GRAD>
+__global__ void __launch_bounds__(64) wave_rows_kernel(DevMesh m, char *wscr,
+                                                       const double *__restrict__ q,
+                                                       const int *__restrict__ items, int n_items,
+                                                       const unsigned *n_items_dev,
+                                                       const int *__restrict__ start_of,
+                                                       unsigned int *work, RowOut out, GradIn g,
+                                                       int *ovf_list, unsigned int *ovf_count) {
+    using namespace wv;
+    __shared__ WaveLds L;
+    const int lane = __lane_id();
+    const WaveScr W = wave_carve(wscr + static_cast<size_t>(blockIdx.x) * wave_scratch_bytes());
+    if (n_items_dev) n_items = static_cast<int>(*n_items_dev);
+    for (;;) {
+        unsigned it = 0;
+        if (lane == 0) it = atomicAdd(work, 1u);
+        it = __shfl(it, 0, 64);
+        if (it >= static_cast<unsigned>(n_items)) break;
+        const int qi = items[it];
+        const int st = start_of[qi];
+        const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
+                       q[3 * static_cast<size_t>(qi) + 2]};
+        int C = 0, B = 0, NB = 0;
+        if (st < 0 || !wave_topology(m, W, L, st, qq, C, B, NB)) {
+            if (lane == 0) ovf_list[atomicAdd(ovf_count, 1u)] = qi;
+            __syncthreads();
+            continue;
+        }
+        auto slot_of_ci = [&](int ci) {
+            const uint2 cs = W.cslot[ci];
+            return [cs](int vi, int) {
+                const unsigned s = u16_of(cs, vi);
+                return s == NONE ? -1 : static_cast<int>(s);
+            };
+        };
+        if constexpr (GRAD) {
+            for (int s = lane; s < NB; s += 64) W.acc[s] = grad_dot(g, qi, W.nbv[s]);
+            wave_sync();
+            GradAcc A;
+            for (int ci = lane; ci < C; ci += 64)
+                old_contributions(m, W.dtet[W.cav[ci]], static_cast<unsigned>(W.cmask[ci]),
+                                  slot_of_ci(ci),
+                                  [&](int slot, double vol) { A.vol(W.acc[slot], vol); });
+            for (int b = lane; b < B; b += 64) {
+                const int code = W.bnd[b];
+                const int ci = code >> 2;
+                new_contributions(
+                    m, W.dtet[W.cav[ci]], code & 3, qq, slot_of_ci(ci),
+                    [&](int slot, double vol) { A.vol(W.acc[slot], -vol); },
+                    moment_fn([&](int slot, const d3 &gm) { A.mom(W.acc[slot], gm); }));
+            }
+            GradAcc T;
+            T.s = wave_sum_f64(A.s);
+            T.as = wave_sum_f64(A.as);
+            T.ag = {wave_sum_f64(A.ag.x), wave_sum_f64(A.ag.y), wave_sum_f64(A.ag.z)};
+            T.tg = {wave_sum_f64(A.tg.x), wave_sum_f64(A.tg.y), wave_sum_f64(A.tg.z)};
+            if (lane == 0) store_dq(g, qi, T.result());
+            __syncthreads();
+        } else {
+            const int I3 = 4 * C;  // step-3 terms first, then 3 per boundary face
+            const int nterm = I3 + 3 * B;
+            // slot of term i (-1: a vertex without one)
+            auto term_slot = [&](int i) {
+                unsigned u = NONE;
+                if (i < I3) {
+                    u = u16_of(W.cslot[i >> 2], i & 3);
+                } else if (i < nterm) {
+                    const int b = (i - I3) / 3, k = (i - I3) % 3;
+                    const int code = W.bnd[b];
+                    u = u16_of(W.cslot[code >> 2], face_slot(code & 3, k));
+                }
+                return u == NONE ? -1 : static_cast<int>(u);
+            };
+            // terms per neighbour (LDS counters reuse the adjacency)
+            int *pos_l = reinterpret_cast<int *>(L.adj);
+            for (int s = lane; s < NB; s += 64) pos_l[s] = 0;
+            __syncthreads();
+            for (int i = lane; i < nterm; i += 64) {
+                const int sl = term_slot(i);
+                if (sl >= 0) atomicAdd(&pos_l[sl], 1);
+            }
+            __syncthreads();
+            // exclusive scan: first position of each neighbour's terms
+            int carry = 0;
+            for (int s0 = 0; s0 < NB; s0 += 64) {
+                const int s = s0 + lane;
+                const int c = s < NB ? pos_l[s] : 0;
+                int total;
+                const int pre = wave_excl_scan(c, total);
+                if (s < NB) {
+                    W.cnt[s] = c;
+                    W.off[s] = carry + pre;
+                    pos_l[s] = carry + pre;
+                }
+                carry += total;
+            }
+            __syncthreads();
+            // stable placement: chunks of 64 terms in sequence order; the lanes
+            // of one neighbour (matched on the 11 bits of its slot) take
+            // consecutive positions in lane order
+            for (int i0 = 0; i0 < nterm; i0 += 64) {
+                const int i = i0 + lane;
+                const int sl = i < nterm ? term_slot(i) : -1;
+                unsigned long long same = __ballot(sl >= 0);
+#pragma unroll
+                for (int b = 0; b < 11; b++) {
+                    const bool bit = (sl >> b) & 1;
+                    const unsigned long long bb = __ballot(bit);
+                    same &= bit ? bb : ~bb;
+                }
+                int base = 0;
+                const int leader = sl >= 0 ? static_cast<int>(__ffsll(same)) - 1 : 0;
+                if (sl >= 0 && lane == leader) base = atomicAdd(&pos_l[sl], __popcll(same));
+                base = __shfl(base, leader, 64);
+                if (sl >= 0) W.tpos[i] = base + __popcll(same & ((1ull << lane) - 1ull));
+            }
+            wave_sync();
+            // the terms: step 3 per cavity tet, step 4 per boundary face
+            // (negated: stolen -= vol is stolen + -vol), each at its position
+            for (int ci = lane; ci < C; ci += 64) {
+                const uint2 cs = W.cslot[ci];
+                old_contributions(m, W.dtet[W.cav[ci]], static_cast<unsigned>(W.cmask[ci]),
+                                  slot_of_ci(ci), [&](int slot, double vol) {
+                                      const int k = slot == static_cast<int>(u16_of(cs, 0))
+                                                        ? 0
+                                                        : slot == static_cast<int>(u16_of(cs, 1))
+                                                              ? 1
+                                                              : slot == static_cast<int>(
+                                                                            u16_of(cs, 2))
+                                                                    ? 2
+                                                                    : 3;
+                                      W.val[W.tpos[4 * ci + k]] = vol;
+                                  });
+            }
+            for (int b = lane; b < B; b += 64) {
+                const int code = W.bnd[b];
+                const int ci = code >> 2, fi = code & 3;
+                const uint2 cs = W.cslot[ci];
+                new_contributions(m, W.dtet[W.cav[ci]], fi, qq, slot_of_ci(ci),
+                                  [&](int slot, double vol) {
+                                      const int k =
+                                          slot == static_cast<int>(u16_of(cs, face_slot(fi, 0)))
+                                              ? 0
+                                              : slot == static_cast<int>(
+                                                            u16_of(cs, face_slot(fi, 1)))
+                                                    ? 1
+                                                    : 2;
+                                      W.val[W.tpos[I3 + 3 * b + k]] = -vol;
+                                  });
+            }
+            wave_sync();
+            // each neighbour's terms in the reference's order
+            for (int s = lane; s < NB; s += 64) {
+                const double *vp = W.val + W.off[s];
+                const int n = W.cnt[s];
+                double v = 0.0;
+                for (int j = 0; j < n; j++) v += vp[j];
+                W.acc[s] = v;
+            }
+            wave_sync();
+            // step 5 (natural_insertionfree.c:721-737): the total in slot order
+            double total = 0.0;
+            for (int s0 = 0; s0 < NB; s0 += 64) {
+                const double x = s0 + lane < NB ? W.acc[s0 + lane] : 0.0;
+                const int n = min(64, NB - s0);
+                for (int l = 0; l < n; l++) total += readlane_f64(x, l);
+            }
+            const int cnt = total > 1e-30 ? NB : 0;
+            const double inv = cnt ? 1.0 / total : 0.0;
+            unsigned long long base = 0;
+            if (lane == 0 && cnt > 0)
+                base = atomicAdd(out.pool_used, static_cast<unsigned long long>(cnt));
+            base = __shfl(base, 0, 64);
+            const long long off = static_cast<long long>(base);
+            if (off + cnt <= out.pool_cap) {
+                for (int s = lane; s < cnt; s += 64) {
+                    out.pool_cols[off + s] = W.nbv[s];
+                    out.pool_vals[off + s] = W.acc[s] * inv;
+                }
+            }
+            if (lane == 0) {
+                out.row_off[qi] = off;
+                out.row_cnt[qi] = cnt;
+                out.row_total[qi] = total;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // ----------------------------------------------------------- geometry kernel
 
 // Work order of the geometry kernels: the Morton-ordered items regrouped,
@@ -1690,6 +2346,36 @@ hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, in
                        hipStream_t st) {
     hipLaunchKernelGGL(sibson_rows_kernel, dim3(nblocks), dim3(256), 0, st, m, caps, sc, level,
                        d_q, d_items, n_items, d_n_items, d_work, out);
+    return hipGetLastError();
+}
+
+size_t wave_scratch_size() { return wave_scratch_bytes(); }
+
+int wave_blocks_per_cu() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, wave_rows_kernel<false>, 64, 0) !=
+        hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
+}
+
+hipError_t launch_wave_rows(const DevMesh &m, char *wscr, int nblocks, const double *d_q,
+                            const int *d_items, int n_items, const unsigned *d_n_items,
+                            const int *d_start, unsigned int *d_work, const RowOut &out,
+                            int *ovf_list, unsigned int *ovf_count, hipStream_t st) {
+    GradIn g{};
+    hipLaunchKernelGGL(wave_rows_kernel<false>, dim3(nblocks), dim3(64), 0, st, m, wscr, d_q,
+                       d_items, n_items, d_n_items, d_start, d_work, out, g, ovf_list, ovf_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_wave_grad(const DevMesh &m, char *wscr, int nblocks, const double *d_q,
+                            const int *d_items, int n_items, const int *d_start,
+                            unsigned int *d_work, const GradIn &g, int *ovf_list,
+                            unsigned int *ovf_count, hipStream_t st) {
+    RowOut out{};
+    hipLaunchKernelGGL(wave_rows_kernel<true>, dim3(nblocks), dim3(64), 0, st, m, wscr, d_q,
+                       d_items, n_items, nullptr, d_start, d_work, out, g, ovf_list, ovf_count);
     return hipGetLastError();
 }
 

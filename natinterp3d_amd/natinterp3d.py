@@ -100,7 +100,9 @@ class MeshAndVerticesHIP:
                 "rows_items": list(t.rows_items)[:n], "emit_ms": t.emit_ms,
                 "grad_ms": list(t.grad_ms), "grad_items": list(t.grad_items),
                 "mixed_fp32": t.mixed_fp32, "mixed_fp64": t.mixed_fp64,
-                "mixed_rows_fp64": t.mixed_rows_fp64}
+                "mixed_rows_fp64": t.mixed_rows_fp64, "wave_ms": t.wave_ms,
+                "wave_items": t.wave_items, "grad_wave_ms": t.grad_wave_ms,
+                "grad_wave_items": t.grad_wave_items}
 
     def set_precision(self, mode="fp64", cond_threshold=0.02, cancel_threshold=20.0):
         """Row arithmetic: "fp64" (default; weights bit-identical to the
@@ -121,8 +123,9 @@ class MeshAndVerticesHIP:
 
     def set_policy(self, **kw):
         """Change the row-path policy: budgets (list of one visited-tet budget,
-        0..96; 0 = thread-per-query rows), first_level (1 or 2), work_sort
-        (bool), topo_grid, geom_grid (1..4).  Results do not depend on it."""
+        0..96; 0 = skip the group kernel), first_level (1 or 2), work_sort
+        (bool), topo_grid, geom_grid (1..4), wave_rows (bool).  Results do
+        not depend on it."""
         set_row_policy(self._m, 1, **kw)
 
     def autotune(self, queries, repeats=3):
@@ -457,11 +460,11 @@ def get_row_policy(handle, num_problems):
                "nat_mesh_get_policy")
     return {"budgets": budgets.tolist(), "first_level": pol.first_level,
             "work_sort": bool(pol.work_sort), "topo_grid": pol.topo_grid,
-            "geom_grid": pol.geom_grid}
+            "geom_grid": pol.geom_grid, "wave_rows": bool(pol.wave_rows)}
 
 
 def set_row_policy(handle, num_problems, budgets=None, first_level=None, work_sort=None,
-                   topo_grid=None, geom_grid=None):
+                   topo_grid=None, geom_grid=None, wave_rows=None):
     """Set the given fields of a mesh handle's row-path policy."""
     cur = get_row_policy(handle, num_problems)
     pol = _lib.Policy()
@@ -469,6 +472,7 @@ def set_row_policy(handle, num_problems, budgets=None, first_level=None, work_so
     pol.work_sort = int(bool(cur["work_sort"] if work_sort is None else work_sort))
     pol.topo_grid = int(cur["topo_grid"] if topo_grid is None else topo_grid)
     pol.geom_grid = int(cur["geom_grid"] if geom_grid is None else geom_grid)
+    pol.wave_rows = int(bool(cur["wave_rows"] if wave_rows is None else wave_rows))
     b = None
     if budgets is not None:
         arr = np.asarray(budgets)

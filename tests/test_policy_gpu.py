@@ -40,6 +40,8 @@ def csr_equal(a, b):
 
 POLICIES = [
     dict(first_level=2),
+    dict(wave_rows=False),
+    dict(wave_rows=False, first_level=2),
     dict(work_sort=False),
     dict(topo_grid=2, geom_grid=3),
     dict(first_level=2, work_sort=False, topo_grid=4, geom_grid=2),
@@ -51,7 +53,7 @@ def test_batch_policies_give_identical_rows(oracle_mod):
     P = len(keys)
     b = natinterp3d_amd.BatchInterpolator(keys)
     assert b.policy() == {"budgets": [96] * P, "first_level": 1, "work_sort": True,
-                          "topo_grid": 1, "geom_grid": 1}
+                          "topo_grid": 1, "geom_grid": 1, "wave_rows": True}
     ref = b.get_weights(queries)
     for p in (0, P - 1):  # a box and a shell against the oracle
         single = natinterp3d_amd.Interpolator(keys[p])
