@@ -1368,7 +1368,7 @@ struct WaveScr {
     int *tpos;             // [IMAX] position of each term in val
     double *val;           // [IMAX] term volumes (step 4 negated) grouped by neighbour,
                            //        each group in sequence order
-    double *acc;           // [NMAX] stolen volumes; <dL/dy, values[j]> in the gradient pass
+    double *acc;           // [NT] stolen volumes; <dL/dy, values[j]> in the gradient pass
 };
 
 __host__ __device__ constexpr size_t wv_round(size_t b) { return (b + 255) & ~size_t(255); }
@@ -1378,7 +1378,7 @@ __host__ __device__ inline size_t wave_scratch_bytes() {
     return wv_round(4 * VT) + wv_round(8 * VT) + wv_round(VT) + 2 * wv_round(4 * VMAX) +
            wv_round(4 * VT) + 4 * wv_round(4 * CMAX) + wv_round(8 * CMAX) + wv_round(4 * CMAX) +
            wv_round(4 * BMAX) + 3 * wv_round(4 * NT) + 3 * wv_round(4 * NMAX) +
-           wv_round(4 * IMAX) + wv_round(8 * IMAX) + wv_round(8 * NMAX);
+           wv_round(4 * IMAX) + wv_round(8 * IMAX) + wv_round(8 * NT);
 }
 
 __device__ inline WaveScr wave_carve(char *p) {
@@ -1410,7 +1410,7 @@ __device__ inline WaveScr wave_carve(char *p) {
     w.cnt = reinterpret_cast<int *>(take(4 * NMAX));
     w.tpos = reinterpret_cast<int *>(take(4 * IMAX));
     w.val = reinterpret_cast<double *>(take(8 * IMAX));
-    w.acc = reinterpret_cast<double *>(take(8 * NMAX));
+    w.acc = reinterpret_cast<double *>(take(8 * NT));
     return w;
 }
 
@@ -1494,8 +1494,11 @@ static_assert(sizeof(uint2) * wv::CMAX >= sizeof(int) * wv::NMAX,
               "the fill positions reuse the adjacency");
 
 // Topology of one query (phases B-D): on success returns true with C cavity
-// tets (W.dtet[L.cav[ci]], W.cslot, W.cmask), B boundary faces (W.bnd) and NB
-// neighbours (W.nbv) in the reference's order.
+// tets (W.dtet[W.cav[ci]], W.cslot, W.cmask), B boundary faces (W.bnd) and NB
+// neighbours (W.nbv), all in the reference's order.  ORDERED = false (the
+// gradient, whose sums do not depend on order): the cavity in search order,
+// and neighbour slots are the neighbour table's own slots (NB = its size).
+template <bool ORDERED>
 __device__ bool wave_topology(const DevMesh &m, const WaveScr &W, WaveLds &L, int st,
                               const d3 &qq, int &C, int &B, int &NB) {
     using namespace wv;
@@ -1636,6 +1639,13 @@ __device__ bool wave_topology(const DevMesh &m, const WaveScr &W, WaveLds &L, in
     }
     const int d0 = __builtin_amdgcn_readfirstlane(W.dense[s0]);
     __syncthreads();
+    if constexpr (!ORDERED) {
+        for (int d = lane; d < D; d += 64) {
+            W.cav[d] = d;
+            W.pos[d] = d;
+        }
+        (void)d0;
+    } else {
     // ---- phase C: the reference's cavity order, run by the whole wave on
     // uniform values.  Popped bits live in registers (lane l holds indices
     // 32 l .. 32 l + 31).  Of the children a tet pushes, the last is popped
@@ -1686,6 +1696,7 @@ __device__ bool wave_topology(const DevMesh &m, const WaveScr &W, WaveLds &L, in
     // every in-sphere tet is reachable through in-sphere tets: nc < D only
     // after a stack overflow
     if (!ok || nc != D) return false;
+    }
     C = D;
     // ---- phase D: boundary faces in (cavity position, face) order
     int nb_ = 0;
@@ -1720,10 +1731,13 @@ __device__ bool wave_topology(const DevMesh &m, const WaveScr &W, WaveLds &L, in
             full = true;
             continue;
         }
-        atomicMin(&W.nmin[h], e);
+        if constexpr (ORDERED) atomicMin(&W.nmin[h], e);
     }
     if (__ballot(full)) return false;
     wave_sync();
+    if constexpr (!ORDERED) {
+        NB = NT;
+    } else {
     // slots in order of first appearance
     int nn = 0;
     for (int e0 = 0; e0 < nent; e0 += 64) {
@@ -1748,6 +1762,7 @@ __device__ bool wave_topology(const DevMesh &m, const WaveScr &W, WaveLds &L, in
     if (nn > NMAX || nn == 0) return false;
     NB = nn;
     wave_sync();
+    }
     // per cavity tet: neighbour slots of its vertices and the circumcentre mask
     // (face opposite p[k] held by an earlier cavity tet)
     for (int ci = lane; ci < C; ci += 64) {
@@ -1759,7 +1774,8 @@ __device__ bool wave_topology(const DevMesh &m, const WaveScr &W, WaveLds &L, in
         for (int k = 0; k < 4; k++) {
             const int v = sel4(tv, k);
             const int h = v < N ? wv_nb_find(W.nkey, v) : -1;
-            sl[k] = h < 0 ? NONE : static_cast<unsigned>(W.nslot[h]);
+            sl[k] = h < 0 ? NONE : ORDERED ? static_cast<unsigned>(W.nslot[h])
+                                           : static_cast<unsigned>(h);
         }
         W.cslot[ci] = pack_u16(sl[0], sl[1], sl[2], sl[3]);
         const uint2 a = L.adj[d];
@@ -1801,7 +1817,7 @@ __global__ void __launch_bounds__(64) wave_rows_kernel(DevMesh m, char *wscr,
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
         int C = 0, B = 0, NB = 0;
-        if (st < 0 || !wave_topology(m, W, L, st, qq, C, B, NB)) {
+        if (st < 0 || !wave_topology<!GRAD>(m, W, L, st, qq, C, B, NB)) {
             if (lane == 0) ovf_list[atomicAdd(ovf_count, 1u)] = qi;
             __syncthreads();
             continue;
@@ -1814,7 +1830,11 @@ __global__ void __launch_bounds__(64) wave_rows_kernel(DevMesh m, char *wscr,
             };
         };
         if constexpr (GRAD) {
-            for (int s = lane; s < NB; s += 64) W.acc[s] = grad_dot(g, qi, W.nbv[s]);
+            // a_j per neighbour-table slot
+            for (int h = lane; h < NT; h += 64) {
+                const int v = ld_coh(&W.nkey[h]);
+                if (v >= 0) W.acc[h] = grad_dot(g, qi, v);
+            }
             wave_sync();
             GradAcc A;
             for (int ci = lane; ci < C; ci += 64)
