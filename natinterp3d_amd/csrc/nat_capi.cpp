@@ -555,10 +555,11 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
             HIP_OR_FAIL(m->rec.ensure(sizeof(int4) * want_rec));
             m->rec_cap = want_rec;
         }
-        // counters: [0..3] fused-level work, [4] topology work, [5] geometry
-        // work, [6] wave-level work, [8..11] overflow counts, [12] wave-level
-        // overflow, u64 pool use at [16], records at [18], mixed-path FP32 /
-        // FP64 counts at [20], [22], rows redone at [24]
+        // counters: [0..3] fused-level work, [6] wave-level work, [8..11]
+        // overflow counts, [12] wave-level overflow, u64 pool use at [16],
+        // records at [18], mixed-path FP32 / FP64 counts at [20], [22], rows
+        // redone at [24]; per-XCD work parts of the topology kernel at
+        // [32..39], of the geometry kernel at [40..47]
         unsigned *ctr = m->counters.as<unsigned>();
         unsigned long long *pool_used = reinterpret_cast<unsigned long long *>(ctr + 16);
         unsigned long long *rec_used = reinterpret_cast<unsigned long long *>(ctr + 18);
@@ -587,7 +588,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         m->times.locate_ms = elapsed(m, 2, 3);
         mark(m, 2, st);
         HIP_OR_FAIL(nat::launch_topo(m->dev, m->topo_blocks * m->topo_grid, d_q,
-                                     m->order.as<int>(), M, m->start.as<int>(), ctr + 4, topo,
+                                     m->order.as<int>(), M, m->start.as<int>(), ctr + 32, topo,
                                      ctr + 8, m->ovf[0].as<int>(), st));
         mark(m, 3, st);
         unsigned long long meta[2] = {0, 0};
@@ -609,7 +610,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
                                               m->geom_order.as<int>(), st));
         HIP_OR_FAIL(nat::launch_geom(
             m->dev, m->nthr_geom / nat::kGeomThreads * m->geom_grid, d_q,
-            (m->work_sort ? m->geom_order : m->order).as<int>(), M, ctr + 5,
+            (m->work_sort ? m->geom_order : m->order).as<int>(), M, ctr + 40,
             topo, out, m->mixed_tau, m->mixed_kappa,
             m->precision == 1 ? reinterpret_cast<unsigned long long *>(ctr + 20) : nullptr, st));
         mark(m, 3, st);
@@ -722,8 +723,9 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
     HIP_OR_FAIL(m->ovf[0].ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->ovf[1].ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->counters.ensure(256));
-    // counters: [0] record work, [1..3] fused-level work, [6] wave-level work,
-    // [8..11] overflow counts, [12] wave-level overflow
+    // counters: [1..3] fused-level work, [6] wave-level work, [8..11]
+    // overflow counts, [12] wave-level overflow, [32..39] per-XCD work parts
+    // of the record kernel
     unsigned *ctr = m->counters.as<unsigned>();
     HIP_OR_FAIL(hipMemsetAsync(m->counters.p, 0, 256, st));
     nat::Topo topo;
@@ -740,7 +742,7 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
     }
     mark(m, 2, st);
     HIP_OR_FAIL(nat::launch_grad_geom(m->dev, m->nthr_geom / nat::kGeomThreads, d_q, order, M,
-                                      ctr, topo, g, m->ovf[0].as<int>(), ctr + 8, st));
+                                      ctr + 32, topo, g, m->ovf[0].as<int>(), ctr + 8, st));
     mark(m, 3, st);
     unsigned novf = 0;
     HIP_OR_FAIL(hipMemcpyAsync(&novf, ctr + 8, sizeof(unsigned), hipMemcpyDeviceToHost, st));
@@ -895,14 +897,14 @@ int issue_graph_rows(nat_mesh *m, nat_graph *g, const double *d_q, hipStream_t s
     HIP_OR_FAIL(nat::launch_locate(m->dev, m->caps[0], sc0, m->scap[0] / 256, d_q,
                                    m->order.as<int>(), M, m->start.as<int>(), st));
     HIP_OR_FAIL(nat::launch_topo(m->dev, m->topo_blocks * m->topo_grid, d_q, m->order.as<int>(),
-                                 M, m->start.as<int>(), ctr + 4, topo, ctr + 8,
+                                 M, m->start.as<int>(), ctr + 32, topo, ctr + 8,
                                  m->ovf[0].as<int>(), st));
     if (m->work_sort)
         HIP_OR_FAIL(
             nat::launch_work_sort(m->order.as<int>(), M, topo, m->geom_order.as<int>(), st));
     HIP_OR_FAIL(nat::launch_geom(
         m->dev, m->nthr_geom / nat::kGeomThreads * m->geom_grid, d_q,
-        (m->work_sort ? m->geom_order : m->order).as<int>(), M, ctr + 5, topo,
+        (m->work_sort ? m->geom_order : m->order).as<int>(), M, ctr + 40, topo,
         out, m->mixed_tau, m->mixed_kappa,
         m->precision == 1 ? reinterpret_cast<unsigned long long *>(ctr + 20) : nullptr, st));
     // the larger levels read their item counts from the device

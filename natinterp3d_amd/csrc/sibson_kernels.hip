@@ -678,6 +678,33 @@ __device__ __forceinline__ long long wave_reserve(unsigned long long *counter, i
     return static_cast<long long>(base) + pre;
 }
 
+// Work split by XCD: items [0, n) in kParts contiguous parts, part p with its
+// own counter ctr[p].  Workgroups are dispatched to the 8 XCDs round-robin,
+// so the blocks of one part share an L2 and a compact run of the (Morton)
+// item order, and the mesh data of the queries in flight on an XCD fits its
+// L2.  A wave drains its own part, then helps the others; k counts the parts
+// it found exhausted.  Returns false when all are; else the batch's first
+// item and size (<= batch).
+constexpr int kParts = 8;
+__device__ __forceinline__ bool part_batch(unsigned int *ctr, int n, int batch, int &k,
+                                           int &first, int &cnt) {
+    while (k < kParts) {
+        const int p = static_cast<int>((blockIdx.x + static_cast<unsigned>(k)) % kParts);
+        const int lo = static_cast<int>(static_cast<long long>(n) * p / kParts);
+        const int hi = static_cast<int>(static_cast<long long>(n) * (p + 1) / kParts);
+        unsigned b = 0;
+        if (__lane_id() == 0) b = atomicAdd(&ctr[p], static_cast<unsigned>(batch));
+        b = __shfl(b, 0, 64);
+        if (static_cast<long long>(lo) + b < hi) {
+            first = lo + static_cast<int>(b);
+            cnt = min(batch, hi - first);
+            return true;
+        }
+        k++;
+    }
+    return false;
+}
+
 // The wave's next 64 work items; returns false when the list is exhausted.
 __device__ __forceinline__ bool next_batch(unsigned int *work, int n_items, int &item) {
     unsigned base = 0;
@@ -1030,13 +1057,10 @@ topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict
     const int gw = lane >> 4;
     GroupMem &g = gmem[threadIdx.x >> 4];
     const int N = m.n_data;
-    for (;;) {
-        unsigned base = 0;
-        if (lane == 0) base = atomicAdd(work, 4u);
-        base = __shfl(base, 0, 64);
-        if (base >= static_cast<unsigned>(n_items)) break;
-        const int item = static_cast<int>(base) + gw;
-        const bool active = item < n_items;
+    int parts_done = 0, first = 0, cnt = 0;
+    while (part_batch(work, n_items, 4, parts_done, first, cnt)) {
+        const int item = first + gw;
+        const bool active = gw < cnt;
         const int qi = active ? items[item] : -1;
         int st = -1;
         d3 qq = {0, 0, 0};
@@ -2058,11 +2082,12 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
     __shared__ double acc_lds[kLevel0Row * kGeomThreads];
     double *acc = acc_lds + threadIdx.x;
     unsigned n32 = 0, n64 = 0, nredo = 0;
-    int item;
-    while (next_batch(work, n_items, item)) {
+    int parts_done = 0, first = 0, nbatch = 0;
+    while (part_batch(work, n_items, 64, parts_done, first, nbatch)) {
+        const int item = first + __lane_id();
         int qi = -1, cnt = 0, nnb = 0;
         long long roff = -1;
-        if (item < n_items) {
+        if (__lane_id() < nbatch) {
             qi = items ? items[item] : item;
             roff = topo.rec_off[qi];
         }
@@ -2186,10 +2211,10 @@ __global__ void __launch_bounds__(kGeomThreads) grad_geom_kernel(
     unsigned int *work, Topo topo, GradIn g, int *ovf_list, unsigned int *ovf_count) {
     __shared__ double a_lds[kLevel0Row * kGeomThreads];
     double *nb_a = a_lds + threadIdx.x;
-    int item;
-    while (next_batch(work, n_items, item)) {
-        if (item >= n_items) continue;
-        const int qi = items ? items[item] : item;
+    int parts_done = 0, first = 0, nbatch = 0;
+    while (part_batch(work, n_items, 64, parts_done, first, nbatch)) {
+        if (__lane_id() >= nbatch) continue;
+        const int qi = items ? items[first + __lane_id()] : first + __lane_id();
         const long long roff = topo.rec_off[qi];
         if (roff < 0) {
             ovf_list[atomicAdd(ovf_count, 1u)] = qi;
