@@ -2320,6 +2320,58 @@ __global__ void __launch_bounds__(256) spmm_t_kernel(int M, int D, const int *__
         atomicAdd(out + static_cast<size_t>(cols[e]) * D + d, vals[e] * gv);
 }
 
+// W^T g with per-block aggregation: a block takes 256 consecutive rows of
+// `order` (neighbouring queries, which share most of their neighbours), sums
+// their contributions per column in an LDS hash table of 2^hlog2 columns x D,
+// and adds each distinct column to out once, with float64 atomics.  An entry
+// whose column finds no table slot within 16 probes goes to out directly.
+__global__ void __launch_bounds__(256) spmm_t_agg_kernel(int M, int D, int hlog2,
+                                                         const int *__restrict__ rowptr,
+                                                         const int *__restrict__ cols,
+                                                         const double *__restrict__ vals,
+                                                         const double *__restrict__ gy,
+                                                         const int *__restrict__ order,
+                                                         double *__restrict__ out) {
+    extern __shared__ double agg_lds[];
+    const int H = 1 << hlog2;
+    double *hv = agg_lds;                                // [H * D]
+    int *hk = reinterpret_cast<int *>(agg_lds + static_cast<size_t>(H) * D);  // [H]
+    for (int i = threadIdx.x; i < H; i += blockDim.x) hk[i] = -1;
+    for (int i = threadIdx.x; i < H * D; i += blockDim.x) hv[i] = 0.0;
+    __syncthreads();
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < M) {
+        const int row = order ? order[i] : static_cast<int>(i);
+        const double *g = gy + static_cast<size_t>(row) * D;
+        for (int e = rowptr[row]; e < rowptr[row + 1]; e++) {
+            const int c = cols[e];
+            const double w = vals[e];
+            unsigned h = hash_key(c, hlog2);
+            int slot = -1;
+            for (int n = 0; n < 16; n++) {
+                const int old = atomicCAS(&hk[h], -1, c);
+                if (old == -1 || old == c) {
+                    slot = static_cast<int>(h);
+                    break;
+                }
+                h = (h + 1) & (H - 1);
+            }
+            if (slot >= 0) {
+                for (int d = 0; d < D; d++) atomicAdd(&hv[slot * D + d], w * g[d]);
+            } else {
+                for (int d = 0; d < D; d++) atomicAdd(out + static_cast<size_t>(c) * D + d, w * g[d]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int sl = threadIdx.x; sl < H; sl += blockDim.x) {
+        const int c = hk[sl];
+        if (c < 0) continue;
+        for (int d = 0; d < D; d++)
+            atomicAdd(out + static_cast<size_t>(c) * D + d, hv[sl * D + d]);
+    }
+}
+
 __global__ void __launch_bounds__(256) fill_zero_kernel(double *p, long long n) {
     const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t < n) p[t] = 0.0;
@@ -2495,9 +2547,20 @@ hipError_t launch_spmm_t(int M, int N, int D, const int *rowptr, const int *cols
         hipLaunchKernelGGL(fill_zero_kernel, dim3(static_cast<unsigned>((nz + 255) / 256)),
                            dim3(256), 0, st, out, nz);
     const long long nt = static_cast<long long>(M) * D;
-    if (nt > 0)
+    if (nt <= 0) return hipGetLastError();
+    // the largest power-of-two table of at most 2048 columns in 48 KB of LDS
+    int hlog2 = 11;
+    while (hlog2 > 0 && (static_cast<size_t>(1) << hlog2) * (4 + 8 * static_cast<size_t>(D)) >
+                            48 * 1024)
+        hlog2--;
+    if (hlog2 >= 7) {
+        const size_t lds = (static_cast<size_t>(1) << hlog2) * (4 + 8 * static_cast<size_t>(D));
+        hipLaunchKernelGGL(spmm_t_agg_kernel, dim3(static_cast<unsigned>((M + 255) / 256)),
+                           dim3(256), lds, st, M, D, hlog2, rowptr, cols, vals, gy, order, out);
+    } else {
         hipLaunchKernelGGL(spmm_t_kernel, dim3(static_cast<unsigned>((nt + 255) / 256)),
                            dim3(256), 0, st, M, D, rowptr, cols, vals, gy, order, out);
+    }
     return hipGetLastError();
 }
 
