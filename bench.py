@@ -512,6 +512,8 @@ def main():
                          "shell: fwd+bwd on a hollow shell (differentiation workload)")
     ap.add_argument("--shell-keys", type=int, default=20_000)
     ap.add_argument("--shell-queries", type=int, default=2_000_000)
+    ap.add_argument("--mixed", choices=["on", "off"], default="on",
+                    help="plain workload: also time the step with mixed-precision rows")
     ap.add_argument("--autotune", choices=["on", "off"], default="on",
                     help="plain workload: tune the row policy on the queries before timing")
     ap.add_argument("--problems", type=int, default=256)
@@ -618,6 +620,26 @@ def main():
         f"({qps / 1e6:.1f} M queries/s), plain fwd {fwd_ms:.2f} ms/step, "
         f"ratio {ms_per_step / fwd_ms:.3f}; last timings {per_step[-1]}")
 
+    # the same step with the mixed-precision rows (neighbour sets unchanged),
+    # against the plain FP64 forward
+    mixed = None
+    if args.mixed == "on":
+        mesh.set_precision("mixed")
+        step()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        mixed_ms = (time.perf_counter() - t3) * 1e3 / args.steps
+        tm = mesh.timings()
+        mesh.set_precision("fp64")
+        mixed = {"fwd_bwd_ms": mixed_ms, "fwd_bwd_over_plain_fp64_fwd": mixed_ms / fwd_ms,
+                 "fp32_tets_faces": tm["mixed_fp32"], "fp64_tets_faces": tm["mixed_fp64"],
+                 "rows_redone_fp64": tm["mixed_rows_fp64"]}
+        log(f"mixed precision: fwd+bwd {mixed_ms:.2f} ms/step, "
+            f"{mixed_ms / fwd_ms:.3f} x the plain FP64 forward")
+
     if args.calib_copy_mib > 0:
         n = args.calib_copy_mib << 20
         src = torch.ones(n, dtype=torch.uint8, device=dev)
@@ -675,9 +697,13 @@ def main():
             "plain_fwd_ms": fwd_ms,
             "fwd_bwd_over_fwd": ms_per_step / fwd_ms,
             "kernel_ms": {**kms, "locate_kernel": per_step[-1]["locate_ms"],
-                          "larger_levels_fwd": float(sum(per_step[-1]["rows_ms"][1:])),
-                          "larger_levels_bwd": float(sum(per_step[-1]["grad_ms"][1:])),
+                          "wave_rows_kernel_fwd": per_step[-1]["wave_ms"],
+                          "wave_rows_kernel_bwd": per_step[-1]["grad_wave_ms"],
+                          "thread_levels_fwd": float(sum(per_step[-1]["rows_ms"][1:])),
+                          "thread_levels_bwd": float(sum(per_step[-1]["grad_ms"][1:])),
                           "emit": per_step[-1]["emit_ms"]},
+            "rows_beyond_group_kernel": per_step[-1]["wave_items"],
+            "mixed_precision": mixed,
             "mesh_build_s": round(build_s, 1),
             "row_policy": tuned if tuned is not None else "default (autotuner off)",
         },
