@@ -954,7 +954,7 @@ constexpr int NMAX = kLevel0Row;  // neighbours
 constexpr int SMAX = 2 * VMAX;  // order-replay stack (shares the fronts' storage)
 }  // namespace grp
 
-struct GroupMem {
+struct alignas(16) GroupMem {
     int vkey[grp::VT];             // tet index, -1 = empty slot
     unsigned vnb[grp::VT];         // visited slots of the 4 neighbours, 8 bits each (0xff none)
     unsigned char vflag[grp::VT];  // 1 tested, 2 inside the circumsphere, 4 popped by the replay
@@ -1016,12 +1016,24 @@ __device__ __forceinline__ int nb_find(const GroupMem &g, int v) {
 
 // Vertex of entry e of the face-vertex sequence (boundary face e / 3,
 // vertex e % 3 in getFaceVerticies3 order).
-__device__ __forceinline__ int face_vertex(const DevMesh &m, const GroupMem &g, int e) {
+// After the boundary faces are known the visited table is no longer read:
+// its storage then holds, per cavity position, the tet's vertices (over
+// vkey and vnb) and the tet (over vflag and vpos); the face-circumcentre
+// masks go to the (dead) replay stack.
+static_assert(sizeof(int) * grp::VT * 2 == sizeof(int4) * grp::CMAX, "cavity vertex cache");
+static_assert(grp::VT * 2 == sizeof(int) * grp::CMAX, "cavity tet cache");
+static_assert(grp::SMAX >= grp::CMAX, "cavity mask cache");
+__device__ __forceinline__ int4 *cav_verts(GroupMem &g) { return reinterpret_cast<int4 *>(g.vkey); }
+__device__ __forceinline__ const int4 *cav_verts(const GroupMem &g) {
+    return reinterpret_cast<const int4 *>(g.vkey);
+}
+__device__ __forceinline__ int *cav_tet(GroupMem &g) { return reinterpret_cast<int *>(g.vflag); }
+__device__ __forceinline__ unsigned char *cav_mask(GroupMem &g) { return g.stk; }
+
+// vertex of face-vertex sequence number e (boundary face e / 3)
+__device__ __forceinline__ int face_vertex(const GroupMem &g, int e) {
     const int j = g.bnd[e / 3];
-    const int t = g.vkey[g.cav[j >> 2]];
-    bool neg;
-    const int4 tv = ld_tet_v(m, t, neg);
-    return sel4(tv, face_slot(j & 3, e % 3));
+    return sel4(cav_verts(g)[j >> 2], face_slot(j & 3, e % 3));
 }
 
 // Record of one query, in 16-byte units at topo.rec[rec_off[qi]]:
@@ -1247,11 +1259,35 @@ topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict
                 }
                 if (nbnd > grp::BMAX) ok = false;
                 group_sync();
+                if (ok) {
+                    // per cavity tet, once: the face-circumcentre mask (face
+                    // opposite p[k] held by an earlier cavity tet), the tet,
+                    // its vertices
+                    for (int ci = gl; ci < ncav; ci += grp::G) {
+                        const unsigned nbs = g.vnb[g.cav[ci]];
+                        unsigned mask = 0;
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            const unsigned ns = (nbs >> (8 * (3 - k))) & 0xffu;
+                            if (ns != 0xffu && (g.vflag[ns] & 2u) && g.vpos[ns] < ci)
+                                mask |= 1u << k;
+                        }
+                        cav_mask(g)[ci] = static_cast<unsigned char>(mask);
+                    }
+                    group_sync();
+                    for (int ci = gl; ci < ncav; ci += grp::G) cav_tet(g)[ci] = g.vkey[g.cav[ci]];
+                    group_sync();
+                    for (int ci = gl; ci < ncav; ci += grp::G) {
+                        bool neg;
+                        cav_verts(g)[ci] = ld_tet_v(m, cav_tet(g)[ci], neg);
+                    }
+                    group_sync();
+                }
                 // neighbours: least sequence number of each data vertex
                 const int nent = ok ? 3 * nbnd : 0;
                 bool full = false;
                 for (int e = gl; e < nent; e += grp::G) {
-                    const int v = face_vertex(m, g, e);
+                    const int v = face_vertex(g, e);
                     if (v >= N) continue;
                     const int h = nb_insert(g, v);
                     if (h < 0) {
@@ -1268,7 +1304,7 @@ topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict
                     bool first = false;
                     int v = -1, h = -1;
                     if (e < nent) {
-                        v = face_vertex(m, g, e);
+                        v = face_vertex(g, e);
                         if (v < N) {
                             h = nb_find(g, v);
                             first = h >= 0 && g.nmin[h] == e;
@@ -1310,23 +1346,16 @@ topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict
         }
         if (ncav == 0) continue;
         for (int ci = gl; ci < ncav; ci += grp::G) {
-            const int s = g.cav[ci];
-            const int t = g.vkey[s];
-            bool neg;
-            const int4 tv = ld_tet_v(m, t, neg);
-            const unsigned nbs = g.vnb[s];
-            unsigned mask = 0;
+            const int4 tv = cav_verts(g)[ci];
             int sl[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const unsigned ns = (nbs >> (8 * (3 - k))) & 0xffu;
-                if (ns != 0xffu && (g.vflag[ns] & 2u) && g.vpos[ns] < ci) mask |= 1u << k;
                 const int v = sel4(tv, k);
                 const int h = v < N ? nb_find(g, v) : -1;
                 sl[k] = h < 0 ? 0xffff : g.nslot[h];
             }
-            r[1 + ci] = make_int4(t, sl[0] | (sl[1] << 16), sl[2] | (sl[3] << 16),
-                                  static_cast<int>(mask));
+            r[1 + ci] = make_int4(cav_tet(g)[ci], sl[0] | (sl[1] << 16), sl[2] | (sl[3] << 16),
+                                  static_cast<int>(cav_mask(g)[ci]));
         }
         int *rf = reinterpret_cast<int *>(r + 1 + ncav);
         for (int bi = gl; bi < nbnd; bi += grp::G) rf[bi] = g.bnd[bi];
