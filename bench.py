@@ -293,8 +293,8 @@ def run_shell(args):
         log(f"step {i + 1}/{args.steps} done")
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     tm = mesh.timings()
-    fwd_ms = float(sum(tm["rows_ms"]))
-    bwd_ms = float(sum(tm["grad_ms"]))
+    fwd_ms = float(sum(tm["rows_ms"])) + tm["wave_ms"]
+    bwd_ms = float(sum(tm["grad_ms"])) + tm["grad_wave_ms"]
     log(f"fwd+bwd {ms:.1f} ms/step ({M / ms * 1e3 / 1e6:.2f} M queries/s); "
         f"timings {tm}")
 
@@ -667,6 +667,19 @@ def main():
     log(f"kernels (ms): {kms}; dominant {dom}: {kb[dom]:.0f} B/query -> "
         f"{achieved:.1f} GB/s, {ops / 1e12:.2f} T FP64 op/s")
 
+    # measured HBM bytes per launch of the dominant kernel (rocprofv3 --pmc on
+    # this workload, committed with the commands that made them)
+    traffic, traffic_src = None, None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         "hbm_traffic_plain_at_scale_1M_10M.txt")
+    if args.keys == 1_000_000 and args.queries == 10_000_000 and os.path.exists(tpath):
+        with open(tpath) as fh:
+            for line in fh:
+                name, _, val = line.partition("=")
+                if not line.startswith("#") and name.strip() == dom and val.strip():
+                    traffic = float(val)
+                    traffic_src = "profiles/hbm_traffic_plain_at_scale_1M_10M.txt"
+
     base = None
     if refbuild is not None:
         log("cpu baseline: reference on the host cores")
@@ -710,7 +723,8 @@ def main():
         "roofline": {
             "kernel": dom,
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_source": traffic_src,
             "bytes_per_query": kb[dom], "queries_per_launch": M, "launch_ms": launch_ms,
             "fp64_ops_per_s": ops, "fp64_frac": ops / FP64_OPS_PEAK,
         },
