@@ -928,8 +928,19 @@ __global__ void __launch_bounds__(256) locate_kernel(DevMesh m, Caps caps, Scrat
                                                      int *__restrict__ start) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= sc.nthr) return;  // sc.nthr: the scratch stride, at least the threads used
-    const int stride = min(static_cast<int>(gridDim.x * blockDim.x), sc.nthr);
-    for (int i = tid; i < n_items; i += stride) {
+    // the blocks of XCD p (block b runs on XCD b mod kParts) stride over part
+    // p of the Morton-ordered items, so each L2 sees one compact region
+    const int nthr = min(static_cast<int>(gridDim.x * blockDim.x), sc.nthr);
+    const int nblk = nthr / static_cast<int>(blockDim.x);
+    const int parts = nblk >= kParts ? kParts : 1;
+    const int p = static_cast<int>(blockIdx.x) % parts;
+    const int lo = static_cast<int>(static_cast<long long>(n_items) * p / parts);
+    const int hi = static_cast<int>(static_cast<long long>(n_items) * (p + 1) / parts);
+    const int pblk = (nblk - p + parts - 1) / parts;  // blocks of this part
+    const int stride = pblk * static_cast<int>(blockDim.x);
+    for (int i = lo + static_cast<int>(blockIdx.x) / parts * static_cast<int>(blockDim.x) +
+                 static_cast<int>(threadIdx.x);
+         i < hi; i += stride) {
         const int qi = items ? items[i] : i;
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
