@@ -523,6 +523,21 @@ __device__ __forceinline__ void new_contributions(const DevMesh &m, int t, int f
     const bool desc = fi & 1;
     d3 cf = {0.0, 0.0, 0.0};
     if constexpr (WithVol) cf = cc_tri(desc ? F[2] : F[0], F[1], desc ? F[0] : F[2]);
+    if constexpr (!WithVol) {
+        // moments only (no bit-exact order to keep): the triangle (F[k], q,
+        // F[k+1]) is c_a of vertex k and c_b of vertex k+1, computed once
+        const d3 t0 = cc_tri(F[0], q, F[1]), t1 = cc_tri(F[1], q, F[2]);
+        const d3 t2 = cc_tri(F[2], q, F[0]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int slot = slot_of(s[k], sel4(tv, s[k]));
+            if (slot < 0) continue;
+            const d3 &ca = k == 0 ? t0 : (k == 1 ? t1 : t2);
+            const d3 &cb = k == 0 ? t2 : (k == 1 ? t0 : t1);
+            if constexpr (Mom::active) mom(slot, facet_moment(F[k], q, ca, cb, cv));
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const int slot = slot_of(s[k], sel4(tv, s[k]));
