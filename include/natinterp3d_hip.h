@@ -70,6 +70,18 @@ int nat_mesh_get_info(const nat_mesh *m, nat_mesh_info *info);
 int nat_mesh_export(const nat_mesh *m, double *verts, int32_t *tet_v, int32_t *tet_n,
                     int32_t *kd_left, int32_t *kd_right, double *kd_pos, double *kd_rect);
 
+/* A mesh from packed arrays of the layout nat_mesh_export writes (no
+ * reference counterpart: the reference has no mesh serialisation).  The
+ * builder does not run: a mesh saved to disk and loaded back gives the same
+ * rows as the one built from the keys.  degeneracies = {coplanar, cospherical,
+ * edge anomalies} for nat_mesh_get_info, or NULL.  The arrays are checked:
+ * indices in range and a k-d tree reaching every node from node 0 exactly
+ * once (depth <= 4096); -1 otherwise.  device = -1: host-only mesh. */
+int nat_mesh_from_packed(const double *verts, int num_data_points, const int32_t *tet_v,
+                         const int32_t *tet_n, int num_tets, const int32_t *kd_left,
+                         const int32_t *kd_right, const double *kd_pos, const double *kd_rect,
+                         const int32_t *degeneracies, int device, nat_mesh **out);
+
 /* Replaces getInsertionFreeWeights / getInsertionFreeWeightsParallel
  * (natural_insertionfree.c:761-940): host queries in, host CSR out, same
  * ownership and return codes.  The rows are computed on the GPU. */

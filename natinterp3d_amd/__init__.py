@@ -7,8 +7,10 @@ CSR matrices) with the weights computed by hand-written HIP kernels.
 """
 from .batch import BatchInterpolator
 from .natinterp3d import (GraphCapacityError, Interpolator, MeshAndVerticesHIP, WeightsGraph,
-                          get_weights, interpolate, spmm_device)
+                          clear_mesh_cache, get_weights, interpolate, mesh_cache_stats,
+                          spmm_device)
 
 __all__ = ["BatchInterpolator", "GraphCapacityError", "Interpolator", "MeshAndVerticesHIP",
-           "WeightsGraph", "get_weights", "interpolate", "spmm_device"]
+           "WeightsGraph", "clear_mesh_cache", "get_weights", "interpolate", "mesh_cache_stats",
+           "spmm_device"]
 __version__ = "0.1.0"

@@ -21,6 +21,7 @@ EXPORTS = (
     "nat_mesh_set_precision", "nat_predicate_audit", "nat_insphere_classify",
     "nat_graph_create", "nat_graph_launch", "nat_graph_copy_out", "nat_graph_free",
     "nat_mesh_set_policy", "nat_mesh_get_policy", "nat_mesh_autotune",
+    "nat_mesh_from_packed",
 )
 
 
@@ -132,6 +133,9 @@ def lib():
     L.nat_insphere_classify.argtypes = [dp, ctypes.c_int, dp, ip]
     L.nat_mesh_set_precision.restype = ctypes.c_int
     L.nat_mesh_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    L.nat_mesh_from_packed.restype = ctypes.c_int
+    L.nat_mesh_from_packed.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, vp, vp, vp, vp,
+                                       vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
     L.nat_mesh_set_policy.restype = ctypes.c_int
     L.nat_mesh_set_policy.argtypes = [vp, ctypes.POINTER(Policy), vp]
     L.nat_mesh_get_policy.restype = ctypes.c_int

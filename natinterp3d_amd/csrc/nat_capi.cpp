@@ -1042,6 +1042,75 @@ int nat_mesh_build(const double *data_points, int num_data_points, int device, n
     return 0;
 }
 
+int nat_mesh_from_packed(const double *verts, int num_data_points, const int32_t *tet_v,
+                         const int32_t *tet_n, int num_tets, const int32_t *kd_left,
+                         const int32_t *kd_right, const double *kd_pos, const double *kd_rect,
+                         const int32_t *degeneracies, int device, nat_mesh **out) {
+    g_err.clear();
+    if (!out) return fail(-1, "out is NULL");
+    *out = nullptr;
+    if (!verts || !tet_v || !tet_n || !kd_left || !kd_right || !kd_pos || !kd_rect)
+        return fail(-1, "NULL array");
+    if (num_data_points <= 0 || num_tets <= 0)
+        return fail(-1, "need data points and tets (got %d, %d)", num_data_points, num_tets);
+    if (static_cast<long long>(num_tets) * 4 > 2147483647LL)
+        return fail(-1, "too many tets (%d)", num_tets);
+    if (int rc = check_device(device)) return rc;
+    const int N = num_data_points, T = num_tets;
+    // every index in range
+    for (long long i = 0; i < 4LL * T; i++) {
+        if (tet_v[i] < 0 || tet_v[i] >= N + 4)
+            return fail(-1, "tet %lld: vertex slot %d outside [0, %d)", i / 4, tet_v[i], N + 4);
+        if (tet_n[i] < -1 || tet_n[i] >= T)
+            return fail(-1, "tet %lld: neighbour %d outside [-1, %d)", i / 4, tet_n[i], T);
+    }
+    for (int t = 0; t < T; t++)
+        if (kd_left[t] < -1 || kd_left[t] >= T || kd_right[t] < -1 || kd_right[t] >= T)
+            return fail(-1, "k-d node %d: child outside [-1, %d)", t, T);
+    // the k-d tree: every node reached from the root (node 0) once, depth bounded
+    constexpr int kMaxDepth = 4096;
+    std::vector<unsigned char> seen(T, 0);
+    std::vector<std::pair<int, int>> stack{{0, 1}};
+    int depth = 0, reached = 0;
+    while (!stack.empty()) {
+        const auto [node, d] = stack.back();
+        stack.pop_back();
+        if (seen[node]) return fail(-1, "k-d node %d reached twice", node);
+        seen[node] = 1;
+        reached++;
+        depth = std::max(depth, d);
+        if (depth > kMaxDepth) return fail(-1, "k-d tree deeper than %d", kMaxDepth);
+        if (kd_left[node] >= 0) stack.emplace_back(kd_left[node], d + 1);
+        if (kd_right[node] >= 0) stack.emplace_back(kd_right[node], d + 1);
+    }
+    if (reached != T) return fail(-1, "k-d tree reaches %d of %d nodes", reached, T);
+    nat_mesh *m = new nat_mesh();
+    m->device = device;
+    m->parts.resize(1);
+    nat::PackedMesh &h = m->parts[0];
+    h.n_data = N;
+    h.n_tets = T;
+    h.verts.assign(verts, verts + 3 * static_cast<size_t>(N + 4));
+    h.tet_v.assign(tet_v, tet_v + 4 * static_cast<size_t>(T));
+    h.tet_n.assign(tet_n, tet_n + 4 * static_cast<size_t>(T));
+    h.kd_left.assign(kd_left, kd_left + T);
+    h.kd_right.assign(kd_right, kd_right + T);
+    h.kd_pos.assign(kd_pos, kd_pos + 3 * static_cast<size_t>(T));
+    std::copy(kd_rect, kd_rect + 6, h.kd_rect);
+    h.kd_depth = depth;
+    if (degeneracies) {
+        h.coplanar = degeneracies[0];
+        h.cospherical = degeneracies[1];
+        h.edge_anomalies = degeneracies[2];
+    }
+    if (int urc = finish_build(m)) {
+        delete m;
+        return urc;
+    }
+    *out = m;
+    return 0;
+}
+
 int nat_batch_build(const double *const *data_points, const int32_t *num_data_points,
                     int num_problems, int device, int num_threads, nat_mesh **out) {
     g_err.clear();

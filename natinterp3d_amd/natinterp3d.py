@@ -8,7 +8,9 @@ kernels of libnatinterp3d_hip.so.  ``parallel`` and ``num_threads`` are
 accepted for compatibility and do not change any result (the reference's
 results do not depend on them either, tests/test_natinterp3d.py:347-373).
 """
+import collections
 import ctypes
+import hashlib
 import os
 import weakref
 from typing import Optional
@@ -173,6 +175,61 @@ class MeshAndVerticesHIP:
         _lib.check(rc, "nat_mesh_export")
         return out
 
+    _FORMAT = "natinterp3d_amd.mesh"
+
+    def save(self, path, **extra):
+        """Write the packed mesh (the builder's output: final coordinates,
+        tets, neighbours, k-d tree) to ``path`` as .npz (plain arrays, no
+        pickles); ``load`` makes the device mesh from it without running the
+        builder, and its rows equal this mesh's."""
+        inf = self.info()
+        np.savez(path, format=np.array(self._FORMAT), version=np.array(1),
+                 degeneracies=np.array([inf["coplanar"], inf["cospherical"],
+                                        inf["edge_anomalies"]], np.int32),
+                 **self.export(), **extra)
+
+    @classmethod
+    def load(cls, path, device: Optional[int] = None):
+        """The mesh saved by ``save`` at ``path``, uploaded to ``device``."""
+        with np.load(path, allow_pickle=False) as z:
+            if "format" not in z or str(z["format"]) != cls._FORMAT or int(z["version"]) != 1:
+                raise ValueError(f"{path}: not a saved natinterp3d_amd mesh")
+            arrays = {k: z[k] for k in ("verts", "tet_v", "tet_n", "kd_left", "kd_right",
+                                        "kd_pos", "kd_rect", "degeneracies")}
+        return cls.from_packed(arrays, device=device)
+
+    @classmethod
+    def from_packed(cls, arrays, device: Optional[int] = None):
+        """A mesh from packed arrays of the layout ``export`` returns (plus an
+        optional ``degeneracies`` [3]); the arrays are checked by the
+        library."""
+        verts = np.ascontiguousarray(arrays["verts"], np.float64).reshape(-1, 3)
+        tet_v = np.ascontiguousarray(arrays["tet_v"], np.int32).reshape(-1, 4)
+        tet_n = np.ascontiguousarray(arrays["tet_n"], np.int32).reshape(-1, 4)
+        kd_left = np.ascontiguousarray(arrays["kd_left"], np.int32).reshape(-1)
+        kd_right = np.ascontiguousarray(arrays["kd_right"], np.int32).reshape(-1)
+        kd_pos = np.ascontiguousarray(arrays["kd_pos"], np.float64).reshape(-1, 3)
+        kd_rect = np.ascontiguousarray(arrays["kd_rect"], np.float64).reshape(-1)
+        T = tet_v.shape[0]
+        if (tet_n.shape[0] != T or kd_left.shape[0] != T or kd_right.shape[0] != T
+                or kd_pos.shape[0] != T or kd_rect.shape[0] != 6 or verts.shape[0] < 5):
+            raise ValueError("packed mesh arrays have inconsistent shapes")
+        deg = arrays.get("degeneracies")
+        deg = None if deg is None else np.ascontiguousarray(deg, np.int32).reshape(3)
+        self = cls.__new__(cls)
+        self.numDataPoints = verts.shape[0] - 4
+        self.device = _default_device() if device is None else int(device)
+        self._m = ctypes.c_void_p()
+        L = _lib.lib()
+        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        rc = L.nat_mesh_from_packed(vp(verts), self.numDataPoints, vp(tet_v), vp(tet_n), T,
+                                    vp(kd_left), vp(kd_right), vp(kd_pos), vp(kd_rect),
+                                    None if deg is None else vp(deg), self.device,
+                                    ctypes.byref(self._m))
+        _lib.check(rc, "nat_mesh_from_packed")
+        self._finalizer = weakref.finalize(self, L.nat_mesh_free, self._m)
+        return self
+
     def get_weights(self, queryPoints):
         queryPoints = _as_c_double_2d(queryPoints)
         numQueryPoints = queryPoints.shape[0]
@@ -267,8 +324,9 @@ def interpolate(
 ) -> np.ndarray:
     """Natural neighbour (Sibson) interpolation of ``values`` known at ``keys``
     [N,3], evaluated at ``queries`` [M,3]; values [N] or [N,D] -> [M] or [M,D].
-    Same contract as natinterp3d.interpolate (natinterp3d.py:9-35)."""
-    interpolator = Interpolator(keys, parallel=parallel, num_threads=num_threads)
+    Same contract as natinterp3d.interpolate (natinterp3d.py:9-35); the mesh of
+    recently used keys is reused (see ``clear_mesh_cache``)."""
+    interpolator = _cached_interpolator(keys, parallel, num_threads)
     return interpolator.interpolate(queries, values)
 
 
@@ -280,9 +338,47 @@ def get_weights(
 ) -> scipy.sparse.csr_matrix:
     """Sibson coordinates of ``queries`` with respect to ``keys`` as a CSR
     matrix [M,N].  Same contract as natinterp3d.get_weights
-    (natinterp3d.py:38-60)."""
-    interpolator = Interpolator(keys, parallel=parallel, num_threads=num_threads)
+    (natinterp3d.py:38-60); the mesh of recently used keys is reused."""
+    interpolator = _cached_interpolator(keys, parallel, num_threads)
     return interpolator.get_weights(queries)
+
+
+# Device meshes of the keys of recent module-level calls, most recent last.
+# The reference builds a mesh per call (natinterp3d.py:34, 59); the build is
+# deterministic, so a mesh built earlier for the same keys gives the same
+# rows.  Size from $NATINTERP3D_MESH_CACHE (default 2; 0 turns it off).
+_MESH_CACHE = collections.OrderedDict()
+_MESH_CACHE_STATS = {"hits": 0, "misses": 0}
+
+
+def _cached_interpolator(keys, parallel, num_threads):
+    size = int(os.environ.get("NATINTERP3D_MESH_CACHE", "2"))
+    if size <= 0:
+        return Interpolator(keys, parallel=parallel, num_threads=num_threads)
+    k = np.ascontiguousarray(keys, np.float64)
+    dev = _default_device()
+    key = (k.shape, hashlib.blake2b(memoryview(k).cast("B"), digest_size=20).hexdigest(), dev)
+    it = _MESH_CACHE.get(key)
+    if it is not None:
+        _MESH_CACHE.move_to_end(key)
+        _MESH_CACHE_STATS["hits"] += 1
+        return it
+    _MESH_CACHE_STATS["misses"] += 1
+    it = Interpolator(k, parallel=parallel, num_threads=num_threads, device=dev)
+    _MESH_CACHE[key] = it
+    while len(_MESH_CACHE) > size:
+        _MESH_CACHE.popitem(last=False)
+    return it
+
+
+def clear_mesh_cache():
+    """Drop the device meshes kept for module-level calls."""
+    _MESH_CACHE.clear()
+
+
+def mesh_cache_stats():
+    """Hits and misses of the module-level mesh cache, and its entries."""
+    return {**_MESH_CACHE_STATS, "entries": len(_MESH_CACHE)}
 
 
 class Interpolator:
@@ -306,6 +402,26 @@ class Interpolator:
         self.parallel = parallel
         self.num_threads = num_threads
         self.mesh_and_vertices = MeshAndVerticesHIP(centered, device=device)
+
+    def save(self, path):
+        """Write the mesh and the key centroid to ``path`` (.npz, plain arrays):
+        ``Interpolator.load`` then skips the host build."""
+        self.mesh_and_vertices.save(path, centroid=np.asarray(self._centroid, np.float64))
+
+    @classmethod
+    def load(cls, path, parallel: bool = True, num_threads: Optional[int] = None,
+             device: Optional[int] = None) -> "Interpolator":
+        """The Interpolator saved by ``save``; its rows equal the original's."""
+        with np.load(path, allow_pickle=False) as z:
+            if "centroid" not in z:
+                raise ValueError(f"{path}: not a saved Interpolator (no centroid)")
+            centroid = np.asarray(z["centroid"], np.float64).reshape(3)
+        self = cls.__new__(cls)
+        self._centroid = centroid
+        self.parallel = parallel
+        self.num_threads = num_threads
+        self.mesh_and_vertices = MeshAndVerticesHIP.load(path, device=device)
+        return self
 
     def autotune(self, query_points: np.ndarray, repeats: int = 3) -> dict:
         """Time the row-path policies on representative ``query_points`` and

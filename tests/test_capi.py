@@ -102,3 +102,34 @@ def test_policy_argument_checks():
     assert _lib.lib().nat_mesh_autotune(m.handle, None, 0, None, 0, None,
                                         ctypes.byref(rep)) < 0
     assert "repeats" in _lib.last_error()
+
+
+def test_mesh_round_trip_through_packed_arrays(tmp_path):
+    """A host-only mesh saved to .npz and loaded back (no builder run) exports
+    the same arrays; corrupted arrays are rejected."""
+    rng = np.random.default_rng(4)
+    k = rng.random((300, 3))
+    m = MeshAndVerticesHIP(np.ascontiguousarray(k - k.mean(0)), device=-1)
+    path = tmp_path / "mesh.npz"
+    m.save(path)
+    m2 = MeshAndVerticesHIP.load(path, device=-1)
+    a, b = m.export(), m2.export()
+    for key in a:
+        assert np.array_equal(a[key], b[key]), key
+    i1, i2 = m.info(), m2.info()
+    for key in ("n_data", "n_tets", "kd_depth", "coplanar", "cospherical", "edge_anomalies"):
+        assert i1[key] == i2[key], key
+    bad = dict(a)
+    bad["tet_v"] = a["tet_v"].copy()
+    bad["tet_v"][5, 2] = a["verts"].shape[0]  # vertex slot out of range
+    with pytest.raises(RuntimeError, match="vertex slot"):
+        MeshAndVerticesHIP.from_packed(bad, device=-1)
+    bad = dict(a)
+    bad["kd_left"] = a["kd_left"].copy()
+    node = int(np.flatnonzero(a["kd_left"] >= 0)[0])
+    bad["kd_left"][node] = 0  # a cycle back to the root
+    with pytest.raises(RuntimeError, match="k-d"):
+        MeshAndVerticesHIP.from_packed(bad, device=-1)
+    with pytest.raises(ValueError, match="not a saved"):
+        np.savez(tmp_path / "other.npz", x=np.zeros(3))
+        MeshAndVerticesHIP.load(tmp_path / "other.npz", device=-1)
