@@ -1749,6 +1749,17 @@ int nat_mesh_autotune(nat_mesh *m, const double *d_queries, int64_t num_query_po
         ~Restore() { m->timing = t; }
     } restore{m, timing};
     const int Mi = static_cast<int>(M);
+    // on an error the caller's policy comes back
+    const Policy orig = policy_of(m);
+    bool tuned = false;
+    struct Keep {
+        nat_mesh *m;
+        const Policy &p;
+        const bool &tuned;
+        ~Keep() {
+            if (!tuned) (void)apply_policy(m, p);
+        }
+    } keep{m, orig, tuned};
 
     Policy def;
     def.budget.assign(P, static_cast<unsigned char>(kGroupBudget));
@@ -1837,6 +1848,7 @@ int nat_mesh_autotune(nat_mesh *m, const double *d_queries, int64_t num_query_po
     }
     // final compute under the chosen policy (nat_weights_emit may follow)
     if (int rc = time_policy(m, best, d_queries, Mi, d_prob, st, 1, &ms)) return rc;
+    tuned = true;
     report->tuned_ms = best_ms;
     report->candidates = tried;
     for (int p = 0; p < P; p++)
