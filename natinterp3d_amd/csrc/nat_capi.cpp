@@ -101,6 +101,8 @@ struct nat_mesh {
     std::vector<nat::LocParams> lph;      // [P]
     nat::DevMesh dev{};
     DevBuf verts, tet_v, tet_n, kd, cell_start, cell_pts, lp;
+    DevBuf old_vol;                       // step-3 table (built on the first compute)
+    bool old_vol_tried = false;
     DevBuf prob, qoff;                    // batch calls: problem of each query
     std::mutex mu;
 
@@ -155,7 +157,8 @@ struct nat_mesh {
 
     int64_t device_bytes() const {
         return static_cast<int64_t>(verts.bytes + tet_v.bytes + tet_n.bytes + kd.bytes +
-                                    cell_start.bytes + cell_pts.bytes + lp.bytes);
+                                    cell_start.bytes + cell_pts.bytes + lp.bytes +
+                                    old_vol.bytes);
     }
     int problems() const { return static_cast<int>(parts.size()); }
     int64_t workspace_bytes() const {
@@ -175,7 +178,7 @@ struct nat_mesh {
                          &row_off,  &row_cnt,   &rowptr64,  &row_total, &ovf[0],    &ovf[1],
                          &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,   &csr_rowptr,
                          &rec,      &rec_off,   &start,     &lp,       &prob,       &qoff,
-                         &route,    &wscr};
+                         &route,    &wscr,      &old_vol};
         for (DevBuf *b : all) b->release();
         for (int l = 0; l < kLevels; l++) scratch[l].release();
         for (hipEvent_t &e : ev) {
@@ -494,9 +497,29 @@ int ensure_wave_scratch(nat_mesh *m, long long want, int *launch) {
 }
 
 // d_prob: problem of each query (batch meshes), NULL for a single problem
+// The step-3 table (32 doubles per tet), built once per mesh on its first
+// compute; without the memory for it the geometry is computed inline.
+int ensure_old_vol(nat_mesh *m, hipStream_t st) {
+    if (m->old_vol_tried) return 0;
+    m->old_vol_tried = true;
+    const size_t bytes = sizeof(double) * 32 * static_cast<size_t>(m->dev.n_tets);
+    if (m->old_vol.ensure(bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        m->old_vol.release();
+        return 0;
+    }
+    nat::DevMesh d = m->dev;
+    d.old_vol = nullptr;
+    HIP_OR_FAIL(nat::launch_old_vol(d, m->old_vol.as<double>(), st));
+    m->dev.old_vol = m->old_vol.as<double>();
+    return 0;
+}
+
 int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipStream_t st,
                  long long *nnz_out) {
     m->last_M = -1;
+    if (M > 0)
+        if (int rc = ensure_old_vol(m, st)) return rc;
     m->dev.prob = d_prob;
     if (M == 0) {
         m->last_M = 0;

@@ -48,6 +48,11 @@ struct DevMesh {
     // thread-per-query levels), high byte = first thread-per-query level for
     // the queries that leave the group kernel.  Read at launch time.
     const unsigned short *route;
+    // Step-3 subcell volumes, which depend on the mesh alone: per tet, per
+    // vertex VI, per variant (bit i: the face opposite the i-th other vertex
+    // of VI takes its circumcentre in the neighbour's vertex order), at
+    // [(t * 4 + VI) * 8 + variant]; NULL until built (computed inline then).
+    const double *old_vol;
 };
 
 __device__ __forceinline__ unsigned route_of(const DevMesh &m, int qi) {
@@ -175,6 +180,8 @@ hipError_t launch_grad_geom(const DevMesh &m, int nblocks, const double *d_q, co
                             int n_items, unsigned int *d_work, const Topo &topo,
                             const GradIn &g, int *ovf_list, unsigned int *ovf_count,
                             hipStream_t st);
+// the step-3 table of DevMesh::old_vol (out: n_tets * 32 doubles)
+hipError_t launch_old_vol(const DevMesh &m, double *out, hipStream_t st);
 // wave-per-query rows (cavities beyond the group kernel): per-wave scratch
 // of wave_scratch_size() bytes for each of nblocks one-wave blocks
 size_t wave_scratch_size();

@@ -476,6 +476,74 @@ __device__ __forceinline__ void old_contributions(const DevMesh &m, int t, unsig
     step3_vertex<3>(P, F, ct, tv, slot_of, add);
 }
 
+// The step-3 table (DevMesh::old_vol): the subcell volume of vertex VI of a
+// tet for each of the 8 choices of vertex order of the three face
+// circumcentres it uses (the faces opposite the other three vertices a, b, c:
+// each in the tet's own ascending order or in the order of the neighbour
+// across it, as old_contributions picks with fcmask).  Same functions, same
+// operands: each entry has the bits old_contributions would compute.
+template <int VI>
+__device__ __forceinline__ int old_variant(unsigned fcmask) {
+    constexpr int a = FaceOf<VI>::a, b = FaceOf<VI>::b, c = FaceOf<VI>::c;
+    return static_cast<int>(((fcmask >> a) & 1u) | (((fcmask >> b) & 1u) << 1) |
+                            (((fcmask >> c) & 1u) << 2));
+}
+
+__device__ __forceinline__ d3 sel3d(bool s, const d3 &a, const d3 &b) {
+    return {s ? a.x : b.x, s ? a.y : b.y, s ? a.z : b.z};
+}
+
+template <int VI>
+__device__ __forceinline__ void old_vol_vertex(const Quad &P, const Quad &Fo, const Quad &Fn,
+                                               const d3 &ct, double *out) {
+    constexpr int a = FaceOf<VI>::a, b = FaceOf<VI>::b, c = FaceOf<VI>::c;
+    for (int v = 0; v < 8; v++) {
+        const d3 fa = sel3d(v & 1, Fn.get<a>(), Fo.get<a>());
+        const d3 fb = sel3d(v & 2, Fn.get<b>(), Fo.get<b>());
+        const d3 fc = sel3d(v & 4, Fn.get<c>(), Fo.get<c>());
+        out[VI * 8 + v] =
+            subcell_volume(P.get<VI>(), P.get<a>(), P.get<b>(), P.get<c>(), ct, fc, fb, fa);
+    }
+}
+
+__global__ void __launch_bounds__(256) old_vol_kernel(DevMesh m, double *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m.n_tets) return;
+    bool neg;
+    const int4 tv = ld_tet_v(m, t, neg);
+    const Quad P = {ld_vert(m, tv.x), ld_vert(m, tv.y), ld_vert(m, tv.z), ld_vert(m, tv.w)};
+    const d3 ct = cc_tet(P.p0, P.p1, P.p2, P.p3);
+    const int4 tn = m.tet_n[t];
+    const Quad Fo = {step3_face_cc<0>(m, P, tv, tn, 0u), step3_face_cc<1>(m, P, tv, tn, 0u),
+                     step3_face_cc<2>(m, P, tv, tn, 0u), step3_face_cc<3>(m, P, tv, tn, 0u)};
+    // face K's neighbour is tn[3 - K]; without one the variant is never used
+    const Quad Fn = {step3_face_cc<0>(m, P, tv, tn, tn.w >= 0 ? 1u : 0u),
+                     step3_face_cc<1>(m, P, tv, tn, tn.z >= 0 ? 2u : 0u),
+                     step3_face_cc<2>(m, P, tv, tn, tn.y >= 0 ? 4u : 0u),
+                     step3_face_cc<3>(m, P, tv, tn, tn.x >= 0 ? 8u : 0u)};
+    double *o = out + static_cast<size_t>(t) * 32;
+    old_vol_vertex<0>(P, Fo, Fn, ct, o);
+    old_vol_vertex<1>(P, Fo, Fn, ct, o);
+    old_vol_vertex<2>(P, Fo, Fn, ct, o);
+    old_vol_vertex<3>(P, Fo, Fn, ct, o);
+}
+
+// Step 3 for one cavity tet from the table; the same calls of add as
+// old_contributions (vertex order, slots from slot_of(VI, .)).
+template <typename SlotOf, typename Add>
+__device__ __forceinline__ void old_contributions_table(const DevMesh &m, int t, unsigned fcmask,
+                                                        SlotOf slot_of, Add add) {
+    const double *ov = m.old_vol + static_cast<size_t>(t) * 32;
+    int s = slot_of(0, -1);
+    if (s >= 0) add(s, ov[0 * 8 + old_variant<0>(fcmask)]);
+    s = slot_of(1, -1);
+    if (s >= 0) add(s, ov[1 * 8 + old_variant<1>(fcmask)]);
+    s = slot_of(2, -1);
+    if (s >= 0) add(s, ov[2 * 8 + old_variant<2>(fcmask)]);
+    s = slot_of(3, -1);
+    if (s >= 0) add(s, ov[3 * 8 + old_variant<3>(fcmask)]);
+}
+
 // Derivative of a stolen volume: d sigma_k / dq = (1 / |pk - q|) times the
 // moment, integral of (x - q) dA, of the new Voronoi facet between q and pk.
 // Inside the virtual tet (pk, q, A, B) that facet is the quadrilateral
@@ -1998,7 +2066,13 @@ __global__ void __launch_bounds__(64) wave_rows_kernel(DevMesh m, char *wscr,
             // (negated: stolen -= vol is stolen + -vol), each at its position
             for (int ci = lane; ci < C; ci += 64) {
                 const uint2 cs = W.cslot[ci];
-                old_contributions(m, W.dtet[W.cav[ci]], static_cast<unsigned>(W.cmask[ci]),
+                auto old_fn = [&](auto... a) {
+                    if (m.old_vol)
+                        old_contributions_table(m, a...);
+                    else
+                        old_contributions(m, a...);
+                };
+                old_fn(W.dtet[W.cav[ci]], static_cast<unsigned>(W.cmask[ci]),
                                   slot_of_ci(ci), [&](int slot, double vol) {
                                       const int k = slot == static_cast<int>(u16_of(cs, 0))
                                                         ? 0
@@ -2167,6 +2241,11 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
                             acc[slot * kGeomThreads] += vol;
                             if constexpr (MIXED) absum += fabs(vol);
                         };
+                        if (m.old_vol) {  // exact FP64 values in both modes
+                            old_contributions_table(m, cr.x, static_cast<unsigned>(cr.w),
+                                                    slot_of, add);
+                            continue;
+                        }
                         if constexpr (MIXED) {
                             if (fp32) {
                                 if (old_contributions32(m, cr.x, qq, tau, slot_of, add)) {
@@ -2498,6 +2577,13 @@ hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, in
                        hipStream_t st) {
     hipLaunchKernelGGL(sibson_rows_kernel, dim3(nblocks), dim3(256), 0, st, m, caps, sc, level,
                        d_q, d_items, n_items, d_n_items, d_work, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_old_vol(const DevMesh &m, double *out, hipStream_t st) {
+    if (m.n_tets > 0)
+        hipLaunchKernelGGL(old_vol_kernel, dim3((m.n_tets + 255) / 256), dim3(256), 0, st, m,
+                           out);
     return hipGetLastError();
 }
 
