@@ -480,19 +480,14 @@ __device__ __forceinline__ void old_contributions(const DevMesh &m, int t, unsig
 // tet for each of the 8 choices of vertex order of the three face
 // circumcentres it uses (the faces opposite the other three vertices a, b, c:
 // each in the tet's own ascending order or in the order of the neighbour
-// across it, as old_contributions picks with fcmask).  Same functions, same
-// operands: each entry has the bits old_contributions would compute.
-template <int VI>
-__device__ __forceinline__ int old_variant(unsigned fcmask) {
-    constexpr int a = FaceOf<VI>::a, b = FaceOf<VI>::b, c = FaceOf<VI>::c;
-    return static_cast<int>(((fcmask >> a) & 1u) | (((fcmask >> b) & 1u) << 1) |
-                            (((fcmask >> c) & 1u) << 2));
-}
-
+// across it, as old_contributions picks with fcmask), stored under each of the
+// 16 masks so that a tet's four values are one 32-byte load.  Same functions,
+// same operands: each entry has the bits old_contributions would compute.
 __device__ __forceinline__ d3 sel3d(bool s, const d3 &a, const d3 &b) {
     return {s ? a.x : b.x, s ? a.y : b.y, s ? a.z : b.z};
 }
 
+// the 8 values of vertex VI, stored for each of the 16 masks they serve
 template <int VI>
 __device__ __forceinline__ void old_vol_vertex(const Quad &P, const Quad &Fo, const Quad &Fn,
                                                const d3 &ct, double *out) {
@@ -501,8 +496,12 @@ __device__ __forceinline__ void old_vol_vertex(const Quad &P, const Quad &Fo, co
         const d3 fa = sel3d(v & 1, Fn.get<a>(), Fo.get<a>());
         const d3 fb = sel3d(v & 2, Fn.get<b>(), Fo.get<b>());
         const d3 fc = sel3d(v & 4, Fn.get<c>(), Fo.get<c>());
-        out[VI * 8 + v] =
+        const double vol =
             subcell_volume(P.get<VI>(), P.get<a>(), P.get<b>(), P.get<c>(), ct, fc, fb, fa);
+        // masks whose bits a, b, c give v (bit VI free)
+        const unsigned base = ((v & 1) << a) | (((v >> 1) & 1) << b) | (((v >> 2) & 1) << c);
+        out[base * 4 + VI] = vol;
+        out[(base | (1u << VI)) * 4 + VI] = vol;
     }
 }
 
@@ -521,7 +520,7 @@ __global__ void __launch_bounds__(256) old_vol_kernel(DevMesh m, double *__restr
                      step3_face_cc<1>(m, P, tv, tn, tn.z >= 0 ? 2u : 0u),
                      step3_face_cc<2>(m, P, tv, tn, tn.y >= 0 ? 4u : 0u),
                      step3_face_cc<3>(m, P, tv, tn, tn.x >= 0 ? 8u : 0u)};
-    double *o = out + static_cast<size_t>(t) * 32;
+    double *o = out + static_cast<size_t>(t) * 64;
     old_vol_vertex<0>(P, Fo, Fn, ct, o);
     old_vol_vertex<1>(P, Fo, Fn, ct, o);
     old_vol_vertex<2>(P, Fo, Fn, ct, o);
@@ -533,15 +532,16 @@ __global__ void __launch_bounds__(256) old_vol_kernel(DevMesh m, double *__restr
 template <typename SlotOf, typename Add>
 __device__ __forceinline__ void old_contributions_table(const DevMesh &m, int t, unsigned fcmask,
                                                         SlotOf slot_of, Add add) {
-    const double *ov = m.old_vol + static_cast<size_t>(t) * 32;
+    const double4 v =
+        reinterpret_cast<const double4 *>(m.old_vol)[static_cast<size_t>(t) * 16 + (fcmask & 15u)];
     int s = slot_of(0, -1);
-    if (s >= 0) add(s, ov[0 * 8 + old_variant<0>(fcmask)]);
+    if (s >= 0) add(s, v.x);
     s = slot_of(1, -1);
-    if (s >= 0) add(s, ov[1 * 8 + old_variant<1>(fcmask)]);
+    if (s >= 0) add(s, v.y);
     s = slot_of(2, -1);
-    if (s >= 0) add(s, ov[2 * 8 + old_variant<2>(fcmask)]);
+    if (s >= 0) add(s, v.z);
     s = slot_of(3, -1);
-    if (s >= 0) add(s, ov[3 * 8 + old_variant<3>(fcmask)]);
+    if (s >= 0) add(s, v.w);
 }
 
 // Derivative of a stolen volume: d sigma_k / dq = (1 / |pk - q|) times the
