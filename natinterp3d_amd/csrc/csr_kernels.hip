@@ -209,13 +209,15 @@ hipError_t launch_scan_counts(const int *d_in, long long *d_out, int n, long lon
 // 16 lanes per row: each row's pool range and CSR range are contiguous, so
 // the lanes of a row read and write consecutive entries.  Rows reaching past
 // cap (a replay whose pool overflowed, reported separately) are skipped.
+// Pool columns are device vertex indices; vmap turns them into the caller's
+// (key) indices.
 constexpr int kGatherLanes = 16;
 
 __global__ void __launch_bounds__(256) gather_csr_kernel(
     const long long *__restrict__ rowptr64, const long long *__restrict__ row_off,
     const int *__restrict__ row_cnt, const double *__restrict__ pv, const int *__restrict__ pc,
-    int M, long long cap, double *__restrict__ vals, int *__restrict__ cols,
-    int *__restrict__ rowptr32) {
+    const int *__restrict__ vmap, int M, long long cap, double *__restrict__ vals,
+    int *__restrict__ cols, int *__restrict__ rowptr32) {
     const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const long long i = t / kGatherLanes;
     const int lane = static_cast<int>(t % kGatherLanes);
@@ -227,18 +229,39 @@ __global__ void __launch_bounds__(256) gather_csr_kernel(
     if (src < 0 || src + c > cap || dst + c > cap) return;
     for (int j = lane; j < c; j += kGatherLanes) {
         vals[dst + j] = pv[src + j];
-        cols[dst + j] = pc[src + j];
+        cols[dst + j] = vmap[pc[src + j]];
     }
 }
 
 hipError_t launch_gather_csr(const long long *d_rowptr64, const long long *d_row_off,
                              const int *d_row_cnt, const double *pool_vals,
-                             const int *pool_cols, int M, long long cap, double *d_vals,
-                             int *d_cols, int *d_rowptr32, hipStream_t st) {
+                             const int *pool_cols, const int *vmap, int M, long long cap,
+                             double *d_vals, int *d_cols, int *d_rowptr32, hipStream_t st) {
     const long long threads = (static_cast<long long>(M) + 1) * kGatherLanes;
     hipLaunchKernelGGL(gather_csr_kernel, dim3(static_cast<unsigned>((threads + 255) / 256)),
                        dim3(256), 0, st, d_rowptr64, d_row_off, d_row_cnt, pool_vals, pool_cols,
-                       M, cap, d_vals, d_cols, d_rowptr32);
+                       vmap, M, cap, d_vals, d_cols, d_rowptr32);
+    return hipGetLastError();
+}
+
+// out[i, :] = in[idx[i], :] for i < n (rows of D doubles): the caller's
+// per-key values in device vertex order.
+__global__ void __launch_bounds__(256) gather_rows_kernel(const double *__restrict__ in,
+                                                          const int *__restrict__ idx, int n,
+                                                          int D, double *__restrict__ out) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<long long>(n) * D) return;
+    const long long i = t / D;
+    const int d = static_cast<int>(t - i * D);
+    out[t] = in[static_cast<long long>(idx[i]) * D + d];
+}
+
+hipError_t launch_gather_rows(const double *in, const int *idx, int n, int D, double *out,
+                              hipStream_t st) {
+    const long long total = static_cast<long long>(n) * D;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(static_cast<unsigned>((total + 255) / 256)),
+                       dim3(256), 0, st, in, idx, n, D, out);
     return hipGetLastError();
 }
 

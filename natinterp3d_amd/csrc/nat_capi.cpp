@@ -101,6 +101,8 @@ struct nat_mesh {
     std::vector<nat::LocParams> lph;      // [P]
     nat::DevMesh dev{};
     DevBuf verts, tet_v, tet_n, kd, cell_start, cell_pts, lp;
+    DevBuf vorig;                         // [N+4P] device vertex -> key index
+    DevBuf vals_dev;                      // backward: values in device vertex order
     DevBuf old_vol;                       // step-3 table (built on the first compute)
     bool old_vol_tried = false;
     DevBuf prob, qoff;                    // batch calls: problem of each query
@@ -158,7 +160,7 @@ struct nat_mesh {
     int64_t device_bytes() const {
         return static_cast<int64_t>(verts.bytes + tet_v.bytes + tet_n.bytes + kd.bytes +
                                     cell_start.bytes + cell_pts.bytes + lp.bytes +
-                                    old_vol.bytes);
+                                    old_vol.bytes + vorig.bytes);
     }
     int problems() const { return static_cast<int>(parts.size()); }
     int64_t workspace_bytes() const {
@@ -168,7 +170,7 @@ struct nat_mesh {
                    ovf[0].bytes +
                    ovf[1].bytes + counters.bytes + pool_vals.bytes + pool_cols.bytes +
                    csr_vals.bytes + csr_cols.bytes + csr_rowptr.bytes +
-                   rec.bytes + rec_off.bytes + start.bytes + wscr.bytes;
+                   rec.bytes + rec_off.bytes + start.bytes + wscr.bytes + vals_dev.bytes;
         for (int l = 0; l < kLevels; l++) s += scratch[l].bytes;
         return static_cast<int64_t>(s);
     }
@@ -178,7 +180,7 @@ struct nat_mesh {
                          &row_off,  &row_cnt,   &rowptr64,  &row_total, &ovf[0],    &ovf[1],
                          &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,   &csr_rowptr,
                          &rec,      &rec_off,   &start,     &lp,       &prob,       &qoff,
-                         &route,    &wscr,      &old_vol};
+                         &route,    &wscr,      &old_vol,   &vorig,    &vals_dev};
         for (DevBuf *b : all) b->release();
         for (int l = 0; l < kLevels; l++) scratch[l].release();
         for (hipEvent_t &e : ev) {
@@ -289,6 +291,46 @@ void build_grid(const nat::PackedMesh &h, double g0[3], double &gh, int gn[3],
     }
 }
 
+// 10-bit coordinate spread to every third bit (x -> bits 0, 3, 6, ...).
+unsigned spread3(unsigned x) {
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// perm[new] = old: the n points pts[3i..3i+2] in ascending 30-bit Morton
+// code over the box lo..hi (10 bits per axis; points outside are clamped
+// onto it), equal codes in index order -- a stable three-pass radix sort.
+void morton_perm(const double *pts, int n, const double lo[3], const double hi[3],
+                 std::vector<int> &perm) {
+    std::vector<unsigned> code(n);
+    double scale[3];
+    for (int k = 0; k < 3; k++) scale[k] = hi[k] > lo[k] ? 1024.0 / (hi[k] - lo[k]) : 0.0;
+    for (int i = 0; i < n; i++) {
+        unsigned c = 0;
+        for (int k = 0; k < 3; k++) {
+            const double f = (pts[3 * i + k] - lo[k]) * scale[k];
+            const unsigned u = f >= 1023.0 ? 1023u : (f > 0.0 ? static_cast<unsigned>(f) : 0u);
+            c |= spread3(u) << k;
+        }
+        code[i] = c;
+    }
+    perm.resize(n);
+    std::vector<int> tmp(n);
+    for (int i = 0; i < n; i++) perm[i] = i;
+    for (int pass = 0; pass < 3; pass++) {
+        const int sh = 10 * pass;
+        std::vector<int> cnt(1025, 0);
+        for (int i = 0; i < n; i++) cnt[((code[perm[i]] >> sh) & 1023u) + 1]++;
+        for (int b = 0; b < 1024; b++) cnt[b + 1] += cnt[b];
+        for (int i = 0; i < n; i++) tmp[cnt[(code[perm[i]] >> sh) & 1023u]++] = perm[i];
+        perm.swap(tmp);
+    }
+}
+
 // Offsets of the parts in the global numbering.
 void index_parts(nat_mesh *m) {
     const int P = m->problems();
@@ -351,6 +393,11 @@ int builder_error(int rc) {
 // Uploads the parts as one mesh with global numbering: data vertices of all
 // problems first (problem p's from vbase[p]), then the four super vertices of
 // each problem; tets and k-d nodes from tbase[p]; per-problem grid blocks.
+// Within a problem, data vertices and tets are renumbered in Morton order of
+// position and centroid over the data bounding box; every record keeps its
+// slot order (vertex slots, adjacency slots, k-d children), so the searches
+// and the arithmetic see the same mesh.  vorig maps each device vertex back
+// to its global key index (super vertices map to themselves).
 int upload_mesh(nat_mesh *m) {
     const int P = m->problems();
     const long long NT = m->vbase[P], TT = m->tbase[P];
@@ -362,17 +409,38 @@ int upload_mesh(nat_mesh *m) {
     std::vector<double4> kd(std::max<size_t>(T, 1));
     std::vector<int> start_all;
     std::vector<double4> pts_all(std::max<size_t>(T, 1));
+    std::vector<int> vorig(nv);
+    std::vector<int> vperm, vnew, tperm, tnew;
     m->lph.assign(P, nat::LocParams{});
     for (int p = 0; p < P; p++) {
         const nat::PackedMesh &h = m->parts[p];
         const int N = h.n_data;
         const int vb = static_cast<int>(m->vbase[p]), tb = static_cast<int>(m->tbase[p]);
         const int sb = static_cast<int>(NT) + 4 * p;
-        auto gv = [&](int v) { return v < N ? vb + v : sb + (v - N); };
-        for (int i = 0; i < N + 4; i++)
+        nat::LocParams &L = m->lph[p];
+        // bounding box of the data points: the Morton grid of the renumbering
+        // and of the queries
+        for (int k = 0; k < 3; k++) L.bbox[k] = L.bbox[3 + k] = h.verts[k];
+        for (int i = 0; i < N; i++)
+            for (int k = 0; k < 3; k++) {
+                L.bbox[k] = std::min(L.bbox[k], h.verts[3 * i + k]);
+                L.bbox[3 + k] = std::max(L.bbox[3 + k], h.verts[3 * i + k]);
+            }
+        morton_perm(h.verts.data(), N, L.bbox, L.bbox + 3, vperm);
+        morton_perm(h.kd_pos.data(), h.n_tets, L.bbox, L.bbox + 3, tperm);
+        vnew.assign(N, 0);
+        for (int i = 0; i < N; i++) vnew[vperm[i]] = i;
+        tnew.assign(h.n_tets, 0);
+        for (int t = 0; t < h.n_tets; t++) tnew[tperm[t]] = t;
+        auto gv = [&](int v) { return v < N ? vb + vnew[v] : sb + (v - N); };
+        auto gt = [&](int x) { return x < 0 ? -1 : tb + tnew[x]; };
+        for (int i = 0; i < N + 4; i++) {
             verts[gv(i)] =
                 make_double4(h.verts[3 * i], h.verts[3 * i + 1], h.verts[3 * i + 2], 0.0);
-        for (int t = 0; t < h.n_tets; t++) {
+            vorig[gv(i)] = i < N ? vb + i : sb + (i - N);
+        }
+        for (int nt = 0; nt < h.n_tets; nt++) {
+            const int t = tperm[nt];
             const int *pv = &h.tet_v[4 * t];
             auto pt = [&](int v) {
                 return nat::d3{h.verts[3 * v], h.verts[3 * v + 1], h.verts[3 * v + 2]};
@@ -381,21 +449,16 @@ int upload_mesh(nat_mesh *m) {
             // in bit 31 of p3 so that the common positive case reads unchanged
             const bool pos = nat::orient3d(pt(pv[0]), pt(pv[1]), pt(pv[2]), pt(pv[3])) > 0;
             const int w = gv(pv[3]);
-            tv[tb + t] = make_int4(gv(pv[0]), gv(pv[1]), gv(pv[2]),
-                                   pos ? w : (w | static_cast<int>(0x80000000u)));
-            int nb[4];
-            for (int k = 0; k < 4; k++) {
-                const int x = h.tet_n[4 * t + k];
-                nb[k] = x < 0 ? -1 : tb + x;
-            }
-            tn[tb + t] = make_int4(nb[0], nb[1], nb[2], nb[3]);
-            int2 lr = make_int2(h.kd_left[t] < 0 ? -1 : tb + h.kd_left[t],
-                                h.kd_right[t] < 0 ? -1 : tb + h.kd_right[t]);
+            tv[tb + nt] = make_int4(gv(pv[0]), gv(pv[1]), gv(pv[2]),
+                                    pos ? w : (w | static_cast<int>(0x80000000u)));
+            tn[tb + nt] = make_int4(gt(h.tet_n[4 * t]), gt(h.tet_n[4 * t + 1]),
+                                    gt(h.tet_n[4 * t + 2]), gt(h.tet_n[4 * t + 3]));
+            int2 lr = make_int2(gt(h.kd_left[t]), gt(h.kd_right[t]));
             double wd;
             std::memcpy(&wd, &lr, sizeof(wd));
-            kd[tb + t] = make_double4(h.kd_pos[3 * t], h.kd_pos[3 * t + 1], h.kd_pos[3 * t + 2], wd);
+            kd[tb + nt] =
+                make_double4(h.kd_pos[3 * t], h.kd_pos[3 * t + 1], h.kd_pos[3 * t + 2], wd);
         }
-        nat::LocParams &L = m->lph[p];
         std::vector<int> start;
         std::vector<double4> pts;
         build_grid(h, L.g0, L.gh, L.gn, start, pts);
@@ -405,23 +468,16 @@ int upload_mesh(nat_mesh *m) {
         for (int t = 0; t < h.n_tets; t++) {
             long long id;
             std::memcpy(&id, &pts[t].w, sizeof(id));
-            id += tb;
+            id = gt(static_cast<int>(id));
             double wd;
             std::memcpy(&wd, &id, sizeof(wd));
             pts_all[tb + t] = make_double4(pts[t].x, pts[t].y, pts[t].z, wd);
         }
-        L.kd_root = h.n_tets > 0 ? tb : -1;
+        L.kd_root = h.n_tets > 0 ? gt(0) : -1;
         for (int k = 0; k < 3; k++) {
             L.rmin[k] = h.kd_rect[k];
             L.rmax[k] = h.kd_rect[3 + k];
         }
-        // bounding box of the data points: the Morton grid of the queries
-        for (int k = 0; k < 3; k++) L.bbox[k] = L.bbox[3 + k] = h.verts[k];
-        for (int i = 0; i < N; i++)
-            for (int k = 0; k < 3; k++) {
-                L.bbox[k] = std::min(L.bbox[k], h.verts[3 * i + k]);
-                L.bbox[3 + k] = std::max(L.bbox[3 + k], h.verts[3 * i + k]);
-            }
         if (start_all.size() > 2147483647ULL) return fail(-1, "batch grid too large");
     }
     HIP_OR_FAIL(m->verts.ensure(sizeof(double4) * nv));
@@ -431,6 +487,8 @@ int upload_mesh(nat_mesh *m) {
     HIP_OR_FAIL(m->cell_start.ensure(sizeof(int) * start_all.size()));
     HIP_OR_FAIL(m->cell_pts.ensure(sizeof(double4) * pts_all.size()));
     HIP_OR_FAIL(m->lp.ensure(sizeof(nat::LocParams) * P));
+    HIP_OR_FAIL(m->vorig.ensure(sizeof(int) * nv));
+    HIP_OR_FAIL(hipMemcpy(m->vorig.p, vorig.data(), sizeof(int) * nv, hipMemcpyHostToDevice));
     HIP_OR_FAIL(hipMemcpy(m->verts.p, verts.data(), sizeof(double4) * nv, hipMemcpyHostToDevice));
     HIP_OR_FAIL(hipMemcpy(m->tet_v.p, tv.data(), sizeof(int4) * tv.size(), hipMemcpyHostToDevice));
     HIP_OR_FAIL(hipMemcpy(m->tet_n.p, tn.data(), sizeof(int4) * tn.size(), hipMemcpyHostToDevice));
@@ -757,7 +815,13 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
     topo.rec_used = nullptr;
     topo.rec_off = reinterpret_cast<long long *>(const_cast<char *>(blob) + kTopoHeader);
     const int *order = reinterpret_cast<const int *>(blob + topo_order_off(M));
-    nat::GradIn g{d_gy, d_values, D, d_dq, d_rowptr, d_w,
+    // the kernels read values by device vertex index: the caller's rows in
+    // that order (data vertices only; super vertices carry no value)
+    const int nd = m->dev.n_data;
+    HIP_OR_FAIL(m->vals_dev.ensure(sizeof(double) * std::max<size_t>(1, static_cast<size_t>(nd) * D)));
+    HIP_OR_FAIL(nat::launch_gather_rows(d_values, m->vorig.as<int>(), nd, D,
+                                        m->vals_dev.as<double>(), st));
+    nat::GradIn g{d_gy, m->vals_dev.as<double>(), D, d_dq, d_rowptr, d_w,
                   reinterpret_cast<const double *>(blob + topo_total_off(M))};
     for (int l = 0; l < kLevels; l++) {
         m->times.grad_ms[l] = 0.0;
@@ -841,8 +905,9 @@ int emit_rows(nat_mesh *m, double *d_vals, int *d_cols, int *d_rowptr, hipStream
     HIP_OR_FAIL(nat::launch_scan_counts(m->row_cnt.as<int>(), rp64, M, m->scan.as<long long>(),
                                         st));
     HIP_OR_FAIL(nat::launch_gather_csr(rp64, m->row_off.as<long long>(), m->row_cnt.as<int>(),
-                                       m->pool_vals.as<double>(), m->pool_cols.as<int>(), M,
-                                       m->last_nnz, d_vals, d_cols, d_rowptr, st));
+                                       m->pool_vals.as<double>(), m->pool_cols.as<int>(),
+                                       m->vorig.as<int>(), M, m->last_nnz, d_vals, d_cols,
+                                       d_rowptr, st));
     mark(m, 1, st);
     m->times.emit_ms = elapsed(m, 0, 1);
     return 0;
@@ -955,9 +1020,9 @@ int issue_graph_rows(nat_mesh *m, nat_graph *g, const double *d_q, hipStream_t s
     HIP_OR_FAIL(nat::launch_scan_counts(m->row_cnt.as<int>(), rp64, M, m->scan.as<long long>(),
                                         st));
     HIP_OR_FAIL(nat::launch_gather_csr(rp64, m->row_off.as<long long>(), m->row_cnt.as<int>(),
-                                       m->pool_vals.as<double>(), m->pool_cols.as<int>(), M,
-                                       g->cap, g->vals.as<double>(), g->cols.as<int>(),
-                                       g->rowptr.as<int>(), st));
+                                       m->pool_vals.as<double>(), m->pool_cols.as<int>(),
+                                       m->vorig.as<int>(), M, g->cap, g->vals.as<double>(),
+                                       g->cols.as<int>(), g->rowptr.as<int>(), st));
     return 0;
 }
 

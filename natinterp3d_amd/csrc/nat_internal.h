@@ -14,19 +14,25 @@ struct LocParams {
     double gh, ginv;     // cell edge and its inverse
     int gn[3];           // cells per axis
     int cell_base;       // first entry of this problem's cells in cell_start
-    int kd_root;         // tet index of the k-d root (the problem's first tet)
+    int kd_root;         // tet index of the k-d root (the builder's first tet)
     int pad_;
     double rmin[3], rmax[3];  // k-d root hyper-rectangle
     double bbox[6];           // data bounding box min[3], max[3]
 };
 
-// Immutable device mesh.  All arrays are packed SoA in tet order (the
-// reference's linked-list order), 16- or 32-byte records so that one tet or
-// one vertex is one aligned vector load.  A batch of independent problems is
-// one mesh with global numbering: the data vertices of every problem first
-// (problem p's at its vertex offset, so CSR columns are global), then the
-// four super vertices of each problem; tets and k-d nodes at the problem's
-// tet offset.  The super-vertex test stays "slot >= n_data".
+// Immutable device mesh.  All arrays are packed SoA, 16- or 32-byte records
+// so that one tet or one vertex is one aligned vector load.  Within each
+// problem the data vertices are numbered by the Morton code of their
+// position and the tets (k-d nodes) by that of their centroid, so that the
+// records neighbouring queries read share cache lines; the host mesh keeps
+// the builder's numbering and CSR columns are mapped back to key indices
+// when rows are gathered.  Indices are labels to every kernel: orders come
+// from vertex slots, adjacency slots and the k-d tree's shape, never from
+// index comparisons.  A batch of independent problems is one mesh with
+// global numbering: the data vertices of every problem first (problem p's at
+// its vertex offset), then the four super vertices of each problem; tets and
+// k-d nodes at the problem's tet offset.  The super-vertex test stays
+// "slot >= n_data".
 struct DevMesh {
     const double4 *verts;  // [N+4P] x,y,z,0; slots N.. are the super vertices
     const int4 *tet_v;     // [T] vertex slots p0..p3; bit 31 of .w set when
@@ -208,11 +214,15 @@ hipError_t launch_spmm_t(int M, int N, int D, const int *rowptr, const int *cols
 hipError_t launch_scan_counts(const int *d_in, long long *d_out, int n, long long *d_tmp,
                               hipStream_t st);
 size_t scan_tmp_elems(int n);
-// rows reaching past cap entries (pool or output) are skipped
+// rows reaching past cap entries (pool or output) are skipped; columns pass
+// through vmap (device vertex index -> key index)
 hipError_t launch_gather_csr(const long long *d_rowptr64, const long long *d_row_off,
                              const int *d_row_cnt, const double *pool_vals,
-                             const int *pool_cols, int M, long long cap, double *d_vals,
-                             int *d_cols, int *d_rowptr32, hipStream_t st);
+                             const int *pool_cols, const int *vmap, int M, long long cap,
+                             double *d_vals, int *d_cols, int *d_rowptr32, hipStream_t st);
+// out[i, :] = in[idx[i], :], rows of D doubles, i < n
+hipError_t launch_gather_rows(const double *in, const int *idx, int n, int D, double *out,
+                              hipStream_t st);
 hipError_t launch_sort_rows(int M, const int *rowptr, int *cols, void *vals, int vdt,
                             hipStream_t st);
 hipError_t launch_spmm(int M, int D, const int *rowptr, const int *cols, const void *vals,
