@@ -156,7 +156,7 @@ def run_precision(args):
 
     import natinterp3d_amd
     from natinterp3d_amd import workloads
-    from natinterp3d_amd.natinterp3d import spmm_device
+    from natinterp3d_amd.natinterp3d import row_order, spmm_device
 
     keys, queries, values = workloads.lattice_jitter(n_queries=args.queries)
     keys = np.ascontiguousarray(keys, np.float64)
@@ -175,7 +175,7 @@ def run_precision(args):
 
     def fwd():
         crow, col, val = mesh.weights_device(qd)
-        return spmm_device(crow, col, val, xd, M), col, val
+        return spmm_device(crow, col, val, xd, M, order=row_order(mesh.handle, M)), col, val
 
     def timed():
         for _ in range(args.warmup):
@@ -374,7 +374,7 @@ def run_batch(args):
 
     import natinterp3d_amd
     from natinterp3d_amd import workloads
-    from natinterp3d_amd.natinterp3d import spmm_device
+    from natinterp3d_amd.natinterp3d import row_order, spmm_device
 
     probs = workloads.batch_problems(n_problems=args.problems, shell_every=args.shell_every)
     nk = sum(len(k) for k, _, _ in probs)
@@ -407,14 +407,16 @@ def run_batch(args):
 
     def batched():
         crow, col, val = batch.weights_device(qcat, off)
-        return spmm_device(crow, col, val, vcat, int(off[-1])), col.numel()
+        return spmm_device(crow, col, val, vcat, int(off[-1]),
+                           order=row_order(batch.handle, int(off[-1]))), col.numel()
 
     def sequential():
         n = 0
         outs = []
         for m, q, v in zip(singles, qd, vd):
             crow, col, val = m.weights_device(q)
-            outs.append(spmm_device(crow, col, val, v, q.shape[0]))
+            outs.append(spmm_device(crow, col, val, v, q.shape[0],
+                                    order=row_order(m.handle, q.shape[0])))
             n += col.numel()
         return outs, n
 
@@ -539,7 +541,7 @@ def main():
     import natinterp3d_amd
     from natinterp3d_amd import workloads
     from natinterp3d_amd.autograd import SibsonInterpolation
-    from natinterp3d_amd.natinterp3d import spmm_device
+    from natinterp3d_amd.natinterp3d import row_order, spmm_device
 
     log(f"workload: {args.keys:,} keys, {args.queries:,} queries")
     keys, queries, values = workloads.plain_at_scale(n_keys=args.keys, n_queries=args.queries)
@@ -588,7 +590,8 @@ def main():
     def plain_forward():
         with torch.no_grad():
             crow, col, val = mesh.weights_device(qd.detach())
-            return spmm_device(crow, col, val, xd.detach(), M), col.numel()
+            return spmm_device(crow, col, val, xd.detach(), M,
+                               order=row_order(mesh.handle, M)), col.numel()
 
     for i in range(args.warmup):
         y = step()

@@ -106,11 +106,22 @@ int nat_weights_emit(nat_mesh *m, double *d_vals, int32_t *d_cols, int32_t *d_ro
  * d_out[M,D] = W[M,N] (CSR, d_rowptr int32[M+1], d_cols int32, d_vals)
  * x d_values[N,D].  dtype codes: 0 = float32, 1 = float64; vals_dtype is the
  * weight type, values_dtype the value type, out_dtype the accumulation and
- * output type (the NumPy result type of the two). */
+ * output type (the NumPy result type of the two).  Rows are visited in
+ * d_row_order when it is given (a permutation of 0..M-1; NULL: 0..M-1); the
+ * result does not depend on it, the speed does: a spatial order such as
+ * nat_weights_row_order's lets neighbouring rows share the rows of d_values
+ * they read in cache. */
 int nat_spmm_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
                  const int32_t *d_cols, const void *d_vals, int vals_dtype,
-                 const void *d_values, int values_dtype, void *d_out, int out_dtype,
-                 void *stream);
+                 const void *d_values, int values_dtype, const int32_t *d_row_order,
+                 void *d_out, int out_dtype, void *stream);
+
+/* Device pointer to the query order of the last nat_weights_compute on this
+ * mesh (int32[M], a permutation of 0..M-1 in which neighbouring queries are
+ * close in space), for nat_spmm_csr.  Valid until the next compute on the
+ * mesh; returns -1 when num_query_points is not that compute's M.  (No
+ * reference counterpart: scipy visits rows in index order.) */
+int nat_weights_row_order(const nat_mesh *m, int num_query_points, const int32_t **out);
 
 /* Sort the entries of every CSR row by column, in place (d_vals dtype codes
  * as above).  The reference's Interpolator.get_weights casts W to the query

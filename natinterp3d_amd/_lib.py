@@ -21,7 +21,7 @@ EXPORTS = (
     "nat_mesh_set_precision", "nat_predicate_audit", "nat_insphere_classify",
     "nat_graph_create", "nat_graph_launch", "nat_graph_copy_out", "nat_graph_free",
     "nat_mesh_set_policy", "nat_mesh_get_policy", "nat_mesh_autotune",
-    "nat_mesh_from_packed",
+    "nat_mesh_from_packed", "nat_weights_row_order",
 )
 
 
@@ -95,7 +95,9 @@ def lib():
     L.nat_weights_emit.argtypes = [vp, vp, vp, vp, vp]
     L.nat_spmm_csr.restype = ctypes.c_int
     L.nat_spmm_csr.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp,
-                               ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, vp]
+                               ctypes.c_int, vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]
+    L.nat_weights_row_order.restype = ctypes.c_int
+    L.nat_weights_row_order.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
     L.nat_csr_sort_rows.restype = ctypes.c_int
     L.nat_csr_sort_rows.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int, vp]
     L.nat_topology_save.restype = ctypes.c_int

@@ -23,7 +23,7 @@ import ctypes
 import torch
 
 from . import _lib
-from .natinterp3d import spmm_device
+from .natinterp3d import row_order, spmm_device
 
 
 def _vp(t):
@@ -55,7 +55,7 @@ class SibsonInterpolation(torch.autograd.Function):
             _lib.check(L.nat_topology_save(mesh.handle, ctypes.c_void_p(topo.data_ptr()),
                                            topo.numel(), ctypes.byref(nbytes),
                                            ctypes.c_void_p(stream)), "nat_topology_save")
-        y = spmm_device(crow, col, val, v, q.shape[0])
+        y = spmm_device(crow, col, val, v, q.shape[0], order=row_order(mesh.handle, q.shape[0]))
         ctx.mesh = mesh
         ctx.values_dtype = values.dtype
         ctx.queries_dtype = queries.dtype

@@ -17,7 +17,7 @@ import scipy.sparse
 
 from . import _lib
 from .natinterp3d import (_DT_CODE, _default_device, autotune_rows, get_row_policy,
-                          set_row_policy, spmm_device)
+                          row_order, set_row_policy, spmm_device)
 
 
 class BatchInterpolator:
@@ -211,7 +211,8 @@ class BatchInterpolator:
                     ctypes.c_void_p(stream)), "nat_csr_sort_rows")
             vd = vdtype if vdtype in _DT_CODE else res_dtype
             xd = torch.from_numpy(np.ascontiguousarray(np.concatenate(v2, axis=0), vd)).to(dev)
-            y = spmm_device(crow, col, val, xd, q.shape[0]).cpu().numpy()
+            y = spmm_device(crow, col, val, xd, q.shape[0],
+                            order=row_order(self.handle, q.shape[0])).cpu().numpy()
         if y.dtype != res_dtype:
             y = y.astype(res_dtype)
         outs = [y[off[p]:off[p + 1]] for p in range(self.num_problems)]

@@ -303,14 +303,18 @@ hipError_t launch_sort_rows(int M, const int *rowptr, int *cols, void *vals, int
 // ------------------------------------------------------------------- SpMM
 
 // y[i,d] = sum over the row's entries in stored order of w * x[col,d],
-// starting from zero, in the output type (scipy csr_matvec(s)).
+// starting from zero, in the output type (scipy csr_matvec(s)).  One thread
+// per (row, d); rows taken in `order` when given.
 template <typename TW, typename TX, typename TY>
 __global__ void spmm_kernel(int M, int D, const int *__restrict__ rowptr,
                             const int *__restrict__ cols, const TW *__restrict__ w,
-                            const TX *__restrict__ x, TY *__restrict__ y) {
+                            const TX *__restrict__ x, const int *__restrict__ order,
+                            TY *__restrict__ y) {
     const long long gid = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (gid >= static_cast<long long>(M) * D) return;
-    const int i = static_cast<int>(gid / D), d = static_cast<int>(gid % D);
+    const int i = order ? order[gid / D] : static_cast<int>(gid / D);
+    const int d = static_cast<int>(gid % D);
+    if (i < 0 || i >= M) return;
     TY acc = 0;
     const int b = rowptr[i], e = rowptr[i + 1];
     for (int j = b; j < e; j++) {
@@ -318,29 +322,102 @@ __global__ void spmm_kernel(int M, int D, const int *__restrict__ rowptr,
         const TY v = static_cast<TY>(x[static_cast<size_t>(cols[j]) * D + d]);
         acc += a * v;
     }
-    y[gid] = acc;
+    y[static_cast<size_t>(i) * D + d] = acc;
+}
+
+// The same sums for D <= 4 with 16 lanes per row: lane l forms the D
+// products of entries b + l, b + 16 + l, ... (the row's weights and columns
+// read by consecutive lanes), and lane d of the row adds column d's products
+// in stored order from LDS -- the same roundings and the same sequence of
+// additions as the loop above.  Rows in `order` (a spatial order keeps the
+// rows of x that neighbouring rows share in L2).
+constexpr int kSpmmLanes = 16;
+constexpr int kSpmmRows = 256 / kSpmmLanes;
+
+__device__ __forceinline__ void lanes_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <typename TW, typename TX, typename TY, int D>
+__global__ void __launch_bounds__(256) spmm_rows_kernel(int M, const int *__restrict__ rowptr,
+                                                        const int *__restrict__ cols,
+                                                        const TW *__restrict__ w,
+                                                        const TX *__restrict__ x,
+                                                        const int *__restrict__ order,
+                                                        TY *__restrict__ y) {
+    __shared__ TY prod[kSpmmRows][kSpmmLanes][D];
+    const int lane = static_cast<int>(threadIdx.x) & (kSpmmLanes - 1);
+    const int r = static_cast<int>(threadIdx.x) / kSpmmLanes;
+    const long long k = static_cast<long long>(blockIdx.x) * kSpmmRows + r;
+    int i = -1;
+    if (k < M) i = order ? order[k] : static_cast<int>(k);
+    if (i < 0 || i >= M) return;  // the row's 16 lanes leave together
+    const int b = rowptr[i], e = rowptr[i + 1];
+    TY acc = 0;
+    for (int c0 = b; c0 < e; c0 += kSpmmLanes) {
+        const int j = c0 + lane;
+        if (j < e) {
+            const TY a = static_cast<TY>(w[j]);
+            const size_t xb = static_cast<size_t>(cols[j]) * D;
+#pragma unroll
+            for (int d = 0; d < D; d++) prod[r][lane][d] = a * static_cast<TY>(x[xb + d]);
+        }
+        lanes_sync();
+        if (lane < D) {
+            const int n = min(kSpmmLanes, e - c0);
+            for (int t = 0; t < n; t++) acc += prod[r][t][lane];
+        }
+        lanes_sync();
+    }
+    if (lane < D) y[static_cast<size_t>(i) * D + lane] = acc;
 }
 
 template <typename TW, typename TX, typename TY>
 static hipError_t spmm_t(int M, int D, const int *rp, const int *c, const void *w, const void *x,
-                         void *y, hipStream_t st) {
+                         const int *order, void *y, hipStream_t st) {
     const long long n = static_cast<long long>(M) * D;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL((spmm_kernel<TW, TX, TY>), dim3(static_cast<unsigned>((n + 255) / 256)),
-                       dim3(256), 0, st, M, D, rp, c, static_cast<const TW *>(w),
-                       static_cast<const TX *>(x), static_cast<TY *>(y));
+    const dim3 rows_grid(static_cast<unsigned>((M + kSpmmRows - 1) / kSpmmRows));
+    const TW *wt = static_cast<const TW *>(w);
+    const TX *xt = static_cast<const TX *>(x);
+    TY *yt = static_cast<TY *>(y);
+    switch (D) {
+        case 1:
+            hipLaunchKernelGGL((spmm_rows_kernel<TW, TX, TY, 1>), rows_grid, dim3(256), 0, st, M,
+                               rp, c, wt, xt, order, yt);
+            break;
+        case 2:
+            hipLaunchKernelGGL((spmm_rows_kernel<TW, TX, TY, 2>), rows_grid, dim3(256), 0, st, M,
+                               rp, c, wt, xt, order, yt);
+            break;
+        case 3:
+            hipLaunchKernelGGL((spmm_rows_kernel<TW, TX, TY, 3>), rows_grid, dim3(256), 0, st, M,
+                               rp, c, wt, xt, order, yt);
+            break;
+        case 4:
+            hipLaunchKernelGGL((spmm_rows_kernel<TW, TX, TY, 4>), rows_grid, dim3(256), 0, st, M,
+                               rp, c, wt, xt, order, yt);
+            break;
+        default:
+            hipLaunchKernelGGL((spmm_kernel<TW, TX, TY>),
+                               dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, M,
+                               D, rp, c, wt, xt, order, yt);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_spmm(int M, int D, const int *rp, const int *c, const void *w, int vdt,
-                       const void *x, int xdt, void *y, int ydt, hipStream_t st) {
+                       const void *x, int xdt, const int *order, void *y, int ydt,
+                       hipStream_t st) {
     const int code = vdt * 4 + xdt * 2 + ydt;
     switch (code) {
-        case 0: return spmm_t<float, float, float>(M, D, rp, c, w, x, y, st);
-        case 1: return spmm_t<float, float, double>(M, D, rp, c, w, x, y, st);
-        case 3: return spmm_t<float, double, double>(M, D, rp, c, w, x, y, st);
-        case 5: return spmm_t<double, float, double>(M, D, rp, c, w, x, y, st);
-        case 7: return spmm_t<double, double, double>(M, D, rp, c, w, x, y, st);
+        case 0: return spmm_t<float, float, float>(M, D, rp, c, w, x, order, y, st);
+        case 1: return spmm_t<float, float, double>(M, D, rp, c, w, x, order, y, st);
+        case 3: return spmm_t<float, double, double>(M, D, rp, c, w, x, order, y, st);
+        case 5: return spmm_t<double, float, double>(M, D, rp, c, w, x, order, y, st);
+        case 7: return spmm_t<double, double, double>(M, D, rp, c, w, x, order, y, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -1411,8 +1411,8 @@ int nat_get_weights(const double *query_points, int num_query_points, nat_mesh *
 
 int nat_spmm_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
                  const int32_t *d_cols, const void *d_vals, int vals_dtype,
-                 const void *d_values, int values_dtype, void *d_out, int out_dtype,
-                 void *stream) {
+                 const void *d_values, int values_dtype, const int32_t *d_row_order,
+                 void *d_out, int out_dtype, void *stream) {
     g_err.clear();
     (void)num_cols;
     if (num_rows < 0 || dim < 0) return fail(-1, "negative size");
@@ -1423,9 +1423,19 @@ int nat_spmm_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
         return fail(-1, "unsupported dtype combination %d %d %d", vals_dtype, values_dtype,
                     out_dtype);
     hipError_t e = nat::launch_spmm(num_rows, dim, d_rowptr, d_cols, d_vals, vals_dtype,
-                                    d_values, values_dtype, d_out, out_dtype,
+                                    d_values, values_dtype, d_row_order, d_out, out_dtype,
                                     static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail(-10, "spmm launch failed: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int nat_weights_row_order(const nat_mesh *m, int num_query_points, const int32_t **out) {
+    g_err.clear();
+    if (!m || !out) return fail(-1, "NULL argument");
+    if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    if (num_query_points != m->last_M || m->last_M < 0)
+        return fail(-1, "last compute had %d queries, not %d", m->last_M, num_query_points);
+    *out = m->order.as<const int32_t>();
     return 0;
 }
 

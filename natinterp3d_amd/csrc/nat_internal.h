@@ -225,7 +225,9 @@ hipError_t launch_gather_rows(const double *in, const int *idx, int n, int D, do
                               hipStream_t st);
 hipError_t launch_sort_rows(int M, const int *rowptr, int *cols, void *vals, int vdt,
                             hipStream_t st);
+// rows in order (a permutation of 0..M-1) when it is given
 hipError_t launch_spmm(int M, int D, const int *rowptr, const int *cols, const void *vals,
-                       int vdt, const void *x, int xdt, void *y, int ydt, hipStream_t st);
+                       int vdt, const void *x, int xdt, const int *order, void *y, int ydt,
+                       hipStream_t st);
 
 }  // namespace nat

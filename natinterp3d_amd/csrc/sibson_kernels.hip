@@ -2455,10 +2455,16 @@ __global__ void __launch_bounds__(256) spmm_t_kernel(int M, int D, const int *__
 }
 
 // W^T g with per-block aggregation: a block takes 256 consecutive rows of
-// `order` (neighbouring queries, which share most of their neighbours), sums
-// their contributions per column in an LDS hash table of 2^hlog2 columns x D,
-// and adds each distinct column to out once, with float64 atomics.  An entry
-// whose column finds no table slot within 16 probes goes to out directly.
+// `order` (neighbouring queries, which share most of their neighbours), 16
+// lanes per row and 16 rows at a time (a row's entries read by consecutive
+// lanes; the entries of one row have distinct columns, so at most the 4 rows
+// of a wave meet on a table slot at once), sums their contributions per
+// column in an LDS hash table of 2^hlog2 columns x D, and adds each distinct
+// column to out once, with float64 atomics.  An entry whose column finds no
+// table slot within 16 probes goes to out directly.
+constexpr int kAggRows = 256;
+constexpr int kAggLanes = 16;
+
 __global__ void __launch_bounds__(256) spmm_t_agg_kernel(int M, int D, int hlog2,
                                                          const int *__restrict__ rowptr,
                                                          const int *__restrict__ cols,
@@ -2473,13 +2479,19 @@ __global__ void __launch_bounds__(256) spmm_t_agg_kernel(int M, int D, int hlog2
     for (int i = threadIdx.x; i < H; i += blockDim.x) hk[i] = -1;
     for (int i = threadIdx.x; i < H * D; i += blockDim.x) hv[i] = 0.0;
     __syncthreads();
-    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < M) {
-        const int row = order ? order[i] : static_cast<int>(i);
+    const int lane = static_cast<int>(threadIdx.x) & (kAggLanes - 1);
+    const long long base = static_cast<long long>(blockIdx.x) * kAggRows;
+    for (int k = static_cast<int>(threadIdx.x) / kAggLanes; k < kAggRows;
+         k += 256 / kAggLanes) {
+        const long long ii = base + k;
+        if (ii >= M) break;
+        const int row = order ? order[ii] : static_cast<int>(ii);
+        if (row < 0 || row >= M) continue;
         const double *g = gy + static_cast<size_t>(row) * D;
-        for (int e = rowptr[row]; e < rowptr[row + 1]; e++) {
-            const int c = cols[e];
-            const double w = vals[e];
+        const int e = rowptr[row + 1];
+        for (int j = rowptr[row] + lane; j < e; j += kAggLanes) {
+            const int c = cols[j];
+            const double w = vals[j];
             unsigned h = hash_key(c, hlog2);
             int slot = -1;
             for (int n = 0; n < 16; n++) {
@@ -2696,7 +2708,8 @@ hipError_t launch_spmm_t(int M, int N, int D, const int *rowptr, const int *cols
         hlog2--;
     if (hlog2 >= 7) {
         const size_t lds = (static_cast<size_t>(1) << hlog2) * (4 + 8 * static_cast<size_t>(D));
-        hipLaunchKernelGGL(spmm_t_agg_kernel, dim3(static_cast<unsigned>((M + 255) / 256)),
+        hipLaunchKernelGGL(spmm_t_agg_kernel,
+                           dim3(static_cast<unsigned>((M + kAggRows - 1) / kAggRows)),
                            dim3(256), lds, st, M, D, hlog2, rowptr, cols, vals, gy, order, out);
     } else {
         hipLaunchKernelGGL(spmm_t_kernel, dim3(static_cast<unsigned>((nt + 255) / 256)),

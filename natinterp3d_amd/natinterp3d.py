@@ -283,9 +283,20 @@ class MeshAndVerticesHIP:
         return crow, col, val
 
 
-def spmm_device(crow, col, val, values, num_rows):
+def row_order(handle, num_rows):
+    """Device pointer (ctypes.c_void_p) to the query order of the last weights
+    compute on the mesh ``handle`` (neighbouring queries close in space), for
+    ``spmm_device``; valid until the next compute on that mesh."""
+    out = ctypes.c_void_p()
+    _lib.check(_lib.lib().nat_weights_row_order(handle, num_rows, ctypes.byref(out)),
+               "nat_weights_row_order")
+    return out
+
+
+def spmm_device(crow, col, val, values, num_rows, order=None):
     """``W @ values`` on the device with scipy's accumulation order and dtype
-    rule; values [N,D] float32/float64, val float32/float64."""
+    rule; values [N,D] float32/float64, val float32/float64.  ``order``
+    (``row_order``) only sets the order rows are visited in."""
     import torch
     out_dt = torch.float64 if (val.dtype == torch.float64 or values.dtype == torch.float64) \
         else torch.float32
@@ -297,7 +308,7 @@ def spmm_device(crow, col, val, values, num_rows):
         num_rows, values.shape[0], D, ctypes.c_void_p(crow.data_ptr()),
         ctypes.c_void_p(col.data_ptr() if col.numel() else 0),
         ctypes.c_void_p(val.data_ptr() if val.numel() else 0), codes[val.dtype],
-        ctypes.c_void_p(values.data_ptr()), codes[values.dtype],
+        ctypes.c_void_p(values.data_ptr()), codes[values.dtype], order,
         ctypes.c_void_p(out.data_ptr()), codes[out_dt], ctypes.c_void_p(stream))
     _lib.check(rc, "nat_spmm_csr")
     return out
@@ -478,7 +489,8 @@ class Interpolator:
                     ctypes.c_void_p(stream)), "nat_csr_sort_rows")
             vdt = vals2.dtype if vals2.dtype in _DT_CODE else res_dtype
             xd = torch.from_numpy(np.ascontiguousarray(vals2, vdt)).to(dev)
-            y = spmm_device(crow, col, val, xd, q.shape[0])
+            y = spmm_device(crow, col, val, xd, q.shape[0],
+                            order=row_order(self.mesh_and_vertices.handle, q.shape[0]))
             out = y.cpu().numpy()
         if out.dtype != res_dtype:
             out = out.astype(res_dtype)

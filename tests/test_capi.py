@@ -69,8 +69,10 @@ def test_argument_checks():
     assert L.nat_mesh_build(dp, 0, -1, ctypes.byref(out)) < 0
     assert "data point" in _lib.last_error()
     assert L.nat_mesh_build(dp, 4, -7, ctypes.byref(out)) < 0
-    assert L.nat_spmm_csr(-1, 1, 1, None, None, None, 1, None, 1, None, 1, None) < 0
-    assert L.nat_spmm_csr(3, 1, 1, None, None, None, 1, None, 0, None, 0, None) < 0
+    assert L.nat_spmm_csr(-1, 1, 1, None, None, None, 1, None, 1, None, None, 1, None) < 0
+    assert L.nat_spmm_csr(3, 1, 1, None, None, None, 1, None, 0, None, None, 0, None) < 0
+    order = ctypes.c_void_p()
+    assert L.nat_weights_row_order(None, 3, ctypes.byref(order)) < 0
 
 
 def test_query_shape_checks():

@@ -160,3 +160,31 @@ def test_interpolate_matches_scipy_bitwise(oracle_mod, vdt, qdt):
     assert np.array_equal(y, expect)
     y1 = it.interpolate(q, vals[:, 0])
     assert y1.shape == (5000,) and np.array_equal(y1, W @ vals[:, 0])
+
+
+@pytest.mark.parametrize("vdt", [np.float64, np.float32])
+def test_spmm_widths_and_row_order_bitwise(vdt):
+    """W @ values for value widths 1..7 (the 16-lane kernel up to 4, one
+    thread per (row, column) beyond) equals scipy's product bit for bit,
+    with rows visited in index order and in the compute's spatial order;
+    rows longer than 16 entries cross the lane chunk."""
+    import torch
+    from natinterp3d_amd.natinterp3d import row_order, spmm_device
+    rng = np.random.default_rng(17)
+    keys = rng.random((2500, 3))
+    q = 0.05 + 0.9 * rng.random((4000, 3))
+    it = natinterp3d_amd.Interpolator(keys)
+    m = it.mesh_and_vertices
+    W = it.get_weights(q)
+    assert np.diff(W.indptr).max() > 16
+    qd = torch.from_numpy(q - it._centroid).cuda()
+    crow, col, val = m.weights_device(qd)
+    order = row_order(m.handle, len(q))
+    for D in range(1, 8):
+        vals = rng.standard_normal((2500, D)).astype(vdt)
+        expect = W @ vals
+        xd = torch.from_numpy(vals).cuda()
+        for o in (None, order):
+            y = spmm_device(crow, col, val, xd, len(q), order=o).cpu().numpy()
+            assert y.dtype == expect.dtype
+            assert np.array_equal(y, expect), (D, o is not None)
