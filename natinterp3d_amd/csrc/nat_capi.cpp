@@ -125,6 +125,7 @@ struct nat_mesh {
     int nthr[kLevels] = {0, 0, 0, 0};  // [0]: locate kernel threads; the most per level
     int scap[kLevels] = {0, 0, 0, 0};  // threads the level's scratch is laid out for
     int nthr_geom = 0;
+    int grad_blocks = 0;               // resident blocks of the gradient geometry kernel
     int topo_blocks = 0;
     DevBuf scratch[kLevels];
     DevBuf start;                      // start tet per query
@@ -214,6 +215,7 @@ void set_levels(nat_mesh *m) {
     m->nthr[0] = cus * nat::locate_blocks_per_cu() * 256;
     m->topo_blocks = cus * nat::topo_blocks_per_cu();
     m->nthr_geom = cus * nat::geom_blocks_per_cu() * nat::kGeomThreads;
+    m->grad_blocks = cus * nat::grad_geom_blocks_per_cu();
     // larger levels (fused kernel) only see the queries that overflowed; a
     // level's scratch holds as many threads as it has queries, up to these
     // counts (level 1 at most ~31 GB, level 2 ~7 GB, level 3 ~2 GB).  Level 1
@@ -828,7 +830,7 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
         m->times.grad_items[l] = 0;
     }
     mark(m, 2, st);
-    HIP_OR_FAIL(nat::launch_grad_geom(m->dev, m->nthr_geom / nat::kGeomThreads, d_q, order, M,
+    HIP_OR_FAIL(nat::launch_grad_geom(m->dev, m->grad_blocks, d_q, order, M,
                                       ctr + 32, topo, g, m->ovf[0].as<int>(), ctr + 8, st));
     mark(m, 3, st);
     unsigned novf = 0;

@@ -153,6 +153,7 @@ hipError_t launch_problem_ids(const long long *d_offsets, int P, int M, int *d_p
 int rows_blocks_per_cu();  // resident 256-thread blocks of the fused row kernel per CU
 int topo_blocks_per_cu();   // 128-thread blocks of the topology kernel
 int geom_blocks_per_cu();  // kGeomThreads-thread blocks
+int grad_geom_blocks_per_cu();  // resident blocks of the gradient geometry kernel
 // fused row kernel: used for the larger capacity levels
 // d_n_items != NULL: the item count is read on the device (graph form)
 hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int level,

@@ -551,20 +551,22 @@ __device__ __forceinline__ void old_contributions_table(const DevMesh &m, int t,
 // faces (pk, q, A) and (pk, q, B), cv the tet's circumcentre.  Returns its
 // moment divided by |pk - q|^2, triangle areas signed along pk - q (the
 // boundary faces' consistent orientation makes the pieces add up).
+// Everything relative to q: with h = m - q = w / 2 and A, V, B the
+// circumcentres minus q, twice the triangle areas are a1 = ((A-h) x (V-h)).w
+// and a2 = ((V-h) x (B-h)).w, their centroids (h+A+V)/3 and (h+V+B)/3, and
+// the result (a1 (h+A+V) + a2 (h+V+B)) / (6 |w|^2): one division.
 __device__ __forceinline__ d3 facet_moment(const d3 &pk, const d3 &q, const d3 &ca,
                                            const d3 &cb, const d3 &cv) {
     const d3 w = sub3(pk, q);
-    const d3 mq = {(pk.x + q.x) * 0.5, (pk.y + q.y) * 0.5, (pk.z + q.z) * 0.5};
-    const d3 u = sub3(ca, mq), v = sub3(cv, mq), x = sub3(cb, mq);
-    const double a1 = 0.5 * dot3(cross3(u, v), w);
-    const double a2 = 0.5 * dot3(cross3(v, x), w);
-    const double inv = 1.0 / dot3(w, w);
-    const d3 g1 = {(mq.x + ca.x + cv.x) / 3.0 - q.x, (mq.y + ca.y + cv.y) / 3.0 - q.y,
-                   (mq.z + ca.z + cv.z) / 3.0 - q.z};
-    const d3 g2 = {(mq.x + cv.x + cb.x) / 3.0 - q.x, (mq.y + cv.y + cb.y) / 3.0 - q.y,
-                   (mq.z + cv.z + cb.z) / 3.0 - q.z};
-    return {(a1 * g1.x + a2 * g2.x) * inv, (a1 * g1.y + a2 * g2.y) * inv,
-            (a1 * g1.z + a2 * g2.z) * inv};
+    const d3 h = {w.x * 0.5, w.y * 0.5, w.z * 0.5};
+    const d3 A = sub3(ca, q), V = sub3(cv, q), B = sub3(cb, q);
+    const d3 u = sub3(A, h), v = sub3(V, h), x = sub3(B, h);
+    const double a1 = dot3(cross3(u, v), w);
+    const double a2 = dot3(cross3(v, x), w);
+    const double s = 1.0 / (6.0 * dot3(w, w));
+    return {(a1 * (h.x + A.x + V.x) + a2 * (h.x + V.x + B.x)) * s,
+            (a1 * (h.y + A.y + V.y) + a2 * (h.y + V.y + B.y)) * s,
+            (a1 * (h.z + A.z + V.z) + a2 * (h.z + V.z + B.z)) * s};
 }
 
 struct NoMoment {
@@ -2338,61 +2340,95 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
 // dL/dq = (sum_j a_j G_j - abar sum_j G_j) / S, abar = sum_j a_j lambda_j,
 // so only the step-4 circumcentres and facet moments G_j are recomputed: no
 // old subcells, no volumes.  Queries without a record (made by a larger
-// level in the forward pass) are listed in ovf_list.  a_j =
-// <dL/dy, values[j]> per neighbour slot lives in LDS.
-__global__ void __launch_bounds__(kGeomThreads) grad_geom_kernel(
+// level in the forward pass) are listed in ovf_list.
+//
+// 16 lanes per row, 4 rows per wave, rows handed out 4 at a time: the rows
+// in flight on an XCD (and the vertex and tet lines they share) stay few
+// enough for its L2, where one thread per row kept ten times as many.  The
+// lanes take the neighbours (a_j = <dL/dy, values[j]>, kept in LDS) and the
+// boundary faces in turn; the sums are reduced over the 16 lanes (the
+// gradient has no prescribed summation order).
+constexpr int kGradLanes = 16;
+constexpr int kGradThreads = 256;
+constexpr int kGradBatch = 32;
+
+__global__ void __launch_bounds__(kGradThreads) grad_geom_kernel(
     DevMesh m, const double *__restrict__ q, const int *__restrict__ items, int n_items,
     unsigned int *work, Topo topo, GradIn g, int *ovf_list, unsigned int *ovf_count) {
-    __shared__ double a_lds[kLevel0Row * kGeomThreads];
-    double *nb_a = a_lds + threadIdx.x;
-    int parts_done = 0, first = 0, nbatch = 0;
-    while (part_batch(work, n_items, 64, parts_done, first, nbatch)) {
-        if (__lane_id() >= nbatch) continue;
-        const int qi = items ? items[first + __lane_id()] : first + __lane_id();
+    __shared__ double a_lds[kGradThreads / kGradLanes][kLevel0Row];
+    const int gl = __lane_id() & (kGradLanes - 1);
+    const int gw = __lane_id() / kGradLanes;
+    double *nb_a = a_lds[threadIdx.x / kGradLanes];
+    int parts_done = 0, first = 0, nbatch = 0, sub = kGradBatch;
+    for (;;) {
+        // a wave claims kGradBatch consecutive items and works through them
+        // 4 at a time: one atomic per claim, few rows in flight
+        if (sub >= nbatch) {
+            if (!part_batch(work, n_items, kGradBatch, parts_done, first, nbatch)) break;
+            sub = 0;
+        }
+        const int item = sub + gw;
+        sub += 64 / kGradLanes;
+        if (item >= nbatch) continue;
+        const int qi = items ? items[first + item] : first + item;
         const long long roff = topo.rec_off[qi];
         if (roff < 0) {
-            ovf_list[atomicAdd(ovf_count, 1u)] = qi;
+            if (gl == 0) ovf_list[atomicAdd(ovf_count, 1u)] = qi;
             continue;
         }
         const int4 *r = topo.rec + roff;
         const int4 hd = r[0];
         const int ncav = hd.x, nbnd = hd.y, nnb = hd.z;
         if (nnb > kLevel0Row) {  // not produced by the topology kernel; recompute
-            ovf_list[atomicAdd(ovf_count, 1u)] = qi;
+            if (gl == 0) ovf_list[atomicAdd(ovf_count, 1u)] = qi;
             continue;
         }
         const int e0 = g.rowptr[qi];
         // coincident rows (constant weights nearby) and empty rows: dq = 0
         if (ncav == 0 || g.rowptr[qi + 1] - e0 != nnb) {
-            store_dq(g, qi, {0.0, 0.0, 0.0});
+            if (gl == 0) store_dq(g, qi, {0.0, 0.0, 0.0});
             continue;
         }
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
         const int *rn = reinterpret_cast<const int *>(r + 1 + ncav + (nbnd + 3) / 4);
         double abar = 0.0;
-        for (int j = 0; j < nnb; j++) {
+        for (int j = gl; j < nnb; j += kGradLanes) {
             const double a = grad_dot(g, qi, rn[j]);
-            nb_a[j * kGeomThreads] = a;
+            nb_a[j] = a;
             abar += a * g.w[e0 + j];
         }
+        group_sync();
         d3 ag = {0.0, 0.0, 0.0}, tg = {0.0, 0.0, 0.0};
         const int *rf = reinterpret_cast<const int *>(r + 1 + ncav);
-        for (int bi = 0; bi < nbnd; bi++) {
+        for (int bi = gl; bi < nbnd; bi += kGradLanes) {
             const int code = rf[bi];
             const int4 cr = r[1 + (code >> 2)];
             new_contributions<false>(
                 m, cr.x, code & 3, qq, [&](int vi, int) { return rec_slot(cr, vi); },
                 [](int, double) {},
                 moment_fn([&](int slot, const d3 &gm) {
-                    const double a = nb_a[slot * kGeomThreads];
+                    const double a = nb_a[slot];
                     ag = {ag.x + a * gm.x, ag.y + a * gm.y, ag.z + a * gm.z};
                     tg = {tg.x + gm.x, tg.y + gm.y, tg.z + gm.z};
                 }));
         }
-        const double inv = 1.0 / g.row_total[qi];
-        store_dq(g, qi, {(ag.x - abar * tg.x) * inv, (ag.y - abar * tg.y) * inv,
-                         (ag.z - abar * tg.z) * inv});
+#pragma unroll
+        for (int d = kGradLanes / 2; d >= 1; d >>= 1) {
+            abar += __shfl_xor(abar, d, kGradLanes);
+            ag.x += __shfl_xor(ag.x, d, kGradLanes);
+            ag.y += __shfl_xor(ag.y, d, kGradLanes);
+            ag.z += __shfl_xor(ag.z, d, kGradLanes);
+            tg.x += __shfl_xor(tg.x, d, kGradLanes);
+            tg.y += __shfl_xor(tg.y, d, kGradLanes);
+            tg.z += __shfl_xor(tg.z, d, kGradLanes);
+        }
+        if (gl == 0) {
+            const double inv = 1.0 / g.row_total[qi];
+            store_dq(g, qi, {(ag.x - abar * tg.x) * inv, (ag.y - abar * tg.y) * inv,
+                             (ag.z - abar * tg.z) * inv});
+        }
+        group_sync();  // nb_a is rewritten by the group's next row
     }
 }
 
@@ -2575,6 +2611,14 @@ int topo_blocks_per_cu() {
     return n > 0 ? n : 1;
 }
 int locate_blocks_per_cu() { return blocks_per_cu(locate_kernel); }
+int grad_geom_blocks_per_cu() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grad_geom_kernel, kGradThreads, 0) !=
+        hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
+}
+
 int geom_blocks_per_cu() {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, geom_kernel<false>, kGeomThreads, 0) !=
@@ -2679,7 +2723,7 @@ hipError_t launch_grad_geom(const DevMesh &m, int nblocks, const double *d_q, co
                             int n_items, unsigned int *d_work, const Topo &topo,
                             const GradIn &g, int *ovf_list, unsigned int *ovf_count,
                             hipStream_t st) {
-    hipLaunchKernelGGL(grad_geom_kernel, dim3(nblocks), dim3(kGeomThreads), 0, st, m, d_q,
+    hipLaunchKernelGGL(grad_geom_kernel, dim3(nblocks), dim3(kGradThreads), 0, st, m, d_q,
                        d_items, n_items, d_work, topo, g, ovf_list, ovf_count);
     return hipGetLastError();
 }
