@@ -518,6 +518,9 @@ def main():
                     help="plain workload: also time the step with mixed-precision rows")
     ap.add_argument("--autotune", choices=["on", "off"], default="on",
                     help="plain workload: tune the row policy on the queries before timing")
+    ap.add_argument("--end-to-end", choices=["on", "off"], default="on",
+                    help="plain workload: also time the drop-in host call (host queries in, "
+                         "scipy CSR out), reported beside the device-resident value")
     ap.add_argument("--problems", type=int, default=256)
     ap.add_argument("--shell-every", type=int, default=0,
                     help="batch workload: every k-th problem a hollow shell (0: none)")
@@ -625,6 +628,25 @@ def main():
 
     # the same step with the mixed-precision rows (neighbour sets unchanged),
     # against the plain FP64 forward
+    # the drop-in host call of the reference's plugin (get_weights on host
+    # queries -> scipy CSR): H2D of the queries, the kernels, D2H of the CSR
+    # over PCIe and the CSR wrap.  Reported only; `value` is device-resident.
+    end_to_end = None
+    if args.end_to_end == "on":
+        runs = []
+        for _ in range(2):
+            t4 = time.perf_counter()
+            W = mesh.get_weights(qc)
+            runs.append(time.perf_counter() - t4)
+            nnz_host = W.nnz
+            del W
+        best = min(runs)
+        end_to_end = {"call": "MeshAndVerticesHIP.get_weights(host queries) -> scipy CSR",
+                      "seconds": best, "queries_per_s": M / best, "nnz": nnz_host,
+                      "csr_bytes": nnz_host * 12 + (M + 1) * 4, "runs_s": runs}
+        log(f"drop-in get_weights (host in, scipy CSR out): {best * 1e3:.1f} ms "
+            f"({M / best / 1e6:.1f} M queries/s, {nnz_host:,} entries)")
+
     mixed = None
     if args.mixed == "on":
         mesh.set_precision("mixed")
@@ -720,6 +742,7 @@ def main():
                           "emit": per_step[-1]["emit_ms"]},
             "rows_beyond_group_kernel": per_step[-1]["wave_items"],
             "mixed_precision": mixed,
+            "drop_in_end_to_end": end_to_end,
             "mesh_build_s": round(build_s, 1),
             "row_policy": tuned if tuned is not None else "default (autotuner off)",
         },

@@ -14,6 +14,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <sys/mman.h>
 #include <vector>
 
 #include "geom.h"
@@ -935,6 +936,132 @@ std::vector<std::pair<const void *, size_t>> workspace_fingerprint(const nat_mes
     return f;
 }
 
+// ---- host transfers of the host-buffer entry point (nat_get_weights)
+//
+// Pageable hipMemcpy moved the CSR at about 4 GB/s (one staging thread, and
+// page faults on the freshly allocated output).  Here kXferLanes host threads
+// each move one contiguous slice through two pinned chunks on a stream of
+// their own: the DMA of one chunk overlaps the host copy of the previous one,
+// and the output pages are first touched by all threads at once.  The pinned
+// chunks are allocated on the first large transfer to a device and kept for
+// the life of the process.
+constexpr int kXferLanes = 8;
+constexpr size_t kXferChunk = static_cast<size_t>(16) << 20;
+constexpr size_t kXferDirect = static_cast<size_t>(8) << 20;  // smaller: one hipMemcpy
+constexpr int kXferMaxDevices = 64;
+
+struct XferLane {
+    hipStream_t st = nullptr;
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+struct XferPool {
+    std::mutex mu;
+    bool ready = false;
+    XferLane lane[kXferLanes];
+};
+
+XferPool *xfer_pool(int device) {
+    static std::mutex mu;
+    static XferPool *pools[kXferMaxDevices] = {};
+    if (device < 0 || device >= kXferMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!pools[device]) pools[device] = new XferPool;  // lives to the process end
+    return pools[device];
+}
+
+hipError_t xfer_ready(XferPool &p) {  // the caller holds p.mu, device current
+    if (p.ready) return hipSuccess;
+    for (XferLane &L : p.lane) {
+        hipError_t e = hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking);
+        for (int k = 0; k < 2 && e == hipSuccess; k++) {
+            e = hipHostMalloc(&L.buf[k], kXferChunk, hipHostMallocDefault);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&L.ev[k], hipEventDisableTiming);
+        }
+        if (e != hipSuccess) return e;
+    }
+    p.ready = true;
+    return hipSuccess;
+}
+
+// One lane's slice: device -> host (to_host) or host -> device, chunk by chunk.
+hipError_t xfer_slice(XferLane &L, char *dst, const char *src, size_t n, bool to_host) {
+    const size_t nch = (n + kXferChunk - 1) / kXferChunk;
+    auto len_of = [&](size_t k) { return std::min(kXferChunk, n - k * kXferChunk); };
+    if (to_host) {
+        for (size_t k = 0; k <= nch; k++) {
+            if (k < nch) {  // buffer k & 1 was emptied by the host copy of chunk k - 2
+                hipError_t e = hipMemcpyAsync(L.buf[k & 1], src + k * kXferChunk, len_of(k),
+                                              hipMemcpyDeviceToHost, L.st);
+                if (e == hipSuccess) e = hipEventRecord(L.ev[k & 1], L.st);
+                if (e != hipSuccess) return e;
+            }
+            if (k > 0) {
+                const size_t j = k - 1;
+                hipError_t e = hipEventSynchronize(L.ev[j & 1]);
+                if (e != hipSuccess) return e;
+                std::memcpy(dst + j * kXferChunk, L.buf[j & 1], len_of(j));
+            }
+        }
+        return hipSuccess;
+    }
+    for (size_t k = 0; k < nch; k++) {
+        if (k >= 2) {  // the DMA of chunk k - 2 has read buffer k & 1
+            hipError_t e = hipEventSynchronize(L.ev[k & 1]);
+            if (e != hipSuccess) return e;
+        }
+        std::memcpy(L.buf[k & 1], src + k * kXferChunk, len_of(k));
+        hipError_t e = hipMemcpyAsync(dst + k * kXferChunk, L.buf[k & 1], len_of(k),
+                                      hipMemcpyHostToDevice, L.st);
+        if (e == hipSuccess) e = hipEventRecord(L.ev[k & 1], L.st);
+        if (e != hipSuccess) return e;
+    }
+    return hipStreamSynchronize(L.st);
+}
+
+// Copies n bytes between device memory and pageable host memory; returns
+// when the copy is complete.  The caller has the device current and has
+// finished (synchronised) the work that produces or consumes the data.
+hipError_t xfer(int device, void *dst, const void *src, size_t n, bool to_host) {
+    const hipMemcpyKind kind = to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+    XferPool *p = n >= kXferDirect ? xfer_pool(device) : nullptr;
+    if (!p) return n ? hipMemcpy(dst, src, n, kind) : hipSuccess;
+    std::lock_guard<std::mutex> lk(p->mu);
+    hipError_t e = xfer_ready(*p);
+    if (e != hipSuccess) return e;
+    // slices of whole chunks, one per lane (fewer lanes for smaller copies)
+    const size_t chunks = (n + kXferChunk - 1) / kXferChunk;
+    const int lanes = static_cast<int>(std::min<size_t>(kXferLanes, chunks));
+    std::vector<hipError_t> err(lanes, hipSuccess);
+    std::vector<std::thread> th;
+    auto run = [&](int l) {
+        const size_t c0 = chunks * l / lanes, c1 = chunks * (l + 1) / lanes;
+        const size_t b0 = c0 * kXferChunk, b1 = std::min(n, c1 * kXferChunk);
+        hipError_t le = hipSetDevice(device);
+        if (le == hipSuccess)
+            le = xfer_slice(p->lane[l], static_cast<char *>(dst) + b0,
+                            static_cast<const char *>(src) + b0, b1 - b0, to_host);
+        err[l] = le;
+    };
+    for (int l = 1; l < lanes; l++) th.emplace_back(run, l);
+    run(0);
+    for (std::thread &t : th) t.join();
+    for (hipError_t le : err)
+        if (le != hipSuccess) return le;
+    return hipSuccess;
+}
+
+// Host output of n bytes that the caller releases with free(): 2 MB aligned
+// with transparent huge pages advised, so that first touch faults ~500 times
+// less often.
+void *alloc_host_out(size_t n) {
+    void *p = nullptr;
+    const size_t huge = static_cast<size_t>(2) << 20;
+    if (posix_memalign(&p, n >= huge ? huge : 64, std::max<size_t>(n, 1)) != 0) return nullptr;
+    if (n >= huge) (void)madvise(p, n, MADV_HUGEPAGE);
+    return p;
+}
+
 }  // namespace
 
 struct nat_graph {
@@ -1374,8 +1501,8 @@ int nat_get_weights(const double *query_points, int num_query_points, nat_mesh *
     hipStream_t st = nullptr;
     if (M > 0) {
         HIP_OR_FAIL(m->q.ensure(sizeof(double) * 3 * Mz));
-        HIP_OR_FAIL(hipMemcpy(m->q.p, query_points, sizeof(double) * 3 * Mz,
-                              hipMemcpyHostToDevice));
+        HIP_OR_FAIL(hipStreamSynchronize(nullptr));  // no earlier work still reads m->q
+        HIP_OR_FAIL(xfer(m->device, m->q.p, query_points, sizeof(double) * 3 * Mz, false));
     }
     long long nnz = 0;
     int rc = compute_rows(m, m->q.as<double>(), M, nullptr, st, &nnz);
@@ -1388,19 +1515,18 @@ int nat_get_weights(const double *query_points, int num_query_points, nat_mesh *
     rc = emit_rows(m, m->csr_vals.as<double>(), m->csr_cols.as<int>(), m->csr_rowptr.as<int>(),
                    st);
     if (rc) return rc;
-    double *hv = static_cast<double *>(std::malloc(sizeof(double) * std::max<size_t>(nz, 1)));
-    int *hc = static_cast<int *>(std::malloc(sizeof(int) * std::max<size_t>(nz, 1)));
+    double *hv = static_cast<double *>(alloc_host_out(sizeof(double) * nz));
+    int *hc = static_cast<int *>(alloc_host_out(sizeof(int) * nz));
     if (!hv || !hc) {
         std::free(hv);
         std::free(hc);
         return fail(-3, "host allocation of %zu entries failed", nz);
     }
-    hipError_t e = hipMemcpy(weight_row_ptrs, m->csr_rowptr.p, sizeof(int) * (Mz + 1),
-                             hipMemcpyDeviceToHost);
-    if (e == hipSuccess && nz)
-        e = hipMemcpy(hv, m->csr_vals.p, sizeof(double) * nz, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && nz)
-        e = hipMemcpy(hc, m->csr_cols.p, sizeof(int) * nz, hipMemcpyDeviceToHost);
+    hipError_t e = hipStreamSynchronize(st);  // the CSR is complete
+    if (e == hipSuccess)
+        e = xfer(m->device, weight_row_ptrs, m->csr_rowptr.p, sizeof(int) * (Mz + 1), true);
+    if (e == hipSuccess && nz) e = xfer(m->device, hv, m->csr_vals.p, sizeof(double) * nz, true);
+    if (e == hipSuccess && nz) e = xfer(m->device, hc, m->csr_cols.p, sizeof(int) * nz, true);
     if (e != hipSuccess) {
         std::free(hv);
         std::free(hc);

@@ -188,3 +188,25 @@ def test_spmm_widths_and_row_order_bitwise(vdt):
             y = spmm_device(crow, col, val, xd, len(q), order=o).cpu().numpy()
             assert y.dtype == expect.dtype
             assert np.array_equal(y, expect), (D, o is not None)
+
+
+def test_host_call_large_transfers_match_device_csr():
+    """get_weights on host buffers large enough for the pinned multi-lane
+    copies (9.6 MB of queries in, ~75 MB of CSR out, slices not a multiple of
+    the chunk) returns the device-resident CSR byte for byte, twice in a row
+    (the second call reuses the staging)."""
+    import torch
+    rng = np.random.default_rng(23)
+    keys = rng.random((20000, 3))
+    q = 0.02 + 0.96 * rng.random((400003, 3))
+    it = natinterp3d_amd.Interpolator(keys)
+    crow, col, val = it.mesh_and_vertices.weights_device(
+        torch.from_numpy(q - it._centroid).cuda())
+    torch.cuda.synchronize()
+    crow, col, val = crow.cpu().numpy(), col.cpu().numpy(), val.cpu().numpy()
+    assert val.nbytes > 8 << 20 and 3 * 8 * len(q) > 8 << 20
+    for _ in range(2):
+        W = it.get_weights(q)
+        assert np.array_equal(W.indptr, crow)
+        assert np.array_equal(W.indices, col)
+        assert np.array_equal(W.data, val)
