@@ -183,6 +183,17 @@ void nat_graph_free(nat_graph *g);
 int nat_topology_save(nat_mesh *m, void *d_dst, int64_t capacity, int64_t *bytes,
                       void *stream);
 
+/* Lets the next nat_weights_compute of num_query_points queries on this mesh
+ * write its topology records straight into d_dst (16-byte aligned, capacity
+ * bytes), the buffer a nat_topology_save(m, d_dst, ...) after that compute
+ * fills: the save then copies only the per-query arrays, not the records
+ * (≈5 GB at 10 M queries).  One-shot: any next compute consumes it.  When the
+ * records do not fit, or M differs, the compute keeps them in its workspace
+ * and the save copies them as without a target.  If the save goes to another
+ * buffer, it copies the records from d_dst, which must stay valid until then.
+ * d_dst NULL clears a pending target.  (No reference counterpart.) */
+int nat_topology_target(nat_mesh *m, void *d_dst, int64_t capacity, int num_query_points);
+
 /* dL/dq [M,3] float64 of y = W(q) @ values, from dL/dy [M,D] float64:
  * dL/dq_i = sum_j <dL/dy_i, values_j> d W_ij / d q_i, with the Sibson
  * coordinate derivative from the Voronoi facets between q_i and its

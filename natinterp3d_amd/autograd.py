@@ -26,6 +26,10 @@ from . import _lib
 from .natinterp3d import row_order, spmm_device
 
 
+# saved-topology bytes per query of the last forward on each mesh (by handle)
+_TOPO_BYTES_PER_QUERY = {}
+
+
 def _vp(t):
     return ctypes.c_void_p(t.data_ptr() if t is not None and t.numel() else 0)
 
@@ -43,18 +47,35 @@ class SibsonInterpolation(torch.autograd.Function):
             raise ValueError("values must have shape (num_data_points, D)")
         q = queries.detach().to(torch.float64).contiguous()
         v = values.detach().to(torch.float64).contiguous()
-        crow, col, val = mesh.weights_device(q)
+        M = q.shape[0]
+        L = _lib.lib()
         topo = None
+        key = mesh.handle.value
+        if ctx.needs_input_grad[1] and M:
+            # the compute writes its records straight into the saved topology
+            # (sized from the last one of this mesh, with a margin); if they
+            # do not fit, the save below copies them into a larger buffer
+            est = int(_TOPO_BYTES_PER_QUERY.get(key, 700.0) * 1.15 * M) + 4096
+            topo = torch.empty(est, dtype=torch.uint8, device=q.device)
+            _lib.check(L.nat_topology_target(mesh.handle, ctypes.c_void_p(topo.data_ptr()),
+                                             topo.numel(), M), "nat_topology_target")
+        try:
+            crow, col, val = mesh.weights_device(q)
+        finally:
+            if topo is not None:  # consumed by the compute; cleared if it never ran
+                L.nat_topology_target(mesh.handle, None, 0, 0)
         if ctx.needs_input_grad[1]:
-            L = _lib.lib()
             nbytes = ctypes.c_int64()
             _lib.check(L.nat_topology_save(mesh.handle, None, 0, ctypes.byref(nbytes), None),
                        "nat_topology_save")
-            topo = torch.empty(int(nbytes.value), dtype=torch.uint8, device=q.device)
+            if topo is None or int(nbytes.value) > topo.numel():
+                topo = torch.empty(int(nbytes.value), dtype=torch.uint8, device=q.device)
             stream = torch.cuda.current_stream(q.device).cuda_stream
             _lib.check(L.nat_topology_save(mesh.handle, ctypes.c_void_p(topo.data_ptr()),
                                            topo.numel(), ctypes.byref(nbytes),
                                            ctypes.c_void_p(stream)), "nat_topology_save")
+            if M:
+                _TOPO_BYTES_PER_QUERY[key] = int(nbytes.value) / M
         y = spmm_device(crow, col, val, v, q.shape[0], order=row_order(mesh.handle, q.shape[0]))
         ctx.mesh = mesh
         ctx.values_dtype = values.dtype

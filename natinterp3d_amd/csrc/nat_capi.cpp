@@ -146,6 +146,12 @@ struct nat_mesh {
     double nnz_per_row = 24.0;
 
     int last_M = -1;
+    // one-shot caller buffer for the next compute's topology records
+    // (nat_topology_target) and the buffer the last compute wrote them to
+    char *ext_topo = nullptr;
+    long long ext_capacity = 0;
+    int ext_M = -1;
+    char *ext_written = nullptr;
     long long last_nnz = 0;
     long long last_units = 0;  // record units of the last compute
 
@@ -576,9 +582,22 @@ int ensure_old_vol(nat_mesh *m, hipStream_t st) {
     return 0;
 }
 
+size_t topo_rec_off(long long M);
+
 int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipStream_t st,
                  long long *nnz_out) {
     m->last_M = -1;
+    // the records go straight into a caller's saved-topology buffer when one
+    // was set for this many queries and they fit; else into the workspace
+    char *ext_base = m->ext_topo && m->ext_M == M ? m->ext_topo : nullptr;
+    long long ext_units = 0;
+    if (ext_base) {
+        ext_units = (m->ext_capacity - static_cast<long long>(topo_rec_off(M))) /
+                    static_cast<long long>(sizeof(int4));
+        if (ext_units < 1) ext_base = nullptr;
+    }
+    m->ext_topo = nullptr;
+    m->ext_written = nullptr;
     if (M > 0)
         if (int rc = ensure_old_vol(m, st)) return rc;
     m->dev.prob = d_prob;
@@ -635,7 +654,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
             HIP_OR_FAIL(m->pool_cols.ensure(sizeof(int) * want));
             m->pool_cap = want;
         }
-        if (m->rec_cap < want_rec) {
+        if (!ext_base && m->rec_cap < want_rec) {
             HIP_OR_FAIL(m->rec.ensure(sizeof(int4) * want_rec));
             m->rec_cap = want_rec;
         }
@@ -661,8 +680,9 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         nat::Scratch sc0;
         nat::scratch_carve(m->scratch[0].p, m->caps[0], m->scap[0], &sc0);
         nat::Topo topo;
-        topo.rec = m->rec.as<int4>();
-        topo.rec_cap = m->rec_cap;
+        topo.rec = ext_base ? reinterpret_cast<int4 *>(ext_base + topo_rec_off(M))
+                            : m->rec.as<int4>();
+        topo.rec_cap = ext_base ? ext_units : m->rec_cap;
         topo.rec_used = rec_used;
         topo.rec_off = m->rec_off.as<long long>();
         mark(m, 2, st);
@@ -682,8 +702,9 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         HIP_OR_FAIL(hipStreamSynchronize(st));
         m->times.topo_ms = elapsed(m, 2, 3);
         const long long units = static_cast<long long>(meta[1]);
-        if (units > m->rec_cap) {
+        if (units > topo.rec_cap) {
             want_rec = units + units / 8 + 1024;
+            ext_base = nullptr;  // the caller's buffer is too small: the workspace
             continue;
         }
         m->units_per_row = std::max(8.0, 1.125 * static_cast<double>(units) / M);
@@ -765,6 +786,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         m->times.mixed_rows_fp64 = static_cast<int64_t>(tail[4]);
         if (static_cast<long long>(used) <= m->pool_cap) {
             m->last_M = M;
+            m->ext_written = ext_base;
             m->last_nnz = static_cast<long long>(used);
             m->nnz_per_row = std::max(4.0, 1.125 * static_cast<double>(used) / M);
             *nnz_out = m->last_nnz;
@@ -1600,13 +1622,32 @@ int nat_topology_save(nat_mesh *m, void *d_dst, int64_t capacity, int64_t *bytes
                                    4 * static_cast<size_t>(M), hipMemcpyDeviceToDevice, st));
         HIP_OR_FAIL(hipMemcpyAsync(dst + topo_total_off(M), m->row_total.p,
                                    8 * static_cast<size_t>(M), hipMemcpyDeviceToDevice, st));
-        if (units > 0)
-            HIP_OR_FAIL(hipMemcpyAsync(dst + topo_rec_off(M), m->rec.p,
+        // records already in place when the compute wrote them to d_dst
+        // (nat_topology_target); else from wherever it wrote them
+        const char *rsrc = m->ext_written ? m->ext_written + topo_rec_off(M)
+                                          : static_cast<const char *>(m->rec.p);
+        if (units > 0 && m->ext_written != dst)
+            HIP_OR_FAIL(hipMemcpyAsync(dst + topo_rec_off(M), rsrc,
                                        sizeof(int4) * static_cast<size_t>(units),
                                        hipMemcpyDeviceToDevice, st));
     }
     // the header comes from the stack: finish before returning
     HIP_OR_FAIL(hipStreamSynchronize(st));
+    return 0;
+}
+
+int nat_topology_target(nat_mesh *m, void *d_dst, int64_t capacity, int num_query_points) {
+    g_err.clear();
+    if (!m) return fail(-1, "NULL argument");
+    if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    if (d_dst && (capacity < 0 || num_query_points < 0))
+        return fail(-1, "bad size (capacity %lld, M %d)", static_cast<long long>(capacity),
+                    num_query_points);
+    if (reinterpret_cast<uintptr_t>(d_dst) & 15u) return fail(-1, "d_dst must be 16-byte aligned");
+    std::lock_guard<std::mutex> lk(m->mu);
+    m->ext_topo = static_cast<char *>(d_dst);
+    m->ext_capacity = capacity;
+    m->ext_M = num_query_points;
     return 0;
 }
 

@@ -73,6 +73,7 @@ def test_argument_checks():
     assert L.nat_spmm_csr(3, 1, 1, None, None, None, 1, None, 0, None, None, 0, None) < 0
     order = ctypes.c_void_p()
     assert L.nat_weights_row_order(None, 3, ctypes.byref(order)) < 0
+    assert L.nat_topology_target(None, None, 0, 0) < 0
 
 
 def test_query_shape_checks():

@@ -226,3 +226,55 @@ def test_grad_queries_rejects_mismatched_topology():
                             None, ctypes.c_void_p(gy.data_ptr()), ctypes.c_void_p(vals.data_ptr()),
                             1, ctypes.c_void_p(dq.data_ptr()), None)
     assert rc == -1 and "100 queries" in _lib.last_error()
+
+
+def test_topology_target_in_place_and_fallbacks():
+    """Records written straight into the saved topology (nat_topology_target)
+    give the same values, products and gradients as a saved copy; a target too
+    small for the records, or set for another number of queries, falls back to
+    the copy."""
+    import ctypes
+    import torch
+    from natinterp3d_amd import _lib, autograd
+    k, q, v = workloads.uniform_small(seed=9, n_keys=3000, n_queries=5000)
+    interp = natinterp3d_amd.Interpolator(k)
+    mesh = interp.mesh_and_vertices
+    key = mesh.handle.value
+    dev = torch.device("cuda", 0)
+    vals = torch.tensor(np.stack([v, -2 * v], 1), device=dev)
+    gy = torch.randn(len(q), 2, dtype=torch.float64, device=dev,
+                     generator=torch.Generator(device=dev).manual_seed(3))
+
+    def run(bytes_per_query):
+        if bytes_per_query is None:
+            autograd._TOPO_BYTES_PER_QUERY.pop(key, None)
+        else:
+            autograd._TOPO_BYTES_PER_QUERY[key] = bytes_per_query
+        qt = torch.tensor(q, device=dev, requires_grad=True)
+        vt = vals.clone().requires_grad_(True)
+        y = interp.interpolate_torch(qt, vt)
+        dq, dv = torch.autograd.grad(y, (qt, vt), gy)
+        return y, dq, dv
+
+    def same(got, ref, what):
+        # y and dL/dq bit for bit; dL/dvalues to 1e-12 (float64 atomics: the
+        # order of the additions into one output is not fixed)
+        assert torch.equal(got[0], ref[0]), what
+        assert torch.equal(got[1], ref[1]), what
+        torch.testing.assert_close(got[2], ref[2], rtol=1e-12, atol=1e-12)
+
+    base = run(1e-3)  # target far too small: the records are copied
+    for per_q in (None, 5000.0):  # default estimate; ample room: written in place
+        same(run(per_q), base, per_q)
+    # a target set for another query count is consumed without being written
+    L = _lib.lib()
+    buf = torch.full((1 << 22,), 7, dtype=torch.uint8, device=dev)
+    assert L.nat_topology_target(mesh.handle, ctypes.c_void_p(buf.data_ptr()), buf.numel(),
+                                 len(q) + 1) == 0
+    crow, col, val = mesh.weights_device(torch.tensor(q - interp._centroid, device=dev))
+    torch.cuda.synchronize()
+    assert bool((buf == 7).all())
+    assert torch.equal(torch.tensor(interp.get_weights(q).data, device=dev), val)
+    same(run(1e-3), base, "after another-M target")
+    assert L.nat_topology_target(mesh.handle, ctypes.c_void_p(buf.data_ptr() + 8), 64, 5) < 0
+    say("[grad] topology target: in place, too small and other-M targets agree")
