@@ -544,7 +544,7 @@ def main():
     import natinterp3d_amd
     from natinterp3d_amd import workloads
     from natinterp3d_amd.autograd import SibsonInterpolation
-    from natinterp3d_amd.natinterp3d import row_order, spmm_device
+    from natinterp3d_amd.natinterp3d import weights_spmm
 
     log(f"workload: {args.keys:,} keys, {args.queries:,} queries")
     keys, queries, values = workloads.plain_at_scale(n_keys=args.keys, n_queries=args.queries)
@@ -593,8 +593,7 @@ def main():
     def plain_forward():
         with torch.no_grad():
             crow, col, val = mesh.weights_device(qd.detach())
-            return spmm_device(crow, col, val, xd.detach(), M,
-                               order=row_order(mesh.handle, M)), col.numel()
+            return weights_spmm(mesh.handle, M, xd.detach()), col.numel()
 
     for i in range(args.warmup):
         y = step()

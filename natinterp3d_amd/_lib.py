@@ -22,6 +22,7 @@ EXPORTS = (
     "nat_graph_create", "nat_graph_launch", "nat_graph_copy_out", "nat_graph_free",
     "nat_mesh_set_policy", "nat_mesh_get_policy", "nat_mesh_autotune",
     "nat_mesh_from_packed", "nat_weights_row_order", "nat_topology_target",
+    "nat_weights_spmm",
 )
 
 
@@ -108,6 +109,8 @@ def lib():
     L.nat_spmm_t_csr.restype = ctypes.c_int
     L.nat_spmm_t_csr.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp,
                                  vp, vp]
+    L.nat_weights_spmm.restype = ctypes.c_int
+    L.nat_weights_spmm.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp, vp]
     L.nat_topology_target.restype = ctypes.c_int
     L.nat_topology_target.argtypes = [vp, vp, ctypes.c_int64, ctypes.c_int]
     L.nat_topology_row_order.restype = ctypes.c_int

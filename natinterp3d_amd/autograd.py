@@ -23,7 +23,7 @@ import ctypes
 import torch
 
 from . import _lib
-from .natinterp3d import row_order, spmm_device
+from .natinterp3d import row_order, spmm_device, weights_spmm
 
 
 # saved-topology bytes per query of the last forward on each mesh (by handle)
@@ -76,7 +76,10 @@ class SibsonInterpolation(torch.autograd.Function):
                                            ctypes.c_void_p(stream)), "nat_topology_save")
             if M:
                 _TOPO_BYTES_PER_QUERY[key] = int(nbytes.value) / M
-        y = spmm_device(crow, col, val, v, q.shape[0], order=row_order(mesh.handle, q.shape[0]))
+        if 1 <= v.shape[1] <= 4:  # from the row pool: same sums, read in order
+            y = weights_spmm(mesh.handle, M, v)
+        else:
+            y = spmm_device(crow, col, val, v, M, order=row_order(mesh.handle, M))
         ctx.mesh = mesh
         ctx.values_dtype = values.dtype
         ctx.queries_dtype = queries.dtype

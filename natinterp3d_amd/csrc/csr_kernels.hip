@@ -246,9 +246,10 @@ hipError_t launch_gather_csr(const long long *d_rowptr64, const long long *d_row
 
 // out[i, :] = in[idx[i], :] for i < n (rows of D doubles): the caller's
 // per-key values in device vertex order.
-__global__ void __launch_bounds__(256) gather_rows_kernel(const double *__restrict__ in,
+template <typename T>
+__global__ void __launch_bounds__(256) gather_rows_kernel(const T *__restrict__ in,
                                                           const int *__restrict__ idx, int n,
-                                                          int D, double *__restrict__ out) {
+                                                          int D, T *__restrict__ out) {
     const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= static_cast<long long>(n) * D) return;
     const long long i = t / D;
@@ -260,8 +261,9 @@ hipError_t launch_gather_rows(const double *in, const int *idx, int n, int D, do
                               hipStream_t st) {
     const long long total = static_cast<long long>(n) * D;
     if (total <= 0) return hipSuccess;
-    hipLaunchKernelGGL(gather_rows_kernel, dim3(static_cast<unsigned>((total + 255) / 256)),
-                       dim3(256), 0, st, in, idx, n, D, out);
+    hipLaunchKernelGGL(gather_rows_kernel<double>,
+                       dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, in,
+                       idx, n, D, out);
     return hipGetLastError();
 }
 
@@ -340,6 +342,31 @@ __device__ __forceinline__ void lanes_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// One row, entries [b, e), by its 16 lanes (prod: the row's LDS staging).
+template <typename TW, typename TX, typename TY, int D>
+__device__ __forceinline__ void spmm_row16(TY (*prod)[D], int lane, int b, int e,
+                                           const int *__restrict__ cols,
+                                           const TW *__restrict__ w, const TX *__restrict__ x,
+                                           TY *__restrict__ yrow) {
+    TY acc = 0;
+    for (int c0 = b; c0 < e; c0 += kSpmmLanes) {
+        const int j = c0 + lane;
+        if (j < e) {
+            const TY a = static_cast<TY>(w[j]);
+            const size_t xb = static_cast<size_t>(cols[j]) * D;
+#pragma unroll
+            for (int d = 0; d < D; d++) prod[lane][d] = a * static_cast<TY>(x[xb + d]);
+        }
+        lanes_sync();
+        if (lane < D) {
+            const int n = min(kSpmmLanes, e - c0);
+            for (int t = 0; t < n; t++) acc += prod[t][lane];
+        }
+        lanes_sync();
+    }
+    if (lane < D) yrow[lane] = acc;
+}
+
 template <typename TW, typename TX, typename TY, int D>
 __global__ void __launch_bounds__(256) spmm_rows_kernel(int M, const int *__restrict__ rowptr,
                                                         const int *__restrict__ cols,
@@ -354,24 +381,86 @@ __global__ void __launch_bounds__(256) spmm_rows_kernel(int M, const int *__rest
     int i = -1;
     if (k < M) i = order ? order[k] : static_cast<int>(k);
     if (i < 0 || i >= M) return;  // the row's 16 lanes leave together
-    const int b = rowptr[i], e = rowptr[i + 1];
-    TY acc = 0;
-    for (int c0 = b; c0 < e; c0 += kSpmmLanes) {
-        const int j = c0 + lane;
-        if (j < e) {
-            const TY a = static_cast<TY>(w[j]);
-            const size_t xb = static_cast<size_t>(cols[j]) * D;
-#pragma unroll
-            for (int d = 0; d < D; d++) prod[r][lane][d] = a * static_cast<TY>(x[xb + d]);
-        }
-        lanes_sync();
-        if (lane < D) {
-            const int n = min(kSpmmLanes, e - c0);
-            for (int t = 0; t < n; t++) acc += prod[r][t][lane];
-        }
-        lanes_sync();
+    spmm_row16<TW, TX, TY, D>(prod[r], lane, rowptr[i], rowptr[i + 1], cols, w, x,
+                              y + static_cast<size_t>(i) * D);
+}
+
+// The same product from the row pool of a compute: row i at
+// [row_off[i], row_off[i] + row_cnt[i]) with device vertex columns, x in
+// device vertex order, rows in the compute's order -- the pool rows are the
+// emitted CSR rows (same entries, same order), so the sums are the same.
+template <typename TX, int D>
+__global__ void __launch_bounds__(256) pool_spmm_kernel(int M, const long long *__restrict__ row_off,
+                                                        const int *__restrict__ row_cnt,
+                                                        const int *__restrict__ cols,
+                                                        const double *__restrict__ w,
+                                                        const TX *__restrict__ x,
+                                                        const int *__restrict__ order,
+                                                        double *__restrict__ y) {
+    __shared__ double prod[kSpmmRows][kSpmmLanes][D];
+    const int lane = static_cast<int>(threadIdx.x) & (kSpmmLanes - 1);
+    const int r = static_cast<int>(threadIdx.x) / kSpmmLanes;
+    const long long k = static_cast<long long>(blockIdx.x) * kSpmmRows + r;
+    if (k >= M) return;
+    const int i = order[k];
+    if (i < 0 || i >= M) return;
+    const long long b = row_off[i];
+    spmm_row16<double, TX, double, D>(prod[r], lane, static_cast<int>(0),
+                                      row_cnt[i], cols + b, w + b, x,
+                                      y + static_cast<size_t>(i) * D);
+}
+
+template <typename TX>
+static hipError_t pool_spmm_t(int M, int D, const long long *off, const int *cnt, const int *c,
+                              const double *w, const void *x, const int *order, double *y,
+                              hipStream_t st) {
+    const dim3 grid(static_cast<unsigned>((M + kSpmmRows - 1) / kSpmmRows));
+    const TX *xt = static_cast<const TX *>(x);
+    switch (D) {
+        case 1:
+            hipLaunchKernelGGL((pool_spmm_kernel<TX, 1>), grid, dim3(256), 0, st, M, off, cnt, c,
+                               w, xt, order, y);
+            break;
+        case 2:
+            hipLaunchKernelGGL((pool_spmm_kernel<TX, 2>), grid, dim3(256), 0, st, M, off, cnt, c,
+                               w, xt, order, y);
+            break;
+        case 3:
+            hipLaunchKernelGGL((pool_spmm_kernel<TX, 3>), grid, dim3(256), 0, st, M, off, cnt, c,
+                               w, xt, order, y);
+            break;
+        case 4:
+            hipLaunchKernelGGL((pool_spmm_kernel<TX, 4>), grid, dim3(256), 0, st, M, off, cnt, c,
+                               w, xt, order, y);
+            break;
+        default:
+            return hipErrorInvalidValue;
     }
-    if (lane < D) y[static_cast<size_t>(i) * D + lane] = acc;
+    return hipGetLastError();
+}
+
+hipError_t launch_pool_spmm(int M, int D, const long long *row_off, const int *row_cnt,
+                            const int *pool_cols, const double *pool_vals, const void *x_dev,
+                            int xdt, const int *order, double *y, hipStream_t st) {
+    if (M <= 0) return hipSuccess;
+    return xdt == 1 ? pool_spmm_t<double>(M, D, row_off, row_cnt, pool_cols, pool_vals, x_dev,
+                                          order, y, st)
+                    : pool_spmm_t<float>(M, D, row_off, row_cnt, pool_cols, pool_vals, x_dev,
+                                         order, y, st);
+}
+
+hipError_t launch_gather_rows_bytes(const void *in, const int *idx, int n, int D, int elt,
+                                    void *out, hipStream_t st) {
+    const long long total = static_cast<long long>(n) * D;
+    if (total <= 0) return hipSuccess;
+    const dim3 grid(static_cast<unsigned>((total + 255) / 256));
+    if (elt == 8)
+        hipLaunchKernelGGL(gather_rows_kernel<double>, grid, dim3(256), 0, st,
+                           static_cast<const double *>(in), idx, n, D, static_cast<double *>(out));
+    else
+        hipLaunchKernelGGL(gather_rows_kernel<float>, grid, dim3(256), 0, st,
+                           static_cast<const float *>(in), idx, n, D, static_cast<float *>(out));
+    return hipGetLastError();
 }
 
 template <typename TW, typename TX, typename TY>

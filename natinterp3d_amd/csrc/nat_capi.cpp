@@ -104,6 +104,7 @@ struct nat_mesh {
     DevBuf verts, tet_v, tet_n, kd, cell_start, cell_pts, lp;
     DevBuf vorig;                         // [N+4P] device vertex -> key index
     DevBuf vals_dev;                      // backward: values in device vertex order
+    DevBuf vals_fwd;                      // nat_weights_spmm: values in device vertex order
     DevBuf old_vol;                       // step-3 table (built on the first compute)
     bool old_vol_tried = false;
     DevBuf prob, qoff;                    // batch calls: problem of each query
@@ -178,7 +179,8 @@ struct nat_mesh {
                    ovf[0].bytes +
                    ovf[1].bytes + counters.bytes + pool_vals.bytes + pool_cols.bytes +
                    csr_vals.bytes + csr_cols.bytes + csr_rowptr.bytes +
-                   rec.bytes + rec_off.bytes + start.bytes + wscr.bytes + vals_dev.bytes;
+                   rec.bytes + rec_off.bytes + start.bytes + wscr.bytes + vals_dev.bytes +
+                   vals_fwd.bytes;
         for (int l = 0; l < kLevels; l++) s += scratch[l].bytes;
         return static_cast<int64_t>(s);
     }
@@ -188,7 +190,8 @@ struct nat_mesh {
                          &row_off,  &row_cnt,   &rowptr64,  &row_total, &ovf[0],    &ovf[1],
                          &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,   &csr_rowptr,
                          &rec,      &rec_off,   &start,     &lp,       &prob,       &qoff,
-                         &route,    &wscr,      &old_vol,   &vorig,    &vals_dev};
+                         &route,    &wscr,      &old_vol,   &vorig,    &vals_dev,
+                         &vals_fwd};
         for (DevBuf *b : all) b->release();
         for (int l = 0; l < kLevels; l++) scratch[l].release();
         for (hipEvent_t &e : ev) {
@@ -1576,6 +1579,34 @@ int nat_spmm_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
                                     d_values, values_dtype, d_row_order, d_out, out_dtype,
                                     static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail(-10, "spmm launch failed: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int nat_weights_spmm(nat_mesh *m, int num_query_points, const void *d_values, int values_dtype,
+                     int dim, double *d_out, void *stream) {
+    g_err.clear();
+    if (!m) return fail(-1, "NULL argument");
+    if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    if (values_dtype < 0 || values_dtype > 1) return fail(-1, "values dtype %d", values_dtype);
+    if (dim < 1 || dim > 4)
+        return fail(-1, "dim %d: nat_weights_spmm takes 1 to 4 value columns", dim);
+    std::lock_guard<std::mutex> lk(m->mu);
+    if (m->last_M < 0 || num_query_points != m->last_M)
+        return fail(-6, "last compute had %d queries, not %d", m->last_M, num_query_points);
+    if (num_query_points == 0) return 0;
+    if (!d_values || !d_out) return fail(-1, "NULL argument");
+    DeviceGuard g(m->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int nd = m->dev.n_data;
+    const int elt = values_dtype ? 8 : 4;
+    HIP_OR_FAIL(m->vals_fwd.ensure(static_cast<size_t>(elt) *
+                                   std::max<size_t>(1, static_cast<size_t>(nd) * dim)));
+    HIP_OR_FAIL(nat::launch_gather_rows_bytes(d_values, m->vorig.as<int>(), nd, dim, elt,
+                                              m->vals_fwd.p, st));
+    HIP_OR_FAIL(nat::launch_pool_spmm(num_query_points, dim, m->row_off.as<long long>(),
+                                      m->row_cnt.as<int>(), m->pool_cols.as<int>(),
+                                      m->pool_vals.as<double>(), m->vals_fwd.p, values_dtype,
+                                      m->order.as<int>(), d_out, st));
     return 0;
 }
 

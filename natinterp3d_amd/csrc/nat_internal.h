@@ -221,6 +221,14 @@ hipError_t launch_gather_csr(const long long *d_rowptr64, const long long *d_row
                              const int *d_row_cnt, const double *pool_vals,
                              const int *pool_cols, const int *vmap, int M, long long cap,
                              double *d_vals, int *d_cols, int *d_rowptr32, hipStream_t st);
+// W @ x from the row pool of a compute (rows in order, device columns, x in
+// device vertex order; xdt 0 = float32, 1 = float64), D = 1..4
+hipError_t launch_pool_spmm(int M, int D, const long long *row_off, const int *row_cnt,
+                            const int *pool_cols, const double *pool_vals, const void *x_dev,
+                            int xdt, const int *order, double *y, hipStream_t st);
+// out[i, :] = in[idx[i], :], rows of D elements of elt (4 or 8) bytes, i < n
+hipError_t launch_gather_rows_bytes(const void *in, const int *idx, int n, int D, int elt,
+                                    void *out, hipStream_t st);
 // out[i, :] = in[idx[i], :], rows of D doubles, i < n
 hipError_t launch_gather_rows(const double *in, const int *idx, int n, int D, double *out,
                               hipStream_t st);

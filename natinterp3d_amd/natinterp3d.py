@@ -254,6 +254,23 @@ class MeshAndVerticesHIP:
 
     # ---- device-resident forms (PyTorch tensors on this mesh's GPU)
 
+    def compute_device(self, queries):
+        """The rows of centred float64 CUDA queries [M,3], kept in the mesh's
+        row pool for ``weights_spmm`` (no CSR emitted); returns M.  Raises
+        OverflowError, as ``get_weights`` would, past INT32_MAX entries."""
+        import torch
+        q = queries.detach().to(dtype=torch.float64).contiguous()
+        if q.dim() != 2 or q.shape[1] != 3:
+            raise ValueError("Query points must have shape (num_query_points, 3)")
+        stream = torch.cuda.current_stream(q.device).cuda_stream
+        nnz = ctypes.c_int64()
+        _lib.check(_lib.lib().nat_weights_compute(self._m, ctypes.c_void_p(q.data_ptr()),
+                                                  q.shape[0], ctypes.c_void_p(stream),
+                                                  ctypes.byref(nnz)), "nat_weights_compute")
+        if nnz.value > 2147483647:
+            raise OverflowError("Total number of nonzero weights exceeds int32 limit")
+        return q.shape[0]
+
     def weights_device(self, queries):
         """CSR of centred float64 CUDA queries [M,3] as three device tensors
         (crow int32[M+1], col int32[nnz], val float64[nnz])."""
@@ -290,6 +307,23 @@ def row_order(handle, num_rows):
     out = ctypes.c_void_p()
     _lib.check(_lib.lib().nat_weights_row_order(handle, num_rows, ctypes.byref(out)),
                "nat_weights_row_order")
+    return out
+
+
+def weights_spmm(handle, num_rows, values):
+    """``W @ values`` (float64 [M,D]) for the rows of the last weights compute
+    on the mesh ``handle``, read from the mesh's row pool: bit for bit
+    ``spmm_device`` on the emitted CSR.  values [N,D] float32/float64 CUDA,
+    D = 1..4."""
+    import torch
+    codes = {torch.float32: 0, torch.float64: 1}
+    values = values.contiguous()
+    out = torch.empty((num_rows, values.shape[1]), dtype=torch.float64, device=values.device)
+    stream = torch.cuda.current_stream(values.device).cuda_stream
+    _lib.check(_lib.lib().nat_weights_spmm(
+        handle, num_rows, ctypes.c_void_p(values.data_ptr()), codes[values.dtype],
+        values.shape[1], ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream)),
+        "nat_weights_spmm")
     return out
 
 
@@ -476,6 +510,16 @@ class Interpolator:
         dev = torch.device("cuda", self.mesh_and_vertices.device)
         with torch.cuda.device(dev):
             qd = torch.from_numpy(q).to(dev)
+            vdt = vals2.dtype if vals2.dtype in _DT_CODE else res_dtype
+            if out_dtype == np.float64 and 1 <= vals2.shape[1] <= 4:
+                # float64 weights: the product straight from the row pool,
+                # without emitting the CSR (same sums, same order)
+                M = self.mesh_and_vertices.compute_device(qd)
+                xd = torch.from_numpy(np.ascontiguousarray(vals2, vdt)).to(dev)
+                out = weights_spmm(self.mesh_and_vertices.handle, M, xd).cpu().numpy()
+                if out.dtype != res_dtype:
+                    out = out.astype(res_dtype)
+                return np.squeeze(out, axis=1) if one_d else out
             crow, col, val = self.mesh_and_vertices.weights_device(qd)
             if out_dtype != np.float64:
                 # get_weights' astype(out_dtype) canonicalises W (rows sorted
@@ -487,7 +531,6 @@ class Interpolator:
                     ctypes.c_void_p(col.data_ptr() if col.numel() else 0),
                     ctypes.c_void_p(val.data_ptr() if val.numel() else 0), 0,
                     ctypes.c_void_p(stream)), "nat_csr_sort_rows")
-            vdt = vals2.dtype if vals2.dtype in _DT_CODE else res_dtype
             xd = torch.from_numpy(np.ascontiguousarray(vals2, vdt)).to(dev)
             y = spmm_device(crow, col, val, xd, q.shape[0],
                             order=row_order(self.mesh_and_vertices.handle, q.shape[0]))

@@ -210,3 +210,34 @@ def test_host_call_large_transfers_match_device_csr():
         assert np.array_equal(W.indptr, crow)
         assert np.array_equal(W.indices, col)
         assert np.array_equal(W.data, val)
+
+
+@pytest.mark.parametrize("vdt", [np.float64, np.float32])
+def test_pool_product_equals_csr_product(vdt):
+    """W @ values read from the row pool (nat_weights_spmm) equals the product
+    on the emitted CSR bit for bit, value widths 1..4, single mesh and batch;
+    a width of 5 and a query count other than the last compute's are refused."""
+    import torch
+    from natinterp3d_amd.natinterp3d import spmm_device, weights_spmm
+    rng = np.random.default_rng(29)
+    keys = rng.random((2500, 3))
+    q = 0.05 + 0.9 * rng.random((4000, 3))
+    it = natinterp3d_amd.Interpolator(keys)
+    m = it.mesh_and_vertices
+    crow, col, val = m.weights_device(torch.from_numpy(q - it._centroid).cuda())
+    for D in range(1, 5):
+        xd = torch.from_numpy(rng.standard_normal((2500, D)).astype(vdt)).cuda()
+        assert torch.equal(weights_spmm(m.handle, len(q), xd),
+                           spmm_device(crow, col, val, xd, len(q))), D
+    with pytest.raises(RuntimeError, match="1 to 4"):
+        weights_spmm(m.handle, len(q), torch.zeros((2500, 5), dtype=torch.float64).cuda())
+    with pytest.raises(RuntimeError, match="queries"):
+        weights_spmm(m.handle, len(q) + 1, xd)
+    ks = [rng.random((n, 3)) for n in (800, 1500, 300)]
+    qs = [0.05 + 0.9 * rng.random((n, 3)) for n in (900, 2000, 400)]
+    b = natinterp3d_amd.BatchInterpolator(ks)
+    qc, off = b._centre(qs)
+    crow, col, val = b.weights_device(torch.from_numpy(qc).cuda(), off)
+    xd = torch.from_numpy(rng.standard_normal((2600, 3)).astype(vdt)).cuda()
+    assert torch.equal(weights_spmm(b.handle, int(off[-1]), xd),
+                       spmm_device(crow, col, val, xd, int(off[-1])))
