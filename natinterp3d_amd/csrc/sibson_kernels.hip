@@ -1007,7 +1007,10 @@ __global__ void __launch_bounds__(256) sibson_grad_rows_kernel(
 // Start tet of every work item: exact nearest centroid (grid; the k-d
 // replica when the grid cannot certify it), then the visibility walk.
 // start[qi] = tet, -1 outside the super simplex, -2 k-d stack overflow.
-__global__ void __launch_bounds__(256) locate_kernel(DevMesh m, Caps caps, Scratch sc,
+// 8 waves per SIMD: the search is latency-bound and uses 32 VGPRs; without
+// the hint the compiler takes 112 SGPRs, which allow only 7 (800 per SIMD).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+locate_kernel(DevMesh m, Caps caps, Scratch sc,
                                                      const double *__restrict__ q,
                                                      const int *__restrict__ items, int n_items,
                                                      int *__restrict__ start) {
