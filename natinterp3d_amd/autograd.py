@@ -30,6 +30,17 @@ from .natinterp3d import row_order, spmm_device, weights_spmm
 _TOPO_BYTES_PER_QUERY = {}
 
 
+_SIDE = {}
+
+
+def _side_stream(device):
+    """One extra stream per device for the backward's W^T g."""
+    key = torch.device(device).index
+    if key not in _SIDE:
+        _SIDE[key] = torch.cuda.Stream(device=device)
+    return _SIDE[key]
+
+
 def _vp(t):
     return ctypes.c_void_p(t.data_ptr() if t is not None and t.numel() else 0)
 
@@ -93,8 +104,28 @@ class SibsonInterpolation(torch.autograd.Function):
         gy = gy.to(torch.float64).contiguous()
         M, D = gy.shape
         L = _lib.lib()
-        stream = ctypes.c_void_p(torch.cuda.current_stream(gy.device).cuda_stream)
+        cur = torch.cuda.current_stream(gy.device)
+        stream = ctypes.c_void_p(cur.cuda_stream)
         dq = dv = None
+        if ctx.needs_input_grad[2]:
+            # W^T g on a second stream, beside the query gradient: that
+            # kernel's resident blocks leave most of each CU's LDS and wave
+            # slots free (default priority, like every stream here)
+            order = ctypes.c_void_p()
+            if topo is not None and M:
+                _lib.check(L.nat_topology_row_order(ctypes.c_void_p(topo.data_ptr()), M,
+                                                    ctypes.byref(order)),
+                           "nat_topology_row_order")
+            side = _side_stream(gy.device)
+            side.wait_stream(cur)
+            with torch.cuda.stream(side):
+                dv = torch.empty(v.shape, dtype=torch.float64, device=gy.device)
+                _lib.check(L.nat_spmm_t_csr(M, v.shape[0], D, _vp(crow), _vp(col), _vp(val),
+                                            _vp(gy), order, _vp(dv),
+                                            ctypes.c_void_p(side.cuda_stream)),
+                           "nat_spmm_t_csr")
+            for t in (crow, col, val, gy):
+                t.record_stream(side)
         if ctx.needs_input_grad[1]:
             dq = torch.empty((M, 3), dtype=torch.float64, device=gy.device)
             if M:
@@ -102,14 +133,8 @@ class SibsonInterpolation(torch.autograd.Function):
                                               _vp(q), M, _vp(crow), _vp(val), _vp(gy), _vp(v),
                                               D, _vp(dq), stream), "nat_grad_queries")
             dq = dq.to(ctx.queries_dtype)
-        if ctx.needs_input_grad[2]:
-            dv = torch.empty(v.shape, dtype=torch.float64, device=gy.device)
-            order = ctypes.c_void_p()
-            if topo is not None and M:
-                _lib.check(L.nat_topology_row_order(ctypes.c_void_p(topo.data_ptr()), M,
-                                                    ctypes.byref(order)),
-                           "nat_topology_row_order")
-            _lib.check(L.nat_spmm_t_csr(M, v.shape[0], D, _vp(crow), _vp(col), _vp(val),
-                                        _vp(gy), order, _vp(dv), stream), "nat_spmm_t_csr")
+        if dv is not None:
+            cur.wait_stream(side)
+            dv.record_stream(cur)
             dv = dv.to(ctx.values_dtype)
         return None, dq, dv
