@@ -555,23 +555,35 @@ __device__ __forceinline__ void old_contributions_table(const DevMesh &m, int t,
 // circumcentres minus q, twice the triangle areas are a1 = ((A-h) x (V-h)).w
 // and a2 = ((V-h) x (B-h)).w, their centroids (h+A+V)/3 and (h+V+B)/3, and
 // the result (a1 (h+A+V) + a2 (h+V+B)) / (6 |w|^2): one division.
-__device__ __forceinline__ d3 facet_moment(const d3 &pk, const d3 &q, const d3 &ca,
-                                           const d3 &cb, const d3 &cv) {
+// facet_moment6 leaves the division to the caller (1 / (6 |w|^2) depends on
+// the neighbour alone: facet_scale).
+__device__ __forceinline__ d3 facet_moment6(const d3 &pk, const d3 &q, const d3 &ca,
+                                            const d3 &cb, const d3 &cv) {
     const d3 w = sub3(pk, q);
     const d3 h = {w.x * 0.5, w.y * 0.5, w.z * 0.5};
     const d3 A = sub3(ca, q), V = sub3(cv, q), B = sub3(cb, q);
     const d3 u = sub3(A, h), v = sub3(V, h), x = sub3(B, h);
     const double a1 = dot3(cross3(u, v), w);
     const double a2 = dot3(cross3(v, x), w);
-    const double s = 1.0 / (6.0 * dot3(w, w));
-    return {(a1 * (h.x + A.x + V.x) + a2 * (h.x + V.x + B.x)) * s,
-            (a1 * (h.y + A.y + V.y) + a2 * (h.y + V.y + B.y)) * s,
-            (a1 * (h.z + A.z + V.z) + a2 * (h.z + V.z + B.z)) * s};
+    return {a1 * (h.x + A.x + V.x) + a2 * (h.x + V.x + B.x),
+            a1 * (h.y + A.y + V.y) + a2 * (h.y + V.y + B.y),
+            a1 * (h.z + A.z + V.z) + a2 * (h.z + V.z + B.z)};
+}
+__device__ __forceinline__ double facet_scale(const d3 &pk, const d3 &q) {
+    const d3 w = sub3(pk, q);
+    return 1.0 / (6.0 * dot3(w, w));
+}
+__device__ __forceinline__ d3 facet_moment(const d3 &pk, const d3 &q, const d3 &ca,
+                                           const d3 &cb, const d3 &cv) {
+    const d3 m6 = facet_moment6(pk, q, ca, cb, cv);
+    const double s = facet_scale(pk, q);
+    return {m6.x * s, m6.y * s, m6.z * s};
 }
 
 struct NoMoment {
     __device__ void operator()(int, const d3 &) const {}
     static constexpr bool active = false;
+    static constexpr bool raw = false;
 };
 
 // Step 4 for one boundary face, face fi of cavity tet t
@@ -604,7 +616,12 @@ __device__ __forceinline__ void new_contributions(const DevMesh &m, int t, int f
             if (slot < 0) continue;
             const d3 &ca = k == 0 ? t0 : (k == 1 ? t1 : t2);
             const d3 &cb = k == 0 ? t2 : (k == 1 ? t0 : t1);
-            if constexpr (Mom::active) mom(slot, facet_moment(F[k], q, ca, cb, cv));
+            if constexpr (Mom::active) {
+                if constexpr (Mom::raw)
+                    mom(slot, facet_moment6(F[k], q, ca, cb, cv));
+                else
+                    mom(slot, facet_moment(F[k], q, ca, cb, cv));
+            }
         }
         return;
     }
@@ -687,15 +704,20 @@ __device__ __forceinline__ bool new_contributions32(const DevMesh &m, int t, int
     return true;
 }
 
-template <typename F>
+template <typename F, bool RAW = false>
 struct MomentFn {
     F f;
     static constexpr bool active = true;
+    static constexpr bool raw = RAW;  // f gets 6 |w|^2 times the moment
     __device__ void operator()(int slot, const d3 &gm) const { f(slot, gm); }
 };
 template <typename F>
 __device__ __forceinline__ MomentFn<F> moment_fn(F f) {
     return MomentFn<F>{f};
+}
+template <typename F>
+__device__ __forceinline__ MomentFn<F, true> moment6_fn(F f) {
+    return MomentFn<F, true>{f};
 }
 
 // a_j = <dL/dy[qi], values[j]>
@@ -2359,9 +2381,11 @@ __global__ void __launch_bounds__(kGradThreads) grad_geom_kernel(
     DevMesh m, const double *__restrict__ q, const int *__restrict__ items, int n_items,
     unsigned int *work, Topo topo, GradIn g, int *ovf_list, unsigned int *ovf_count) {
     __shared__ double a_lds[kGradThreads / kGradLanes][kLevel0Row];
+    __shared__ double s_lds[kGradThreads / kGradLanes][kLevel0Row];  // 1 / (6 |p_j - q|^2)
     const int gl = __lane_id() & (kGradLanes - 1);
     const int gw = __lane_id() / kGradLanes;
     double *nb_a = a_lds[threadIdx.x / kGradLanes];
+    double *nb_s = s_lds[threadIdx.x / kGradLanes];
     int parts_done = 0, first = 0, nbatch = 0, sub = kGradBatch;
     for (;;) {
         // a wave claims kGradBatch consecutive items and works through them
@@ -2399,6 +2423,7 @@ __global__ void __launch_bounds__(kGradThreads) grad_geom_kernel(
         for (int j = gl; j < nnb; j += kGradLanes) {
             const double a = grad_dot(g, qi, rn[j]);
             nb_a[j] = a;
+            nb_s[j] = facet_scale(ld_vert(m, rn[j]), qq);
             abar += a * g.w[e0 + j];
         }
         group_sync();
@@ -2410,7 +2435,9 @@ __global__ void __launch_bounds__(kGradThreads) grad_geom_kernel(
             new_contributions<false>(
                 m, cr.x, code & 3, qq, [&](int vi, int) { return rec_slot(cr, vi); },
                 [](int, double) {},
-                moment_fn([&](int slot, const d3 &gm) {
+                moment6_fn([&](int slot, const d3 &g6) {
+                    const double sc = nb_s[slot];
+                    const d3 gm = {g6.x * sc, g6.y * sc, g6.z * sc};
                     const double a = nb_a[slot];
                     ag = {ag.x + a * gm.x, ag.y + a * gm.y, ag.z + a * gm.z};
                     tg = {tg.x + gm.x, tg.y + gm.y, tg.z + gm.z};
