@@ -1005,7 +1005,7 @@ __global__ void __launch_bounds__(256) sibson_grad_rows_kernel(
         epoch = epoch == 0x7ffffffe ? 1 : epoch + 1;
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
-        d3 dq;
+        d3 dq = {0.0, 0.0, 0.0};  // rows outside the super simplex return before setting it
         if (fused_row<true>(m, caps, sc, tid, epoch, qq, qi, &g, &dq) < 0)
             ovf_list[atomicAdd(ovf_count, 1u)] = qi;
         else

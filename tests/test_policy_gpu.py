@@ -138,3 +138,38 @@ def test_autotune_single_mesh_and_graph_replay():
         assert np.array_equal(col.cpu().numpy(), ref.indices)
         assert np.array_equal(val.cpu().numpy(), ref.data)
     mesh.set_policy(budgets=[96], first_level=1)
+
+
+def test_routed_rows_outside_the_hull_have_zero_gradient():
+    """With every row routed past the group kernel (budget 0), queries outside
+    the data's hull and outside the super simplex (empty rows) get a zero
+    query gradient from the larger levels, and the other rows the same
+    gradient as under the default policy."""
+    rng = np.random.default_rng(41)
+    keys = rng.random((1500, 3))
+    q = np.vstack([0.05 + 0.9 * rng.random((400, 3)),
+                   1.2 + 0.1 * rng.random((20, 3)),   # outside the hull
+                   np.full((2, 3), 1e6)])               # outside the super simplex
+    interp = natinterp3d_amd.Interpolator(keys)
+    dev = torch.device("cuda", 0)
+    vt = torch.tensor(rng.standard_normal((1500, 2)), device=dev)
+    g = torch.tensor(rng.standard_normal((len(q), 2)), device=dev)
+
+    def grads():
+        qt = torch.tensor(q, device=dev, requires_grad=True)
+        y = interp.interpolate_torch(qt, vt)
+        (dq,) = torch.autograd.grad(y, (qt,), g)
+        return y, dq
+
+    y0, dq0 = grads()
+    mesh = interp.mesh_and_vertices
+    mesh.set_policy(budgets=[0])
+    try:
+        y1, dq1 = grads()
+    finally:
+        mesh.set_policy(budgets=[96])
+    assert torch.equal(y0, y1)
+    nnz = np.diff(interp.get_weights(q).indptr)
+    assert (nnz[400:] == 0).all()
+    assert bool((dq1[400:] == 0).all()) and bool(torch.isfinite(dq1).all())
+    torch.testing.assert_close(dq1[:400], dq0[:400], rtol=1e-12, atol=1e-12)
