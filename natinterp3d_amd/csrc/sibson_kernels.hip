@@ -374,6 +374,29 @@ __device__ int coincident(const DevMesh &m, int start, const d3 &q, int out[4]) 
     return n;
 }
 
+// The same in an int4 (unused entries 0), built with fixed indices so that
+// nothing goes to private memory.
+__device__ __forceinline__ int coincident4(const DevMesh &m, int start, const d3 &q, int4 &out) {
+    bool neg;
+    const int4 tv = ld_tet_v(m, start, neg);
+    int n = 0;
+    int o0 = 0, o1 = 0, o2 = 0, o3 = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int p = sel4(tv, i);
+        const d3 v = ld_vert(m, p);
+        if (p < m.n_data && v.x == q.x && v.y == q.y && v.z == q.z) {
+            o0 = n == 0 ? p : o0;
+            o1 = n == 1 ? p : o1;
+            o2 = n == 2 ? p : o2;
+            o3 = n == 3 ? p : o3;
+            n++;
+        }
+    }
+    out = make_int4(o0, o1, o2, o3);
+    return n;
+}
+
 // Slot of vertex v in a tet's vertex record (v is one of the four).
 __device__ __forceinline__ int slot_in(const int4 &tv, int v) {
     return tv.x == v ? 0 : (tv.y == v ? 1 : (tv.z == v ? 2 : 3));
@@ -1206,7 +1229,7 @@ topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict
         const int vcap = active ? min(grp::VMAX, static_cast<int>(route_of(m, qi) & 0xffu)) : 0;
         bool ok = st != -2 && vcap > 0;
         int ncav = 0, nbnd = 0, nnb = 0;
-        int coin[4] = {0, 0, 0, 0};
+        int4 coin = make_int4(0, 0, 0, 0);
         if (active && st >= 0 && ok) {
             // ---- phase 0: empty tables
             for (int i = gl; i < grp::VT; i += grp::G) {
@@ -1360,7 +1383,7 @@ topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict
             ncav = ok ? nc : 0;
             if (ok && ncav == 0) {
                 // coincident-vertex path, natural_insertionfree.c:596-622
-                nnb = coincident(m, st, qq, coin);
+                nnb = coincident4(m, st, qq, coin);
             } else if (ok) {
                 // ---- phase D: boundary faces
                 for (int c0 = 0; c0 < 4 * ncav; c0 += grp::G) {
@@ -1463,7 +1486,7 @@ topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict
         if (gl == 0) {
             topo.rec_off[qi] = off;
             r[0] = make_int4(ncav, nbnd, nnb, 0);
-            if (ncav == 0 && nnb > 0) r[1] = make_int4(coin[0], coin[1], coin[2], coin[3]);
+            if (ncav == 0 && nnb > 0) r[1] = coin;
         }
         if (ncav == 0) continue;
         for (int ci = gl; ci < ncav; ci += grp::G) {
