@@ -125,14 +125,16 @@ int nat_weights_row_order(const nat_mesh *m, int num_query_points, const int32_t
 
 /* W @ values for the rows of the last nat_weights_compute on this mesh, read
  * from the mesh's row pool rather than an emitted CSR: d_out[M, dim] float64
- * = W (float64) x d_values[N, dim] (values_dtype 0 = float32, 1 = float64;
+ * = W (float64) x d_values[N, dim] (N = num_data_points, all problems of a
+ * batch, checked against the mesh; values_dtype 0 = float32, 1 = float64;
  * dim 1..4), bit for bit what nat_spmm_csr gives on nat_weights_emit's CSR
  * (scipy's stored-order sums, natinterp3d.py:131-137).  The pool rows are the
  * CSR rows in compute order with the mesh's own vertex numbering, so the
  * product reads them nearly in sequence.  Valid until the next compute.
  * (No reference counterpart.) */
-int nat_weights_spmm(nat_mesh *m, int num_query_points, const void *d_values, int values_dtype,
-                     int dim, double *d_out, void *stream);
+int nat_weights_spmm(nat_mesh *m, int num_query_points, const void *d_values,
+                     int num_data_points, int values_dtype, int dim, double *d_out,
+                     void *stream);
 
 /* Sort the entries of every CSR row by column, in place (d_vals dtype codes
  * as above).  The reference's Interpolator.get_weights casts W to the query

@@ -110,7 +110,8 @@ def lib():
     L.nat_spmm_t_csr.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp,
                                  vp, vp]
     L.nat_weights_spmm.restype = ctypes.c_int
-    L.nat_weights_spmm.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp, vp]
+    L.nat_weights_spmm.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, vp, vp]
     L.nat_topology_target.restype = ctypes.c_int
     L.nat_topology_target.argtypes = [vp, vp, ctypes.c_int64, ctypes.c_int]
     L.nat_topology_row_order.restype = ctypes.c_int

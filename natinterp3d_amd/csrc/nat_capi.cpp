@@ -1582,14 +1582,18 @@ int nat_spmm_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
     return 0;
 }
 
-int nat_weights_spmm(nat_mesh *m, int num_query_points, const void *d_values, int values_dtype,
-                     int dim, double *d_out, void *stream) {
+int nat_weights_spmm(nat_mesh *m, int num_query_points, const void *d_values,
+                     int num_data_points, int values_dtype, int dim, double *d_out,
+                     void *stream) {
     g_err.clear();
     if (!m) return fail(-1, "NULL argument");
     if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
     if (values_dtype < 0 || values_dtype > 1) return fail(-1, "values dtype %d", values_dtype);
     if (dim < 1 || dim > 4)
         return fail(-1, "dim %d: nat_weights_spmm takes 1 to 4 value columns", dim);
+    if (num_data_points != m->dev.n_data)
+        return fail(-1, "values have %d rows, the mesh %d data points", num_data_points,
+                    m->dev.n_data);
     std::lock_guard<std::mutex> lk(m->mu);
     if (m->last_M < 0 || num_query_points != m->last_M)
         return fail(-6, "last compute had %d queries, not %d", m->last_M, num_query_points);

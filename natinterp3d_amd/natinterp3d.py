@@ -321,9 +321,9 @@ def weights_spmm(handle, num_rows, values):
     out = torch.empty((num_rows, values.shape[1]), dtype=torch.float64, device=values.device)
     stream = torch.cuda.current_stream(values.device).cuda_stream
     _lib.check(_lib.lib().nat_weights_spmm(
-        handle, num_rows, ctypes.c_void_p(values.data_ptr()), codes[values.dtype],
-        values.shape[1], ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream)),
-        "nat_weights_spmm")
+        handle, num_rows, ctypes.c_void_p(values.data_ptr()), values.shape[0],
+        codes[values.dtype], values.shape[1], ctypes.c_void_p(out.data_ptr()),
+        ctypes.c_void_p(stream)), "nat_weights_spmm")
     return out
 
 

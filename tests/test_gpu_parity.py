@@ -233,6 +233,8 @@ def test_pool_product_equals_csr_product(vdt):
         weights_spmm(m.handle, len(q), torch.zeros((2500, 5), dtype=torch.float64).cuda())
     with pytest.raises(RuntimeError, match="queries"):
         weights_spmm(m.handle, len(q) + 1, xd)
+    with pytest.raises(RuntimeError, match="data points"):
+        weights_spmm(m.handle, len(q), xd[:-1])
     ks = [rng.random((n, 3)) for n in (800, 1500, 300)]
     qs = [0.05 + 0.9 * rng.random((n, 3)) for n in (900, 2000, 400)]
     b = natinterp3d_amd.BatchInterpolator(ks)
