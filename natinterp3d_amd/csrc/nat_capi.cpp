@@ -1068,9 +1068,30 @@ hipError_t xfer(int device, void *dst, const void *src, size_t n, bool to_host) 
                             static_cast<const char *>(src) + b0, b1 - b0, to_host);
         err[l] = le;
     };
-    for (int l = 1; l < lanes; l++) th.emplace_back(run, l);
-    run(0);
+    bool spawned = true;
+    try {
+        for (int l = 1; l < lanes; l++) th.emplace_back(run, l);
+    } catch (...) {  // no thread: the lanes started so far finish, the rest in one copy
+        spawned = false;
+    }
+    if (spawned) run(0);
     for (std::thread &t : th) t.join();
+    if (!spawned) {
+        const size_t b0 = chunks * (th.size() + 1) / lanes * kXferChunk;
+        hipError_t le = hipSetDevice(device);
+        if (le == hipSuccess) {
+            // slices 1..th.size() are done; slice 0 and those past them remain
+            const size_t s0 = chunks * 1 / lanes * kXferChunk;
+            le = hipMemcpy(dst, src, std::min(n, s0), kind);
+            if (le == hipSuccess && b0 < n)
+                le = hipMemcpy(static_cast<char *>(dst) + b0,
+                               static_cast<const char *>(src) + b0, n - b0, kind);
+        }
+        if (le != hipSuccess) return le;
+        for (size_t l = 1; l <= th.size(); l++)
+            if (err[l] != hipSuccess) return err[l];
+        return hipSuccess;
+    }
     for (hipError_t le : err)
         if (le != hipSuccess) return le;
     return hipSuccess;
