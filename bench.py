@@ -64,16 +64,16 @@ def kernel_bytes_per_query(w, rec_q, D):
     topo_group_kernel: order + start (8), query (24), tet vertex ids of every
       tested tet (16 T), their vertices (24 V), neighbour ids of cavity tets
       (16 C), the record written (rec_q) and its offset (8);
-    geom_kernel: order + record offset (12), query (24), record (rec_q), tet
+    geom_rows_kernel: order + record offset (12), query (24), record (rec_q), tet
       vertex ids of cavity tets (16 C), vertices (24 V), row written (12 nnz)
       and its offset/length (12);
-    grad_geom_kernel: as geom_kernel without the row, plus dL/dy (8 D), the
+    grad_geom_kernel: as geom_rows_kernel without the row, plus dL/dy (8 D), the
       neighbours' values (8 D nnz) and dL/dq written (24)."""
     geo = 12 + 24 + rec_q + 16 * w["cavity"] + 24 * w["verts_read"]
     return {
         "topo_group_kernel": 8 + 24 + 16 * w["tets_read"] + 24 * w["verts_read"]
                              + 16 * w["cavity"] + rec_q + 8,
-        "geom_kernel": geo + 12 * w["nnz"] + 12,
+        "geom_rows_kernel": geo + 12 * w["nnz"] + 12,
         "grad_geom_kernel": geo + 8 * D + 8 * D * w["nnz"] + 24,
     }
 
@@ -88,7 +88,7 @@ def kernel_fp64_ops_per_query(w):
            + 89 * (w["sub3"] + w["sub4"]))
     return {
         "topo_group_kernel": 77 * w["insphere"],
-        "geom_kernel": geo + 2 * w["nnz"] + 1,
+        "geom_rows_kernel": geo + 2 * w["nnz"] + 1,
         "grad_geom_kernel": geo + 90 * w["sub4"] + 15 * (w["sub3"] + w["sub4"]),
     }
 
@@ -682,7 +682,7 @@ def main():
     kb = kernel_bytes_per_query(WORK_PER_QUERY, rec_q, D)
     kops = kernel_fp64_ops_per_query(WORK_PER_QUERY)
     kms = {"topo_group_kernel": float(np.mean([t["topo_ms"] for t in per_step])),
-           "geom_kernel": float(np.mean([t["geom_ms"] for t in per_step])),
+           "geom_rows_kernel": float(np.mean([t["geom_ms"] for t in per_step])),
            "grad_geom_kernel": float(np.mean([t["grad_ms"][0] for t in per_step]))}
     dom = max(kms, key=kms.get)
     launch_ms = kms[dom]

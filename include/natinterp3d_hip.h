@@ -315,12 +315,16 @@ typedef struct {
     int32_t topo_grid;    /* group-kernel grid = resident blocks x this (1..4) */
     int32_t geom_grid;    /* geometry-kernel grid = resident blocks x this (1..4) */
     int32_t wave_rows;    /* 1: the wave-per-query level runs */
-    int32_t reserved[3];
+    int32_t geom_lanes;   /* FP64 geometry kernel lanes per row: 2 (0 also
+                             means 2) or 1 (one thread per row); mixed
+                             precision and a mesh without the step-3 table
+                             always run one thread per row */
+    int32_t reserved[2];
 } nat_policy;
 
 /* Sets the policy and, when budgets is given, the group kernel's visited-tet
  * budget per problem (budgets[P], 0..96; 0 = thread-per-query rows only).
- * A graph keeps the grid sizes and work sort of its capture and reads the
+ * A graph keeps the grid sizes, work sort and geometry lanes of its capture and reads the
  * budgets and first level at every replay. */
 int nat_mesh_set_policy(nat_mesh *m, const nat_policy *policy, const uint8_t *budgets);
 int nat_mesh_get_policy(const nat_mesh *m, nat_policy *policy, uint8_t *budgets);

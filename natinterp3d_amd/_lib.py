@@ -44,7 +44,8 @@ class Policy(ctypes.Structure):
     _fields_ = [
         ("first_level", ctypes.c_int32), ("work_sort", ctypes.c_int32),
         ("topo_grid", ctypes.c_int32), ("geom_grid", ctypes.c_int32),
-        ("wave_rows", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
+        ("wave_rows", ctypes.c_int32), ("geom_lanes", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 

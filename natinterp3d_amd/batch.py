@@ -69,7 +69,8 @@ class BatchInterpolator:
     def set_policy(self, **kw):
         """Change the row-path policy: budgets (per problem, 0..96; 0 = skip
         the group kernel), first_level (1 or 2), work_sort (bool), topo_grid,
-        geom_grid (1..4), wave_rows (bool).  Results do not depend on it."""
+        geom_grid (1..4), wave_rows (bool), geom_lanes (2 or 1).  Results do
+        not depend on it."""
         set_row_policy(self._m, self.num_problems, **kw)
 
     def autotune(self, query_sets, repeats=3):

@@ -117,7 +117,7 @@ struct nat_mesh {
     DevBuf route;
     int first_level = 1;
     int work_sort = 1;
-    int topo_grid = 1, geom_grid = 1;
+    int topo_grid = 1, geom_grid = 1, geom_lanes = 2;
     int wave_rows = 1;                  // wave-per-query level before the thread-per-query ones
     DevBuf wscr;                        // its per-wave scratch
     int wave_cap = 0;                   // waves the scratch holds
@@ -127,6 +127,7 @@ struct nat_mesh {
     int nthr[kLevels] = {0, 0, 0, 0};  // [0]: locate kernel threads; the most per level
     int scap[kLevels] = {0, 0, 0, 0};  // threads the level's scratch is laid out for
     int nthr_geom = 0;
+    int geom_rows_blocks = 0;
     int grad_blocks = 0;               // resident blocks of the gradient geometry kernel
     int topo_blocks = 0;
     DevBuf scratch[kLevels];
@@ -225,6 +226,7 @@ void set_levels(nat_mesh *m) {
     m->nthr[0] = cus * nat::locate_blocks_per_cu() * 256;
     m->topo_blocks = cus * nat::topo_blocks_per_cu();
     m->nthr_geom = cus * nat::geom_blocks_per_cu() * nat::kGeomThreads;
+    m->geom_rows_blocks = cus * nat::geom_rows_blocks_per_cu();
     m->grad_blocks = cus * nat::grad_geom_blocks_per_cu();
     // larger levels (fused kernel) only see the queries that overflowed; a
     // level's scratch holds as many threads as it has queries, up to these
@@ -717,7 +719,8 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
             HIP_OR_FAIL(nat::launch_work_sort(m->order.as<int>(), M, topo,
                                               m->geom_order.as<int>(), st));
         HIP_OR_FAIL(nat::launch_geom(
-            m->dev, m->nthr_geom / nat::kGeomThreads * m->geom_grid, d_q,
+            m->dev, m->nthr_geom / nat::kGeomThreads * m->geom_grid,
+            m->geom_lanes == 2 ? m->geom_rows_blocks * m->geom_grid : 0, d_q,
             (m->work_sort ? m->geom_order : m->order).as<int>(), M, ctr + 40,
             topo, out, m->mixed_tau, m->mixed_kappa,
             m->precision == 1 ? reinterpret_cast<unsigned long long *>(ctr + 20) : nullptr, st));
@@ -1166,7 +1169,8 @@ int issue_graph_rows(nat_mesh *m, nat_graph *g, const double *d_q, hipStream_t s
         HIP_OR_FAIL(
             nat::launch_work_sort(m->order.as<int>(), M, topo, m->geom_order.as<int>(), st));
     HIP_OR_FAIL(nat::launch_geom(
-        m->dev, m->nthr_geom / nat::kGeomThreads * m->geom_grid, d_q,
+        m->dev, m->nthr_geom / nat::kGeomThreads * m->geom_grid,
+        m->geom_lanes == 2 ? m->geom_rows_blocks * m->geom_grid : 0, d_q,
         (m->work_sort ? m->geom_order : m->order).as<int>(), M, ctr + 40, topo,
         out, m->mixed_tau, m->mixed_kappa,
         m->precision == 1 ? reinterpret_cast<unsigned long long *>(ctr + 20) : nullptr, st));
@@ -1235,6 +1239,7 @@ int problem_ids(nat_mesh *m, const int64_t *query_offsets, long long M, hipStrea
 struct Policy {
     std::vector<unsigned char> budget;
     int first_level = 1, work_sort = 1, topo_grid = 1, geom_grid = 1, wave_rows = 1;
+    int geom_lanes = 2;
 };
 
 Policy policy_of(const nat_mesh *m) {
@@ -1245,6 +1250,7 @@ Policy policy_of(const nat_mesh *m) {
     p.topo_grid = m->topo_grid;
     p.geom_grid = m->geom_grid;
     p.wave_rows = m->wave_rows;
+    p.geom_lanes = m->geom_lanes;
     return p;
 }
 
@@ -1255,6 +1261,7 @@ int apply_policy(nat_mesh *m, const Policy &p) {
     m->topo_grid = p.topo_grid;
     m->geom_grid = p.geom_grid;
     m->wave_rows = p.wave_rows;
+    m->geom_lanes = p.geom_lanes;
     return upload_route(m);
 }
 
@@ -2002,6 +2009,8 @@ int nat_mesh_set_policy(nat_mesh *m, const nat_policy *policy, const uint8_t *bu
         return fail(-1, "work_sort %d (0 or 1)", policy->work_sort);
     if (policy->wave_rows != 0 && policy->wave_rows != 1)
         return fail(-1, "wave_rows %d (0 or 1)", policy->wave_rows);
+    if (policy->geom_lanes < 0 || policy->geom_lanes > 2)
+        return fail(-1, "geom_lanes %d (0, 1 or 2)", policy->geom_lanes);
     if (policy->topo_grid < 1 || policy->topo_grid > 4 || policy->geom_grid < 1 ||
         policy->geom_grid > 4)
         return fail(-1, "grid multipliers %d, %d outside 1..4", policy->topo_grid,
@@ -2020,6 +2029,7 @@ int nat_mesh_set_policy(nat_mesh *m, const nat_policy *policy, const uint8_t *bu
     p.topo_grid = policy->topo_grid;
     p.geom_grid = policy->geom_grid;
     p.wave_rows = policy->wave_rows;
+    p.geom_lanes = policy->geom_lanes == 1 ? 1 : 2;
     return apply_policy(m, p);
 }
 
@@ -2032,6 +2042,7 @@ int nat_mesh_get_policy(const nat_mesh *m, nat_policy *policy, uint8_t *budgets)
     policy->topo_grid = m->topo_grid;
     policy->geom_grid = m->geom_grid;
     policy->wave_rows = m->wave_rows;
+    policy->geom_lanes = m->geom_lanes;
     if (budgets)
         for (int p = 0; p < m->problems(); p++)
             budgets[p] = p < static_cast<int>(m->budget.size()) ? m->budget[p] : kGroupBudget;

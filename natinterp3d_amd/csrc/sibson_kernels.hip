@@ -7,8 +7,10 @@
 //                      reference's LIFO order, boundary faces, neighbour
 //                      first-appearance order; a compact record per query
 //                                             natural_insertionfree.c:486-584
-//   geom_kernel   one thread per query: circumcentres and Voronoi subcell
-//                 volumes from the record (steps 3-5), row into the pool
+//   geom_rows_kernel  two lanes per query: Voronoi subcell volumes from the
+//                 record and the step-3 table (steps 3-5), row into the pool;
+//                 geom_kernel, one thread per query, where mixed precision is
+//                 on or the table is absent
 //                                             natural_insertionfree.c:628-737
 // Queries whose cavity or row exceeds the level-0 capacities are finished by
 // sibson_rows_kernel, which does all of it in one thread with larger scratch.
@@ -2383,6 +2385,157 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
     }
 }
 
+// The same rows (FP64, step-3 table present) with G lanes per row and 64 / G
+// rows per wave.  At G = 2 a CU holds 512 rows in flight (4 waves per SIMD at
+// 128 VGPRs) against 640 of one thread per row, and a wave works on 32
+// consecutive rows of the Morton order: the mesh data of the rows in flight
+// fits L2 better, and each row's tet and vertex loads are spread over two
+// lanes.  Every
+// accumulator sees the additions of the loop above in the same order:
+//   step 3  the cavity tets in order; lane g adds the table values of
+//           vertices g, g + G, ... (the vertices of a tet have distinct
+//           slots);
+//   step 4  the lanes compute G boundary faces at a time, then subtract
+//           their contributions face by face in sequence order (the three
+//           slots of a face are distinct);
+//   step 5  one lane sums the accumulators in slot order.
+// The lanes of a row share a wave and its LDS operations execute in issue
+// order; each lane's accesses to acc[slot][row] take data-dependent slots, so
+// the compiler keeps them in program order.  Only the face-by-face
+// subtraction needs a compiler barrier (per lane, "if (g == t)" in a loop
+// over t is one subtraction that could be hoisted out of the loop); no
+// memory fence is placed, so the table and vertex loads of later tets and
+// faces can be issued early.
+constexpr int kGeomRowsThreads = 256;
+constexpr int kGeomRowsBatch = 64;
+constexpr int kGeomRowLanes = 2;  // 14.9 ms at 1M keys / 10M queries; 4 lanes 15.1, 8 lanes 18.8
+
+template <int G>
+__global__ void __launch_bounds__(kGeomRowsThreads) geom_rows_kernel(
+    DevMesh m, const double *__restrict__ q, const int *__restrict__ items, int n_items,
+    unsigned int *work, Topo topo, RowOut out) {
+    constexpr int RPW = 64 / G;  // rows per wave
+    __shared__ double acc[kLevel0Row][kGeomRowsThreads / G];
+    const int lane = __lane_id();
+    const int g = lane & (G - 1);
+    const int lead = lane & ~(G - 1);
+    const int rr = static_cast<int>(threadIdx.x) / G;
+    int parts_done = 0, first = 0, nbatch = 0, sub = kGeomRowsBatch;
+    for (;;) {
+        if (sub >= nbatch) {
+            if (!part_batch(work, n_items, kGeomRowsBatch, parts_done, first, nbatch)) break;
+            sub = 0;
+        }
+        const int item = sub + lane / G;
+        sub += RPW;
+        int qi = -1, cnt = 0;
+        long long roff = -1;
+        if (item < nbatch) {
+            qi = items ? items[first + item] : first + item;
+            roff = topo.rec_off[qi];
+        }
+        const int4 *r = roff >= 0 ? topo.rec + roff : nullptr;
+        if (r) {
+            const int4 hd = r[0];
+            const int ncav = hd.x, nbnd = hd.y, nnb = hd.z;
+            if (ncav == 0) {
+                cnt = nnb;  // coincident vertices (or an empty row)
+            } else {
+                const d3 qq = {q[3 * static_cast<size_t>(qi)],
+                               q[3 * static_cast<size_t>(qi) + 1],
+                               q[3 * static_cast<size_t>(qi) + 2]};
+                for (int j = g; j < nnb; j += G) acc[j][rr] = 0.0;
+                // step 3
+                if (g < 4) {
+                    for (int ci = 0; ci < ncav; ci++) {
+                        const int4 cr = r[1 + ci];
+                        const size_t base = (static_cast<size_t>(cr.x) * 16 +
+                                             (static_cast<unsigned>(cr.w) & 15u)) * 4;
+#pragma unroll
+                        for (int vi = g; vi < 4; vi += G) {
+                            const double v = m.old_vol[base + vi];
+                            const int slot = rec_slot(cr, vi);
+                            if (slot >= 0) acc[slot][rr] += v;
+                        }
+                    }
+                }
+                // step 4
+                const int *rf = reinterpret_cast<const int *>(r + 1 + ncav);
+                for (int c0 = 0; c0 < nbnd; c0 += G) {
+                    const int bi = c0 + g;
+                    // up to three (slot, volume) pairs, in call order; filled
+                    // with value selects (stores merged through a selected
+                    // address would put the pairs in scratch memory)
+                    int s0 = -1, s1 = -1, s2 = -1;
+                    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+                    if (bi < nbnd) {
+                        const int code = rf[bi];
+                        const int4 cr = r[1 + (code >> 2)];
+                        new_contributions(
+                            m, cr.x, code & 3, qq, [&](int vi, int) { return rec_slot(cr, vi); },
+                            [&](int slot, double vol) {
+                                const bool f0 = s0 < 0;
+                                const bool f1 = !f0 && s1 < 0;
+                                const bool f2 = !f0 && !f1;
+                                s0 = f0 ? slot : s0;
+                                v0 = f0 ? vol : v0;
+                                s1 = f1 ? slot : s1;
+                                v1 = f1 ? vol : v1;
+                                s2 = f2 ? slot : s2;
+                                v2 = f2 ? vol : v2;
+                            });
+                    }
+                    const int nf = min(G, nbnd - c0);
+                    for (int t = 0; t < nf; t++) {
+                        if (g == t) {
+                            if (s0 >= 0) acc[s0][rr] -= v0;
+                            if (s1 >= 0) acc[s1][rr] -= v1;
+                            if (s2 >= 0) acc[s2][rr] -= v2;
+                        }
+                        // keeps face t's subtractions between those of faces
+                        // t - 1 and t + 1 (no instruction is emitted)
+                        asm volatile("" ::: "memory");
+                    }
+                }
+                // step 5
+                double total = 0.0;
+                if (g == 0)
+                    for (int j = 0; j < nnb; j++) total += acc[j][rr];
+                total = __shfl(total, lead, 64);
+                if (g == 0) out.row_total[qi] = total;
+                if (total > 1e-30) {
+                    const double inv = 1.0 / total;
+                    for (int j = g; j < nnb; j += G) acc[j][rr] *= inv;
+                    cnt = nnb;
+                }
+            }
+        }
+        const long long off_w = wave_reserve(out.pool_used, g == 0 ? cnt : 0);
+        const long long off = __shfl(off_w, lead, 64);
+        if (!r) continue;  // another level makes this row
+        if (g == 0) {
+            out.row_off[qi] = off;
+            out.row_cnt[qi] = cnt;
+        }
+        if (cnt == 0 || off + cnt > out.pool_cap) continue;
+        const int4 hd = r[0];
+        if (hd.x == 0) {
+            const int4 co = r[1];
+            const double w = 1.0 / cnt;
+            for (int j = g; j < cnt; j += G) {
+                out.pool_cols[off + j] = sel4(co, j);
+                out.pool_vals[off + j] = w;
+            }
+        } else {
+            const int *rn = reinterpret_cast<const int *>(r + 1 + hd.x + (hd.y + 3) / 4);
+            for (int j = g; j < cnt; j += G) {
+                out.pool_cols[off + j] = rn[j];
+                out.pool_vals[off + j] = acc[j][rr];
+            }
+        }
+    }
+}
+
 // Gradient of the rows that have a topology record (the cached cavity).
 // With lambda_j the forward's weights and S its total stolen volume,
 // dL/dq = (sum_j a_j G_j - abar sum_j G_j) / S, abar = sum_j a_j lambda_j,
@@ -2679,6 +2832,13 @@ int geom_blocks_per_cu() {
         return 1;
     return n > 0 ? n : 1;
 }
+int geom_rows_blocks_per_cu() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, geom_rows_kernel<kGeomRowLanes>,
+                                                     kGeomRowsThreads, 0) != hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
+}
 
 hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, int level,
                        int nblocks, const double *d_q, const int *d_items, int n_items,
@@ -2750,10 +2910,16 @@ hipError_t launch_work_sort(const int *d_items, int n, const Topo &topo, int *d_
     return hipGetLastError();
 }
 
-hipError_t launch_geom(const DevMesh &m, int nblocks, const double *d_q, const int *d_items,
-                       int n_items, unsigned int *d_work, const Topo &topo, const RowOut &out,
-                       float mixed_tau, float mixed_kappa, unsigned long long *mix,
-                       hipStream_t st) {
+hipError_t launch_geom(const DevMesh &m, int nblocks, int nblocks_rows, const double *d_q,
+                       const int *d_items, int n_items, unsigned int *d_work, const Topo &topo,
+                       const RowOut &out, float mixed_tau, float mixed_kappa,
+                       unsigned long long *mix, hipStream_t st) {
+    if (!mix && m.old_vol && nblocks_rows > 0) {
+        hipLaunchKernelGGL(geom_rows_kernel<kGeomRowLanes>, dim3(nblocks_rows),
+                           dim3(kGeomRowsThreads), 0, st, m, d_q, d_items, n_items, d_work, topo,
+                           out);
+        return hipGetLastError();
+    }
     if (mix)
         hipLaunchKernelGGL(geom_kernel<true>, dim3(nblocks), dim3(kGeomThreads), 0, st, m, d_q,
                            d_items, n_items, d_work, topo, out, mixed_tau, mixed_kappa, mix);

@@ -126,8 +126,9 @@ class MeshAndVerticesHIP:
     def set_policy(self, **kw):
         """Change the row-path policy: budgets (list of one visited-tet budget,
         0..96; 0 = skip the group kernel), first_level (1 or 2), work_sort
-        (bool), topo_grid, geom_grid (1..4), wave_rows (bool).  Results do
-        not depend on it."""
+        (bool), topo_grid, geom_grid (1..4), wave_rows (bool), geom_lanes
+        (2 or 1: FP64 geometry kernel lanes per row).  Results do not depend
+        on it."""
         set_row_policy(self._m, 1, **kw)
 
     def autotune(self, queries, repeats=3):
@@ -631,11 +632,12 @@ def get_row_policy(handle, num_problems):
                "nat_mesh_get_policy")
     return {"budgets": budgets.tolist(), "first_level": pol.first_level,
             "work_sort": bool(pol.work_sort), "topo_grid": pol.topo_grid,
-            "geom_grid": pol.geom_grid, "wave_rows": bool(pol.wave_rows)}
+            "geom_grid": pol.geom_grid, "wave_rows": bool(pol.wave_rows),
+            "geom_lanes": pol.geom_lanes}
 
 
 def set_row_policy(handle, num_problems, budgets=None, first_level=None, work_sort=None,
-                   topo_grid=None, geom_grid=None, wave_rows=None):
+                   topo_grid=None, geom_grid=None, wave_rows=None, geom_lanes=None):
     """Set the given fields of a mesh handle's row-path policy."""
     cur = get_row_policy(handle, num_problems)
     pol = _lib.Policy()
@@ -644,6 +646,7 @@ def set_row_policy(handle, num_problems, budgets=None, first_level=None, work_so
     pol.topo_grid = int(cur["topo_grid"] if topo_grid is None else topo_grid)
     pol.geom_grid = int(cur["geom_grid"] if geom_grid is None else geom_grid)
     pol.wave_rows = int(bool(cur["wave_rows"] if wave_rows is None else wave_rows))
+    pol.geom_lanes = int(cur["geom_lanes"] if geom_lanes is None else geom_lanes)
     b = None
     if budgets is not None:
         arr = np.asarray(budgets)

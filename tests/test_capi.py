@@ -93,8 +93,9 @@ def test_policy_argument_checks():
     m = MeshAndVerticesHIP(np.ascontiguousarray(k - k.mean(0)), device=-1)
     assert get_row_policy(m.handle, 1) == {"budgets": [96], "first_level": 1,
                                            "work_sort": True, "topo_grid": 1, "geom_grid": 1,
-                                           "wave_rows": True}
+                                           "wave_rows": True, "geom_lanes": 2}
     for kw, msg in (({"first_level": 3}, "first_level"), ({"topo_grid": 0}, "grid"),
+                    ({"geom_lanes": 3}, "geom_lanes"), ({"geom_lanes": -1}, "geom_lanes"),
                     ({"geom_grid": 5}, "grid"), ({"budgets": [97]}, "budget"),
                     ({"first_level": 2}, "host-only")):
         with pytest.raises(RuntimeError, match=msg):

@@ -45,6 +45,8 @@ POLICIES = [
     dict(work_sort=False),
     dict(topo_grid=2, geom_grid=3),
     dict(first_level=2, work_sort=False, topo_grid=4, geom_grid=2),
+    dict(geom_lanes=1),
+    dict(geom_lanes=1, work_sort=False, geom_grid=2),
 ]
 
 
@@ -53,7 +55,8 @@ def test_batch_policies_give_identical_rows(oracle_mod):
     P = len(keys)
     b = natinterp3d_amd.BatchInterpolator(keys)
     assert b.policy() == {"budgets": [96] * P, "first_level": 1, "work_sort": True,
-                          "topo_grid": 1, "geom_grid": 1, "wave_rows": True}
+                          "topo_grid": 1, "geom_grid": 1, "wave_rows": True,
+                          "geom_lanes": 2}
     ref = b.get_weights(queries)
     for p in (0, P - 1):  # a box and a shell against the oracle
         single = natinterp3d_amd.Interpolator(keys[p])
@@ -69,7 +72,8 @@ def test_batch_policies_give_identical_rows(oracle_mod):
             Ws = b.get_weights(queries)
             for p in range(P):
                 assert csr_equal(Ws[p], ref[p]), f"budgets {budgets} {extra}: problem {p}"
-            b.set_policy(first_level=1, work_sort=True, topo_grid=1, geom_grid=1)
+            b.set_policy(first_level=1, work_sort=True, topo_grid=1, geom_grid=1,
+                         geom_lanes=2)
         say(f"[policy] budgets {budgets}: all rows equal")
 
 
@@ -84,7 +88,8 @@ def test_single_mesh_routed_rows_and_gradients():
     y0 = interp.interpolate_torch(qt, vt)
     dq0, dv0 = torch.autograd.grad(y0, (qt, vt), g)
     mesh = interp.mesh_and_vertices
-    for pol in ({"budgets": [0]}, {"budgets": [0], "first_level": 2}, {"budgets": [24]}):
+    for pol in ({"budgets": [0]}, {"budgets": [0], "first_level": 2}, {"budgets": [24]},
+                {"geom_lanes": 1}):
         mesh.set_policy(**pol)
         assert csr_equal(interp.get_weights(q), ref), pol
         y = interp.interpolate_torch(qt, vt)
@@ -92,8 +97,31 @@ def test_single_mesh_routed_rows_and_gradients():
         dq, dv = torch.autograd.grad(y, (qt, vt), g)
         torch.testing.assert_close(dq, dq0, rtol=1e-12, atol=1e-12)
         torch.testing.assert_close(dv, dv0, rtol=1e-12, atol=1e-12)
-        mesh.set_policy(budgets=[96], first_level=1)
+        mesh.set_policy(budgets=[96], first_level=1, geom_lanes=2)
         say(f"[policy] single mesh {pol}: rows, products and gradients equal")
+
+
+def test_geometry_lanes_give_identical_rows_at_scale():
+    """The FP64 geometry kernel with two lanes per row (accumulations split
+    over the lanes in the reference's order) against one thread per row, on
+    100k keys (10% in Gaussian blobs) and 1M queries: the device CSR bit for
+    bit."""
+    keys, queries, _ = workloads.plain_at_scale(n_keys=100_000, n_queries=1_000_000, seed=5)
+    c = keys.mean(0)
+    mesh = natinterp3d_amd.MeshAndVerticesHIP(np.ascontiguousarray(keys - c), device=0)
+    qd = torch.from_numpy(np.ascontiguousarray(queries - c)).to(torch.device("cuda", 0))
+    out = {}
+    for lanes in (2, 1, 2):
+        mesh.set_policy(geom_lanes=lanes)
+        assert mesh.policy()["geom_lanes"] == lanes
+        crow, col, val = mesh.weights_device(qd)
+        torch.cuda.synchronize()
+        cur = (crow.cpu(), col.cpu(), val.cpu())
+        if lanes in out:
+            assert all(torch.equal(a, b) for a, b in zip(cur, out[lanes])), "not repeatable"
+        out[lanes] = cur
+    assert all(torch.equal(a, b) for a, b in zip(out[1], out[2]))
+    say(f"[policy] geometry lanes 1 and 2: {int(out[2][0][-1])} entries equal")
 
 
 def test_autotune_batch_keeps_rows_and_reports():
