@@ -315,10 +315,10 @@ typedef struct {
     int32_t topo_grid;    /* group-kernel grid = resident blocks x this (1..4) */
     int32_t geom_grid;    /* geometry-kernel grid = resident blocks x this (1..4) */
     int32_t wave_rows;    /* 1: the wave-per-query level runs */
-    int32_t geom_lanes;   /* FP64 geometry kernel lanes per row: 2 (0 also
-                             means 2) or 1 (one thread per row); mixed
-                             precision and a mesh without the step-3 table
-                             always run one thread per row */
+    int32_t geom_lanes;   /* geometry kernel lanes per row: 2 (0 also
+                             means 2) or 1 (one thread per row); a mesh
+                             without the step-3 table always runs one
+                             thread per row */
     int32_t reserved[2];
 } nat_policy;
 

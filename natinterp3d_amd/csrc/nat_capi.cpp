@@ -127,7 +127,7 @@ struct nat_mesh {
     int nthr[kLevels] = {0, 0, 0, 0};  // [0]: locate kernel threads; the most per level
     int scap[kLevels] = {0, 0, 0, 0};  // threads the level's scratch is laid out for
     int nthr_geom = 0;
-    int geom_rows_blocks = 0;
+    int geom_rows_blocks = 0, geom_rows_blocks_mixed = 0;
     int grad_blocks = 0;               // resident blocks of the gradient geometry kernel
     int topo_blocks = 0;
     DevBuf scratch[kLevels];
@@ -226,7 +226,8 @@ void set_levels(nat_mesh *m) {
     m->nthr[0] = cus * nat::locate_blocks_per_cu() * 256;
     m->topo_blocks = cus * nat::topo_blocks_per_cu();
     m->nthr_geom = cus * nat::geom_blocks_per_cu() * nat::kGeomThreads;
-    m->geom_rows_blocks = cus * nat::geom_rows_blocks_per_cu();
+    m->geom_rows_blocks = cus * nat::geom_rows_blocks_per_cu(false);
+    m->geom_rows_blocks_mixed = cus * nat::geom_rows_blocks_per_cu(true);
     m->grad_blocks = cus * nat::grad_geom_blocks_per_cu();
     // larger levels (fused kernel) only see the queries that overflowed; a
     // level's scratch holds as many threads as it has queries, up to these
@@ -571,6 +572,11 @@ int ensure_wave_scratch(nat_mesh *m, long long want, int *launch) {
 // d_prob: problem of each query (batch meshes), NULL for a single problem
 // The step-3 table (64 doubles per tet), built once per mesh on its first
 // compute; without the memory for it the geometry is computed inline.
+// Grid of the two-lanes-per-row geometry kernel of the mesh's precision.
+int geom_rows_grid(const nat_mesh *m) {
+    return (m->precision == 1 ? m->geom_rows_blocks_mixed : m->geom_rows_blocks) * m->geom_grid;
+}
+
 int ensure_old_vol(nat_mesh *m, hipStream_t st) {
     if (m->old_vol_tried) return 0;
     m->old_vol_tried = true;
@@ -720,7 +726,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
                                               m->geom_order.as<int>(), st));
         HIP_OR_FAIL(nat::launch_geom(
             m->dev, m->nthr_geom / nat::kGeomThreads * m->geom_grid,
-            m->geom_lanes == 2 ? m->geom_rows_blocks * m->geom_grid : 0, d_q,
+            m->geom_lanes == 2 ? geom_rows_grid(m) : 0, d_q,
             (m->work_sort ? m->geom_order : m->order).as<int>(), M, ctr + 40,
             topo, out, m->mixed_tau, m->mixed_kappa,
             m->precision == 1 ? reinterpret_cast<unsigned long long *>(ctr + 20) : nullptr, st));
@@ -1170,7 +1176,7 @@ int issue_graph_rows(nat_mesh *m, nat_graph *g, const double *d_q, hipStream_t s
             nat::launch_work_sort(m->order.as<int>(), M, topo, m->geom_order.as<int>(), st));
     HIP_OR_FAIL(nat::launch_geom(
         m->dev, m->nthr_geom / nat::kGeomThreads * m->geom_grid,
-        m->geom_lanes == 2 ? m->geom_rows_blocks * m->geom_grid : 0, d_q,
+        m->geom_lanes == 2 ? geom_rows_grid(m) : 0, d_q,
         (m->work_sort ? m->geom_order : m->order).as<int>(), M, ctr + 40, topo,
         out, m->mixed_tau, m->mixed_kappa,
         m->precision == 1 ? reinterpret_cast<unsigned long long *>(ctr + 20) : nullptr, st));

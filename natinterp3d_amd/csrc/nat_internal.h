@@ -153,7 +153,7 @@ hipError_t launch_problem_ids(const long long *d_offsets, int P, int M, int *d_p
 int rows_blocks_per_cu();  // resident 256-thread blocks of the fused row kernel per CU
 int topo_blocks_per_cu();   // 128-thread blocks of the topology kernel
 int geom_blocks_per_cu();       // kGeomThreads-thread blocks
-int geom_rows_blocks_per_cu();  // blocks of the two-lanes-per-row geometry kernel
+int geom_rows_blocks_per_cu(bool mixed);  // blocks of the two-lanes-per-row geometry kernel
 int grad_geom_blocks_per_cu();  // resident blocks of the gradient geometry kernel
 // fused row kernel: used for the larger capacity levels
 // d_n_items != NULL: the item count is read on the device (graph form)
@@ -178,9 +178,9 @@ hipError_t launch_work_sort(const int *d_items, int n, const Topo &topo, int *d_
 // mix != NULL: mixed precision (FP32 where the condition estimate is at least
 // mixed_tau; rows with sum |vol| > mixed_kappa |sum vol| redone in FP64),
 // counts of FP32 / FP64 tets and faces and of redone rows added to mix[0..2]
-// nblocks: grid of the thread-per-row kernel (mixed precision, no step-3
-// table, or nblocks_rows = 0); nblocks_rows: grid of the two-lanes-per-row
-// FP64 kernel.
+// nblocks: grid of the thread-per-row kernel (no step-3 table, or
+// nblocks_rows = 0); nblocks_rows: grid of the two-lanes-per-row kernel
+// (FP64, or mixed precision when mix is given).
 hipError_t launch_geom(const DevMesh &m, int nblocks, int nblocks_rows, const double *d_q,
                        const int *d_items, int n_items, unsigned int *d_work, const Topo &topo,
                        const RowOut &out, float mixed_tau, float mixed_kappa,

@@ -9,8 +9,8 @@
 //                                             natural_insertionfree.c:486-584
 //   geom_rows_kernel  two lanes per query: Voronoi subcell volumes from the
 //                 record and the step-3 table (steps 3-5), row into the pool;
-//                 geom_kernel, one thread per query, where mixed precision is
-//                 on or the table is absent
+//                 geom_kernel, one thread per query, where the table is
+//                 absent or the row policy asks for one lane
 //                                             natural_insertionfree.c:628-737
 // Queries whose cavity or row exceeds the level-0 capacities are finished by
 // sibson_rows_kernel, which does all of it in one thread with larger scratch.
@@ -2385,8 +2385,10 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
     }
 }
 
-// The same rows (FP64, step-3 table present) with G lanes per row and 64 / G
-// rows per wave.  At G = 2 a CU holds 512 rows in flight (4 waves per SIMD at
+// The same rows (step-3 table present; MIXED as above, each lane deciding
+// FP32 or FP64 for its own boundary faces, the lanes' sum |vol| shares added
+// across the row before the cancellation test) with G lanes per row and
+// 64 / G rows per wave.  At G = 2 a CU holds 512 rows in flight (4 waves per SIMD at
 // 128 VGPRs) against 640 of one thread per row, and a wave works on 32
 // consecutive rows of the Morton order: the mesh data of the rows in flight
 // fits L2 better, and each row's tet and vertex loads are spread over two
@@ -2410,11 +2412,13 @@ constexpr int kGeomRowsThreads = 256;
 constexpr int kGeomRowsBatch = 64;
 constexpr int kGeomRowLanes = 2;  // 14.9 ms at 1M keys / 10M queries; 4 lanes 15.1, 8 lanes 18.8
 
-template <int G>
+template <int G, bool MIXED>
 __global__ void __launch_bounds__(kGeomRowsThreads) geom_rows_kernel(
     DevMesh m, const double *__restrict__ q, const int *__restrict__ items, int n_items,
-    unsigned int *work, Topo topo, RowOut out) {
+    unsigned int *work, Topo topo, RowOut out, float tau, float kappa,
+    unsigned long long *mix) {
     constexpr int RPW = 64 / G;  // rows per wave
+    unsigned n32 = 0, n64 = 0, nredo = 0;
     __shared__ double acc[kLevel0Row][kGeomRowsThreads / G];
     const int lane = __lane_id();
     const int g = lane & (G - 1);
@@ -2444,36 +2448,40 @@ __global__ void __launch_bounds__(kGeomRowsThreads) geom_rows_kernel(
                 const d3 qq = {q[3 * static_cast<size_t>(qi)],
                                q[3 * static_cast<size_t>(qi) + 1],
                                q[3 * static_cast<size_t>(qi) + 2]};
-                for (int j = g; j < nnb; j += G) acc[j][rr] = 0.0;
-                // step 3
-                if (g < 4) {
-                    for (int ci = 0; ci < ncav; ci++) {
-                        const int4 cr = r[1 + ci];
-                        const size_t base = (static_cast<size_t>(cr.x) * 16 +
-                                             (static_cast<unsigned>(cr.w) & 15u)) * 4;
+                const int *rf = reinterpret_cast<const int *>(r + 1 + ncav);
+                double absum = 0.0;  // this lane's share of sum |vol| (MIXED)
+                auto run = [&](bool fp32) {
+                    for (int j = g; j < nnb; j += G) acc[j][rr] = 0.0;
+                    // step 3 (the exact table in both modes)
+                    if (g < 4) {
+                        for (int ci = 0; ci < ncav; ci++) {
+                            const int4 cr = r[1 + ci];
+                            const size_t base = (static_cast<size_t>(cr.x) * 16 +
+                                                 (static_cast<unsigned>(cr.w) & 15u)) * 4;
 #pragma unroll
-                        for (int vi = g; vi < 4; vi += G) {
-                            const double v = m.old_vol[base + vi];
-                            const int slot = rec_slot(cr, vi);
-                            if (slot >= 0) acc[slot][rr] += v;
+                            for (int vi = g; vi < 4; vi += G) {
+                                const double v = m.old_vol[base + vi];
+                                const int slot = rec_slot(cr, vi);
+                                if (slot >= 0) {
+                                    acc[slot][rr] += v;
+                                    if constexpr (MIXED) absum += fabs(v);
+                                }
+                            }
                         }
                     }
-                }
-                // step 4
-                const int *rf = reinterpret_cast<const int *>(r + 1 + ncav);
-                for (int c0 = 0; c0 < nbnd; c0 += G) {
-                    const int bi = c0 + g;
-                    // up to three (slot, volume) pairs, in call order; filled
-                    // with value selects (stores merged through a selected
-                    // address would put the pairs in scratch memory)
-                    int s0 = -1, s1 = -1, s2 = -1;
-                    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-                    if (bi < nbnd) {
-                        const int code = rf[bi];
-                        const int4 cr = r[1 + (code >> 2)];
-                        new_contributions(
-                            m, cr.x, code & 3, qq, [&](int vi, int) { return rec_slot(cr, vi); },
-                            [&](int slot, double vol) {
+                    // step 4
+                    for (int c0 = 0; c0 < nbnd; c0 += G) {
+                        const int bi = c0 + g;
+                        // up to three (slot, volume) pairs, in call order;
+                        // filled with value selects (stores merged through a
+                        // selected address would put the pairs in scratch)
+                        int s0 = -1, s1 = -1, s2 = -1;
+                        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+                        if (bi < nbnd) {
+                            const int code = rf[bi];
+                            const int4 cr = r[1 + (code >> 2)];
+                            auto slot_of = [&](int vi, int) { return rec_slot(cr, vi); };
+                            auto take = [&](int slot, double vol) {
                                 const bool f0 = s0 < 0;
                                 const bool f1 = !f0 && s1 < 0;
                                 const bool f2 = !f0 && !f1;
@@ -2483,25 +2491,54 @@ __global__ void __launch_bounds__(kGeomRowsThreads) geom_rows_kernel(
                                 v1 = f1 ? vol : v1;
                                 s2 = f2 ? slot : s2;
                                 v2 = f2 ? vol : v2;
-                            });
-                    }
-                    const int nf = min(G, nbnd - c0);
-                    for (int t = 0; t < nf; t++) {
-                        if (g == t) {
-                            if (s0 >= 0) acc[s0][rr] -= v0;
-                            if (s1 >= 0) acc[s1][rr] -= v1;
-                            if (s2 >= 0) acc[s2][rr] -= v2;
+                            };
+                            bool done = false;
+                            if constexpr (MIXED) {
+                                if (fp32) {
+                                    done = new_contributions32(m, cr.x, code & 3, qq, tau,
+                                                               slot_of, take);
+                                    if (done)
+                                        n32++;
+                                    else
+                                        n64++;
+                                }
+                            }
+                            if (!done) new_contributions(m, cr.x, code & 3, qq, slot_of, take);
+                            if constexpr (MIXED) absum += fabs(v0) + fabs(v1) + fabs(v2);
                         }
-                        // keeps face t's subtractions between those of faces
-                        // t - 1 and t + 1 (no instruction is emitted)
-                        asm volatile("" ::: "memory");
+                        const int nf = min(G, nbnd - c0);
+                        for (int t = 0; t < nf; t++) {
+                            if (g == t) {
+                                if (s0 >= 0) acc[s0][rr] -= v0;
+                                if (s1 >= 0) acc[s1][rr] -= v1;
+                                if (s2 >= 0) acc[s2][rr] -= v2;
+                            }
+                            // keeps face t's subtractions between those of
+                            // faces t - 1 and t + 1 (no instruction is emitted)
+                            asm volatile("" ::: "memory");
+                        }
+                    }
+                };
+                auto row_total = [&]() {
+                    double t = 0.0;
+                    if (g == 0)
+                        for (int j = 0; j < nnb; j++) t += acc[j][rr];
+                    return __shfl(t, lead, 64);
+                };
+                run(MIXED);
+                double total = row_total();
+                if constexpr (MIXED) {
+                    // a row whose volumes cancel carries the FP32 rounding of
+                    // its largest terms into its weights: redo it in FP64
+                    double ab = absum;
+#pragma unroll
+                    for (int d = G / 2; d >= 1; d >>= 1) ab += __shfl_xor(ab, d, 64);
+                    if (!(ab <= kappa * fabs(total))) {
+                        run(false);
+                        total = row_total();
+                        if (g == 0) nredo++;
                     }
                 }
-                // step 5
-                double total = 0.0;
-                if (g == 0)
-                    for (int j = 0; j < nnb; j++) total += acc[j][rr];
-                total = __shfl(total, lead, 64);
                 if (g == 0) out.row_total[qi] = total;
                 if (total > 1e-30) {
                     const double inv = 1.0 / total;
@@ -2532,6 +2569,17 @@ __global__ void __launch_bounds__(kGeomRowsThreads) geom_rows_kernel(
                 out.pool_cols[off + j] = rn[j];
                 out.pool_vals[off + j] = acc[j][rr];
             }
+        }
+    }
+    if constexpr (MIXED) {
+        int t32, t64, tre;
+        wave_excl_scan(static_cast<int>(n32), t32);
+        wave_excl_scan(static_cast<int>(n64), t64);
+        wave_excl_scan(static_cast<int>(nredo), tre);
+        if (lane == 0) {
+            atomicAdd(mix, static_cast<unsigned long long>(t32));
+            atomicAdd(mix + 1, static_cast<unsigned long long>(t64));
+            atomicAdd(mix + 2, static_cast<unsigned long long>(tre));
         }
     }
 }
@@ -2832,11 +2880,14 @@ int geom_blocks_per_cu() {
         return 1;
     return n > 0 ? n : 1;
 }
-int geom_rows_blocks_per_cu() {
+int geom_rows_blocks_per_cu(bool mixed) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, geom_rows_kernel<kGeomRowLanes>,
-                                                     kGeomRowsThreads, 0) != hipSuccess)
-        return 1;
+    const hipError_t e =
+        mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &n, geom_rows_kernel<kGeomRowLanes, true>, kGeomRowsThreads, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &n, geom_rows_kernel<kGeomRowLanes, false>, kGeomRowsThreads, 0);
+    if (e != hipSuccess) return 1;
     return n > 0 ? n : 1;
 }
 
@@ -2914,10 +2965,15 @@ hipError_t launch_geom(const DevMesh &m, int nblocks, int nblocks_rows, const do
                        const int *d_items, int n_items, unsigned int *d_work, const Topo &topo,
                        const RowOut &out, float mixed_tau, float mixed_kappa,
                        unsigned long long *mix, hipStream_t st) {
-    if (!mix && m.old_vol && nblocks_rows > 0) {
-        hipLaunchKernelGGL(geom_rows_kernel<kGeomRowLanes>, dim3(nblocks_rows),
-                           dim3(kGeomRowsThreads), 0, st, m, d_q, d_items, n_items, d_work, topo,
-                           out);
+    if (m.old_vol && nblocks_rows > 0) {
+        if (mix)
+            hipLaunchKernelGGL((geom_rows_kernel<kGeomRowLanes, true>), dim3(nblocks_rows),
+                               dim3(kGeomRowsThreads), 0, st, m, d_q, d_items, n_items, d_work,
+                               topo, out, mixed_tau, mixed_kappa, mix);
+        else
+            hipLaunchKernelGGL((geom_rows_kernel<kGeomRowLanes, false>), dim3(nblocks_rows),
+                               dim3(kGeomRowsThreads), 0, st, m, d_q, d_items, n_items, d_work,
+                               topo, out, 0.0f, 0.0f, nullptr);
         return hipGetLastError();
     }
     if (mix)
