@@ -2591,13 +2591,14 @@ __global__ void __launch_bounds__(kGeomRowsThreads) geom_rows_kernel(
 // old subcells, no volumes.  Queries without a record (made by a larger
 // level in the forward pass) are listed in ovf_list.
 //
-// 16 lanes per row, 4 rows per wave, rows handed out 4 at a time: the rows
-// in flight on an XCD (and the vertex and tet lines they share) stay few
-// enough for its L2, where one thread per row kept ten times as many.  The
-// lanes take the neighbours (a_j = <dL/dy, values[j]>, kept in LDS) and the
-// boundary faces in turn; the sums are reduced over the 16 lanes (the
-// gradient has no prescribed summation order).
-constexpr int kGradLanes = 16;
+// 8 lanes per row, 8 rows per wave: the rows in flight on an XCD (and the
+// vertex and tet lines they share) stay few enough for its L2, where one
+// thread per row kept 8 times as many.  The lanes take the neighbours
+// (a_j = <dL/dy, values[j]>, kept in LDS) and the boundary faces in turn;
+// the sums are reduced over the 8 lanes (the gradient has no prescribed
+// summation order).  At 1M keys / 10M queries: 12.0 ms with 8 lanes, 12.5
+// with 4, 13.5 with 16, 17.1 with 32.
+constexpr int kGradLanes = 8;
 constexpr int kGradThreads = 256;
 constexpr int kGradBatch = 32;
 
@@ -2613,7 +2614,7 @@ __global__ void __launch_bounds__(kGradThreads) grad_geom_kernel(
     int parts_done = 0, first = 0, nbatch = 0, sub = kGradBatch;
     for (;;) {
         // a wave claims kGradBatch consecutive items and works through them
-        // 4 at a time: one atomic per claim, few rows in flight
+        // 64 / kGradLanes at a time: one atomic per claim
         if (sub >= nbatch) {
             if (!part_batch(work, n_items, kGradBatch, parts_done, first, nbatch)) break;
             sub = 0;
