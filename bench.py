@@ -94,15 +94,31 @@ def kernel_fp64_ops_per_query(w):
 
 
 def cpu_threads():
-    n = len(os.sched_getaffinity(0))
+    """Host threads for the CPU baseline: the affinity count of this process
+    (the cores it may run on), and OMP_NUM_THREADS when the environment sets
+    one (reported beside it, not applied: the reference's OpenMP loop is
+    given the thread count explicitly)."""
+    n = max(1, len(os.sched_getaffinity(0)))
     env = os.environ.get("OMP_NUM_THREADS")
-    if env and env.isdigit():
-        n = min(n, int(env))
-    return max(1, n)
+    return n, (int(env) if env and env.isdigit() else None)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
 
 
 class RefBuild(threading.Thread):
-    """Builds the reference mesh for the CPU baseline while the GPU runs."""
+    """Builds the reference mesh for the CPU baseline beside the product's
+    own (host) mesh build; joined before any GPU warm-up, so that nothing
+    runs on the host during the timed GPU steps."""
 
     def __init__(self, kc):
         super().__init__(daemon=True)
@@ -118,30 +134,46 @@ class RefBuild(threading.Thread):
             self.err = repr(e)
 
 
-def cpu_baseline(refbuild, qc, target_s):
+def join_refbuild(refbuild):
+    if refbuild is not None:
+        t = time.time()
+        refbuild.join()
+        log(f"reference mesh for the cpu baseline: built in {refbuild.seconds:.1f} s "
+            f"(waited {time.time() - t:.1f} s before the GPU warm-up)")
+
+
+def cpu_baseline(refbuild, qc, target_s, runs=5):
+    """The compiled reference's OpenMP forward (getInsertionFreeWeightsParallel)
+    on the host cores: first a warm-up call (it packs the mesh, as the
+    reference's first call does), then the median of `runs` timed calls over
+    the first n queries, n sized so that the runs take about target_s."""
     refbuild.join()
     if refbuild.mesh is None:
         log(f"cpu baseline unavailable: {refbuild.err}")
         return None
-    th = cpu_threads()
+    th, omp = cpu_threads()
     m = refbuild.mesh
     m.get_weights(qc[:2000], num_threads=th)  # first call packs the mesh
     n = min(len(qc), 100_000)
     t = time.perf_counter()
     m.get_weights(qc[:n], num_threads=th)
     rate = n / (time.perf_counter() - t)
-    n = int(min(len(qc), max(n, rate * target_s)))
+    n = int(min(len(qc), max(n, rate * target_s / runs)))
     times = []
-    for _ in range(3):
+    for _ in range(runs):
         t = time.perf_counter()
         m.get_weights(qc[:n], num_threads=th)
         times.append(time.perf_counter() - t)
-    med = sorted(times)[1]
-    import platform
+    med = float(np.median(times))
+    model = cpu_model()
     return {"value": n / med, "unit": "queries/s", "cores": th, "kind": "reference",
-            "sample": f"first {n:,} queries of the workload, median of 3 runs after a "
-                      f"warm-up call, mesh built by the reference in {refbuild.seconds:.0f} s; "
-                      f"host {platform.processor() or platform.machine()}"}
+            "threads": th, "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads_env": omp, "cpu_model": model, "runs": runs,
+            "run_seconds": [round(x, 3) for x in times],
+            "sample": f"first {n:,} queries of the workload, median of {runs} runs after a "
+                      f"warm-up call, {th} OpenMP threads (the process's affinity count), "
+                      f"mesh built by the reference in {refbuild.seconds:.0f} s (before the "
+                      f"GPU warm-up); host CPU: {model}"}
 
 
 def run_precision(args):
@@ -267,6 +299,7 @@ def run_shell(args):
             refbuild = RefBuild(kc)
             refbuild.start()
     mesh = natinterp3d_amd.MeshAndVerticesHIP(kc, device=0)
+    join_refbuild(refbuild)
     dev = torch.device("cuda", 0)
     M = qc.shape[0]
     qd = torch.from_numpy(qc).to(dev).requires_grad_(True)
@@ -380,7 +413,8 @@ def run_batch(args):
     nk = sum(len(k) for k, _, _ in probs)
     nq = sum(len(q) for _, q, _ in probs)
     log(f"batch workload: {len(probs)} problems, {nk:,} keys, {nq:,} queries")
-    th = cpu_threads()
+    aff, omp = cpu_threads()
+    th = min(aff, omp or 16)  # host builder pool: the box's CPU share
     t = time.time()
     batch = natinterp3d_amd.BatchInterpolator([k for k, _, _ in probs], device=0, num_threads=th)
     build_s = time.time() - t
@@ -505,7 +539,7 @@ def main():
     ap.add_argument("--keys", type=int, default=1_000_000)
     ap.add_argument("--queries", type=int, default=10_000_000)
     ap.add_argument("--cpu-baseline", choices=["reference", "none"], default="reference")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["plain", "batch", "precision", "shell"],
                     default="plain",
                     help="plain: the fwd+bwd headline; batch: 256 independent problems in "
@@ -567,6 +601,7 @@ def main():
     build_s = time.time() - t
     info = mesh.info()
     log(f"mesh: {info['n_tets']:,} tets, k-d depth {info['kd_depth']}, built in {build_s:.1f} s")
+    join_refbuild(refbuild)
 
     dev = torch.device("cuda", 0)
     M, D = qc.shape[0], values.shape[1]

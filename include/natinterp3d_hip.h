@@ -172,8 +172,11 @@ int nat_batch_weights_compute(nat_mesh *m, const double *d_queries,
  * larger levels and the CSR assembly -- in a hipGraph on a private stream.
  * nat_graph_launch replays it on `stream` for whatever d_queries holds now
  * (same M and offsets) and returns nnz; it returns -8 when a capacity was
- * exceeded (compute eagerly instead, or make a new graph) and -9 when an
- * eager call on the mesh has reallocated its workspace since.
+ * exceeded (compute eagerly instead, or make a new graph), -9 when an
+ * eager call on the mesh has reallocated its workspace since, and -11 when
+ * nat_mesh_set_precision has changed the row arithmetic (mode or thresholds)
+ * since the capture: the captured kernels keep the capture's arithmetic, so
+ * such a replay would not give the rows the mesh now promises.
  * nat_graph_copy_out copies the last replay's CSR to caller buffers.  The
  * graph uses the mesh's workspace: calls on the mesh are serialised with it. */
 typedef struct nat_graph nat_graph;
@@ -351,6 +354,12 @@ typedef struct {
 int nat_mesh_autotune(nat_mesh *m, const double *d_queries, int64_t num_query_points,
                       const int64_t *query_offsets, int repeats, void *stream,
                       nat_tune_report *report);
+
+/* Test hook of the overflow convention: when the environment variable
+ * NATINTERP3D_TEST_FORGE_NNZ holds a positive integer at a compute, the
+ * nnz > INT32_MAX checks of nat_get_weights, nat_weights_compute's *nnz_out
+ * and nat_weights_emit see the true total plus that many entries (so a test
+ * reaches the -1 branches without 2^31 entries); the rows are unchanged. */
 
 /* Release memory returned by nat_get_weights. */
 void nat_free_host(void *p);

@@ -6,11 +6,12 @@ Drop-in for natinterp3d's API (natinterp3d/__init__.py:7): ``Interpolator``,
 CSR matrices) with the weights computed by hand-written HIP kernels.
 """
 from .batch import BatchInterpolator
-from .natinterp3d import (GraphCapacityError, Interpolator, MeshAndVerticesHIP, WeightsGraph,
-                          clear_mesh_cache, get_weights, interpolate, mesh_cache_stats,
-                          spmm_device)
+from .natinterp3d import (GraphCapacityError, GraphPrecisionError, Interpolator,
+                          MeshAndVerticesHIP, WeightsGraph, clear_mesh_cache, get_weights,
+                          interpolate, mesh_cache_stats, spmm_device)
 
-__all__ = ["BatchInterpolator", "GraphCapacityError", "Interpolator", "MeshAndVerticesHIP",
+__all__ = ["BatchInterpolator", "GraphCapacityError", "GraphPrecisionError", "Interpolator",
+           "MeshAndVerticesHIP",
            "WeightsGraph", "clear_mesh_cache", "get_weights", "interpolate", "mesh_cache_stats",
            "spmm_device"]
 __version__ = "0.1.0"

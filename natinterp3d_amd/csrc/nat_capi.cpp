@@ -155,6 +155,10 @@ struct nat_mesh {
     int ext_M = -1;
     char *ext_written = nullptr;
     long long last_nnz = 0;
+    // test hook: entries added to the row total the overflow checks see
+    // (NATINTERP3D_TEST_FORGE_NNZ, read by each compute); the rows are untouched
+    long long nnz_forge = 0;
+    long long reported_nnz() const { return last_nnz + nnz_forge; }
     long long last_units = 0;  // record units of the last compute
 
     // row arithmetic: 0 = FP64, bit-identical to the reference; 1 = mixed
@@ -595,9 +599,22 @@ int ensure_old_vol(nat_mesh *m, hipStream_t st) {
 
 size_t topo_rec_off(long long M);
 
+// Entries the overflow checks add to a compute's true total: the value of
+// NATINTERP3D_TEST_FORGE_NNZ (a non-negative integer), else 0.  It lets a
+// test take the nnz > INT32_MAX branches of the boundary without 2^31
+// entries; the rows themselves are never changed.
+long long forged_entries() {
+    const char *s = std::getenv("NATINTERP3D_TEST_FORGE_NNZ");
+    if (!s || !*s) return 0;
+    char *end = nullptr;
+    const long long v = std::strtoll(s, &end, 10);
+    return (end && *end == '\0' && v > 0) ? v : 0;
+}
+
 int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipStream_t st,
                  long long *nnz_out) {
     m->last_M = -1;
+    m->nnz_forge = forged_entries();
     // the records go straight into a caller's saved-topology buffer when one
     // was set for this many queries and they fit; else into the workspace
     char *ext_base = m->ext_topo && m->ext_M == M ? m->ext_topo : nullptr;
@@ -615,7 +632,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
     if (M == 0) {
         m->last_M = 0;
         m->last_nnz = 0;
-        *nnz_out = 0;
+        *nnz_out = m->reported_nnz();
         return 0;
     }
     const size_t Mz = static_cast<size_t>(M);
@@ -801,7 +818,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
             m->ext_written = ext_base;
             m->last_nnz = static_cast<long long>(used);
             m->nnz_per_row = std::max(4.0, 1.125 * static_cast<double>(used) / M);
-            *nnz_out = m->last_nnz;
+            *nnz_out = m->reported_nnz();
             return 0;
         }
         want = static_cast<long long>(used) + 1024;
@@ -931,7 +948,7 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
 
 int emit_rows(nat_mesh *m, double *d_vals, int *d_cols, int *d_rowptr, hipStream_t st) {
     if (m->last_M < 0) return fail(-6, "nat_weights_emit without a preceding compute");
-    if (m->last_nnz > 2147483647LL) return -1;
+    if (m->reported_nnz() > 2147483647LL) return -1;
     const int M = m->last_M;
     if (M == 0) {
         HIP_OR_FAIL(hipMemsetAsync(d_rowptr, 0, sizeof(int), st));
@@ -1130,6 +1147,10 @@ struct nat_graph {
     long long cap = 0;                  // CSR capacity (entries)
     std::vector<std::pair<const void *, size_t>> fingerprint;
     long long last_nnz = -1;
+    // row arithmetic baked into the captured kernels (kernel choice and its
+    // thresholds): a replay after nat_mesh_set_precision changed them is refused
+    int precision = 0;
+    float mixed_tau = 0.f, mixed_kappa = 0.f;
 };
 
 namespace {
@@ -1566,8 +1587,8 @@ int nat_get_weights(const double *query_points, int num_query_points, nat_mesh *
     long long nnz = 0;
     int rc = compute_rows(m, m->q.as<double>(), M, nullptr, st, &nnz);
     if (rc) return rc;
-    if (nnz > 2147483647LL) return -1;
-    const size_t nz = static_cast<size_t>(nnz);
+    if (nnz > 2147483647LL) return -1;  // nothing allocated (natural_insertionfree.c:805-812)
+    const size_t nz = static_cast<size_t>(m->last_nnz);
     HIP_OR_FAIL(m->csr_vals.ensure(sizeof(double) * std::max<size_t>(nz, 1)));
     HIP_OR_FAIL(m->csr_cols.ensure(sizeof(int) * std::max<size_t>(nz, 1)));
     HIP_OR_FAIL(m->csr_rowptr.ensure(sizeof(int) * (Mz + 1)));
@@ -1822,6 +1843,9 @@ int nat_graph_create(nat_mesh *m, const double *d_queries, int num_query_points,
         return bail(fail(-10, "graph output allocation failed"));
     if (hipStreamSynchronize(st) != hipSuccess) return bail(fail(-10, "warm-up failed"));
     g->fingerprint = workspace_fingerprint(m);
+    g->precision = m->precision;
+    g->mixed_tau = m->mixed_tau;
+    g->mixed_kappa = m->mixed_kappa;
     if (hipStreamCreate(&g->st) != hipSuccess) return bail(fail(-10, "stream creation failed"));
     if (hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return bail(fail(-10, "hipStreamBeginCapture failed"));
@@ -1848,6 +1872,11 @@ int nat_graph_launch(nat_graph *g, void *stream, int64_t *nnz_out) {
     DeviceGuard dg(m->device);
     if (workspace_fingerprint(m) != g->fingerprint)
         return fail(-9, "the mesh workspace was reallocated since the graph was made");
+    if (m->precision != g->precision ||
+        (g->precision == 1 && (m->mixed_tau != g->mixed_tau || m->mixed_kappa != g->mixed_kappa)))
+        return fail(-11, "the mesh's precision changed since the graph was made (captured %s, "
+                         "now %s)", g->precision ? "mixed" : "fp64",
+                    m->precision ? "mixed" : "fp64");
     hipStream_t st = static_cast<hipStream_t>(stream);
     HIP_OR_FAIL(hipGraphLaunch(g->exec, st));
     unsigned h[20];

@@ -576,7 +576,8 @@ class WeightsGraph:
     it in place) and returns the CSR (crow, col, val) as new tensors, or
     raises ``GraphCapacityError`` when the new queries need more workspace
     than the graph was made with (then compute them with ``weights_device``
-    or make a new graph)."""
+    or make a new graph), and ``GraphPrecisionError`` when the mesh's
+    precision was changed after the capture."""
 
     def __init__(self, mesh, queries, query_offsets=None):
         import torch
@@ -607,6 +608,8 @@ class WeightsGraph:
         rc = L.nat_graph_launch(self._g, stream, ctypes.byref(nnz))
         if rc == -8:
             raise GraphCapacityError(_lib.last_error())
+        if rc == -11:
+            raise GraphPrecisionError(_lib.last_error())
         _lib.check(rc, "nat_graph_launch")
         n = int(nnz.value)
         val = torch.empty(n, dtype=torch.float64, device=dev)
@@ -621,6 +624,11 @@ class WeightsGraph:
 
 class GraphCapacityError(RuntimeError):
     """A graph replay needed more workspace than the graph was made with."""
+
+
+class GraphPrecisionError(RuntimeError):
+    """A graph replay after ``set_precision`` changed the mesh's row
+    arithmetic: the graph holds the kernels of its capture (make a new one)."""
 
 
 def get_row_policy(handle, num_problems):

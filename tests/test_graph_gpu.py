@@ -78,3 +78,29 @@ def test_graph_capacity_overflow_is_reported():
         g.run()
     crow2, col2, val2 = m.weights_device(far)  # the eager path still works
     assert col2.numel() > 2000 * 100
+
+
+def test_graph_refuses_replay_after_precision_change():
+    """A graph captured under one row arithmetic is refused once the mesh's
+    precision changes (its kernels keep the capture's arithmetic); switching
+    back makes it replay again, equal to eager."""
+    import torch
+    k, q, _ = workloads.uniform_small(n_keys=3000, n_queries=20000)
+    c = k.mean(axis=0)
+    m = MeshAndVerticesHIP(np.ascontiguousarray(k - c), device=0)
+    qd = torch.tensor(q - c, device="cuda")
+    m.set_precision("mixed")
+    ref_mixed = m.weights_device(qd)
+    g = natinterp3d_amd.WeightsGraph(m, qd.clone())
+    assert eq(g.run(), ref_mixed)
+    m.set_precision("fp64")
+    with pytest.raises(natinterp3d_amd.GraphPrecisionError):
+        g.run()
+    m.set_precision("mixed", cond_threshold=0.05)
+    with pytest.raises(natinterp3d_amd.GraphPrecisionError):
+        g.run()
+    m.set_precision("mixed")
+    ref_mixed = m.weights_device(qd)
+    torch.cuda.synchronize()
+    assert eq(g.run(), ref_mixed), "replay after switching back"
+    say("[graph precision] replay refused after fp64/threshold switch, equal after switch back")

@@ -82,6 +82,27 @@ def test_oracle_vs_compiled_reference(oracle_mod, ref_mod, seed, n, m):
     _csr_eq(W, Wr.indptr, Wr.indices, Wr.data)
 
 
+@pytest.mark.parametrize("label", ["plain_at_scale_100k", "lattice_jitter_50"])
+def test_oracle_vs_compiled_reference_on_benchmark_recipes(oracle_mod, ref_mod, label):
+    """The oracle's rows equal the compiled reference's on the two large
+    benchmark recipes (the blob mix at 100k keys; the jittered 50^3 lattice,
+    whose perturbed coordinates and near-co-spherical insphere tests are
+    where a restatement would first drift), 200k queries each."""
+    from natinterp3d_amd import workloads
+    if label.startswith("plain"):
+        keys, q, _ = workloads.plain_at_scale(n_keys=100_000, n_queries=200_000)
+    else:
+        keys, q, _ = workloads.lattice_jitter(n_side=50, n_queries=200_000)
+    c = keys.mean(0)
+    kc = np.ascontiguousarray(keys - c)
+    qc = np.ascontiguousarray(q - c)
+    r = ref_mod.RefMesh(kc)
+    om = oracle_mod.OracleMesh(r.export())  # export before the reference packs its mesh
+    Wr = r.get_weights(qc, num_threads=8)
+    W = om.get_weights(qc, num_threads=8)
+    _csr_eq(W, Wr.indptr, Wr.indices, Wr.data)
+
+
 # ---------------------------------------------------------------- mathematics
 
 def _det3(m):
