@@ -101,6 +101,28 @@ int nat_weights_compute(nat_mesh *m, const double *d_queries, int num_query_poin
 int nat_weights_emit(nat_mesh *m, double *d_vals, int32_t *d_cols, int32_t *d_rowptr,
                      void *stream);
 
+/* nat_weights_compute + nat_weights_emit in one call that allocates no
+ * memory.  No reference counterpart: it is for callers that cannot allocate
+ * -- code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Its workspace is a fixed 64 MiB device array that each GPU the
+ * library runs on holds until the process ends; every call on a GPU shares
+ * it (its contents are those of the last call), so calls that overlap on one
+ * GPU, from several threads or streams and on any mesh, race.  It takes at
+ * most 32768 queries per call (-1 beyond) and writes the CSR of d_queries
+ * [M,3] into caller-owned device buffers d_vals[capacity] (float64),
+ * d_cols[capacity] (int32) and d_rowptr[M+1] (int32); *nnz_out = the
+ * entries.  Returns 0; -7 when nnz > capacity (nothing written, *nnz_out
+ * says how many are needed); -8 when the rows need more topology records or
+ * pool entries than the array holds (fewer queries per call); -4 when a row
+ * needs a cavity beyond 4096 tets (use nat_weights_compute).  The rows are
+ * those of nat_weights_compute + nat_weights_emit, bit for bit, under the
+ * mesh's precision and policy; the mesh's own workspace is not touched (a
+ * later nat_weights_emit still refers to the last nat_weights_compute), and
+ * the step-3 table is used only if an earlier compute built it. */
+int nat_weights_noalloc(nat_mesh *m, const double *d_queries, int num_query_points,
+                        double *d_vals, int32_t *d_cols, int32_t *d_rowptr, int64_t capacity,
+                        void *stream, int64_t *nnz_out);
+
 /* W @ values on the device, accumulating each row in stored order from zero
  * like scipy's CSR x dense product (natinterp3d.py:131-137):
  * d_out[M,D] = W[M,N] (CSR, d_rowptr int32[M+1], d_cols int32, d_vals)

@@ -22,7 +22,7 @@ EXPORTS = (
     "nat_graph_create", "nat_graph_launch", "nat_graph_copy_out", "nat_graph_free",
     "nat_mesh_set_policy", "nat_mesh_get_policy", "nat_mesh_autotune",
     "nat_mesh_from_packed", "nat_weights_row_order", "nat_topology_target",
-    "nat_weights_spmm",
+    "nat_weights_spmm", "nat_weights_noalloc",
 )
 
 
@@ -95,6 +95,9 @@ def lib():
     L.nat_weights_compute.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_int64)]
     L.nat_weights_emit.restype = ctypes.c_int
     L.nat_weights_emit.argtypes = [vp, vp, vp, vp, vp]
+    L.nat_weights_noalloc.restype = ctypes.c_int
+    L.nat_weights_noalloc.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, ctypes.c_int64, vp,
+                                      ctypes.POINTER(ctypes.c_int64)]
     L.nat_spmm_csr.restype = ctypes.c_int
     L.nat_spmm_csr.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp,
                                ctypes.c_int, vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]

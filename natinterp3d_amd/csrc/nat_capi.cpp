@@ -2251,3 +2251,209 @@ void nat_free_host(void *p) { std::free(p); }
 const char *nat_last_error(void) { return g_err.c_str(); }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The entry point that allocates nothing (nat_weights_noalloc).
+//
+// Its workspace is `scratch`, a fixed device array that every GPU the library
+// runs on holds from module load to process end.  The kernels get its device
+// address, looked up once per device with hipGetSymbolAddress (the array's
+// name is only a host-side stand-in).  Every call on a GPU shares that GPU's
+// copy and leaves its contents behind; what must start at zero (counters, the
+// thread-per-query level's tables and stamps) is cleared by fill kernels at
+// each call, so calls may be captured in a graph.  Calls that overlap on one
+// GPU -- from several threads or on several streams, on any mesh -- race on it.
+
+namespace {
+
+constexpr size_t kNoAllocBytes = static_cast<size_t>(64) << 20;
+constexpr int kNoAllocMaxQueries = 32768;
+constexpr int kNoAllocLocateThreads = 4096;  // k-d fallback stacks of the start-tet search
+constexpr int kNoAllocWaves = 8;             // wave-per-query rows in flight
+constexpr int kNoAllocLevel1Threads = 64;    // thread-per-query rows (level 1) in flight
+
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+std::mutex g_noalloc_mu;
+char *g_noalloc_ptr[kXferMaxDevices] = {};
+
+// Device address of this device's `scratch` (the device is current).
+int noalloc_scratch(int device, char **out) {
+    if (device < 0 || device >= kXferMaxDevices) return fail(-1, "device %d out of range", device);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(-10, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                        hipGetErrorString(e));
+        g_noalloc_ptr[device] = static_cast<char *>(p);
+    }
+    *out = g_noalloc_ptr[device];
+    return 0;
+}
+
+// Bump allocation of the scratch for one call (256-byte aligned pieces).
+struct Carve {
+    char *base;
+    size_t used = 0, cap;
+    char *take(size_t bytes) {
+        char *r = base + used;
+        used += (bytes + 255) & ~static_cast<size_t>(255);
+        return used <= cap ? r : nullptr;
+    }
+};
+
+int noalloc_rows(nat_mesh *m, char *base, const double *d_q, int M, double *d_vals,
+                 int32_t *d_cols, int32_t *d_rowptr, int64_t capacity, hipStream_t st,
+                 int64_t *nnz_out) {
+    const size_t Mz = static_cast<size_t>(M);
+    int bits = 1;
+    while (bits < 7 && (1ull << (3 * (bits + 1))) <= Mz * 2) bits++;
+    const int nbuckets = 1 << (3 * bits);
+    const size_t scan_elems = std::max(nat::scan_tmp_elems(M) + Mz + 1,
+                                       static_cast<size_t>(nbuckets) + 1 +
+                                           nat::scan_tmp_elems(nbuckets));
+    Carve c{base, 0, kNoAllocBytes};
+    unsigned *ctr = reinterpret_cast<unsigned *>(c.take(256));
+    int *order = reinterpret_cast<int *>(c.take(4 * Mz));
+    int *geom_order = reinterpret_cast<int *>(c.take(4 * Mz));
+    int *keys = reinterpret_cast<int *>(c.take(4 * Mz));
+    int *hist = reinterpret_cast<int *>(c.take(4 * static_cast<size_t>(nbuckets)));
+    long long *scan = reinterpret_cast<long long *>(c.take(8 * scan_elems));
+    long long *row_off = reinterpret_cast<long long *>(c.take(8 * Mz));
+    int *row_cnt = reinterpret_cast<int *>(c.take(4 * Mz));
+    double *row_total = reinterpret_cast<double *>(c.take(8 * Mz));
+    long long *rowptr64 = reinterpret_cast<long long *>(c.take(8 * (Mz + 1)));
+    int *ovf0 = reinterpret_cast<int *>(c.take(4 * Mz));
+    int *ovf1 = reinterpret_cast<int *>(c.take(4 * Mz));
+    int *start = reinterpret_cast<int *>(c.take(4 * Mz));
+    long long *rec_off = reinterpret_cast<long long *>(c.take(8 * Mz));
+    // start-tet search: k-d fallback stacks for up to kNoAllocLocateThreads threads
+    const int nthr0 = std::min(kNoAllocLocateThreads, (M + 255) / 256 * 256);
+    char *sc0_base = c.take(nat::scratch_bytes_per_thread(m->caps[0]) * nthr0 + 16 * 256);
+    // wave-per-query rows and thread-per-query rows of level 1
+    char *wscr = c.take(nat::wave_scratch_size() * kNoAllocWaves);
+    const size_t l1_bytes =
+        nat::scratch_bytes_per_thread(m->caps[1]) * kNoAllocLevel1Threads + 16 * 256;
+    char *sc1_base = c.take(l1_bytes);
+    if (!sc1_base)
+        return fail(-1, "nat_weights_noalloc: %d queries leave no room in the %zu-byte scratch",
+                    M, kNoAllocBytes);
+    // the rest: topology records (60 %) and the row pool (40 %, 12 bytes per entry)
+    const size_t rest = kNoAllocBytes - c.used;
+    const long long rec_cap = static_cast<long long>(rest * 6 / 10 / sizeof(int4));
+    int4 *rec = reinterpret_cast<int4 *>(c.take(sizeof(int4) * static_cast<size_t>(rec_cap)));
+    const long long pool_cap =
+        static_cast<long long>((kNoAllocBytes - c.used - 2 * 256) / (sizeof(double) + sizeof(int)));
+    double *pool_vals =
+        reinterpret_cast<double *>(c.take(sizeof(double) * static_cast<size_t>(pool_cap)));
+    int *pool_cols = reinterpret_cast<int *>(c.take(sizeof(int) * static_cast<size_t>(pool_cap)));
+    if (!rec || !pool_vals || !pool_cols) return fail(-1, "nat_weights_noalloc: scratch carve");
+
+    // counters as compute_rows lays them out
+    unsigned long long *pool_used = reinterpret_cast<unsigned long long *>(ctr + 16);
+    unsigned long long *rec_used = reinterpret_cast<unsigned long long *>(ctr + 18);
+    HIP_OR_FAIL(nat::launch_zero_u32(ctr, 64, st));
+    nat::DevMesh d = m->dev;  // the step-3 table is used when an earlier compute built it
+    d.prob = nullptr;
+    HIP_OR_FAIL(nat::launch_morton_sort(d_q, M, m->lp.as<nat::LocParams>(), nullptr, 1, bits,
+                                        order, hist, keys, scan, st));
+    nat::RowOut out{};
+    out.pool_vals = pool_vals;
+    out.pool_cols = pool_cols;
+    out.pool_cap = pool_cap;
+    out.pool_used = pool_used;
+    out.row_off = row_off;
+    out.row_cnt = row_cnt;
+    out.row_total = row_total;
+    nat::Scratch sc0;
+    nat::scratch_carve(sc0_base, m->caps[0], nthr0, &sc0);
+    nat::Topo topo;
+    topo.rec = rec;
+    topo.rec_cap = rec_cap;
+    topo.rec_used = rec_used;
+    topo.rec_off = rec_off;
+    HIP_OR_FAIL(nat::launch_locate(d, m->caps[0], sc0, nthr0 / 256, d_q, order, M, start, st));
+    HIP_OR_FAIL(nat::launch_topo(d, m->topo_blocks * m->topo_grid, d_q, order, M, start,
+                                 ctr + 32, topo, ctr + 8, ovf0, st));
+    if (m->work_sort) HIP_OR_FAIL(nat::launch_work_sort(order, M, topo, geom_order, st));
+    HIP_OR_FAIL(nat::launch_geom(
+        d, m->nthr_geom / nat::kGeomThreads * m->geom_grid,
+        m->geom_lanes == 2 ? geom_rows_grid(m) : 0, d_q, m->work_sort ? geom_order : order, M,
+        ctr + 40, topo, out, m->mixed_tau, m->mixed_kappa,
+        m->precision == 1 ? reinterpret_cast<unsigned long long *>(ctr + 20) : nullptr, st));
+    // rows beyond the group kernel: a wave each, then one thread each (level 1);
+    // both read their item counts on the device
+    const unsigned *n_dev = ctr + 8;
+    int *items = ovf0, *next = ovf1;
+    if (m->wave_rows) {
+        HIP_OR_FAIL(nat::launch_wave_rows(d, wscr, kNoAllocWaves, d_q, ovf0, 0, n_dev, start,
+                                          ctr + 6, out, ovf1, ctr + 12, st));
+        n_dev = ctr + 12;
+        std::swap(items, next);
+    }
+    // the level's hash tables and per-thread stamps start from zero
+    HIP_OR_FAIL(nat::launch_zero_u32(sc1_base, l1_bytes / 4, st));
+    nat::Scratch sc1;
+    nat::scratch_carve(sc1_base, m->caps[1], kNoAllocLevel1Threads, &sc1);
+    out.ovf_list = next;
+    out.ovf_count = ctr + 9;
+    HIP_OR_FAIL(nat::launch_rows(d, m->caps[1], sc1, 1, 1, d_q, items, 0, n_dev, ctr + 1, out,
+                                 st));
+    // counters: [9] rows beyond level 1, u64 pool use at [16], records at [18]
+    unsigned h[20];
+    HIP_OR_FAIL(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    unsigned long long used = 0, rec_units = 0;
+    std::memcpy(&used, h + 16, sizeof(used));
+    std::memcpy(&rec_units, h + 18, sizeof(rec_units));
+    if (static_cast<long long>(rec_units) > rec_cap)
+        return fail(-8, "nat_weights_noalloc: topology records need %llu of %lld units of the "
+                        "scratch (fewer queries per call)", rec_units, rec_cap);
+    if (h[9] > 0)
+        return fail(-4, "nat_weights_noalloc: %u queries need more than %d cavity tets "
+                        "(use nat_weights_compute)", h[9], m->caps[1].cav);
+    if (static_cast<long long>(used) > pool_cap)
+        return fail(-8, "nat_weights_noalloc: rows need %llu of %lld pool entries of the "
+                        "scratch (fewer queries per call)", used, pool_cap);
+    *nnz_out = static_cast<int64_t>(used);
+    if (static_cast<long long>(used) > capacity)
+        return fail(-7, "nat_weights_noalloc: %llu entries, output capacity %lld", used,
+                    static_cast<long long>(capacity));
+    if (used > 0 && (!d_vals || !d_cols)) return fail(-1, "NULL output");
+    HIP_OR_FAIL(nat::launch_scan_counts(row_cnt, rowptr64, M, scan, st));
+    HIP_OR_FAIL(nat::launch_gather_csr(rowptr64, row_off, row_cnt, pool_vals, pool_cols,
+                                       m->vorig.as<int>(), M, static_cast<long long>(used),
+                                       d_vals, d_cols, d_rowptr, st));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int nat_weights_noalloc(nat_mesh *m, const double *d_queries, int num_query_points,
+                                   double *d_vals, int32_t *d_cols, int32_t *d_rowptr,
+                                   int64_t capacity, void *stream, int64_t *nnz_out) {
+    g_err.clear();
+    if (!m || !nnz_out || !d_rowptr) return fail(-1, "NULL argument");
+    *nnz_out = 0;
+    if (m->device < 0) return fail(-1, "host-only mesh has no device copy");
+    if (m->problems() != 1) return fail(-1, "nat_weights_noalloc needs a single-problem mesh");
+    if (num_query_points < 0 || capacity < 0) return fail(-1, "negative size");
+    if (num_query_points > kNoAllocMaxQueries)
+        return fail(-1, "nat_weights_noalloc takes at most %d queries per call (got %d)",
+                    kNoAllocMaxQueries, num_query_points);
+    if (num_query_points > 0 && !d_queries) return fail(-1, "NULL queries");
+    std::lock_guard<std::mutex> lk(m->mu);
+    DeviceGuard g(m->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    char *base = nullptr;
+    if (int rc = noalloc_scratch(m->device, &base)) return rc;
+    if (num_query_points == 0) {
+        // rowptr[0] = 0 by a fill kernel (no memset node if the caller captures)
+        HIP_OR_FAIL(nat::launch_zero_u32(d_rowptr, 1, st));
+        return 0;
+    }
+    return noalloc_rows(m, base, d_queries, num_query_points, d_vals, d_cols, d_rowptr, capacity,
+                        st, nnz_out);
+}
