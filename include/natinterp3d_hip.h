@@ -282,6 +282,8 @@ typedef struct {
     double grad_wave_ms;      /* ... and in nat_grad_queries (with its start tets) */
     int32_t wave_items;
     int32_t grad_wave_items;
+    double topo_search_ms;    /* level 0 topology: the cavity search (in topo_ms) */
+    double topo_order_ms;     /* ... and the order rules (in topo_ms) */
 } nat_timings;
 
 int nat_mesh_set_timing(nat_mesh *m, int enable);

@@ -37,6 +37,7 @@ class Timings(ctypes.Structure):
         ("mixed_fp64", ctypes.c_int64), ("mixed_rows_fp64", ctypes.c_int64),
         ("wave_ms", ctypes.c_double), ("grad_wave_ms", ctypes.c_double),
         ("wave_items", ctypes.c_int32), ("grad_wave_items", ctypes.c_int32),
+        ("topo_search_ms", ctypes.c_double), ("topo_order_ms", ctypes.c_double),
     ]
 
 

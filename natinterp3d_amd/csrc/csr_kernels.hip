@@ -21,9 +21,11 @@ __device__ __forceinline__ unsigned spread3(unsigned v) {
     return v;
 }
 
+// Morton key of each query: problem-major, then the 3*bits-bit Morton code of
+// its cell in the problem's data bounding box.
 __global__ void morton_key_kernel(const double *__restrict__ q, int M,
                                   const LocParams *__restrict__ lp, const int *__restrict__ prob,
-                                  int bits, int *__restrict__ keys, int *__restrict__ hist) {
+                                  int bits, int *__restrict__ keys) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     const int p = prob ? prob[i] : 0;
@@ -40,20 +42,72 @@ __global__ void morton_key_kernel(const double *__restrict__ q, int M,
     const unsigned ix = cell(q[3 * static_cast<size_t>(i)], 0);
     const unsigned iy = cell(q[3 * static_cast<size_t>(i) + 1], 1);
     const unsigned iz = cell(q[3 * static_cast<size_t>(i) + 2], 2);
-    const int key = (p << (3 * bits)) |
-                    static_cast<int>(spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2));
-    keys[i] = key;
-    atomicAdd(&hist[key], 1);
+    keys[i] = (p << (3 * bits)) |
+              static_cast<int>(spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2));
 }
 
-__global__ void morton_scatter_kernel(const int *__restrict__ keys, int M,
-                                      const long long *__restrict__ start, int *cursor,
-                                      int *__restrict__ order) {
+// The query order is the stable sort of the keys (equal keys in index order):
+// an LSD radix sort, 8 bits per pass.  Each pass counts the digits of every
+// tile of kSortTile consecutive items, scans the counts digit-major (digit d
+// of tile t lands after every smaller digit and after digit d of the earlier
+// tiles), and scatters each tile in index order.  Every count is written, no
+// buffer needs clearing, and the order is a function of the keys alone: the
+// same queries always form the same runs of the order.
+constexpr int kSortTile = 1024;  // one 64-lane wave, 16 rounds
+
+__global__ void __launch_bounds__(64) radix_hist_kernel(const int *__restrict__ keys, int M,
+                                                        int shift, int ntiles,
+                                                        int *__restrict__ hist) {
+    __shared__ int h[256];
+    const int lane = threadIdx.x;
+    const int tile = blockIdx.x;
+    for (int d = lane; d < 256; d += 64) h[d] = 0;
+    __syncthreads();
+    const int base = tile * kSortTile;
+    for (int r = 0; r < kSortTile / 64; r++) {
+        const int i = base + r * 64 + lane;
+        if (i < M) atomicAdd(&h[(keys[i] >> shift) & 255], 1);
+    }
+    __syncthreads();
+    for (int d = lane; d < 256; d += 64) hist[d * ntiles + tile] = h[d];
+}
+
+// vals_in NULL: the values are the item indices
+__global__ void __launch_bounds__(64) radix_scatter_kernel(
+    const int *__restrict__ keys_in, const int *__restrict__ vals_in, int M, int shift, int ntiles,
+    const long long *__restrict__ offs, int *__restrict__ keys_out, int *__restrict__ vals_out) {
+    __shared__ int base[256];
+    const int lane = threadIdx.x;
+    const int tile = blockIdx.x;
+    for (int d = lane; d < 256; d += 64) base[d] = static_cast<int>(offs[d * ntiles + tile]);
+    __syncthreads();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int r = 0; r < kSortTile / 64; r++) {
+        const int i = tile * kSortTile + r * 64 + lane;
+        const bool valid = i < M;
+        const int k = valid ? keys_in[i] : 0;
+        const int v = valid ? (vals_in ? vals_in[i] : i) : 0;
+        const unsigned d = (static_cast<unsigned>(k) >> shift) & 255u;
+        // the lanes holding the same digit, by its 8 bits
+        unsigned long long same = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const unsigned long long bal = __ballot((d >> b) & 1u);
+            same &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        const int rank = __popcll(same & lt);
+        const int pos = base[d] + rank;  // every lane reads before the leader writes
+        if (valid && rank == 0) base[d] += __popcll(same);
+        if (valid) {
+            keys_out[pos] = k;
+            vals_out[pos] = v;
+        }
+    }
+}
+
+__global__ void iota_kernel(int *__restrict__ out, int M) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    const int key = keys[i];
-    const int k = atomicAdd(&cursor[key], 1);
-    order[start[key] + k] = i;
+    if (i < M) out[i] = i;
 }
 
 __global__ void zero_u32_kernel(unsigned *__restrict__ p, size_t n) {
@@ -68,25 +122,68 @@ hipError_t launch_zero_u32(void *d_p, size_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
+namespace {
+size_t rnd256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+}  // namespace
+
+size_t morton_sort_bytes(int M) {
+    const size_t Mz = static_cast<size_t>(M > 0 ? M : 1);
+    const size_t nt = (Mz + kSortTile - 1) / kSortTile;
+    const int nh = static_cast<int>(256 * nt);
+    return 3 * rnd256(4 * Mz) + rnd256(4 * static_cast<size_t>(nh)) +
+           rnd256(8 * (static_cast<size_t>(nh) + 1)) + rnd256(8 * scan_tmp_elems(nh));
+}
+
 hipError_t launch_morton_sort(const double *d_q, int M, const LocParams *d_lp,
-                              const int *d_prob, int P, int bits, int *d_order, int *d_hist,
-                              int *d_keys, long long *d_scan_tmp, hipStream_t st) {
-    const int nb = P << (3 * bits);
-    hipError_t e = launch_zero_u32(d_hist, nb, st);
-    if (e != hipSuccess) return e;
+                              const int *d_prob, int P, int bits, int *d_order, void *d_work,
+                              hipStream_t st) {
+    if (M <= 0) return hipSuccess;
+    const size_t Mz = static_cast<size_t>(M);
+    const int ntiles = static_cast<int>((Mz + kSortTile - 1) / kSortTile);
+    const int nh = 256 * ntiles;
+    char *w = static_cast<char *>(d_work);
+    int *keys[2], *vals_tmp;
+    keys[0] = reinterpret_cast<int *>(w);
+    w += rnd256(4 * Mz);
+    keys[1] = reinterpret_cast<int *>(w);
+    w += rnd256(4 * Mz);
+    vals_tmp = reinterpret_cast<int *>(w);
+    w += rnd256(4 * Mz);
+    int *hist = reinterpret_cast<int *>(w);
+    w += rnd256(4 * static_cast<size_t>(nh));
+    long long *offs = reinterpret_cast<long long *>(w);
+    w += rnd256(8 * (static_cast<size_t>(nh) + 1));
+    long long *tmp = reinterpret_cast<long long *>(w);
     hipLaunchKernelGGL(morton_key_kernel, dim3((M + 255) / 256), dim3(256), 0, st, d_q, M, d_lp,
-                       d_prob, bits, d_keys, d_hist);
-    e = hipGetLastError();
+                       d_prob, bits, keys[0]);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    long long *starts = d_scan_tmp;  // [nb + 1]
-    long long *tmp = d_scan_tmp + nb + 1;
-    e = launch_scan_counts(d_hist, starts, nb, tmp, st);
-    if (e != hipSuccess) return e;
-    e = launch_zero_u32(d_hist, nb, st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(morton_scatter_kernel, dim3((M + 255) / 256), dim3(256), 0, st, d_keys,
-                       M, starts, d_hist, d_order);
-    return hipGetLastError();
+    // key bits: problem index above the 3*bits Morton bits
+    const long long nkeys = static_cast<long long>(P) << (3 * bits);
+    int keybits = 0;
+    while ((1ll << keybits) < nkeys) keybits++;
+    const int passes = (keybits + 7) / 8;
+    if (passes == 0) {
+        hipLaunchKernelGGL(iota_kernel, dim3((M + 255) / 256), dim3(256), 0, st, d_order, M);
+        return hipGetLastError();
+    }
+    // values ping-pong so that the last pass writes d_order
+    for (int k = 0; k < passes; k++) {
+        const int shift = 8 * k;
+        int *vin = k == 0 ? nullptr : ((passes - k) % 2 == 0 ? d_order : vals_tmp);
+        int *vout = (passes - 1 - k) % 2 == 0 ? d_order : vals_tmp;
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(ntiles), dim3(64), 0, st, keys[k & 1], M,
+                           shift, ntiles, hist);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        e = launch_scan_counts(hist, offs, nh, tmp, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(ntiles), dim3(64), 0, st, keys[k & 1], vin,
+                           M, shift, ntiles, offs, keys[(k + 1) & 1], vout);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 __global__ void problem_ids_kernel(const long long *__restrict__ off, int P, int M,

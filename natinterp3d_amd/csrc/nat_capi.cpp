@@ -101,7 +101,7 @@ struct nat_mesh {
     int kd_depth_max = 0;
     std::vector<nat::LocParams> lph;      // [P]
     nat::DevMesh dev{};
-    DevBuf verts, tet_v, tet_n, kd, cell_start, cell_pts, lp;
+    DevBuf verts, tet_v, tet_n, fat, kd, cell_start, cell_pts, lp;
     DevBuf vorig;                         // [N+4P] device vertex -> key index
     DevBuf vals_dev;                      // backward: values in device vertex order
     DevBuf vals_fwd;                      // nat_weights_spmm: values in device vertex order
@@ -136,7 +136,7 @@ struct nat_mesh {
     long long rec_cap = 0;
     double units_per_row = 40.0;
     DevBuf q;                    // host-path query upload
-    DevBuf order, keys, hist;    // Morton bucketing
+    DevBuf order, sortw;         // query order and its sort's workspace
     DevBuf geom_order;           // work order of the forward geometry kernel
     DevBuf scan;                 // int64 scan scratch
     DevBuf row_off, row_cnt, rowptr64, row_total;
@@ -172,13 +172,13 @@ struct nat_mesh {
     nat_timings times{};
 
     int64_t device_bytes() const {
-        return static_cast<int64_t>(verts.bytes + tet_v.bytes + tet_n.bytes + kd.bytes +
+        return static_cast<int64_t>(verts.bytes + tet_v.bytes + tet_n.bytes + fat.bytes + kd.bytes +
                                     cell_start.bytes + cell_pts.bytes + lp.bytes +
                                     old_vol.bytes + vorig.bytes);
     }
     int problems() const { return static_cast<int>(parts.size()); }
     int64_t workspace_bytes() const {
-        size_t s = q.bytes + order.bytes + geom_order.bytes + keys.bytes + hist.bytes + scan.bytes +
+        size_t s = q.bytes + order.bytes + geom_order.bytes + sortw.bytes + scan.bytes +
                    prob.bytes + qoff.bytes +
                    row_off.bytes + row_cnt.bytes + rowptr64.bytes + row_total.bytes +
                    ovf[0].bytes +
@@ -190,8 +190,8 @@ struct nat_mesh {
         return static_cast<int64_t>(s);
     }
     void release_all() {
-        DevBuf *all[] = {&verts,    &tet_v,     &tet_n,     &kd,       &cell_start, &cell_pts,
-                         &q,        &order,     &geom_order, &keys,    &hist,       &scan,
+        DevBuf *all[] = {&verts,    &tet_v,     &tet_n,     &fat,      &kd,       &cell_start, &cell_pts,
+                         &q,        &order,     &geom_order, &sortw,   &scan,
                          &row_off,  &row_cnt,   &rowptr64,  &row_total, &ovf[0],    &ovf[1],
                          &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,   &csr_rowptr,
                          &rec,      &rec_off,   &start,     &lp,       &prob,       &qoff,
@@ -503,6 +503,7 @@ int upload_mesh(nat_mesh *m) {
     HIP_OR_FAIL(m->tet_v.ensure(sizeof(int4) * tv.size()));
     HIP_OR_FAIL(m->tet_n.ensure(sizeof(int4) * tn.size()));
     HIP_OR_FAIL(m->kd.ensure(sizeof(double4) * kd.size()));
+    HIP_OR_FAIL(m->fat.ensure(sizeof(nat::FatTet) * tv.size()));
     HIP_OR_FAIL(m->cell_start.ensure(sizeof(int) * start_all.size()));
     HIP_OR_FAIL(m->cell_pts.ensure(sizeof(double4) * pts_all.size()));
     HIP_OR_FAIL(m->lp.ensure(sizeof(nat::LocParams) * P));
@@ -512,6 +513,28 @@ int upload_mesh(nat_mesh *m) {
     HIP_OR_FAIL(hipMemcpy(m->tet_v.p, tv.data(), sizeof(int4) * tv.size(), hipMemcpyHostToDevice));
     HIP_OR_FAIL(hipMemcpy(m->tet_n.p, tn.data(), sizeof(int4) * tn.size(), hipMemcpyHostToDevice));
     HIP_OR_FAIL(hipMemcpy(m->kd.p, kd.data(), sizeof(double4) * kd.size(), hipMemcpyHostToDevice));
+    {
+        // the tets with their vertex coordinates inline, in device numbering
+        std::vector<nat::FatTet> ft(tv.size());
+        for (size_t t = 0; t < tv.size(); t++) {
+            const int4 v = tv[t];
+            const int s[4] = {v.x, v.y, v.z, v.w & 0x7fffffff};
+            double c[12];
+            for (int k = 0; k < 4; k++) {
+                const double4 p = T ? verts[s[k]] : make_double4(0, 0, 0, 0);
+                c[3 * k] = p.x;
+                c[3 * k + 1] = p.y;
+                c[3 * k + 2] = p.z;
+            }
+            ft[t].c0 = make_double4(c[0], c[1], c[2], c[3]);
+            ft[t].c1 = make_double4(c[4], c[5], c[6], c[7]);
+            ft[t].c2 = make_double4(c[8], c[9], c[10], c[11]);
+            ft[t].v = v;
+            ft[t].n = tn[t];
+        }
+        HIP_OR_FAIL(hipMemcpy(m->fat.p, ft.data(), sizeof(nat::FatTet) * ft.size(),
+                              hipMemcpyHostToDevice));
+    }
     HIP_OR_FAIL(hipMemcpy(m->cell_start.p, start_all.data(), sizeof(int) * start_all.size(),
                           hipMemcpyHostToDevice));
     HIP_OR_FAIL(hipMemcpy(m->cell_pts.p, pts_all.data(), sizeof(double4) * pts_all.size(),
@@ -522,6 +545,7 @@ int upload_mesh(nat_mesh *m) {
     d.verts = m->verts.as<double4>();
     d.tet_v = m->tet_v.as<int4>();
     d.tet_n = m->tet_n.as<int4>();
+    d.fat = m->fat.as<nat::FatTet>();
     d.kd = m->kd.as<double4>();
     d.n_data = static_cast<int>(NT);
     d.n_tets = static_cast<int>(TT);
@@ -644,14 +668,10 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         bits++;
     if ((static_cast<unsigned long long>(P) << (3 * bits)) > (1ull << 26))
         return fail(-1, "too many problems in one batch (%d)", P);
-    const int nbuckets = P << (3 * bits);
-    const size_t scan_elems =
-        std::max(nat::scan_tmp_elems(M) + Mz + 1,
-                 static_cast<size_t>(nbuckets) + 1 + nat::scan_tmp_elems(nbuckets));
+    const size_t scan_elems = nat::scan_tmp_elems(M) + Mz + 1;
     HIP_OR_FAIL(m->order.ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->geom_order.ensure(sizeof(int) * Mz));
-    HIP_OR_FAIL(m->keys.ensure(sizeof(int) * Mz));
-    HIP_OR_FAIL(m->hist.ensure(sizeof(int) * nbuckets));
+    HIP_OR_FAIL(m->sortw.ensure(nat::morton_sort_bytes(M)));
     HIP_OR_FAIL(m->scan.ensure(sizeof(long long) * scan_elems));
     HIP_OR_FAIL(m->row_off.ensure(sizeof(long long) * Mz));
     HIP_OR_FAIL(m->row_cnt.ensure(sizeof(int) * Mz));
@@ -666,8 +686,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
     m->times = nat_timings{};
     mark(m, 0, st);
     HIP_OR_FAIL(nat::launch_morton_sort(d_q, M, m->lp.as<nat::LocParams>(), d_prob, P, bits,
-                                        m->order.as<int>(), m->hist.as<int>(),
-                                        m->keys.as<int>(), m->scan.as<long long>(), st));
+                                        m->order.as<int>(), m->sortw.p, st));
     mark(m, 1, st);
     m->times.order_ms = elapsed(m, 0, 1);
 
@@ -719,9 +738,12 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         mark(m, 3, st);
         m->times.locate_ms = elapsed(m, 2, 3);
         mark(m, 2, st);
-        HIP_OR_FAIL(nat::launch_topo(m->dev, m->topo_blocks * m->topo_grid, d_q,
-                                     m->order.as<int>(), M, m->start.as<int>(), ctr + 32, topo,
-                                     ctr + 8, m->ovf[0].as<int>(), st));
+        HIP_OR_FAIL(nat::launch_topo_search(m->dev, m->topo_blocks * m->topo_grid, d_q,
+                                            m->order.as<int>(), M, m->start.as<int>(), ctr + 32,
+                                            topo, ctr + 8, m->ovf[0].as<int>(), st));
+        mark(m, 1, st);
+        HIP_OR_FAIL(nat::launch_topo_order(m->dev, m->order.as<int>(), M, topo, ctr + 8,
+                                           m->ovf[0].as<int>(), st));
         mark(m, 3, st);
         unsigned long long meta[2] = {0, 0};
         HIP_OR_FAIL(hipMemcpyAsync(meta, ctr + 16, sizeof(meta), hipMemcpyDeviceToHost, st));
@@ -729,6 +751,8 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         HIP_OR_FAIL(hipMemcpyAsync(&novf, ctr + 8, sizeof(unsigned), hipMemcpyDeviceToHost, st));
         HIP_OR_FAIL(hipStreamSynchronize(st));
         m->times.topo_ms = elapsed(m, 2, 3);
+        m->times.topo_search_ms = elapsed(m, 2, 1);
+        m->times.topo_order_ms = elapsed(m, 1, 3);
         const long long units = static_cast<long long>(meta[1]);
         if (units > topo.rec_cap) {
             want_rec = units + units / 8 + 1024;
@@ -977,7 +1001,7 @@ int emit_rows(nat_mesh *m, double *d_vals, int *d_cols, int *d_rowptr, hipStream
 // Device buffers of the mesh workspace a graph's nodes point to.
 std::vector<std::pair<const void *, size_t>> workspace_fingerprint(const nat_mesh *m) {
     std::vector<std::pair<const void *, size_t>> f;
-    const DevBuf *bufs[] = {&m->order, &m->geom_order, &m->keys, &m->hist, &m->scan,
+    const DevBuf *bufs[] = {&m->order, &m->geom_order, &m->sortw, &m->scan,
                             &m->row_off, &m->row_cnt, &m->row_total, &m->rowptr64, &m->ovf[0],
                             &m->ovf[1], &m->counters, &m->pool_vals, &m->pool_cols, &m->rec,
                             &m->rec_off, &m->start};
@@ -1170,8 +1194,7 @@ int issue_graph_rows(nat_mesh *m, nat_graph *g, const double *d_q, hipStream_t s
         HIP_OR_FAIL(nat::launch_problem_ids(g->qoff.as<long long>(), P, M, g->prob.as<int>(), st));
     m->dev.prob = d_prob;
     HIP_OR_FAIL(nat::launch_morton_sort(d_q, M, m->lp.as<nat::LocParams>(), d_prob, P, bits,
-                                        m->order.as<int>(), m->hist.as<int>(),
-                                        m->keys.as<int>(), m->scan.as<long long>(), st));
+                                        m->order.as<int>(), m->sortw.p, st));
     nat::RowOut out;
     out.pool_vals = m->pool_vals.as<double>();
     out.pool_cols = m->pool_cols.as<int>();
@@ -2310,16 +2333,12 @@ int noalloc_rows(nat_mesh *m, char *base, const double *d_q, int M, double *d_va
     const size_t Mz = static_cast<size_t>(M);
     int bits = 1;
     while (bits < 7 && (1ull << (3 * (bits + 1))) <= Mz * 2) bits++;
-    const int nbuckets = 1 << (3 * bits);
-    const size_t scan_elems = std::max(nat::scan_tmp_elems(M) + Mz + 1,
-                                       static_cast<size_t>(nbuckets) + 1 +
-                                           nat::scan_tmp_elems(nbuckets));
+    const size_t scan_elems = nat::scan_tmp_elems(M) + Mz + 1;
     Carve c{base, 0, kNoAllocBytes};
     unsigned *ctr = reinterpret_cast<unsigned *>(c.take(256));
     int *order = reinterpret_cast<int *>(c.take(4 * Mz));
     int *geom_order = reinterpret_cast<int *>(c.take(4 * Mz));
-    int *keys = reinterpret_cast<int *>(c.take(4 * Mz));
-    int *hist = reinterpret_cast<int *>(c.take(4 * static_cast<size_t>(nbuckets)));
+    char *sortw = c.take(nat::morton_sort_bytes(M));
     long long *scan = reinterpret_cast<long long *>(c.take(8 * scan_elems));
     long long *row_off = reinterpret_cast<long long *>(c.take(8 * Mz));
     int *row_cnt = reinterpret_cast<int *>(c.take(4 * Mz));
@@ -2358,7 +2377,7 @@ int noalloc_rows(nat_mesh *m, char *base, const double *d_q, int M, double *d_va
     nat::DevMesh d = m->dev;  // the step-3 table is used when an earlier compute built it
     d.prob = nullptr;
     HIP_OR_FAIL(nat::launch_morton_sort(d_q, M, m->lp.as<nat::LocParams>(), nullptr, 1, bits,
-                                        order, hist, keys, scan, st));
+                                        order, sortw, st));
     nat::RowOut out{};
     out.pool_vals = pool_vals;
     out.pool_cols = pool_cols;

@@ -33,11 +33,23 @@ struct LocParams {
 // its vertex offset), then the four super vertices of each problem; tets and
 // k-d nodes at the problem's tet offset.  The super-vertex test stays
 // "slot >= n_data".
+// One tet in one 128-byte line: its vertices' coordinates (p0..p3, x y z
+// each, packed into three double4), its vertex slots (tet_v, orientation bit
+// included) and its neighbours (tet_n).  The cavity search reads a tested
+// tet with one request instead of a tet load followed by four vertex loads.
+struct alignas(128) FatTet {
+    double4 c0, c1, c2;  // x0 y0 z0 x1 | y1 z1 x2 y2 | z2 x3 y3 z3
+    int4 v;
+    int4 n;
+};
+static_assert(sizeof(FatTet) == 128, "one line per tet");
+
 struct DevMesh {
     const double4 *verts;  // [N+4P] x,y,z,0; slots N.. are the super vertices
     const int4 *tet_v;     // [T] vertex slots p0..p3; bit 31 of .w set when
                            //     orient3d(p0,p1,p2,p3) <= 0
     const int4 *tet_n;     // [T] neighbour across face 0..3, -1 = none
+    const FatTet *fat;     // [T] the same tets with their vertex coordinates inline
     const double4 *kd;     // [T] k-d node i = tet i: centroid x,y,z and
                            //     .w = bit pattern of int2(left, right)
     int n_data;            // N (all problems)
@@ -143,15 +155,18 @@ struct GradIn {
 size_t scratch_bytes_per_thread(const Caps &c);
 void scratch_carve(void *base, const Caps &c, int nthr, Scratch *s);
 
-// buckets: problem-major, then 3*bits-bit Morton cells of the problem's box
+// the query order: a stable sort of the keys problem-major, then 3*bits-bit
+// Morton cells of the problem's box (equal keys in index order), in
+// d_work of morton_sort_bytes(M) bytes
+size_t morton_sort_bytes(int M);
 hipError_t launch_morton_sort(const double *d_q, int M, const LocParams *d_lp,
-                              const int *d_prob, int P, int bits, int *d_order, int *d_hist,
-                              int *d_keys, long long *d_scan_tmp, hipStream_t st);
+                              const int *d_prob, int P, int bits, int *d_order, void *d_work,
+                              hipStream_t st);
 // prob[i] = p with offsets[p] <= i < offsets[p+1]
 hipError_t launch_problem_ids(const long long *d_offsets, int P, int M, int *d_prob,
                               hipStream_t st);
 int rows_blocks_per_cu();  // resident 256-thread blocks of the fused row kernel per CU
-int topo_blocks_per_cu();   // 128-thread blocks of the topology kernel
+int topo_blocks_per_cu();   // 64-thread blocks of the topology search kernel
 int geom_blocks_per_cu();       // kGeomThreads-thread blocks
 int geom_rows_blocks_per_cu(bool mixed);  // blocks of the two-lanes-per-row geometry kernel
 int grad_geom_blocks_per_cu();  // resident blocks of the gradient geometry kernel
@@ -168,7 +183,14 @@ int locate_blocks_per_cu();  // 256-thread blocks
 hipError_t launch_locate(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
                          const double *d_q, const int *d_items, int n_items, int *d_start,
                          hipStream_t st);
-// topology: 128-thread blocks of 8 queries (16 lanes each)
+// topology: the quad search (64-thread blocks of 4 quads of 4 queries, 16 lanes
+// per quad), then one thread per query for the order rules
+hipError_t launch_topo_search(const DevMesh &m, int nblocks, const double *d_q,
+                              const int *d_items, int n_items, const int *d_start,
+                              unsigned int *d_work, const Topo &topo, unsigned int *ovf_count,
+                              int *ovf_list, hipStream_t st);
+hipError_t launch_topo_order(const DevMesh &m, const int *d_items, int n_items, const Topo &topo,
+                             unsigned int *ovf_count, int *ovf_list, hipStream_t st);
 hipError_t launch_topo(const DevMesh &m, int nblocks, const double *d_q, const int *d_items,
                        int n_items, const int *d_start, unsigned int *d_work, const Topo &topo,
                        unsigned int *ovf_count, int *ovf_list, hipStream_t st);

@@ -1085,37 +1085,109 @@ locate_kernel(DevMesh m, Caps caps, Scratch sc,
     }
 }
 
-// Wave-cooperative topology: 16 lanes per query, 4 queries per wave, all
-// per-query state in LDS.
+// Topology of a query, in two kernels.
+//
+// topo_search_kernel: 16 lanes for a quad of four consecutive queries of the
+// Morton order (4 quads per 64-thread block, tables in LDS).  The queries of
+// a quad are close together and their cavities overlap, so one search serves
+// all four: it tests every tet any of them pops in the reference's search --
+// each start tet and every neighbour of a tet whose circumsphere holds one of
+// the queries -- against each query, front by front, the tests of one front
+// in parallel; a tet is expanded when it is inside any query's sphere.  A
+// query's decisions are the same FP64 insphere tests the reference makes, and
+// every tet its own search pops is among those tested.  Per query, the tets
+// inside its sphere get dense indices (its start tet 0) and its record is
+// reserved; entry d holds, for now, the vertex slots of the tet, and a
+// handoff pair (tet, in-sphere adjacency: the dense index of the in-sphere
+// neighbour across each face, 0xff across a face to a tet outside the sphere
+// or to none, i.e. a boundary face) goes to the end of the record.
+//
+// topo_order_kernel (one thread per query, tables in LDS) turns the
+// adjacency into what the reference's order rules fix: the cavity order (the
+// LIFO search, natural_insertionfree.c:498-530, is a depth-first preorder of
+// the in-sphere tets that enters neighbour slots 3, 2, 1, 0 in that order:
+// a tet outside the sphere is never appended or expanded, so it does not
+// change the order of the others), the boundary faces in (cavity position,
+// face) order, the face-circumcentre masks, and the neighbours in order of
+// first appearance in the face-vertex sequence
+// (natural_insertionfree.c:373-382, 559-584); then it completes the record.
+// A query whose in-sphere tets are not all reachable from its start tet
+// through in-sphere tets (the quad's search can find such tets; a search of
+// its own would not) goes to the larger levels, which search it alone.
+//
+// Record of one query, in 16-byte units at topo.rec[rec_off[qi]]:
+//   [0]            (C, B, NB, 0): cavity tets, boundary faces, neighbours
+//   [1 .. C]       per cavity tet in dense order: (tet, slot(p0) | slot(p1)
+//                  << 16, slot(p2) | slot(p3) << 16, fcmask) -- slot 0xffff
+//                  = none; fcmask bit k: the face opposite p[k] is held by a
+//                  cavity tet earlier in cavity order (its circumcentre uses
+//                  that tet's vertex order)
+//   next ceil(C/8)   the cavity order, 16 bits per position: its dense index
+//                    | the fcmask of that tet << 6
+//   next ceil(B/16)  boundary faces in sequence order, a byte each: dense
+//                    index << 2 | face
+//   next ceil(NB/4)  neighbour vertex indices in slot order (room is reserved
+//                    for min(32, B/2 + 4): a cavity bounded by one closed
+//                    surface has B/2 + 2 boundary vertices)
+//   next 2 ceil(C/4) handoff: the in-sphere adjacency, then the tet, per dense
+//                    index (int each)
+// C = 0 rows: NB = vertices coinciding with the query (weight 1/NB each, in
+// unit [1]), or NB = 0 for an empty row.  Between the kernels the header is
+// (C, B, neighbour room, 1) and entry d holds the tet's vertex slots.
 namespace grp {
-constexpr int G = 16;         // lanes per query
-constexpr int GPB = 8;        // queries per 128-thread block
-constexpr int VT = 128;       // visited-table slots
-constexpr int VMAX = 96;      // visited tets
-constexpr int CMAX = 64;      // cavity tets
-constexpr int BMAX = 136;     // boundary faces (a cavity of C tets has about 2C + 2)
-constexpr int NT_LOG2 = 6;
-constexpr int NT = 1 << NT_LOG2;  // neighbour-table slots
+constexpr int G = 16;             // lanes per quad
+constexpr int Q = 4;              // queries per quad
+constexpr int QPB = 4;            // quads per 64-thread block
+constexpr int VT = 255;           // visited-table slots (0xff = none)
+constexpr int VMAX = 192;         // visited tets of a quad
+constexpr int VQ = 96;            // ... counted per query of the quad (budget)
+constexpr int CMAX = 64;          // cavity tets of a query
+constexpr int BMAX = 136;         // boundary faces (a cavity of C tets has about 2C + 2)
 constexpr int NMAX = kLevel0Row;  // neighbours
-constexpr int SMAX = 2 * VMAX;  // order-replay stack (shares the fronts' storage)
+constexpr int NT_LOG2 = 6;
+constexpr int NT = 1 << NT_LOG2;  // vertex-table slots of the order kernel
 }  // namespace grp
 
-struct alignas(16) GroupMem {
-    int vkey[grp::VT];             // tet index, -1 = empty slot
-    unsigned vnb[grp::VT];         // visited slots of the 4 neighbours, 8 bits each (0xff none)
-    unsigned char vflag[grp::VT];  // 1 tested, 2 inside the circumsphere, 4 popped by the replay
-    unsigned char vpos[grp::VT];   // cavity position
-    union {  // the fronts live during the search, the stack during the replay
-        unsigned char front[2][grp::VMAX];
-        unsigned char stk[grp::SMAX];
-    };
-    unsigned char cav[grp::CMAX];  // visited slots in cavity order
-    unsigned short bnd[grp::BMAX]; // boundary faces, cavity position << 2 | face
-    int nkey[grp::NT];             // vertex index, -1 = empty slot
-    int nmin[grp::NT];             // first appearance (face-vertex sequence number)
-    unsigned char nslot[grp::NT];  // neighbour slot
-    int nbv[grp::NMAX];            // neighbours in slot order
-    int cnt[4];                    // frontier sizes, visited count, overflow flag
+__device__ __forceinline__ int nb_room(int B) { return min(grp::NMAX, B / 2 + 4); }
+// offsets of the sections of a record with C cavity tets, B boundary faces and
+// room for `room` neighbours, in 16-byte units; every kernel that reads or
+// writes a section takes its offset from here
+__device__ __forceinline__ int rec_ord_off(int C) { return 1 + C; }
+__device__ __forceinline__ int rec_bnd_off(int C) { return rec_ord_off(C) + (C + 7) / 8; }
+__device__ __forceinline__ int rec_nbv_off(int C, int B) { return rec_bnd_off(C) + (B + 15) / 16; }
+__device__ __forceinline__ int rec_x_off(int C, int B, int room) {
+    return rec_nbv_off(C, B) + (room + 3) / 4;
+}
+__device__ __forceinline__ int rec_units(int C, int B, int room) {
+    return rec_x_off(C, B, room) + 2 * ((C + 3) / 4);
+}
+// sections of a complete record whose header is hd
+__device__ __forceinline__ const unsigned short *rec_order(const int4 *r, const int4 &hd) {
+    return reinterpret_cast<const unsigned short *>(r + rec_ord_off(hd.x));
+}
+__device__ __forceinline__ const unsigned char *rec_bnd(const int4 *r, const int4 &hd) {
+    return reinterpret_cast<const unsigned char *>(r + rec_bnd_off(hd.x));
+}
+__device__ __forceinline__ const int *rec_nbv(const int4 *r, const int4 &hd) {
+    return reinterpret_cast<const int *>(r + rec_nbv_off(hd.x, hd.y));
+}
+// the handoff arrays of a record in flight (header (C, B, room, 1))
+__device__ __forceinline__ int *rec_xadj(int4 *r, int C, int B, int room) {
+    return reinterpret_cast<int *>(r + rec_x_off(C, B, room));
+}
+__device__ __forceinline__ int *rec_xtet(int4 *r, int C, int B, int room) {
+    return reinterpret_cast<int *>(r + rec_x_off(C, B, room) + (C + 3) / 4);
+}
+
+struct alignas(16) QuadMem {
+    int vkey[256];                 // tet index, -1 = empty slot (slot 255 unused)
+    unsigned vnb[256];             // visited slots of the 4 neighbours, 8 bits each (0xff none)
+    unsigned char vflag[256];      // bit 0 tested; bit 1 + j inside query j's circumsphere
+    unsigned char dense[grp::Q][256];  // per query: dense index of an in-sphere slot
+    unsigned char vlist[grp::VMAX];    // visited slots in test order; fronts are ranges
+    int vtet[grp::VMAX];           // their tets
+    int sslot[grp::Q];             // slot of each query's start tet
+    int cnt[4];                    // [0] first front, [3] overflow flag
 };
 
 __device__ __forceinline__ void group_sync() {
@@ -1124,169 +1196,153 @@ __device__ __forceinline__ void group_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+__device__ __forceinline__ unsigned vt_hash(int t) {
+    return (hash_key(t, 8) * 255u) >> 8;  // 0..254
+}
 // visited-table insert of tet t, probing from slot h: its slot (isnew when
 // this call put it there), -1 when the table is full
-__device__ __forceinline__ int vis_probe(GroupMem &g, int t, unsigned h, bool &isnew) {
+__device__ __forceinline__ int vis_probe(QuadMem &g, int t, unsigned h, bool &isnew) {
     for (int n = 0; n < grp::VT; n++) {
         const int old = atomicCAS(&g.vkey[h], -1, t);
         if (old == -1 || old == t) {
             isnew = old == -1;
             return static_cast<int>(h);
         }
-        h = (h + 1) & (grp::VT - 1);
+        h = h + 1 == grp::VT ? 0 : h + 1;
     }
     isnew = false;
     return -1;
 }
 
-__device__ __forceinline__ int nb_insert(GroupMem &g, int v) {
-    unsigned h = hash_key(v, grp::NT_LOG2);
-    for (int n = 0; n < grp::NT; n++) {
-        const int old = atomicCAS(&g.nkey[h], -1, v);
-        if (old == -1 || old == v) return static_cast<int>(h);
-        h = (h + 1) & (grp::NT - 1);
+// query j's in-sphere adjacency word of visited slot s: byte k = its dense
+// index of the neighbour across face k when that is inside its sphere, else 0xff
+__device__ __forceinline__ unsigned adjacency(const QuadMem &g, int s, int j) {
+    const unsigned nbs = g.vnb[s];
+    const unsigned bit = 2u << j;
+    unsigned a = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const unsigned ns = (nbs >> (8 * k)) & 0xffu;
+        const unsigned d = (ns != 0xffu && (g.vflag[ns] & bit)) ? g.dense[j][ns] : 0xffu;
+        a |= d << (8 * k);
     }
-    return -1;
+    return a;
 }
 
-__device__ __forceinline__ int nb_find(const GroupMem &g, int v) {
-    unsigned h = hash_key(v, grp::NT_LOG2);
-    for (int n = 0; n < grp::NT; n++) {
-        const int k = g.nkey[h];
-        if (k == v) return static_cast<int>(h);
-        if (k == -1) return -1;
-        h = (h + 1) & (grp::NT - 1);
-    }
-    return -1;
+// the four vertices of a fat tet
+__device__ __forceinline__ void fat_points(const FatTet &f, d3 &p0, d3 &p1, d3 &p2, d3 &p3) {
+    const double4 a = f.c0, b = f.c1, c = f.c2;
+    p0 = {a.x, a.y, a.z};
+    p1 = {a.w, b.x, b.y};
+    p2 = {b.z, b.w, c.x};
+    p3 = {c.y, c.z, c.w};
 }
 
-// Vertex of entry e of the face-vertex sequence (boundary face e / 3,
-// vertex e % 3 in getFaceVerticies3 order).
-// After the boundary faces are known the visited table is no longer read:
-// its storage then holds, per cavity position, the tet's vertices (over
-// vkey and vnb) and the tet (over vflag and vpos); the face-circumcentre
-// masks go to the (dead) replay stack.
-static_assert(sizeof(int) * grp::VT * 2 == sizeof(int4) * grp::CMAX, "cavity vertex cache");
-static_assert(grp::VT * 2 == sizeof(int) * grp::CMAX, "cavity tet cache");
-static_assert(grp::SMAX >= grp::CMAX, "cavity mask cache");
-__device__ __forceinline__ int4 *cav_verts(GroupMem &g) { return reinterpret_cast<int4 *>(g.vkey); }
-__device__ __forceinline__ const int4 *cav_verts(const GroupMem &g) {
-    return reinterpret_cast<const int4 *>(g.vkey);
-}
-__device__ __forceinline__ int *cav_tet(GroupMem &g) { return reinterpret_cast<int *>(g.vflag); }
-__device__ __forceinline__ unsigned char *cav_mask(GroupMem &g) { return g.stk; }
-
-// vertex of face-vertex sequence number e (boundary face e / 3)
-__device__ __forceinline__ int face_vertex(const GroupMem &g, int e) {
-    const int j = g.bnd[e / 3];
-    return sel4(cav_verts(g)[j >> 2], face_slot(j & 3, e % 3));
-}
-
-// Record of one query, in 16-byte units at topo.rec[rec_off[qi]]:
-//   [0]            (C, B, NB, 0): cavity tets, boundary faces, neighbours
-//   [1 .. C]       per cavity tet: (tet, slot(p0) | slot(p1) << 16,
-//                  slot(p2) | slot(p3) << 16, fcmask) -- slot 0xffff = none;
-//                  fcmask bit k: face opposite p[k] is held by an earlier
-//                  cavity tet (its circumcentre uses that tet's vertex order)
-//   next ceil(B/4) boundary faces, cavity position << 2 | face, 4 per unit
-//   next ceil(NB/4) neighbour vertex indices in slot order, 4 per unit
-// C = 0 rows: NB = vertices coinciding with the query (weight 1/NB each),
-// or NB = 0 for an empty row.
-//
-// Phases per query (16 lanes):
-//   B  breadth-first expansion: every tet the reference's search pops -- the
-//      start tet and every neighbour of a tet whose circumsphere holds q --
-//      is tested once, the tests of one front in parallel
-//   C  one lane replays the reference's LIFO order over the tested set
-//      (natural_insertionfree.c:498-530) from LDS alone: cavity order
-//   D  boundary faces in (cavity position, face) order by ballot compaction;
-//      neighbour slots by least first appearance, ranked in sequence order
-//      (natural_insertionfree.c:559-584)
-//   E  the record, written by all 16 lanes
-// 4 waves per SIMD (the 20 KB of LDS per block allow it) matter more here than
-// the few registers the compiler has to give up for them
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4)))
-topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict__ items,
-                  int n_items, const int *__restrict__ start_of, unsigned int *work, Topo topo,
-                  unsigned int *ovf_count, int *ovf_list) {
-    __shared__ GroupMem gmem[grp::GPB];
+// 64-thread blocks of 4 quads; 17 KB of LDS per block allow 9 per CU
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict__ items,
+                   int n_items, const int *__restrict__ start_of, unsigned int *work, Topo topo,
+                   unsigned int *ovf_count, int *ovf_list) {
+    __shared__ QuadMem gmem[grp::QPB];
     const int lane = __lane_id();
     const int gl = lane & (grp::G - 1);
     const int gw = lane >> 4;
-    GroupMem &g = gmem[threadIdx.x >> 4];
-    const int N = m.n_data;
+    QuadMem &g = gmem[gw];
     int parts_done = 0, first = 0, cnt = 0;
-    while (part_batch(work, n_items, 4, parts_done, first, cnt)) {
-        const int item = first + gw;
-        const bool active = gw < cnt;
-        const int qi = active ? items[item] : -1;
-        int st = -1;
-        d3 qq = {0, 0, 0};
-        if (active) {
-            st = start_of[qi];
-            qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
-                  q[3 * static_cast<size_t>(qi) + 2]};
+    while (part_batch(work, n_items, grp::Q * grp::QPB, parts_done, first, cnt)) {
+        // query j of this quad: item first + 4 gw + j
+        int qi[grp::Q], st[grp::Q];
+        d3 qq[grp::Q];
+        bool act[grp::Q], ok[grp::Q];
+        int budget = 0;
+#pragma unroll
+        for (int j = 0; j < grp::Q; j++) {
+            const int item = grp::Q * gw + j;
+            qi[j] = item < cnt ? items[first + item] : -1;
+            st[j] = -1;
+            qq[j] = {0, 0, 0};
+            int b = 0;
+            if (qi[j] >= 0) {
+                st[j] = start_of[qi[j]];
+                qq[j] = {q[3 * static_cast<size_t>(qi[j])], q[3 * static_cast<size_t>(qi[j]) + 1],
+                         q[3 * static_cast<size_t>(qi[j]) + 2]};
+                b = min(grp::VQ, static_cast<int>(route_of(m, qi[j]) & 0xffu));
+            }
+            // a query outside the super simplex (st -1) has an empty row; k-d
+            // overflow (st -2) and budget 0 go to the larger levels
+            act[j] = qi[j] >= 0 && st[j] >= 0 && b > 0;
+            ok[j] = qi[j] >= 0 && st[j] != -2 && b > 0;
+            if (act[j]) budget += b;
         }
-        // the problem's visited-tet budget; 0 = thread-per-query rows only
-        const int vcap = active ? min(grp::VMAX, static_cast<int>(route_of(m, qi) & 0xffu)) : 0;
-        bool ok = st != -2 && vcap > 0;
-        int ncav = 0, nbnd = 0, nnb = 0;
-        int4 coin = make_int4(0, 0, 0, 0);
-        if (active && st >= 0 && ok) {
-            // ---- phase 0: empty tables
-            for (int i = gl; i < grp::VT; i += grp::G) {
+        const int vcap = min(grp::VMAX, budget);
+        const bool any = act[0] || act[1] || act[2] || act[3];
+        int ncav[grp::Q] = {0, 0, 0, 0}, nbnd[grp::Q] = {0, 0, 0, 0}, nnb[grp::Q] = {0, 0, 0, 0};
+        int nvis = 0;
+        if (any) {
+            for (int i = gl; i < 256; i += grp::G) {
                 g.vkey[i] = -1;
                 g.vflag[i] = 0;
             }
-            for (int i = gl; i < grp::NT; i += grp::G) {
-                g.nkey[i] = -1;
-                g.nmin[i] = 0x7fffffff;
-            }
-            if (gl == 0) g.cnt[0] = g.cnt[1] = g.cnt[2] = g.cnt[3] = 0;
+            if (gl == 0) g.cnt[3] = 0;
             group_sync();
-            // ---- phase B: test every tet the search pops, front by front.
-            // The start tet contains the query, so it is in the cavity and its
-            // neighbours are tested with it in the first front.  (Should the
-            // rounded test still put it outside, the replay below never leaves
-            // it, and the extra tests only cost capacity.)
-            int s0 = 0;
+            // the first front: the start tets and their neighbours
             if (gl == 0) {
                 bool isnew;
-                s0 = vis_probe(g, st, hash_key(st, 7), isnew);
-                g.front[0][0] = static_cast<unsigned char>(s0);
-                int n = 1;
-                const int4 tn0 = m.tet_n[st];
+                int n = 0;
+                for (int j = 0; j < grp::Q; j++) {
+                    g.sslot[j] = -1;
+                    if (!act[j]) continue;
+                    const int s0 = vis_probe(g, st[j], vt_hash(st[j]), isnew);
+                    g.sslot[j] = s0;
+                    if (isnew) {
+                        g.vlist[n] = static_cast<unsigned char>(s0);
+                        g.vtet[n++] = st[j];
+                    }
+                }
+                for (int j = 0; j < grp::Q; j++) {
+                    if (!act[j]) continue;
+                    const int4 tn0 = m.tet_n[st[j]];
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int nb = sel4(tn0, k);
-                    if (nb < 0) continue;
-                    const int s2 = vis_probe(g, nb, hash_key(nb, 7), isnew);
-                    if (s2 >= 0 && isnew) g.front[0][n++] = static_cast<unsigned char>(s2);
+                    for (int k = 0; k < 4; k++) {
+                        const int nb = sel4(tn0, k);
+                        if (nb < 0) continue;
+                        const int s2 = vis_probe(g, nb, vt_hash(nb), isnew);
+                        if (s2 >= 0 && isnew) {
+                            g.vlist[n] = static_cast<unsigned char>(s2);
+                            g.vtet[n++] = nb;
+                        }
+                    }
                 }
                 g.cnt[0] = n;
             }
             group_sync();
-            int cur = 0;
-            int nf = g.cnt[0];
-            int visited = nf;
-            while (nf > 0) {
+            int f0 = 0, f1 = g.cnt[0];
+            while (f0 < f1) {
                 int nnext = 0;  // size of the next front (uniform over the 16 lanes)
-                for (int i0 = 0; i0 < nf; i0 += grp::G) {
+                for (int i0 = f0; i0 < f1; i0 += grp::G) {
                     const int i = i0 + gl;
                     unsigned newslots = 0;
                     int nnew = 0;
-                    if (i < nf) {
-                        const int slot = g.front[cur][i];
-                        const int t = g.vkey[slot];
-                        bool neg;
-                        const int4 tv = ld_tet_v(m, t, neg);
-                        const int4 tn = m.tet_n[t];  // issued with tv, used if inside
-                        const d3 p0 = ld_vert(m, tv.x), p1 = ld_vert(m, tv.y);
-                        const d3 p2 = ld_vert(m, tv.z), p3 = ld_vert(m, tv.w);
-                        const double ins =
-                            neg ? insphere(p1, p0, p2, p3, qq) : insphere(p0, p1, p2, p3, qq);
+                    int nt0 = 0, nt1 = 0, nt2 = 0, nt3 = 0;  // tets of the new slots
+                    if (i < f1) {
+                        const int slot = g.vlist[i];
+                        const int t = g.vtet[i];
+                        const FatTet &ft = m.fat[t];
+                        const bool neg = ft.v.w < 0;
+                        const int4 tn = ft.n;
+                        d3 p0, p1, p2, p3;
+                        fat_points(ft, p0, p1, p2, p3);
+                        if (neg) {
+                            const d3 tmp = p0;
+                            p0 = p1;
+                            p1 = tmp;
+                        }
+                        unsigned in4 = 0;
+#pragma unroll
+                        for (int j = 0; j < grp::Q; j++)
+                            if (act[j] && insphere(p0, p1, p2, p3, qq[j]) > 0) in4 |= 1u << j;
                         unsigned packed = 0xffffffffu;
-                        if (ins > 0) {
+                        if (in4) {
                             packed = 0;
                             // first probes of the four neighbours issued together
                             int nb[4], old[4];
@@ -1294,7 +1350,7 @@ topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict
 #pragma unroll
                             for (int k = 0; k < 4; k++) {
                                 nb[k] = sel4(tn, k);
-                                hh[k] = hash_key(nb[k], 7);
+                                hh[k] = vt_hash(nb[k]);
                                 old[k] = nb[k] >= 0 ? atomicCAS(&g.vkey[hh[k]], -1, nb[k]) : -2;
                             }
 #pragma unroll
@@ -1304,211 +1360,435 @@ topo_group_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict
                                     bool isnew = old[k] == -1;
                                     int s2 = static_cast<int>(hh[k]);
                                     if (old[k] != -1 && old[k] != nb[k])
-                                        s2 = vis_probe(g, nb[k], (hh[k] + 1) & (grp::VT - 1),
-                                                       isnew);
+                                        s2 = vis_probe(g, nb[k],
+                                                       hh[k] + 1 == grp::VT ? 0 : hh[k] + 1, isnew);
                                     if (s2 < 0) {
                                         g.cnt[3] = 1;
                                     } else {
                                         ns = static_cast<unsigned>(s2);
-                                        if (isnew) newslots |= ns << (8 * nnew++);
+                                        if (isnew) {
+                                            nt0 = nnew == 0 ? nb[k] : nt0;
+                                            nt1 = nnew == 1 ? nb[k] : nt1;
+                                            nt2 = nnew == 2 ? nb[k] : nt2;
+                                            nt3 = nnew == 3 ? nb[k] : nt3;
+                                            newslots |= ns << (8 * nnew++);
+                                        }
                                     }
                                 }
                                 packed |= ns << (8 * k);
                             }
                         }
                         g.vnb[slot] = packed;
-                        g.vflag[slot] = ins > 0 ? 3 : 1;
+                        g.vflag[slot] = static_cast<unsigned char>(1u | (in4 << 1));
                     }
-                    // next-front positions: exclusive scan over the 16 lanes
+                    // positions in the next front: exclusive scan over the 16 lanes
                     int incl = nnew;
 #pragma unroll
                     for (int d = 1; d < grp::G; d <<= 1) {
                         const int y = __shfl_up(incl, d, grp::G);
                         if (gl >= d) incl += y;
                     }
-                    const int p = nnext + incl - nnew;
+                    const int p = f1 + nnext + incl - nnew;
                     for (int j = 0; j < nnew; j++) {
-                        if (p + j < grp::VMAX)
-                            g.front[cur ^ 1][p + j] =
+                        if (p + j < grp::VMAX) {
+                            g.vlist[p + j] =
                                 static_cast<unsigned char>((newslots >> (8 * j)) & 0xffu);
-                        else
+                            g.vtet[p + j] = j == 0 ? nt0 : (j == 1 ? nt1 : (j == 2 ? nt2 : nt3));
+                        } else {
                             g.cnt[3] = 1;
+                        }
                     }
                     nnext += __shfl(incl, grp::G - 1, grp::G);
                 }
                 group_sync();
-                visited += nnext;
-                if (visited > vcap) nnext = 0;
-                nf = g.cnt[3] ? 0 : nnext;
-                cur ^= 1;
+                f0 = f1;
+                f1 += nnext;
+                if (f1 > vcap || g.cnt[3]) break;
             }
-            ok = g.cnt[3] == 0 && visited <= vcap;
-            // ---- phase C: the reference's cavity order
-            int nc = 0;
-            if (ok && gl == 0) {
-                int top = 0;
-                g.stk[top++] = static_cast<unsigned char>(s0);
-                while (top > 0) {
-                    const int s = g.stk[--top];
-                    const unsigned f = g.vflag[s];
-                    if (f & 4u) continue;
-                    g.vflag[s] = static_cast<unsigned char>(f | 4u);
-                    if (!(f & 2u)) continue;
-                    if (nc >= grp::CMAX) {
-                        nc = -1;
-                        break;
-                    }
-                    g.cav[nc] = static_cast<unsigned char>(s);
-                    g.vpos[s] = static_cast<unsigned char>(nc);
-                    nc++;
-                    const unsigned nbs = g.vnb[s];
-                    // only in-sphere neighbours are pushed: a tet outside the
-                    // sphere is never appended or expanded, so leaving it out
-                    // of the stack does not change the order of the others
+            const bool qok = g.cnt[3] == 0 && f1 <= vcap;
+            nvis = qok ? f1 : 0;  // the list is valid up to f1 only when nothing overflowed
+            // per query: its start tet inside its sphere, else the
+            // coincident-vertex path (natural_insertionfree.c:596-622)
+            unsigned cav_q = 0;  // queries with a cavity
 #pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        const unsigned ns = (nbs >> (8 * k)) & 0xffu;
-                        if (ns == 0xffu || (g.vflag[ns] & 6u) != 2u) continue;
-                        if (top >= grp::SMAX) {
-                            nc = -1;
-                            top = 0;
-                            break;
-                        }
-                        g.stk[top++] = static_cast<unsigned char>(ns);
-                    }
-                    if (nc < 0) break;
+            for (int j = 0; j < grp::Q; j++) {
+                if (!act[j]) continue;
+                if (!qok) {
+                    ok[j] = false;
+                    continue;
+                }
+                if (g.vflag[g.sslot[j]] & (2u << j)) {
+                    cav_q |= 1u << j;
+                } else {
+                    int4 coin;
+                    nnb[j] = coincident4(m, st[j], qq[j], coin);
                 }
             }
-            nc = __shfl(nc, 0, grp::G);
-            group_sync();
-            if (ok && nc < 0) ok = false;
-            ncav = ok ? nc : 0;
-            if (ok && ncav == 0) {
-                // coincident-vertex path, natural_insertionfree.c:596-622
-                nnb = coincident4(m, st, qq, coin);
-            } else if (ok) {
-                // ---- phase D: boundary faces
-                for (int c0 = 0; c0 < 4 * ncav; c0 += grp::G) {
-                    const int j = c0 + gl;
-                    bool isb = false;
-                    if (j < 4 * ncav) {
-                        const unsigned ns = (g.vnb[g.cav[j >> 2]] >> (8 * (j & 3))) & 0xffu;
-                        isb = ns == 0xffu || !(g.vflag[ns] & 2u);
-                    }
-                    const unsigned mine =
-                        static_cast<unsigned>(__ballot(isb) >> (16 * gw)) & 0xffffu;
-                    if (isb) {
-                        const int p = nbnd + __popc(mine & ((1u << gl) - 1u));
-                        if (p < grp::BMAX) g.bnd[p] = static_cast<unsigned short>(j);
-                    }
-                    nbnd += __popc(mine);
-                }
-                if (nbnd > grp::BMAX) ok = false;
-                group_sync();
-                if (ok) {
-                    // per cavity tet, once: the face-circumcentre mask (face
-                    // opposite p[k] held by an earlier cavity tet), the tet,
-                    // its vertices
-                    for (int ci = gl; ci < ncav; ci += grp::G) {
-                        const unsigned nbs = g.vnb[g.cav[ci]];
-                        unsigned mask = 0;
+            if (cav_q) {
+                // dense indices in test order, the start tet 0
 #pragma unroll
-                        for (int k = 0; k < 4; k++) {
-                            const unsigned ns = (nbs >> (8 * (3 - k))) & 0xffu;
-                            if (ns != 0xffu && (g.vflag[ns] & 2u) && g.vpos[ns] < ci)
-                                mask |= 1u << k;
+                for (int j = 0; j < grp::Q; j++) ncav[j] = (cav_q >> j) & 1u;
+                for (int i0 = 0; i0 < nvis; i0 += grp::G) {
+                    const int i = i0 + gl;
+                    const int s = i < nvis ? g.vlist[i] : -1;
+                    const unsigned fl = s >= 0 ? g.vflag[s] : 0u;
+#pragma unroll
+                    for (int j = 0; j < grp::Q; j++) {
+                        if (!((cav_q >> j) & 1u)) continue;
+                        const bool in = (fl & (2u << j)) && s != g.sslot[j];
+                        const unsigned mine =
+                            static_cast<unsigned>(__ballot(in) >> (16 * gw)) & 0xffffu;
+                        if (in) {
+                            const int d = ncav[j] + __popc(mine & ((1u << gl) - 1u));
+                            g.dense[j][s] = static_cast<unsigned char>(min(d, 255));
                         }
-                        cav_mask(g)[ci] = static_cast<unsigned char>(mask);
+                        ncav[j] += __popc(mine);
                     }
-                    group_sync();
-                    for (int ci = gl; ci < ncav; ci += grp::G) cav_tet(g)[ci] = g.vkey[g.cav[ci]];
-                    group_sync();
-                    for (int ci = gl; ci < ncav; ci += grp::G) {
-                        bool neg;
-                        cav_verts(g)[ci] = ld_tet_v(m, cav_tet(g)[ci], neg);
-                    }
-                    group_sync();
                 }
-                // neighbours: least sequence number of each data vertex
-                const int nent = ok ? 3 * nbnd : 0;
-                bool full = false;
-                for (int e = gl; e < nent; e += grp::G) {
-                    const int v = face_vertex(g, e);
-                    if (v >= N) continue;
-                    const int h = nb_insert(g, v);
-                    if (h < 0) {
-                        full = true;
-                        continue;
-                    }
-                    atomicMin(&g.nmin[h], e);
-                }
-                if (__ballot(full) >> (16 * gw) & 0xffffull) ok = false;
+                if (gl == 0)
+#pragma unroll
+                    for (int j = 0; j < grp::Q; j++)
+                        if ((cav_q >> j) & 1u) g.dense[j][g.sslot[j]] = 0;
                 group_sync();
-                // slots in order of first appearance
-                for (int c0 = 0; c0 < nent && ok; c0 += grp::G) {
-                    const int e = c0 + gl;
-                    bool first = false;
-                    int v = -1, h = -1;
-                    if (e < nent) {
-                        v = face_vertex(g, e);
-                        if (v < N) {
-                            h = nb_find(g, v);
-                            first = h >= 0 && g.nmin[h] == e;
+                // boundary faces: the 0xff bytes of the in-sphere adjacency
+                int b[grp::Q] = {0, 0, 0, 0};
+                for (int i = gl; i < nvis; i += grp::G) {
+                    const int s = g.vlist[i];
+                    const unsigned fl = g.vflag[s];
+#pragma unroll
+                    for (int j = 0; j < grp::Q; j++) {
+                        if (!((cav_q >> j) & 1u) || !(fl & (2u << j)) || ncav[j] > grp::CMAX)
+                            continue;
+                        const unsigned a = adjacency(g, s, j);
+#pragma unroll
+                        for (int k = 0; k < 4; k++) b[j] += ((a >> (8 * k)) & 0xffu) == 0xffu;
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < grp::Q; j++) {
+#pragma unroll
+                    for (int d = 1; d < grp::G; d <<= 1) b[j] += __shfl_xor(b[j], d, grp::G);
+                    nbnd[j] = b[j];
+                    if ((cav_q >> j) & 1u) {
+                        if (ncav[j] > grp::CMAX || nbnd[j] > grp::BMAX) {
+                            ok[j] = false;
+                            cav_q &= ~(1u << j);
                         }
+                    } else {
+                        ncav[j] = 0;
                     }
-                    const unsigned mine =
-                        static_cast<unsigned>(__ballot(first) >> (16 * gw)) & 0xffffu;
-                    if (first) {
-                        const int slot = nnb + __popc(mine & ((1u << gl) - 1u));
-                        g.nslot[h] = static_cast<unsigned char>(slot);
-                        if (slot < grp::NMAX) g.nbv[slot] = v;
-                    }
-                    nnb += __popc(mine);
                 }
-                if (nnb > grp::NMAX) ok = false;
-                if (ok && nnb == 0) ncav = nbnd = 0;  // no data vertex: empty row
-                group_sync();
             }
         }
-        // ---- phase E: reserve and write the record
+        // reserve the records: lane j of the quad reserves query j's
         int units = 0;
-        if (active && ok) units = 1 + ncav + (nbnd + 3) / 4 + (nnb + 3) / 4;
-        const int lead_units = gl == 0 ? units : 0;
-        const long long off0 = wave_reserve(topo.rec_used, lead_units);
-        const long long off = __shfl(off0, 0, grp::G);
-        if (!active) continue;
-        if (!ok || off + units > topo.rec_cap) {
-            if (gl == 0) {
-                topo.rec_off[qi] = -1;
-                if (!ok) ovf_list[atomicAdd(ovf_count, 1u)] = qi;
-            }
-            continue;
-        }
-        int4 *r = topo.rec + off;
-        if (gl == 0) {
-            topo.rec_off[qi] = off;
-            r[0] = make_int4(ncav, nbnd, nnb, 0);
-            if (ncav == 0 && nnb > 0) r[1] = coin;
-        }
-        if (ncav == 0) continue;
-        for (int ci = gl; ci < ncav; ci += grp::G) {
-            const int4 tv = cav_verts(g)[ci];
-            int sl[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int v = sel4(tv, k);
-                const int h = v < N ? nb_find(g, v) : -1;
-                sl[k] = h < 0 ? 0xffff : g.nslot[h];
-            }
-            r[1 + ci] = make_int4(cav_tet(g)[ci], sl[0] | (sl[1] << 16), sl[2] | (sl[3] << 16),
-                                  static_cast<int>(cav_mask(g)[ci]));
+        for (int j = 0; j < grp::Q; j++) {
+            int u = 0;
+            if (qi[j] >= 0 && ok[j])
+                u = ncav[j] > 0 ? rec_units(ncav[j], nbnd[j], nb_room(nbnd[j]))
+                                : 1 + (nnb[j] + 3) / 4;
+            if (gl == j) units = u;
         }
-        int *rf = reinterpret_cast<int *>(r + 1 + ncav);
-        for (int bi = gl; bi < nbnd; bi += grp::G) rf[bi] = g.bnd[bi];
-        int *rn = reinterpret_cast<int *>(r + 1 + ncav + (nbnd + 3) / 4);
-        for (int j = gl; j < nnb; j += grp::G) rn[j] = g.nbv[j];
+        const long long off_l = wave_reserve(topo.rec_used, units);
+        long long off[grp::Q];
+        bool fits[grp::Q];
+#pragma unroll
+        for (int j = 0; j < grp::Q; j++) {
+            off[j] = __shfl(off_l, j, grp::G);
+            int u = 0;
+            if (qi[j] >= 0 && ok[j])
+                u = ncav[j] > 0 ? rec_units(ncav[j], nbnd[j], nb_room(nbnd[j]))
+                                : 1 + (nnb[j] + 3) / 4;
+            fits[j] = off[j] + u <= topo.rec_cap;
+            if (gl == j && qi[j] >= 0) {
+                if (!ok[j] || !fits[j]) {
+                    topo.rec_off[qi[j]] = -1;
+                    if (!ok[j]) ovf_list[atomicAdd(ovf_count, 1u)] = qi[j];
+                } else {
+                    topo.rec_off[qi[j]] = off[j];
+                    int4 *r = topo.rec + off[j];
+                    if (ncav[j] == 0) {
+                        int4 coin = make_int4(0, 0, 0, 0);
+                        if (nnb[j] > 0) coincident4(m, st[j], qq[j], coin);
+                        r[0] = make_int4(0, 0, nnb[j], 0);
+                        if (nnb[j] > 0) r[1] = coin;
+                    } else {
+                        r[0] = make_int4(ncav[j], nbnd[j], nb_room(nbnd[j]), 1);
+                    }
+                }
+            }
+        }
+        // entries and handoff pairs of the queries with a cavity
+        for (int i = gl; i < nvis; i += grp::G) {
+            const int s = g.vlist[i];
+            const unsigned fl = g.vflag[s];
+            if (!(fl & 0x1eu)) continue;
+            const int t = g.vtet[i];
+            const int4 tv = m.fat[t].v;
+#pragma unroll
+            for (int j = 0; j < grp::Q; j++) {
+                if (qi[j] < 0 || !ok[j] || !fits[j] || ncav[j] == 0 || !(fl & (2u << j)))
+                    continue;
+                int4 *r = topo.rec + off[j];
+                const int d = g.dense[j][s];
+                const int room = nb_room(nbnd[j]);
+                r[1 + d] = tv;
+                rec_xadj(r, ncav[j], nbnd[j], room)[d] = static_cast<int>(adjacency(g, s, j));
+                rec_xtet(r, ncav[j], nbnd[j], room)[d] = t;
+            }
+        }
         group_sync();
     }
+}
+
+// The order rules, in two kernels.  Block b of either takes a run of part
+// b mod 8 of the items (the parts of the search kernel), so a query's record
+// is read on the XCD whose L2 the search kernel wrote it through.
+//
+// topo_dfs_kernel, one thread per query (the walk is serial, and 64 of them
+// share a wave): the depth-first preorder from the start tet (dense 0),
+// entering neighbour slots 3, 2, 1, 0.  Appending the tet at position p also
+// settles its part of the boundary sequence (its faces whose adjacency byte
+// is 0xff, in face order: positions are appended in order) and its
+// face-circumcentre mask (bit k: the neighbour across face 3 - k was appended
+// before it).  Both go out as packed streams, a 32-bit store per two
+// positions or four boundary faces.
+constexpr int kDfsThreads = 64;
+constexpr int kDfsLds = 32;  // adjacency words kept in LDS; the rest read from the record
+
+__global__ void __launch_bounds__(kDfsThreads)
+topo_dfs_kernel(const int *__restrict__ items, int n_items, Topo topo, unsigned int *ovf_count,
+                int *ovf_list) {
+    // adjacency (dense < kDfsLds) and depth-first parent per dense index, one
+    // column per thread ([k][t]: the lanes of a wave hit distinct banks)
+    __shared__ unsigned adjl[kDfsLds][kDfsThreads];
+    __shared__ unsigned char parl[grp::CMAX][kDfsThreads];
+    const int t = static_cast<int>(threadIdx.x);
+    const int part = static_cast<int>(blockIdx.x) % kParts;
+    const int lo = static_cast<int>(static_cast<long long>(n_items) * part / kParts);
+    const int hi = static_cast<int>(static_cast<long long>(n_items) * (part + 1) / kParts);
+    const int i = lo + static_cast<int>(blockIdx.x) / kParts * kDfsThreads + t;
+    if (i >= hi) return;
+    const int qi = items ? items[i] : i;
+    const long long roff = topo.rec_off[qi];
+    if (roff < 0) return;
+    int4 *r = topo.rec + roff;
+    const int4 hd = r[0];
+    if (hd.w != 1) return;  // coincident or empty row: complete
+    const int C = hd.x, B = hd.y, room = hd.z;
+    const int *xadj = rec_xadj(r, C, B, room);
+    const int4 *xa4 = reinterpret_cast<const int4 *>(xadj);
+    const int cl = min(C, kDfsLds);
+    for (int d0 = 0; d0 < cl; d0 += 4) {
+        const int4 v = xa4[d0 >> 2];
+        adjl[d0][t] = static_cast<unsigned>(v.x);
+        if (d0 + 1 < cl) adjl[d0 + 1][t] = static_cast<unsigned>(v.y);
+        if (d0 + 2 < cl) adjl[d0 + 2][t] = static_cast<unsigned>(v.z);
+        if (d0 + 3 < cl) adjl[d0 + 3][t] = static_cast<unsigned>(v.w);
+    }
+    auto adj_of = [&](int d) {
+        return d < kDfsLds ? adjl[d][t] : static_cast<unsigned>(xadj[d]);
+    };
+    unsigned *ro = reinterpret_cast<unsigned *>(r + rec_ord_off(C));
+    unsigned *rb = reinterpret_cast<unsigned *>(r + rec_bnd_off(C));
+    unsigned long long vis = 1ull;
+    int n = 0, nb = 0;
+    unsigned ow = 0, bw = 0;
+    // append dense index c at position n: its order entry, boundary faces, mask
+    auto append = [&](int c, unsigned a) {
+        unsigned fc = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (((a >> (8 * k)) & 0xffu) == 0xffu) {
+                if (nb < B) {
+                    bw |= static_cast<unsigned>((c << 2) | k) << (8 * (nb & 3));
+                    if ((nb & 3) == 3) {
+                        rb[nb >> 2] = bw;
+                        bw = 0;
+                    }
+                }
+                nb++;
+            }
+            const unsigned ns = (a >> (8 * (3 - k))) & 0xffu;
+            if (ns != 0xffu && ((vis >> ns) & 1ull)) fc |= 1u << k;
+        }
+        if (n < C) {
+            ow |= (static_cast<unsigned>(c) | (fc << 6)) << (16 * (n & 1));
+            if (n & 1) {
+                ro[n >> 1] = ow;
+                ow = 0;
+            }
+        }
+        n++;
+    };
+    int u = 0;
+    unsigned a = adj_of(0);
+    append(0, a);
+    for (;;) {
+        int c = -1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {  // the highest free slot wins
+            const unsigned ck = (a >> (8 * k)) & 0xffu;
+            if (ck < static_cast<unsigned>(C) && !((vis >> ck) & 1ull)) c = static_cast<int>(ck);
+        }
+        if (c >= 0) {
+            parl[c][t] = static_cast<unsigned char>(u);
+            a = adj_of(c);
+            append(c, a);  // the mask sees the tets appended before c
+            vis |= 1ull << c;
+            u = c;
+        } else {
+            if (u == 0) break;
+            u = parl[u][t];
+            a = adj_of(u);
+        }
+    }
+    if (n != C || nb != B) {
+        topo.rec_off[qi] = -1;
+        ovf_list[atomicAdd(ovf_count, 1u)] = qi;
+        return;
+    }
+    if (n & 1) ro[n >> 1] = ow;
+    if (nb & 3) rb[nb >> 2] = bw;
+}
+
+// topo_order_kernel, 16 lanes per query, 4 queries per 64-thread block (one
+// wave): the neighbours in order of first appearance in the face-vertex
+// sequence (least sequence number by atomicMin, then ballot ranks), the
+// cavity entries' vertex slots, the header.
+constexpr int kOrderLanes = 16;
+constexpr int kOrderQueries = 4;
+
+struct alignas(16) OrderMem {
+    int4 cv[grp::CMAX];            // vertex slots per dense index
+    int dtet[grp::CMAX];           // tet per dense index
+    int nkey[grp::NT];             // vertex table: keys, -1 = empty
+    int nmin[grp::NT];             // first appearance (face-vertex sequence number)
+    unsigned char fcm[grp::CMAX];  // face-circumcentre mask per dense index
+    unsigned char nslot[grp::NT];  // neighbour slot per vertex-table slot
+    unsigned char bnd[grp::BMAX];  // boundary faces in sequence order, dense << 2 | face
+};
+
+__device__ __forceinline__ int order_vertex(const OrderMem &g, int e) {
+    const int code = g.bnd[e / 3];
+    return sel4(g.cv[code >> 2], face_slot(code & 3, e % 3)) & 0x7fffffff;
+}
+
+__global__ void __launch_bounds__(64)
+topo_order_kernel(DevMesh m, const int *__restrict__ items, int n_items, Topo topo,
+                  unsigned int *ovf_count, int *ovf_list) {
+    __shared__ OrderMem omem[kOrderQueries];
+    const int lane = __lane_id();
+    const int gl = lane & (kOrderLanes - 1);
+    const int gw = lane >> 4;
+    OrderMem &g = omem[gw];
+    const int part = static_cast<int>(blockIdx.x) % kParts;
+    const int lo = static_cast<int>(static_cast<long long>(n_items) * part / kParts);
+    const int hi = static_cast<int>(static_cast<long long>(n_items) * (part + 1) / kParts);
+    const int i = lo + static_cast<int>(blockIdx.x) / kParts * kOrderQueries + gw;
+    if (i >= hi) return;  // whole groups leave: nothing below spans groups
+    const int qi = items ? items[i] : i;
+    const long long roff = topo.rec_off[qi];
+    if (roff < 0) return;
+    int4 *r = topo.rec + roff;
+    const int4 hd = r[0];
+    if (hd.w != 1) return;  // coincident or empty row: complete
+    const int C = hd.x, B = hd.y, room = hd.z;
+    const int N = m.n_data;
+    const unsigned gmask = 0xffffu;
+    const int *xtet = rec_xtet(r, C, B, room);
+    const unsigned short *ro = rec_order(r, hd);
+    const unsigned char *rb = rec_bnd(r, hd);
+    for (int d = gl; d < C; d += kOrderLanes) {
+        g.dtet[d] = xtet[d];
+        g.cv[d] = r[1 + d];
+        const unsigned o = ro[d];  // position d: its dense index and mask
+        g.fcm[o & 63u] = static_cast<unsigned char>(o >> 6);
+    }
+    for (int e = gl; e < B; e += kOrderLanes) g.bnd[e] = rb[e];
+    for (int h = gl; h < grp::NT; h += kOrderLanes) {
+        g.nkey[h] = -1;
+        g.nmin[h] = 0x7fffffff;
+    }
+    group_sync();
+    // first appearance of each data vertex in the face-vertex sequence
+    const int nent = 3 * B;
+    bool full = false;
+    for (int e = gl; e < nent; e += kOrderLanes) {
+        const int v = order_vertex(g, e);
+        if (v >= N) continue;
+        unsigned h = hash_key(v, grp::NT_LOG2);
+        int k = 0;
+        for (; k < grp::NT; k++) {
+            const int old = atomicCAS(&g.nkey[h], -1, v);
+            if (old == -1 || old == v) break;
+            h = (h + 1) & (grp::NT - 1);
+        }
+        if (k == grp::NT) {
+            full = true;
+            continue;
+        }
+        atomicMin(&g.nmin[h], e);
+    }
+    bool ok = !((__ballot(full) >> (16 * gw)) & gmask);
+    group_sync();
+    // slots in order of first appearance
+    int nnb = 0;
+    int *rn = reinterpret_cast<int *>(r + rec_nbv_off(C, B));
+    for (int e0 = 0; ok && e0 < nent; e0 += kOrderLanes) {
+        const int e = e0 + gl;
+        bool first = false;
+        int v = -1;
+        unsigned h = 0;
+        if (e < nent) {
+            v = order_vertex(g, e);
+            if (v < N) {
+                h = hash_key(v, grp::NT_LOG2);
+                while (g.nkey[h] != v) h = (h + 1) & (grp::NT - 1);
+                first = g.nmin[h] == e;
+            }
+        }
+        const unsigned mine = static_cast<unsigned>(__ballot(first) >> (16 * gw)) & gmask;
+        if (first) {
+            const int slot = nnb + __popc(mine & ((1u << gl) - 1u));
+            g.nslot[h] = static_cast<unsigned char>(min(slot, 255));
+            if (slot < room) rn[slot] = v;
+        }
+        nnb += __popc(mine);
+    }
+    ok = ok && nnb <= room;
+    group_sync();
+    if (!ok) {
+        if (gl == 0) {
+            topo.rec_off[qi] = -1;
+            ovf_list[atomicAdd(ovf_count, 1u)] = qi;
+        }
+        return;
+    }
+    if (nnb == 0) {  // no data vertex on the boundary: an empty row
+        if (gl == 0) r[0] = make_int4(0, 0, 0, 0);
+        return;
+    }
+    // the cavity entries: tet, vertex slots, face-circumcentre mask
+    for (int d = gl; d < C; d += kOrderLanes) {
+        const int4 tv = g.cv[d];
+        int sl[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int v = sel4(tv, k) & 0x7fffffff;
+            sl[k] = 0xffff;
+            if (v < N) {
+                unsigned h = hash_key(v, grp::NT_LOG2);
+                int key;
+                while ((key = g.nkey[h]) != -1 && key != v) h = (h + 1) & (grp::NT - 1);
+                if (key == v) sl[k] = g.nslot[h];
+            }
+        }
+        r[1 + d] = make_int4(g.dtet[d], sl[0] | (sl[1] << 16), sl[2] | (sl[3] << 16),
+                             static_cast<int>(g.fcm[d]));
+    }
+    if (gl == 0) r[0] = make_int4(C, B, nnb, 0);
 }
 
 // ------------------------------------------------------- wave-per-query rows
@@ -2282,12 +2562,13 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
             } else {
                 const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                                q[3 * static_cast<size_t>(qi) + 2]};
-                const int *rf = reinterpret_cast<const int *>(r + 1 + ncav);
+                const unsigned char *rf = rec_bnd(r, hd);
+                const unsigned short *ro = rec_order(r, hd);
                 double absum = 0.0;
                 auto run = [&](bool fp32) {
                     for (int j = 0; j < nnb; j++) acc[j * kGeomThreads] = 0.0;
                     for (int ci = 0; ci < ncav; ci++) {
-                        const int4 cr = r[1 + ci];
+                        const int4 cr = r[1 + (ro[ci] & 63u)];
                         auto slot_of = [&](int vi, int) { return rec_slot(cr, vi); };
                         auto add = [&](int slot, double vol) {
                             acc[slot * kGeomThreads] += vol;
@@ -2365,7 +2646,7 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
                 out.pool_vals[off + j] = w;
             }
         } else {
-            const int *rn = reinterpret_cast<const int *>(r + 1 + hd.x + (hd.y + 3) / 4);
+            const int *rn = rec_nbv(r, hd);
             for (int j = 0; j < cnt; j++) {
                 out.pool_cols[off + j] = rn[j];
                 out.pool_vals[off + j] = acc[j * kGeomThreads];
@@ -2448,14 +2729,15 @@ __global__ void __launch_bounds__(kGeomRowsThreads) geom_rows_kernel(
                 const d3 qq = {q[3 * static_cast<size_t>(qi)],
                                q[3 * static_cast<size_t>(qi) + 1],
                                q[3 * static_cast<size_t>(qi) + 2]};
-                const int *rf = reinterpret_cast<const int *>(r + 1 + ncav);
+                const unsigned char *rf = rec_bnd(r, hd);
+                const unsigned short *ro = rec_order(r, hd);
                 double absum = 0.0;  // this lane's share of sum |vol| (MIXED)
                 auto run = [&](bool fp32) {
                     for (int j = g; j < nnb; j += G) acc[j][rr] = 0.0;
-                    // step 3 (the exact table in both modes)
+                    // step 3 (the exact table in both modes), cavity order
                     if (g < 4) {
                         for (int ci = 0; ci < ncav; ci++) {
-                            const int4 cr = r[1 + ci];
+                            const int4 cr = r[1 + (ro[ci] & 63u)];
                             const size_t base = (static_cast<size_t>(cr.x) * 16 +
                                                  (static_cast<unsigned>(cr.w) & 15u)) * 4;
 #pragma unroll
@@ -2564,7 +2846,7 @@ __global__ void __launch_bounds__(kGeomRowsThreads) geom_rows_kernel(
                 out.pool_vals[off + j] = w;
             }
         } else {
-            const int *rn = reinterpret_cast<const int *>(r + 1 + hd.x + (hd.y + 3) / 4);
+            const int *rn = rec_nbv(r, hd);
             for (int j = g; j < cnt; j += G) {
                 out.pool_cols[off + j] = rn[j];
                 out.pool_vals[off + j] = acc[j][rr];
@@ -2643,7 +2925,7 @@ __global__ void __launch_bounds__(kGradThreads) grad_geom_kernel(
         }
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
-        const int *rn = reinterpret_cast<const int *>(r + 1 + ncav + (nbnd + 3) / 4);
+        const int *rn = rec_nbv(r, hd);
         double abar = 0.0;
         for (int j = gl; j < nnb; j += kGradLanes) {
             const double a = grad_dot(g, qi, rn[j]);
@@ -2653,7 +2935,7 @@ __global__ void __launch_bounds__(kGradThreads) grad_geom_kernel(
         }
         group_sync();
         d3 ag = {0.0, 0.0, 0.0}, tg = {0.0, 0.0, 0.0};
-        const int *rf = reinterpret_cast<const int *>(r + 1 + ncav);
+        const unsigned char *rf = rec_bnd(r, hd);
         for (int bi = gl; bi < nbnd; bi += kGradLanes) {
             const int code = rf[bi];
             const int4 cr = r[1 + (code >> 2)];
@@ -2715,7 +2997,7 @@ __global__ void __launch_bounds__(256) predicate_audit_kernel(DevMesh m,
                 uncertain++;
         };
         for (int ci = 0; ci < ncav; ci++) check(r[1 + ci].x);
-        const int *rf = reinterpret_cast<const int *>(r + 1 + ncav);
+        const unsigned char *rf = rec_bnd(r, hd);
         for (int bi = 0; bi < nbnd; bi++) {
             const int code = rf[bi];
             const int nb = sel4(m.tet_n[r[1 + (code >> 2)].x], code & 3);
@@ -2860,7 +3142,7 @@ static int blocks_per_cu(K kernel) {
 int rows_blocks_per_cu() { return blocks_per_cu(sibson_rows_kernel); }
 int topo_blocks_per_cu() {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, topo_group_kernel, 128, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, topo_search_kernel, 64, 0) !=
         hipSuccess)
         return 1;
     return n > 0 ? n : 1;
@@ -2946,12 +3228,38 @@ hipError_t launch_locate(const DevMesh &m, const Caps &caps, const Scratch &sc, 
     return hipGetLastError();
 }
 
+hipError_t launch_topo_search(const DevMesh &m, int nblocks, const double *d_q,
+                              const int *d_items, int n_items, const int *d_start,
+                              unsigned int *d_work, const Topo &topo, unsigned int *ovf_count,
+                              int *ovf_list, hipStream_t st) {
+    hipLaunchKernelGGL(topo_search_kernel, dim3(nblocks), dim3(64), 0, st, m, d_q, d_items,
+                       n_items, d_start, d_work, topo, ovf_count, ovf_list);
+    return hipGetLastError();
+}
+
+hipError_t launch_topo_order(const DevMesh &m, const int *d_items, int n_items, const Topo &topo,
+                             unsigned int *ovf_count, int *ovf_list, hipStream_t st) {
+    if (n_items <= 0) return hipSuccess;
+    // blocks per part: the items of the largest part over the queries per block
+    const int per_part = (n_items + kParts - 1) / kParts;
+    const int nb1 = kParts * ((per_part + kDfsThreads - 1) / kDfsThreads);
+    hipLaunchKernelGGL(topo_dfs_kernel, dim3(nb1), dim3(kDfsThreads), 0, st, d_items, n_items,
+                       topo, ovf_count, ovf_list);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int nb2 = kParts * ((per_part + kOrderQueries - 1) / kOrderQueries);
+    hipLaunchKernelGGL(topo_order_kernel, dim3(nb2), dim3(64), 0, st, m, d_items, n_items, topo,
+                       ovf_count, ovf_list);
+    return hipGetLastError();
+}
+
 hipError_t launch_topo(const DevMesh &m, int nblocks, const double *d_q, const int *d_items,
                        int n_items, const int *d_start, unsigned int *d_work, const Topo &topo,
                        unsigned int *ovf_count, int *ovf_list, hipStream_t st) {
-    hipLaunchKernelGGL(topo_group_kernel, dim3(nblocks), dim3(128), 0, st, m, d_q, d_items,
-                       n_items, d_start, d_work, topo, ovf_count, ovf_list);
-    return hipGetLastError();
+    hipError_t e = launch_topo_search(m, nblocks, d_q, d_items, n_items, d_start, d_work, topo,
+                                      ovf_count, ovf_list, st);
+    if (e != hipSuccess) return e;
+    return launch_topo_order(m, d_items, n_items, topo, ovf_count, ovf_list, st);
 }
 
 hipError_t launch_work_sort(const int *d_items, int n, const Topo &topo, int *d_out,

@@ -104,7 +104,8 @@ class MeshAndVerticesHIP:
                 "mixed_fp32": t.mixed_fp32, "mixed_fp64": t.mixed_fp64,
                 "mixed_rows_fp64": t.mixed_rows_fp64, "wave_ms": t.wave_ms,
                 "wave_items": t.wave_items, "grad_wave_ms": t.grad_wave_ms,
-                "grad_wave_items": t.grad_wave_items}
+                "grad_wave_items": t.grad_wave_items, "topo_search_ms": t.topo_search_ms,
+                "topo_order_ms": t.topo_order_ms}
 
     def set_precision(self, mode="fp64", cond_threshold=0.02, cancel_threshold=20.0):
         """Row arithmetic: "fp64" (default; weights bit-identical to the
