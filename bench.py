@@ -56,38 +56,45 @@ def record_bytes(units, M):
     return 16.0 * units / M
 
 
-def kernel_bytes_per_query(w, rec_q, D):
+def kernel_bytes_per_query(w, D):
     """Algorithmic HBM bytes per query of each level-0 kernel, from the mean
-    work counters w (cavity tets C, insphere-tested tets T, distinct vertices
-    V, row entries nnz) and the mean record size rec_q:
+    work counters w (cavity tets C, boundary faces B, insphere-tested tets T,
+    distinct vertices V, row entries nnz, which is also the neighbour count):
 
-    topo_group_kernel: order + start (8), query (24), tet vertex ids of every
-      tested tet (16 T), their vertices (24 V), neighbour ids of cavity tets
-      (16 C), the record written (rec_q) and its offset (8);
-    geom_rows_kernel: order + record offset (12), query (24), record (rec_q), tet
-      vertex ids of cavity tets (16 C), vertices (24 V), row written (12 nnz)
-      and its offset/length (12);
-    grad_geom_kernel: as geom_rows_kernel without the row, plus dL/dy (8 D), the
-      neighbours' values (8 D nnz) and dL/dq written (24)."""
-    geo = 12 + 24 + rec_q + 16 * w["cavity"] + 24 * w["verts_read"]
+    topo_search_kernel: order + start (8), query (24), vertex and neighbour
+      ids of every tested tet (32 T), their vertices (24 V), the record it
+      writes: header, C entries and the handoff (tet, adjacency) (16 + 24 C);
+    topo_dfs_kernel + topo_order_kernel: the handoff and entries read back
+      (16 + 24 C), the cavity order (2 C) and boundary faces (B) written and
+      read, the entries, neighbour list and header written (16 + 16 C + 4 nnz);
+    geom_rows_kernel: order + record offset (12), query (24), the record
+      sections it reads (16 + 18 C + B + 4 nnz), tet vertex ids of cavity tets
+      (16 C), vertices (24 V), row written (12 nnz) and its offset/length (12);
+    grad_geom_kernel: as geom_rows_kernel without the row, plus dL/dy (8 D),
+      the neighbours' values (8 D nnz) and dL/dq written (24)."""
+    C, B, T, V, nnz = w["cavity"], w["boundary"], w["tets_read"], w["verts_read"], w["nnz"]
+    rec_read = 16 + 18 * C + B + 4 * nnz
+    geo = 12 + 24 + rec_read + 16 * C + 24 * V
     return {
-        "topo_group_kernel": 8 + 24 + 16 * w["tets_read"] + 24 * w["verts_read"]
-                             + 16 * w["cavity"] + rec_q + 8,
-        "geom_rows_kernel": geo + 12 * w["nnz"] + 12,
-        "grad_geom_kernel": geo + 8 * D + 8 * D * w["nnz"] + 24,
+        "topo_search_kernel": 8 + 24 + 32 * T + 24 * V + 16 + 24 * C,
+        "topo_dfs_kernel+topo_order_kernel": (16 + 24 * C) + 2 * (2 * C + B)
+                                             + (16 + 16 * C + 4 * nnz),
+        "geom_rows_kernel": geo + 12 * nnz + 12,
+        "grad_geom_kernel": geo + 8 * D + 8 * D * nnz + 24,
     }
 
 
 def kernel_fp64_ops_per_query(w):
-    """Non-fused FP64 operations per query of each level-0 kernel: insphere
-    77; tet circumcentre 79 (cavity and virtual tets); face circumcentre 56
-    (distinct faces plus two per new subcell); subcell volume 89;
-    normalisation 2 per entry; facet moment ~90 per new subcell and the
-    gradient sums ~15 per volume."""
+    """Non-fused FP64 operations per query of each level-0 kernel, counted
+    as the reference's path does them: insphere 77; tet circumcentre 79
+    (cavity and virtual tets); face circumcentre 56 (distinct faces plus two
+    per new subcell); subcell volume 89; normalisation 2 per entry; facet
+    moment ~90 per new subcell and the gradient sums ~15 per volume."""
     geo = (79 * (w["cavity"] + w["boundary"]) + 56 * (w["face_cc"] + 2 * w["sub4"])
            + 89 * (w["sub3"] + w["sub4"]))
     return {
-        "topo_group_kernel": 77 * w["insphere"],
+        "topo_search_kernel": 77 * w["insphere"],
+        "topo_dfs_kernel+topo_order_kernel": 0,
         "geom_rows_kernel": geo + 2 * w["nnz"] + 1,
         "grad_geom_kernel": geo + 90 * w["sub4"] + 15 * (w["sub3"] + w["sub4"]),
     }
@@ -714,17 +721,19 @@ def main():
                                                  None)
     units = (int(nbytes.value) - 32 - 12 * M) / 16.0
     rec_q = record_bytes(units, M)
-    kb = kernel_bytes_per_query(WORK_PER_QUERY, rec_q, D)
+    kb = kernel_bytes_per_query(WORK_PER_QUERY, D)
     kops = kernel_fp64_ops_per_query(WORK_PER_QUERY)
-    kms = {"topo_group_kernel": float(np.mean([t["topo_ms"] for t in per_step])),
+    kms = {"topo_search_kernel": float(np.mean([t["topo_search_ms"] for t in per_step])),
+           "topo_dfs_kernel+topo_order_kernel":
+               float(np.mean([t["topo_order_ms"] for t in per_step])),
            "geom_rows_kernel": float(np.mean([t["geom_ms"] for t in per_step])),
            "grad_geom_kernel": float(np.mean([t["grad_ms"][0] for t in per_step]))}
     dom = max(kms, key=kms.get)
     launch_ms = kms[dom]
     achieved = kb[dom] * M / (launch_ms / 1e3) / 1e9
     ops = kops[dom] * M / (launch_ms / 1e3)
-    log(f"kernels (ms): {kms}; dominant {dom}: {kb[dom]:.0f} B/query -> "
-        f"{achieved:.1f} GB/s, {ops / 1e12:.2f} T FP64 op/s")
+    log(f"kernels (ms): {kms}; record {rec_q:.0f} B/query; dominant {dom}: "
+        f"{kb[dom]:.0f} B/query -> {achieved:.1f} GB/s, {ops / 1e12:.2f} T FP64 op/s")
 
     # measured HBM bytes per launch of the dominant kernel (rocprofv3 --pmc on
     # this workload, committed with the commands that made them)

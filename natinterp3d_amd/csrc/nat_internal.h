@@ -67,9 +67,10 @@ struct DevMesh {
     // the queries that leave the group kernel.  Read at launch time.
     const unsigned short *route;
     // Step-3 subcell volumes, which depend on the mesh alone: per tet, per
-    // face mask (bit K: the face opposite vertex K takes its circumcentre in
-    // the neighbour's vertex order, as the record's fcmask), per vertex VI, at
-    // [(t * 16 + mask) * 4 + VI]; NULL until built (computed inline then).
+    // vertex VI, per choice of vertex order of the three face circumcentres
+    // it uses (the face mask's bits of the other three vertices, as the
+    // record's fcmask), at [(t * 4 + VI) * 8 + v]; NULL until built
+    // (computed inline then).
     const double *old_vol;
 };
 
@@ -213,7 +214,7 @@ hipError_t launch_grad_geom(const DevMesh &m, int nblocks, const double *d_q, co
                             int n_items, unsigned int *d_work, const Topo &topo,
                             const GradIn &g, int *ovf_list, unsigned int *ovf_count,
                             hipStream_t st);
-// the step-3 table of DevMesh::old_vol (out: n_tets * 64 doubles)
+// the step-3 table of DevMesh::old_vol (out: n_tets * 32 doubles)
 hipError_t launch_old_vol(const DevMesh &m, double *out, hipStream_t st);
 // wave-per-query rows (cavities beyond the group kernel): per-wave scratch
 // of wave_scratch_size() bytes for each of nblocks one-wave blocks

@@ -41,6 +41,15 @@ __device__ __forceinline__ int4 ld_tet_v(const DevMesh &m, int t, bool &neg) {
     v.w &= 0x7fffffff;
     return v;
 }
+// the four vertices of a fat tet
+__device__ __forceinline__ void fat_points(const FatTet &f, d3 &p0, d3 &p1, d3 &p2, d3 &p3) {
+    const double4 a = f.c0, b = f.c1, c = f.c2;
+    p0 = {a.x, a.y, a.z};
+    p1 = {a.w, b.x, b.y};
+    p2 = {b.z, b.w, c.x};
+    p3 = {c.y, c.z, c.w};
+}
+
 __device__ __forceinline__ double seld(const d3 &v, int i) {
     return i == 0 ? v.x : (i == 1 ? v.y : v.z);
 }
@@ -505,14 +514,23 @@ __device__ __forceinline__ void old_contributions(const DevMesh &m, int t, unsig
 // tet for each of the 8 choices of vertex order of the three face
 // circumcentres it uses (the faces opposite the other three vertices a, b, c:
 // each in the tet's own ascending order or in the order of the neighbour
-// across it, as old_contributions picks with fcmask), stored under each of the
-// 16 masks so that a tet's four values are one 32-byte load.  Same functions,
-// same operands: each entry has the bits old_contributions would compute.
+// across it, as old_contributions picks with fcmask), at [t][VI][v] with v =
+// the fcmask bits of faces a, b, c as bits 0, 1, 2 (32 doubles, 256 bytes per
+// tet).  Same functions, same operands: each entry has the bits
+// old_contributions would compute.
 __device__ __forceinline__ d3 sel3d(bool s, const d3 &a, const d3 &b) {
     return {s ? a.x : b.x, s ? a.y : b.y, s ? a.z : b.z};
 }
 
-// the 8 values of vertex VI, stored for each of the 16 masks they serve
+// index of vertex vi's value for face mask `mask` (bit vi itself is ignored)
+__host__ __device__ __forceinline__ size_t old_vol_index(int t, unsigned mask, int vi) {
+    // the mask with bit vi removed: bits of the faces opposite the other three
+    // vertices, in ascending vertex order
+    const unsigned v = (mask & ((1u << vi) - 1u)) | ((mask >> (vi + 1)) << vi);
+    return static_cast<size_t>(t) * 32 + static_cast<size_t>(vi) * 8 + (v & 7u);
+}
+
+// the 8 values of vertex VI
 template <int VI>
 __device__ __forceinline__ void old_vol_vertex(const Quad &P, const Quad &Fo, const Quad &Fn,
                                                const d3 &ct, double *out) {
@@ -521,12 +539,8 @@ __device__ __forceinline__ void old_vol_vertex(const Quad &P, const Quad &Fo, co
         const d3 fa = sel3d(v & 1, Fn.get<a>(), Fo.get<a>());
         const d3 fb = sel3d(v & 2, Fn.get<b>(), Fo.get<b>());
         const d3 fc = sel3d(v & 4, Fn.get<c>(), Fo.get<c>());
-        const double vol =
+        out[VI * 8 + v] =
             subcell_volume(P.get<VI>(), P.get<a>(), P.get<b>(), P.get<c>(), ct, fc, fb, fa);
-        // masks whose bits a, b, c give v (bit VI free)
-        const unsigned base = ((v & 1) << a) | (((v >> 1) & 1) << b) | (((v >> 2) & 1) << c);
-        out[base * 4 + VI] = vol;
-        out[(base | (1u << VI)) * 4 + VI] = vol;
     }
 }
 
@@ -545,7 +559,7 @@ __global__ void __launch_bounds__(256) old_vol_kernel(DevMesh m, double *__restr
                      step3_face_cc<1>(m, P, tv, tn, tn.z >= 0 ? 2u : 0u),
                      step3_face_cc<2>(m, P, tv, tn, tn.y >= 0 ? 4u : 0u),
                      step3_face_cc<3>(m, P, tv, tn, tn.x >= 0 ? 8u : 0u)};
-    double *o = out + static_cast<size_t>(t) * 64;
+    double *o = out + static_cast<size_t>(t) * 32;
     old_vol_vertex<0>(P, Fo, Fn, ct, o);
     old_vol_vertex<1>(P, Fo, Fn, ct, o);
     old_vol_vertex<2>(P, Fo, Fn, ct, o);
@@ -557,16 +571,11 @@ __global__ void __launch_bounds__(256) old_vol_kernel(DevMesh m, double *__restr
 template <typename SlotOf, typename Add>
 __device__ __forceinline__ void old_contributions_table(const DevMesh &m, int t, unsigned fcmask,
                                                         SlotOf slot_of, Add add) {
-    const double4 v =
-        reinterpret_cast<const double4 *>(m.old_vol)[static_cast<size_t>(t) * 16 + (fcmask & 15u)];
-    int s = slot_of(0, -1);
-    if (s >= 0) add(s, v.x);
-    s = slot_of(1, -1);
-    if (s >= 0) add(s, v.y);
-    s = slot_of(2, -1);
-    if (s >= 0) add(s, v.z);
-    s = slot_of(3, -1);
-    if (s >= 0) add(s, v.w);
+#pragma unroll
+    for (int vi = 0; vi < 4; vi++) {
+        const int s = slot_of(vi, -1);
+        if (s >= 0) add(s, m.old_vol[old_vol_index(t, fcmask, vi)]);
+    }
 }
 
 // Derivative of a stolen volume: d sigma_k / dq = (1 / |pk - q|) times the
@@ -1227,15 +1236,6 @@ __device__ __forceinline__ unsigned adjacency(const QuadMem &g, int s, int j) {
         a |= d << (8 * k);
     }
     return a;
-}
-
-// the four vertices of a fat tet
-__device__ __forceinline__ void fat_points(const FatTet &f, d3 &p0, d3 &p1, d3 &p2, d3 &p3) {
-    const double4 a = f.c0, b = f.c1, c = f.c2;
-    p0 = {a.x, a.y, a.z};
-    p1 = {a.w, b.x, b.y};
-    p2 = {b.z, b.w, c.x};
-    p3 = {c.y, c.z, c.w};
 }
 
 // 64-thread blocks of 4 quads; 17 KB of LDS per block allow 9 per CU
@@ -2738,11 +2738,10 @@ __global__ void __launch_bounds__(kGeomRowsThreads) geom_rows_kernel(
                     if (g < 4) {
                         for (int ci = 0; ci < ncav; ci++) {
                             const int4 cr = r[1 + (ro[ci] & 63u)];
-                            const size_t base = (static_cast<size_t>(cr.x) * 16 +
-                                                 (static_cast<unsigned>(cr.w) & 15u)) * 4;
 #pragma unroll
                             for (int vi = g; vi < 4; vi += G) {
-                                const double v = m.old_vol[base + vi];
+                                const double v = m.old_vol[old_vol_index(
+                                    cr.x, static_cast<unsigned>(cr.w) & 15u, vi)];
                                 const int slot = rec_slot(cr, vi);
                                 if (slot >= 0) {
                                     acc[slot][rr] += v;
