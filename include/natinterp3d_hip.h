@@ -196,7 +196,8 @@ int nat_batch_weights_compute(nat_mesh *m, const double *d_queries,
  * (same M and offsets) and returns nnz; it returns -8 when a capacity was
  * exceeded (compute eagerly instead, or make a new graph), -9 when an
  * eager call on the mesh has reallocated its workspace since, and -11 when
- * nat_mesh_set_precision has changed the row arithmetic (mode or thresholds)
+ * nat_mesh_set_precision or nat_mesh_set_predicates has changed the row
+ * arithmetic (mode, thresholds or predicates)
  * since the capture: the captured kernels keep the capture's arithmetic, so
  * such a replay would not give the rows the mesh now promises.
  * nat_graph_copy_out copies the last replay's CSR to caller buffers.  The
@@ -284,6 +285,15 @@ typedef struct {
     int32_t grad_wave_items;
     double topo_search_ms;    /* level 0 topology: the cavity search (in topo_ms) */
     double topo_order_ms;     /* ... and the order rules (in topo_ms) */
+    int64_t robust_exact;     /* robust predicates (nat_mesh_set_predicates): tests the
+                                 static error bound left open, decided exactly */
+    int64_t robust_changed;   /* ... of which the exact sign differs from the FP64
+                                 sign the reference would have used */
+    int64_t robust_unresolved; /* ... too wide for the exact evaluator (FP64 sign kept) */
+    double mixed_err_max;     /* mixed precision: max |w_mixed - w_fp64| over the
+                                 sampled rows (nat_mesh_set_mixed_check) */
+    int64_t mixed_err_rows;   /* ... number of rows sampled (0: no statement) */
+    int64_t mixed_err_entries; /* ... and their entries */
 } nat_timings;
 
 int nat_mesh_set_timing(nat_mesh *m, int enable);
@@ -304,6 +314,28 @@ int nat_predicate_audit(nat_mesh *m, const void *d_topo, const double *d_queries
  * by the static error bound.  No GPU needed. */
 int nat_insphere_classify(const double *points, int n, double *det, int32_t *uncertain);
 
+/* Predicates of the query path (no reference counterpart; the reference's
+ * are the plain FP64 determinants, predicates.c:1687-1710 and 3301-3347,
+ * and its adaptive ones, predicates.c:3940-4265, are never called).  mode 0
+ * (default): the plain FP64 signs, rows bit-identical to the reference.
+ * mode 1 (robust): every insphere test of the cavity search and orient3d
+ * test of the walk whose FP64 sign the static error bound (Shewchuk's
+ * o3derrboundA / isperrboundA) does not certify is decided by the sign of
+ * the exact determinant instead; nat_timings counts those tests, the
+ * decisions that changed, and the tests whose exponent range is too wide
+ * for the exact evaluator (they keep the FP64 sign).  The weights stay FP64
+ * (or mixed) geometry over the robust cavity. */
+int nat_mesh_set_predicates(nat_mesh *m, int mode);
+
+/* Host evaluation of the robust decision for n tuples (no GPU needed):
+ * kind 0 = orient3d(a, b, c, d), 12 doubles per tuple; kind 1 =
+ * insphere(a, b, c, d, e), 15 doubles.  sign[i] = the sign of the exact
+ * determinant (-1, 0, 1), or 2 when its exponent range is too wide;
+ * uncertain[i] = 1 when the static error bound does not certify the FP64
+ * sign (the tests the device evaluates exactly). */
+int nat_predicate_exact(const double *points, int n, int kind, int32_t *sign,
+                        int32_t *uncertain);
+
 /* Row arithmetic of a mesh (no reference counterpart).  mode 0 (default):
  * FP64 with the reference's operation order, weights bit-identical.  mode 1:
  * mixed -- the cavity, boundary and neighbour order still come from the FP64
@@ -319,6 +351,15 @@ int nat_insphere_classify(const double *points, int n, double *det, int32_t *unc
 int nat_mesh_set_precision(nat_mesh *m, int mode, double cond_threshold,
                            double cancel_threshold);
 int nat_mesh_get_timings(const nat_mesh *m, nat_timings *out);
+
+/* Error statement of the mixed rows (no reference counterpart): every
+ * stride-th query, in the spatial order the queries are processed in, is
+ * computed again in FP64 from the same topology after each mixed
+ * nat_weights_compute / nat_get_weights, and nat_timings reports the largest
+ * |w_mixed - w_fp64| over those rows (mixed_err_max) with the number of rows
+ * and entries it covers.  stride 0 turns it off; default 1024.  Graph
+ * replays and nat_weights_noalloc make no statement. */
+int nat_mesh_set_mixed_check(nat_mesh *m, int stride);
 
 /* ---- row-path policy and its autotuner (no reference counterpart).
  * Which kernel makes a row changes speed, never the row: every policy gives

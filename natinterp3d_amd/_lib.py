@@ -22,7 +22,8 @@ EXPORTS = (
     "nat_graph_create", "nat_graph_launch", "nat_graph_copy_out", "nat_graph_free",
     "nat_mesh_set_policy", "nat_mesh_get_policy", "nat_mesh_autotune",
     "nat_mesh_from_packed", "nat_weights_row_order", "nat_topology_target",
-    "nat_weights_spmm", "nat_weights_noalloc",
+    "nat_weights_spmm", "nat_weights_noalloc", "nat_mesh_set_predicates",
+    "nat_predicate_exact", "nat_mesh_set_mixed_check",
 )
 
 
@@ -38,6 +39,9 @@ class Timings(ctypes.Structure):
         ("wave_ms", ctypes.c_double), ("grad_wave_ms", ctypes.c_double),
         ("wave_items", ctypes.c_int32), ("grad_wave_items", ctypes.c_int32),
         ("topo_search_ms", ctypes.c_double), ("topo_order_ms", ctypes.c_double),
+        ("robust_exact", ctypes.c_int64), ("robust_changed", ctypes.c_int64),
+        ("robust_unresolved", ctypes.c_int64), ("mixed_err_max", ctypes.c_double),
+        ("mixed_err_rows", ctypes.c_int64), ("mixed_err_entries", ctypes.c_int64),
     ]
 
 
@@ -146,6 +150,12 @@ def lib():
     L.nat_insphere_classify.argtypes = [dp, ctypes.c_int, dp, ip]
     L.nat_mesh_set_precision.restype = ctypes.c_int
     L.nat_mesh_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    L.nat_mesh_set_predicates.restype = ctypes.c_int
+    L.nat_mesh_set_predicates.argtypes = [vp, ctypes.c_int]
+    L.nat_mesh_set_mixed_check.restype = ctypes.c_int
+    L.nat_mesh_set_mixed_check.argtypes = [vp, ctypes.c_int]
+    L.nat_predicate_exact.restype = ctypes.c_int
+    L.nat_predicate_exact.argtypes = [dp, ctypes.c_int, ctypes.c_int, ip, ip]
     L.nat_mesh_from_packed.restype = ctypes.c_int
     L.nat_mesh_from_packed.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, vp, vp, vp, vp,
                                        vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]

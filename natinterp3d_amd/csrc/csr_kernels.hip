@@ -126,6 +126,92 @@ namespace {
 size_t rnd256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 }  // namespace
 
+// Error statement of mixed rows, step 1: the sampled queries (every
+// stride-th of the processing order) and their mixed rows' pool offset,
+// length and total, kept aside while the FP64 pass rewrites them.
+__global__ void mixed_check_save_kernel(const int *__restrict__ order, int n, int stride,
+                                        const long long *__restrict__ row_off,
+                                        const int *__restrict__ row_cnt,
+                                        const double *__restrict__ row_total,
+                                        int *__restrict__ items, long long *__restrict__ s_off,
+                                        int *__restrict__ s_cnt, double *__restrict__ s_total) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int qi = order[static_cast<size_t>(i) * stride];
+    items[i] = qi;
+    s_off[i] = row_off[qi];
+    s_cnt[i] = row_cnt[qi];
+    s_total[i] = row_total[qi];
+}
+
+// Step 2, after the FP64 pass over items: per sampled row with a record, the
+// largest |w_mixed - w_fp64| (columns must agree; a differing column or
+// length counts as an infinite error), then the mixed row's pool offset,
+// length and total are put back.  out[0] = max error (bits of a
+// non-negative double, so unsigned order is numeric order), out[1] = rows,
+// out[2] = entries.
+__global__ void mixed_check_compare_kernel(int n, const int *__restrict__ items,
+                                           const long long *__restrict__ rec_off,
+                                           long long *__restrict__ row_off,
+                                           int *__restrict__ row_cnt,
+                                           double *__restrict__ row_total,
+                                           const long long *__restrict__ s_off,
+                                           const int *__restrict__ s_cnt,
+                                           const double *__restrict__ s_total,
+                                           const double *__restrict__ pool_vals,
+                                           const int *__restrict__ pool_cols, long long pool_cap,
+                                           unsigned long long *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int qi = items[i];
+    if (rec_off[qi] >= 0) {
+        const long long o64 = row_off[qi], o32 = s_off[i];
+        const int c64 = row_cnt[qi], c32 = s_cnt[i];
+        if (o64 >= 0 && o32 >= 0 && o64 + c64 <= pool_cap && o32 + c32 <= pool_cap) {
+            double err = 0.0;
+            if (c64 != c32) {
+                err = __longlong_as_double(0x7ff0000000000000ll);
+            } else {
+                for (int j = 0; j < c64; j++) {
+                    if (pool_cols[o64 + j] != pool_cols[o32 + j])
+                        err = __longlong_as_double(0x7ff0000000000000ll);
+                    else
+                        err = fmax(err, fabs(pool_vals[o64 + j] - pool_vals[o32 + j]));
+                }
+            }
+            atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(err)));
+            atomicAdd(out + 1, 1ull);
+            atomicAdd(out + 2, static_cast<unsigned long long>(c32));
+        }
+    }
+    row_off[qi] = s_off[i];
+    row_cnt[qi] = s_cnt[i];
+    row_total[qi] = s_total[i];
+}
+
+hipError_t launch_mixed_check_save(const int *d_order, int n, int stride,
+                                   const long long *row_off, const int *row_cnt,
+                                   const double *row_total, int *items, long long *s_off,
+                                   int *s_cnt, double *s_total, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mixed_check_save_kernel, dim3((n + 255) / 256), dim3(256), 0, st, d_order,
+                       n, stride, row_off, row_cnt, row_total, items, s_off, s_cnt, s_total);
+    return hipGetLastError();
+}
+
+hipError_t launch_mixed_check_compare(int n, const int *items, const long long *rec_off,
+                                      long long *row_off, int *row_cnt, double *row_total,
+                                      const long long *s_off, const int *s_cnt,
+                                      const double *s_total, const double *pool_vals,
+                                      const int *pool_cols, long long pool_cap,
+                                      unsigned long long *out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mixed_check_compare_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                       items, rec_off, row_off, row_cnt, row_total, s_off, s_cnt, s_total,
+                       pool_vals, pool_cols, pool_cap, out);
+    return hipGetLastError();
+}
+
 size_t morton_sort_bytes(int M) {
     const size_t Mz = static_cast<size_t>(M > 0 ? M : 1);
     const size_t nt = (Mz + kSortTile - 1) / kSortTile;

@@ -17,6 +17,7 @@
 #include <sys/mman.h>
 #include <vector>
 
+#include "exact.h"
 #include "geom.h"
 #include "mesh_builder.h"
 #include "nat_internal.h"
@@ -165,6 +166,10 @@ struct nat_mesh {
     int precision = 0;
     float mixed_tau = 0.02f;
     float mixed_kappa = 20.0f;
+    // error statement of mixed rows: every mixed_check-th query again in
+    // FP64 (0 = off); chk holds the sampled rows' items and saved rows
+    int mixed_check = 1024;
+    DevBuf chk;
 
     // optional kernel timing (HIP events on the launch stream)
     bool timing = false;
@@ -185,7 +190,7 @@ struct nat_mesh {
                    ovf[1].bytes + counters.bytes + pool_vals.bytes + pool_cols.bytes +
                    csr_vals.bytes + csr_cols.bytes + csr_rowptr.bytes +
                    rec.bytes + rec_off.bytes + start.bytes + wscr.bytes + vals_dev.bytes +
-                   vals_fwd.bytes;
+                   vals_fwd.bytes + chk.bytes;
         for (int l = 0; l < kLevels; l++) s += scratch[l].bytes;
         return static_cast<int64_t>(s);
     }
@@ -196,7 +201,7 @@ struct nat_mesh {
                          &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,   &csr_rowptr,
                          &rec,      &rec_off,   &start,     &lp,       &prob,       &qoff,
                          &route,    &wscr,      &old_vol,   &vorig,    &vals_dev,
-                         &vals_fwd};
+                         &vals_fwd, &chk};
         for (DevBuf *b : all) b->release();
         for (int l = 0; l < kLevels; l++) scratch[l].release();
         for (hipEvent_t &e : ev) {
@@ -635,6 +640,48 @@ long long forged_entries() {
     return (end && *end == '\0' && v > 0) ? v : 0;
 }
 
+// Error statement of a mixed compute whose rows are all in the pool: the
+// sampled rows (every m->mixed_check-th query of the processing order) are
+// made again by the FP64 geometry kernel from the same records, behind the
+// compute's entries in the pool, compared with the mixed rows, and the mixed
+// rows' offsets are put back.
+int mixed_check(nat_mesh *m, const double *d_q, int M, int n_check, const nat::Topo &topo,
+                nat::RowOut out, unsigned *ctr, hipStream_t st) {
+    const size_t n = static_cast<size_t>(n_check);
+    const size_t a = (n + 63) & ~static_cast<size_t>(63);
+    HIP_OR_FAIL(m->chk.ensure(a * (4 + 4 + 8 + 8)));
+    int *items = m->chk.as<int>();
+    int *s_cnt = items + a;
+    long long *s_off = reinterpret_cast<long long *>(s_cnt + a);
+    double *s_total = reinterpret_cast<double *>(s_off + a);
+    unsigned long long *res = reinterpret_cast<unsigned long long *>(ctr + 54);
+    HIP_OR_FAIL(nat::launch_zero_u32(ctr + 40, 8, st));
+    HIP_OR_FAIL(nat::launch_zero_u32(res, 6, st));
+    HIP_OR_FAIL(nat::launch_mixed_check_save(m->order.as<int>(), n_check, m->mixed_check,
+                                             out.row_off, out.row_cnt, out.row_total, items,
+                                             s_off, s_cnt, s_total, st));
+    nat::Topo t = topo;
+    t.rec_used = nullptr;
+    HIP_OR_FAIL(nat::launch_geom(m->dev, m->nthr_geom / nat::kGeomThreads * m->geom_grid,
+                                 m->geom_lanes == 2 ? m->geom_rows_blocks * m->geom_grid : 0,
+                                 d_q, items, n_check, ctr + 40, t, out, 0.0f, 0.0f, nullptr,
+                                 st));
+    HIP_OR_FAIL(nat::launch_mixed_check_compare(n_check, items, topo.rec_off, out.row_off,
+                                                out.row_cnt, out.row_total, s_off, s_cnt,
+                                                s_total, out.pool_vals, out.pool_cols,
+                                                out.pool_cap, res, st));
+    unsigned long long h[3] = {0, 0, 0};
+    HIP_OR_FAIL(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    double err;
+    std::memcpy(&err, &h[0], sizeof(err));
+    m->times.mixed_err_max = err;
+    m->times.mixed_err_rows = static_cast<int64_t>(h[1]);
+    m->times.mixed_err_entries = static_cast<int64_t>(h[2]);
+    (void)M;
+    return 0;
+}
+
 int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipStream_t st,
                  long long *nnz_out) {
     m->last_M = -1;
@@ -680,8 +727,14 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
     HIP_OR_FAIL(m->ovf[0].ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->ovf[1].ensure(sizeof(int) * Mz));
     HIP_OR_FAIL(m->counters.ensure(256));
+    m->dev.robust_ctr = reinterpret_cast<unsigned long long *>(m->counters.as<unsigned>() + 48);
     int nthr0 = 0;
     if (int rc = ensure_level_scratch(m, 0, M, &nthr0)) return rc;
+    // mixed rows' error statement: the sampled rows are made again in FP64
+    // behind the compute's own entries in the pool
+    const int n_check = m->precision == 1 && m->mixed_check > 0
+                            ? static_cast<int>((Mz + m->mixed_check - 1) / m->mixed_check)
+                            : 0;
 
     m->times = nat_timings{};
     mark(m, 0, st);
@@ -692,7 +745,9 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
 
     HIP_OR_FAIL(m->rec_off.ensure(sizeof(long long) * Mz));
     HIP_OR_FAIL(m->start.ensure(sizeof(int) * Mz));
-    long long want = static_cast<long long>(std::ceil(m->nnz_per_row * M)) + 1024;
+    const long long check_room =
+        n_check ? static_cast<long long>(std::ceil(2.0 * m->nnz_per_row * n_check)) + 1024 : 0;
+    long long want = static_cast<long long>(std::ceil(m->nnz_per_row * M)) + 1024 + check_room;
     long long want_rec = static_cast<long long>(std::ceil(m->units_per_row * M)) + 1024;
     for (int attempt = 0; attempt < 4; attempt++) {
         m->times.levels = 0;
@@ -709,7 +764,8 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         // overflow counts, [12] wave-level overflow, u64 pool use at [16],
         // records at [18], mixed-path FP32 / FP64 counts at [20], [22], rows
         // redone at [24]; per-XCD work parts of the topology kernel at
-        // [32..39], of the geometry kernel at [40..47]
+        // [32..39], of the geometry kernel at [40..47]; u64 robust-predicate
+        // counts at [48], [50], [52]; u64 mixed error statement at [54..59]
         unsigned *ctr = m->counters.as<unsigned>();
         unsigned long long *pool_used = reinterpret_cast<unsigned long long *>(ctr + 16);
         unsigned long long *rec_used = reinterpret_cast<unsigned long long *>(ctr + 18);
@@ -832,12 +888,19 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         // the mixed path
         unsigned long long tail[5] = {0, 0, 0, 0, 0};
         HIP_OR_FAIL(hipMemcpyAsync(tail, pool_used, sizeof(tail), hipMemcpyDeviceToHost, st));
+        unsigned long long rob[3] = {0, 0, 0};
+        HIP_OR_FAIL(hipMemcpyAsync(rob, ctr + 48, sizeof(rob), hipMemcpyDeviceToHost, st));
         HIP_OR_FAIL(hipStreamSynchronize(st));
         const unsigned long long used = tail[0];
         m->times.mixed_fp32 = static_cast<int64_t>(tail[2]);
         m->times.mixed_fp64 = static_cast<int64_t>(tail[3]);
         m->times.mixed_rows_fp64 = static_cast<int64_t>(tail[4]);
+        m->times.robust_exact = static_cast<int64_t>(rob[0]);
+        m->times.robust_changed = static_cast<int64_t>(rob[1]);
+        m->times.robust_unresolved = static_cast<int64_t>(rob[2]);
         if (static_cast<long long>(used) <= m->pool_cap) {
+            if (n_check > 0 && static_cast<long long>(used) + check_room <= m->pool_cap)
+                if (int rc = mixed_check(m, d_q, M, n_check, topo, out, ctr, st)) return rc;
             m->last_M = M;
             m->ext_written = ext_base;
             m->last_nnz = static_cast<long long>(used);
@@ -845,7 +908,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
             *nnz_out = m->reported_nnz();
             return 0;
         }
-        want = static_cast<long long>(used) + 1024;
+        want = static_cast<long long>(used) + 1024 + check_room;
     }
     return fail(-5, "row pool did not converge");
 }
@@ -887,6 +950,7 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
     // of the record kernel
     unsigned *ctr = m->counters.as<unsigned>();
     HIP_OR_FAIL(hipMemsetAsync(m->counters.p, 0, 256, st));
+    m->dev.robust_ctr = reinterpret_cast<unsigned long long *>(ctr + 48);
     nat::Topo topo;
     topo.rec = reinterpret_cast<int4 *>(const_cast<char *>(blob) + topo_rec_off(M));
     topo.rec_cap = hdr[2];
@@ -1175,6 +1239,7 @@ struct nat_graph {
     // thresholds): a replay after nat_mesh_set_precision changed them is refused
     int precision = 0;
     float mixed_tau = 0.f, mixed_kappa = 0.f;
+    int robust = 0;
 };
 
 namespace {
@@ -1869,6 +1934,7 @@ int nat_graph_create(nat_mesh *m, const double *d_queries, int num_query_points,
     g->precision = m->precision;
     g->mixed_tau = m->mixed_tau;
     g->mixed_kappa = m->mixed_kappa;
+    g->robust = m->dev.robust;
     if (hipStreamCreate(&g->st) != hipSuccess) return bail(fail(-10, "stream creation failed"));
     if (hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return bail(fail(-10, "hipStreamBeginCapture failed"));
@@ -1900,6 +1966,10 @@ int nat_graph_launch(nat_graph *g, void *stream, int64_t *nnz_out) {
         return fail(-11, "the mesh's precision changed since the graph was made (captured %s, "
                          "now %s)", g->precision ? "mixed" : "fp64",
                     m->precision ? "mixed" : "fp64");
+    if (m->dev.robust != g->robust)
+        return fail(-11, "the mesh's predicates changed since the graph was made (captured %s, "
+                         "now %s)", g->robust ? "robust" : "fp64",
+                    m->dev.robust ? "robust" : "fp64");
     hipStream_t st = static_cast<hipStream_t>(stream);
     HIP_OR_FAIL(hipGraphLaunch(g->exec, st));
     unsigned h[20];
@@ -2054,6 +2124,46 @@ int nat_mesh_set_precision(nat_mesh *m, int mode, double cond_threshold,
     if (mode == 1) {
         m->mixed_tau = static_cast<float>(cond_threshold);
         m->mixed_kappa = static_cast<float>(cancel_threshold);
+    }
+    return 0;
+}
+
+int nat_mesh_set_predicates(nat_mesh *m, int mode) {
+    g_err.clear();
+    if (!m) return fail(-1, "NULL mesh");
+    if (mode != 0 && mode != 1) return fail(-1, "predicates mode %d (0 = fp64, 1 = robust)", mode);
+    std::lock_guard<std::mutex> lk(m->mu);
+    m->dev.robust = mode;
+    return 0;
+}
+
+int nat_mesh_set_mixed_check(nat_mesh *m, int stride) {
+    g_err.clear();
+    if (!m) return fail(-1, "NULL mesh");
+    if (stride < 0) return fail(-1, "mixed check stride %d below 0", stride);
+    std::lock_guard<std::mutex> lk(m->mu);
+    m->mixed_check = stride;
+    return 0;
+}
+
+int nat_predicate_exact(const double *points, int n, int kind, int32_t *sign,
+                        int32_t *uncertain) {
+    g_err.clear();
+    if (n < 0 || (kind != 0 && kind != 1) || (n > 0 && (!points || !sign || !uncertain)))
+        return fail(-1, "bad arguments");
+    const int w = kind == 0 ? 12 : 15;
+    for (int i = 0; i < n; i++) {
+        const double *p = points + static_cast<size_t>(w) * i;
+        const nat::d3 a = {p[0], p[1], p[2]}, b = {p[3], p[4], p[5]}, c = {p[6], p[7], p[8]};
+        const nat::d3 d = {p[9], p[10], p[11]};
+        if (kind == 0) {
+            sign[i] = nat::orient3d_exact_sign(a, b, c, d);
+            uncertain[i] = nat::orient3d_uncertain(a, b, c, d) ? 1 : 0;
+        } else {
+            const nat::d3 e = {p[12], p[13], p[14]};
+            sign[i] = nat::insphere_exact_sign(a, b, c, d, e);
+            uncertain[i] = nat::insphere_uncertain(a, b, c, d, e) ? 1 : 0;
+        }
     }
     return 0;
 }
@@ -2376,6 +2486,7 @@ int noalloc_rows(nat_mesh *m, char *base, const double *d_q, int M, double *d_va
     HIP_OR_FAIL(nat::launch_zero_u32(ctr, 64, st));
     nat::DevMesh d = m->dev;  // the step-3 table is used when an earlier compute built it
     d.prob = nullptr;
+    d.robust_ctr = reinterpret_cast<unsigned long long *>(ctr + 48);
     HIP_OR_FAIL(nat::launch_morton_sort(d_q, M, m->lp.as<nat::LocParams>(), nullptr, 1, bits,
                                         order, sortw, st));
     nat::RowOut out{};

@@ -72,6 +72,12 @@ struct DevMesh {
     // record's fcmask), at [(t * 4 + VI) * 8 + v]; NULL until built
     // (computed inline then).
     const double *old_vol;
+    // Predicates of the query path: 0 = the plain FP64 signs, 1 = FP64 where
+    // the static error bound certifies the sign, else the exact sign.  With
+    // 1, robust_ctr[0..2] count exact evaluations, decisions that differ
+    // from the FP64 sign, and tests too wide for the exact evaluator.
+    int robust;
+    unsigned long long *robust_ctr;
 };
 
 __device__ __forceinline__ unsigned route_of(const DevMesh &m, int qi) {
@@ -238,6 +244,19 @@ hipError_t launch_predicate_audit(const DevMesh &m, const double *d_q, int n, co
 hipError_t launch_spmm_t(int M, int N, int D, const int *rowptr, const int *cols,
                          const double *vals, const double *gy, const int *order, double *out,
                          hipStream_t st);
+// error statement of mixed rows: save the sampled rows (order[i * stride],
+// i < n) aside, then, after an FP64 geometry pass over them, compare and put
+// the mixed rows back; out[0..2] = max |dw| bits, rows, entries
+hipError_t launch_mixed_check_save(const int *d_order, int n, int stride,
+                                   const long long *row_off, const int *row_cnt,
+                                   const double *row_total, int *items, long long *s_off,
+                                   int *s_cnt, double *s_total, hipStream_t st);
+hipError_t launch_mixed_check_compare(int n, const int *items, const long long *rec_off,
+                                      long long *row_off, int *row_cnt, double *row_total,
+                                      const long long *s_off, const int *s_cnt,
+                                      const double *s_total, const double *pool_vals,
+                                      const int *pool_cols, long long pool_cap,
+                                      unsigned long long *out, hipStream_t st);
 // exclusive scan of n int32 counts into int64 offsets; total in out[n]
 hipError_t launch_scan_counts(const int *d_in, long long *d_out, int n, long long *d_tmp,
                               hipStream_t st);

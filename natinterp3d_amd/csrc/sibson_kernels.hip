@@ -20,6 +20,7 @@
 // with -ffp-contract=off.
 #include <hip/hip_runtime.h>
 
+#include "exact.h"
 #include "geom.h"
 #include "geom32.h"
 #include "nat_internal.h"
@@ -235,8 +236,59 @@ __device__ int grid_nearest(const DevMesh &m, const LocParams &L, const d3 &q) {
     return -1;
 }
 
+// The two decisions of the query path.  ROBUST = false: the reference's plain
+// FP64 signs.  ROBUST = true (opt-in): the FP64 sign where the static error
+// bound certifies it, else the exact sign (exact.h); m.robust_ctr counts the
+// exact evaluations [0], the decisions that differ from the FP64 sign [1] and
+// the tests too wide for the exact evaluator, which keep the FP64 sign [2].
+__device__ __noinline__ bool robust_inside(const DevMesh &m, const d3 &a, const d3 &b,
+                                           const d3 &c, const d3 &d, const d3 &e, double fast) {
+    const int sg = insphere_exact_sign(a, b, c, d, e);
+    unsigned long long *ctr = m.robust_ctr;
+    if (ctr) atomicAdd(&ctr[0], 1ull);
+    if (sg == 2) {
+        if (ctr) atomicAdd(&ctr[2], 1ull);
+        return fast > 0;
+    }
+    if (ctr && (sg > 0) != (fast > 0)) atomicAdd(&ctr[1], 1ull);
+    return sg > 0;
+}
+__device__ __noinline__ bool robust_separated(const DevMesh &m, const d3 &a, const d3 &b,
+                                              const d3 &c, const d3 &q, double fast) {
+    const int sg = orient3d_exact_sign(a, b, c, q);
+    unsigned long long *ctr = m.robust_ctr;
+    if (ctr) atomicAdd(&ctr[0], 1ull);
+    if (sg == 2) {
+        if (ctr) atomicAdd(&ctr[2], 1ull);
+        return fast < 0;
+    }
+    if (ctr && (sg < 0) != (fast < 0)) atomicAdd(&ctr[1], 1ull);
+    return sg < 0;
+}
+// e inside the circumsphere of the positively oriented (a, b, c, d)
+template <bool ROBUST>
+__device__ __forceinline__ bool inside_sphere(const DevMesh &m, const d3 &a, const d3 &b,
+                                              const d3 &c, const d3 &d, const d3 &e) {
+    const double ins = insphere(a, b, c, d, e);
+    if constexpr (ROBUST) {
+        if (insphere_uncertain(a, b, c, d, e)) return robust_inside(m, a, b, c, d, e, ins);
+    }
+    return ins > 0;
+}
+// q on the outer side of face (a, b, c)
+template <bool ROBUST>
+__device__ __forceinline__ bool separated(const DevMesh &m, const d3 &a, const d3 &b,
+                                          const d3 &c, const d3 &q) {
+    const double o = orient3d(a, b, c, q);
+    if constexpr (ROBUST) {
+        if (orient3d_uncertain(a, b, c, q)) return robust_separated(m, a, b, c, q, o);
+    }
+    return o < 0;
+}
+
 // findContainingSimplex (delaunay.c:383-407): leave through the first face
 // whose plane separates the query, restart the face loop after each move.
+template <bool ROBUST>
 __device__ int walk(const DevMesh &m, int s, const d3 &q, bool &ovf) {
     bool neg;
     int4 tv = ld_tet_v(m, s, neg);
@@ -246,7 +298,7 @@ __device__ int walk(const DevMesh &m, int s, const d3 &q, bool &ovf) {
         const d3 a = ld_vert(m, sel4(tv, face_slot(i, 0)));
         const d3 b = ld_vert(m, sel4(tv, face_slot(i, 1)));
         const d3 c = ld_vert(m, sel4(tv, face_slot(i, 2)));
-        if (orient3d(a, b, c, q) < 0) {
+        if (separated<ROBUST>(m, a, b, c, q)) {
             const int nb = sel4(m.tet_n[s], i);
             if (nb < 0) return -1;
             s = nb;
@@ -264,12 +316,13 @@ __device__ int walk(const DevMesh &m, int s, const d3 &q, bool &ovf) {
 }
 
 // Start tet of the cavity search (findContainingSimplex with the k-d seed).
+template <bool ROBUST>
 __device__ int locate(const DevMesh &m, const LocParams &L, const d3 &q, const Scratch &sc,
                       int tid, const Caps &caps, bool &ovf) {
     int s = grid_nearest(m, L, q);
     if (s < 0) s = kd_nearest(m, L, q, sc, tid, caps.kd, ovf);
     if (ovf || s < 0) return -1;
-    return walk(m, s, q, ovf);
+    return walk<ROBUST>(m, s, q, ovf);
 }
 
 // ----------------------------------------------------------- hash tables
@@ -331,6 +384,7 @@ struct HashTab {
 // a tet is marked visited when popped, pushed neighbours are those not yet
 // visited at push time.  Returns the cavity size (tets in sc.cav in cavity
 // order, cavity positions in the visited table) or -1 on overflow.
+template <bool ROBUST>
 __device__ int cavity_search(const DevMesh &m, const Caps &caps, const Scratch &sc, int tid,
                              const HashTab &vis, int start, const d3 &q) {
     const size_t S = static_cast<size_t>(sc.nthr);
@@ -348,8 +402,9 @@ __device__ int cavity_search(const DevMesh &m, const Caps &caps, const Scratch &
         const int4 tv = ld_tet_v(m, cur, neg);
         const d3 p0 = ld_vert(m, tv.x), p1 = ld_vert(m, tv.y);
         const d3 p2 = ld_vert(m, tv.z), p3 = ld_vert(m, tv.w);
-        const double ins = neg ? insphere(p1, p0, p2, p3, q) : insphere(p0, p1, p2, p3, q);
-        if (ins > 0) {
+        const bool ins = neg ? inside_sphere<ROBUST>(m, p1, p0, p2, p3, q)
+                             : inside_sphere<ROBUST>(m, p0, p1, p2, p3, q);
+        if (ins) {
             if (ncav >= caps.cav) return -1;
             sc.cav[il(ncav, S, tid)] = cur;
             vis.set(h, ncav);
@@ -862,18 +917,18 @@ __device__ __forceinline__ bool next_batch(unsigned int *work, int n_items, int 
 // row length (row in nb_idx / nb_vol) or -1 when a capacity was exceeded.
 // GRAD: the same cavity and volumes, but the row is reduced to dL/dq of
 // query qi (written to *dq; nb_vol then holds <dL/dy, values[j]> per slot).
-template <bool GRAD = false>
+template <bool GRAD, bool ROBUST>
 __device__ int fused_row(const DevMesh &m, const Caps &caps, const Scratch &sc, int tid,
                          int stamp, const d3 &q, int qi, const GradIn *g = nullptr,
                          d3 *dq = nullptr) {
     const size_t S = static_cast<size_t>(sc.nthr);
     const int N = m.n_data;
     bool ovf = false;
-    const int start = locate(m, loc_of(m, qi), q, sc, tid, caps, ovf);
+    const int start = locate<ROBUST>(m, loc_of(m, qi), q, sc, tid, caps, ovf);
     if (ovf) return -1;
     if (start < 0) return 0;  // outside the super simplex: empty row
     const HashTab vis(sc.vis, sc.vis_pos, tid, caps.vis_log2, stamp);
-    const int ncav = cavity_search(m, caps, sc, tid, vis, start, q);
+    const int ncav = cavity_search<ROBUST>(m, caps, sc, tid, vis, start, q);
     if (ncav < 0) return -1;
     if (GRAD) *dq = {0.0, 0.0, 0.0};  // empty and coincident rows
     if (ncav == 0) {
@@ -973,6 +1028,7 @@ __device__ int fused_row(const DevMesh &m, const Caps &caps, const Scratch &sc, 
 // n_items_dev, when given, holds the item count (written by an earlier kernel
 // of the same stream: the graph form of the launch sequence).  Queries whose
 // problem starts its thread-per-query rows at a later level pass straight on.
+template <bool ROBUST>
 __global__ void __launch_bounds__(256) sibson_rows_kernel(DevMesh m, Caps caps, Scratch sc,
                                                           int level,
                                                           const double *__restrict__ q,
@@ -996,7 +1052,7 @@ __global__ void __launch_bounds__(256) sibson_rows_kernel(DevMesh m, Caps caps, 
                 epoch = epoch == 0x7ffffffe ? 1 : epoch + 1;
                 const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                                q[3 * static_cast<size_t>(qi) + 2]};
-                cnt = fused_row(m, caps, sc, tid, epoch, qq, qi);
+                cnt = fused_row<false, ROBUST>(m, caps, sc, tid, epoch, qq, qi);
             }
             if (cnt < 0) {
                 out.ovf_list[atomicAdd(out.ovf_count, 1u)] = qi;
@@ -1021,6 +1077,7 @@ __global__ void __launch_bounds__(256) sibson_rows_kernel(DevMesh m, Caps caps, 
 // Gradient of the rows of the given queries (those without a topology
 // record): the fused row with the gradient reduction.  Queries that exceed
 // the level's capacities go to ovf_list for the next level.
+template <bool ROBUST>
 __global__ void __launch_bounds__(256) sibson_grad_rows_kernel(
     DevMesh m, Caps caps, Scratch sc, int level, const double *__restrict__ q,
     const int *__restrict__ items, int n_items, unsigned int *work, GradIn g, int *ovf_list,
@@ -1040,7 +1097,7 @@ __global__ void __launch_bounds__(256) sibson_grad_rows_kernel(
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
         d3 dq = {0.0, 0.0, 0.0};  // rows outside the super simplex return before setting it
-        if (fused_row<true>(m, caps, sc, tid, epoch, qq, qi, &g, &dq) < 0)
+        if (fused_row<true, ROBUST>(m, caps, sc, tid, epoch, qq, qi, &g, &dq) < 0)
             ovf_list[atomicAdd(ovf_count, 1u)] = qi;
         else
             store_dq(g, qi, dq);
@@ -1048,23 +1105,14 @@ __global__ void __launch_bounds__(256) sibson_grad_rows_kernel(
     sc.epoch[tid] = epoch;
 }
 
-// ----------------------------------------------------------- topology kernel
+// ------------------------------------------------------------- start tets
 
-// Record of one query, in 16-byte units at topo.rec[rec_off[qi]]:
-//   [0]            (C, B, NB, 0): cavity tets, boundary faces, neighbours
-//   [1 .. C]       per cavity tet: (tet, slot(p0) | slot(p1) << 16,
-//                  slot(p2) | slot(p3) << 16, fcmask) -- slot 0xffff = none;
-//                  fcmask bit k: face opposite p[k] is held by an earlier
-//                  cavity tet (its circumcentre uses that tet's vertex order)
-//   next ceil(B/4) boundary faces, cavity position << 2 | face, 4 per unit
-//   next ceil(NB/4) neighbour vertex indices in slot order, 4 per unit
-// C = 0 rows: NB = vertices coinciding with the query (weight 1/NB each),
-// or NB = 0 for an empty row.
 // Start tet of every work item: exact nearest centroid (grid; the k-d
 // replica when the grid cannot certify it), then the visibility walk.
 // start[qi] = tet, -1 outside the super simplex, -2 k-d stack overflow.
 // 8 waves per SIMD: the search is latency-bound and uses 32 VGPRs; without
 // the hint the compiler takes 112 SGPRs, which allow only 7 (800 per SIMD).
+template <bool ROBUST>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 locate_kernel(DevMesh m, Caps caps, Scratch sc,
                                                      const double *__restrict__ q,
@@ -1089,7 +1137,7 @@ locate_kernel(DevMesh m, Caps caps, Scratch sc,
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
         bool ovf = false;
-        const int s = locate(m, loc_of(m, qi), qq, sc, tid, caps, ovf);
+        const int s = locate<ROBUST>(m, loc_of(m, qi), qq, sc, tid, caps, ovf);
         start[qi] = ovf ? -2 : s;
     }
 }
@@ -1239,6 +1287,7 @@ __device__ __forceinline__ unsigned adjacency(const QuadMem &g, int s, int j) {
 }
 
 // 64-thread blocks of 4 quads; 17 KB of LDS per block allow 9 per CU
+template <bool ROBUST>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict__ items,
                    int n_items, const int *__restrict__ start_of, unsigned int *work, Topo topo,
@@ -1340,7 +1389,8 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
                         unsigned in4 = 0;
 #pragma unroll
                         for (int j = 0; j < grp::Q; j++)
-                            if (act[j] && insphere(p0, p1, p2, p3, qq[j]) > 0) in4 |= 1u << j;
+                            if (act[j] && inside_sphere<ROBUST>(m, p0, p1, p2, p3, qq[j]))
+                                in4 |= 1u << j;
                         unsigned packed = 0xffffffffu;
                         if (in4) {
                             packed = 0;
@@ -1977,7 +2027,7 @@ static_assert(sizeof(uint2) * wv::CMAX >= sizeof(int) * wv::NMAX,
 // neighbours (W.nbv), all in the reference's order.  ORDERED = false (the
 // gradient, whose sums do not depend on order): the cavity in search order,
 // and neighbour slots are the neighbour table's own slots (NB = its size).
-template <bool ORDERED>
+template <bool ORDERED, bool ROBUST>
 __device__ bool wave_topology(const DevMesh &m, const WaveScr &W, WaveLds &L, int st,
                               const d3 &qq, int &C, int &B, int &NB) {
     using namespace wv;
@@ -2037,9 +2087,9 @@ __device__ bool wave_topology(const DevMesh &m, const WaveScr &W, WaveLds &L, in
                 const int4 tn = m.tet_n[t];
                 const d3 p0 = ld_vert(m, tv.x), p1 = ld_vert(m, tv.y);
                 const d3 p2 = ld_vert(m, tv.z), p3 = ld_vert(m, tv.w);
-                const double ins =
-                    neg ? insphere(p1, p0, p2, p3, qq) : insphere(p0, p1, p2, p3, qq);
-                if (ins > 0) {
+                const bool ins = neg ? inside_sphere<ROBUST>(m, p1, p0, p2, p3, qq)
+                                     : inside_sphere<ROBUST>(m, p0, p1, p2, p3, qq);
+                if (ins) {
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
                         const int nb = sel4(tn, k);
@@ -2273,7 +2323,7 @@ __device__ bool wave_topology(const DevMesh &m, const WaveScr &W, WaveLds &L, in
 // Forward (GRAD = false): the row into the pool, bit-identical to the
 // reference.  Gradient: dL/dq from the same cavity (rows left empty by the
 // reference get 0).  Queries that exceed a capacity go to ovf_list.
-template <bool GRAD>
+template <bool GRAD, bool ROBUST>
 __global__ void __launch_bounds__(64) wave_rows_kernel(DevMesh m, char *wscr,
                                                        const double *__restrict__ q,
                                                        const int *__restrict__ items, int n_items,
@@ -2296,7 +2346,7 @@ __global__ void __launch_bounds__(64) wave_rows_kernel(DevMesh m, char *wscr,
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
         int C = 0, B = 0, NB = 0;
-        if (st < 0 || !wave_topology<!GRAD>(m, W, L, st, qq, C, B, NB)) {
+        if (st < 0 || !wave_topology<!GRAD, ROBUST>(m, W, L, st, qq, C, B, NB)) {
             if (lane == 0) ovf_list[atomicAdd(ovf_count, 1u)] = qi;
             __syncthreads();
             continue;
@@ -3138,15 +3188,15 @@ static int blocks_per_cu(K kernel) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, 256, 0) != hipSuccess) return 1;
     return n > 0 ? n : 1;
 }
-int rows_blocks_per_cu() { return blocks_per_cu(sibson_rows_kernel); }
+int rows_blocks_per_cu() { return blocks_per_cu(sibson_rows_kernel<false>); }
 int topo_blocks_per_cu() {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, topo_search_kernel, 64, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, topo_search_kernel<false>, 64, 0) !=
         hipSuccess)
         return 1;
     return n > 0 ? n : 1;
 }
-int locate_blocks_per_cu() { return blocks_per_cu(locate_kernel); }
+int locate_blocks_per_cu() { return blocks_per_cu(locate_kernel<false>); }
 int grad_geom_blocks_per_cu() {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grad_geom_kernel, kGradThreads, 0) !=
@@ -3177,8 +3227,12 @@ hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, in
                        int nblocks, const double *d_q, const int *d_items, int n_items,
                        const unsigned *d_n_items, unsigned int *d_work, const RowOut &out,
                        hipStream_t st) {
-    hipLaunchKernelGGL(sibson_rows_kernel, dim3(nblocks), dim3(256), 0, st, m, caps, sc, level,
-                       d_q, d_items, n_items, d_n_items, d_work, out);
+    if (m.robust)
+        hipLaunchKernelGGL(sibson_rows_kernel<true>, dim3(nblocks), dim3(256), 0, st, m, caps, sc,
+                           level, d_q, d_items, n_items, d_n_items, d_work, out);
+    else
+        hipLaunchKernelGGL(sibson_rows_kernel<false>, dim3(nblocks), dim3(256), 0, st, m, caps,
+                           sc, level, d_q, d_items, n_items, d_n_items, d_work, out);
     return hipGetLastError();
 }
 
@@ -3193,7 +3247,7 @@ size_t wave_scratch_size() { return wave_scratch_bytes(); }
 
 int wave_blocks_per_cu() {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, wave_rows_kernel<false>, 64, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, wave_rows_kernel<false, false>, 64, 0) !=
         hipSuccess)
         return 1;
     return n > 0 ? n : 1;
@@ -3204,8 +3258,14 @@ hipError_t launch_wave_rows(const DevMesh &m, char *wscr, int nblocks, const dou
                             const int *d_start, unsigned int *d_work, const RowOut &out,
                             int *ovf_list, unsigned int *ovf_count, hipStream_t st) {
     GradIn g{};
-    hipLaunchKernelGGL(wave_rows_kernel<false>, dim3(nblocks), dim3(64), 0, st, m, wscr, d_q,
-                       d_items, n_items, d_n_items, d_start, d_work, out, g, ovf_list, ovf_count);
+    if (m.robust)
+        hipLaunchKernelGGL((wave_rows_kernel<false, true>), dim3(nblocks), dim3(64), 0, st, m,
+                           wscr, d_q, d_items, n_items, d_n_items, d_start, d_work, out, g,
+                           ovf_list, ovf_count);
+    else
+        hipLaunchKernelGGL((wave_rows_kernel<false, false>), dim3(nblocks), dim3(64), 0, st, m,
+                           wscr, d_q, d_items, n_items, d_n_items, d_start, d_work, out, g,
+                           ovf_list, ovf_count);
     return hipGetLastError();
 }
 
@@ -3214,16 +3274,26 @@ hipError_t launch_wave_grad(const DevMesh &m, char *wscr, int nblocks, const dou
                             unsigned int *d_work, const GradIn &g, int *ovf_list,
                             unsigned int *ovf_count, hipStream_t st) {
     RowOut out{};
-    hipLaunchKernelGGL(wave_rows_kernel<true>, dim3(nblocks), dim3(64), 0, st, m, wscr, d_q,
-                       d_items, n_items, nullptr, d_start, d_work, out, g, ovf_list, ovf_count);
+    if (m.robust)
+        hipLaunchKernelGGL((wave_rows_kernel<true, true>), dim3(nblocks), dim3(64), 0, st, m,
+                           wscr, d_q, d_items, n_items, nullptr, d_start, d_work, out, g,
+                           ovf_list, ovf_count);
+    else
+        hipLaunchKernelGGL((wave_rows_kernel<true, false>), dim3(nblocks), dim3(64), 0, st, m,
+                           wscr, d_q, d_items, n_items, nullptr, d_start, d_work, out, g,
+                           ovf_list, ovf_count);
     return hipGetLastError();
 }
 
 hipError_t launch_locate(const DevMesh &m, const Caps &caps, const Scratch &sc, int nblocks,
                          const double *d_q, const int *d_items, int n_items, int *d_start,
                          hipStream_t st) {
-    hipLaunchKernelGGL(locate_kernel, dim3(nblocks), dim3(256), 0, st, m, caps, sc, d_q, d_items,
-                       n_items, d_start);
+    if (m.robust)
+        hipLaunchKernelGGL(locate_kernel<true>, dim3(nblocks), dim3(256), 0, st, m, caps, sc, d_q,
+                           d_items, n_items, d_start);
+    else
+        hipLaunchKernelGGL(locate_kernel<false>, dim3(nblocks), dim3(256), 0, st, m, caps, sc,
+                           d_q, d_items, n_items, d_start);
     return hipGetLastError();
 }
 
@@ -3231,8 +3301,12 @@ hipError_t launch_topo_search(const DevMesh &m, int nblocks, const double *d_q,
                               const int *d_items, int n_items, const int *d_start,
                               unsigned int *d_work, const Topo &topo, unsigned int *ovf_count,
                               int *ovf_list, hipStream_t st) {
-    hipLaunchKernelGGL(topo_search_kernel, dim3(nblocks), dim3(64), 0, st, m, d_q, d_items,
-                       n_items, d_start, d_work, topo, ovf_count, ovf_list);
+    if (m.robust)
+        hipLaunchKernelGGL(topo_search_kernel<true>, dim3(nblocks), dim3(64), 0, st, m, d_q,
+                           d_items, n_items, d_start, d_work, topo, ovf_count, ovf_list);
+    else
+        hipLaunchKernelGGL(topo_search_kernel<false>, dim3(nblocks), dim3(64), 0, st, m, d_q,
+                           d_items, n_items, d_start, d_work, topo, ovf_count, ovf_list);
     return hipGetLastError();
 }
 
@@ -3297,8 +3371,14 @@ hipError_t launch_grad_rows(const DevMesh &m, const Caps &caps, const Scratch &s
                             int nblocks, const double *d_q, const int *d_items, int n_items,
                             unsigned int *d_work, const GradIn &g, int *ovf_list,
                             unsigned int *ovf_count, hipStream_t st) {
-    hipLaunchKernelGGL(sibson_grad_rows_kernel, dim3(nblocks), dim3(256), 0, st, m, caps, sc,
-                       level, d_q, d_items, n_items, d_work, g, ovf_list, ovf_count);
+    if (m.robust)
+        hipLaunchKernelGGL(sibson_grad_rows_kernel<true>, dim3(nblocks), dim3(256), 0, st, m,
+                           caps, sc, level, d_q, d_items, n_items, d_work, g, ovf_list,
+                           ovf_count);
+    else
+        hipLaunchKernelGGL(sibson_grad_rows_kernel<false>, dim3(nblocks), dim3(256), 0, st, m,
+                           caps, sc, level, d_q, d_items, n_items, d_work, g, ovf_list,
+                           ovf_count);
     return hipGetLastError();
 }
 

@@ -105,7 +105,10 @@ class MeshAndVerticesHIP:
                 "mixed_rows_fp64": t.mixed_rows_fp64, "wave_ms": t.wave_ms,
                 "wave_items": t.wave_items, "grad_wave_ms": t.grad_wave_ms,
                 "grad_wave_items": t.grad_wave_items, "topo_search_ms": t.topo_search_ms,
-                "topo_order_ms": t.topo_order_ms}
+                "topo_order_ms": t.topo_order_ms, "robust_exact": t.robust_exact,
+                "robust_changed": t.robust_changed, "robust_unresolved": t.robust_unresolved,
+                "mixed_err_max": t.mixed_err_max, "mixed_err_rows": t.mixed_err_rows,
+                "mixed_err_entries": t.mixed_err_entries}
 
     def set_precision(self, mode="fp64", cond_threshold=0.02, cancel_threshold=20.0):
         """Row arithmetic: "fp64" (default; weights bit-identical to the
@@ -119,6 +122,28 @@ class MeshAndVerticesHIP:
         _lib.check(_lib.lib().nat_mesh_set_precision(self._m, modes[mode], cond_threshold,
                                                      cancel_threshold),
                    "nat_mesh_set_precision")
+
+    def set_predicates(self, mode="fp64"):
+        """Predicates of the query path: "fp64" (default; the reference's
+        plain FP64 signs, rows bit-identical to it) or "robust" (every
+        insphere / orient3d sign the static error bound leaves uncertain is
+        decided by the exact determinant).  timings() counts the exact
+        evaluations (robust_exact), the decisions that changed against FP64
+        (robust_changed) and the tests too wide to evaluate exactly
+        (robust_unresolved, FP64 sign kept)."""
+        modes = {"fp64": 0, "robust": 1}
+        if mode not in modes:
+            raise ValueError(f"predicates must be one of {sorted(modes)}")
+        _lib.check(_lib.lib().nat_mesh_set_predicates(self._m, modes[mode]),
+                   "nat_mesh_set_predicates")
+
+    def set_mixed_check(self, stride=1024):
+        """Error statement of mixed rows: every stride-th query is made
+        again in FP64 after each mixed compute and timings() reports the
+        largest |w_mixed - w_fp64| over them (mixed_err_max, with
+        mixed_err_rows / mixed_err_entries covered).  0 turns it off."""
+        _lib.check(_lib.lib().nat_mesh_set_mixed_check(self._m, int(stride)),
+                   "nat_mesh_set_mixed_check")
 
     def policy(self):
         """The row-path policy (see set_policy)."""

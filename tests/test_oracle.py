@@ -244,3 +244,40 @@ def test_oracle_plants_are_caught(oracle_mod):
     W = oracle_mod.OracleMesh(mesh).get_weights(small[f"{name}__queries"] - keys.mean(0))
     with pytest.raises(AssertionError):
         _csr_eq(W, small[f"{name}__indptr"], small[f"{name}__indices"], small[f"{name}__data"])
+
+
+def test_exact_cavity_oracle_is_pinned(oracle_mod):
+    """The checker of the robust predicate mode (oracle/exact_predicates.py):
+    on a jittered lattice, (a) for random queries its neighbour sets equal an
+    independent Delaunay triangulation's (Qhull on the mesh's vertices + q);
+    (b) for queries on tet circumspheres its depth-first cavity equals the
+    brute-force set of all tets holding q strictly inside (exact tests);
+    (c) there it differs from the FP64 cavity on some queries -- the
+    decisions the robust mode changes -- and agrees where the FP64 search's
+    tests are all certified."""
+    from natinterp3d_amd import workloads
+    from natinterp3d_amd.natinterp3d import MeshAndVerticesHIP
+    from oracle import exact_predicates as X
+    from tests.robust_cases import borderline_queries
+    keys, _, _ = workloads.lattice_jitter(n_side=10, n_queries=1)
+    kc = np.ascontiguousarray(keys - keys.mean(0))
+    pk = MeshAndVerticesHIP(kc, device=-1).export()
+    om = oracle_mod.OracleMesh(pk)
+    n = pk["verts"].shape[0] - 4
+    rng = np.random.default_rng(8)
+    qr = (rng.random((60, 3)) - 0.5) * 0.6
+    starts = om.locate(qr)
+    for i, q in enumerate(qr):
+        tri = scipy.spatial.Delaunay(np.vstack([pk["verts"][:n], q]))
+        ptr, nb = tri.vertex_neighbor_vertices
+        assert X.exact_neighbours(pk, q, starts[i]) == set(nb[ptr[n]:ptr[n + 1]].tolist()), i
+    qb = borderline_queries(pk, 200, seed=3)
+    starts = om.locate(qb)
+    differ = 0
+    for i, q in enumerate(qb):
+        cav, _ = X.exact_cavity(pk, q, starts[i])
+        if i < 60:
+            assert sorted(cav) == X.in_sphere_brute(pk, q), i
+        if set(cav) != set(om.cavity(q).tolist()):
+            differ += 1
+    assert differ > 0
