@@ -228,10 +228,27 @@ def run_precision(args):
 
     ms64, (y64, col64, w64) = timed()
     geom64 = mesh.timings()["geom_ms"]
+    crow64 = mesh.weights_device(qd)[0].clone()
     mesh.set_precision("mixed")
     msmx, (ymx, colmx, wmx) = timed()
     tm = mesh.timings()
     mesh.set_precision("fp64")
+    # robust predicates: exact signs for the tests the static bound leaves open
+    mesh.set_predicates("robust")
+    msrb, (yrb, colrb, wrb) = timed()
+    tr = mesh.timings()
+    crowrb = mesh.weights_device(qd)[0]
+    mesh.set_predicates("fp64")
+    if torch.equal(crowrb, crow64):
+        diff = (colrb != col64) | (wrb != w64)
+        rid = torch.searchsorted(crow64[1:].long(), torch.nonzero(diff).flatten(), right=True)
+        rows_rb = int(torch.unique(rid).numel())
+    else:
+        rows_rb = int((torch.diff(crowrb) != torch.diff(crow64)).sum().item())
+    log(f"robust predicates {msrb:.2f} ms/step: exact evaluations {tr['robust_exact']:,}, "
+        f"changed decisions {tr['robust_changed']:,}, unresolved {tr['robust_unresolved']:,}, "
+        f"rows differing from FP64 {rows_rb:,}; mixed error statement "
+        f"{tm['mixed_err_max']:.3g} over {tm['mixed_err_rows']:,} sampled rows")
     same_cols = bool(torch.equal(col64, colmx))
     werr = float((wmx - w64).abs().max()) if w64.numel() else 0.0
     yerr = float((ymx - y64).abs().max()) if M else 0.0
@@ -271,6 +288,14 @@ def run_precision(args):
             "mixed_rows_redone_fp64": tm["mixed_rows_fp64"],
             "insphere_decisions": decisions,
             "insphere_uncertified": uncertain,
+            "mixed_error_statement": {"max_abs_weight_error": tm["mixed_err_max"],
+                                      "rows_sampled": tm["mixed_err_rows"],
+                                      "entries_sampled": tm["mixed_err_entries"]},
+            "robust_ms_per_step": msrb,
+            "robust_exact_evaluations": tr["robust_exact"],
+            "robust_changed_decisions": tr["robust_changed"],
+            "robust_unresolved": tr["robust_unresolved"],
+            "robust_rows_differing_from_fp64": rows_rb,
             "mesh_build_s": round(build_s, 1),
         },
     }

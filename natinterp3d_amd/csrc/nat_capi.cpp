@@ -603,7 +603,7 @@ int ensure_wave_scratch(nat_mesh *m, long long want, int *launch) {
 }
 
 // d_prob: problem of each query (batch meshes), NULL for a single problem
-// The step-3 table (32 doubles per tet), built once per mesh on its first
+// The step-3 table (64 doubles per tet), built once per mesh on its first
 // compute; without the memory for it the geometry is computed inline.
 // Grid of the two-lanes-per-row geometry kernel of the mesh's precision.
 int geom_rows_grid(const nat_mesh *m) {
@@ -613,7 +613,7 @@ int geom_rows_grid(const nat_mesh *m) {
 int ensure_old_vol(nat_mesh *m, hipStream_t st) {
     if (m->old_vol_tried) return 0;
     m->old_vol_tried = true;
-    const size_t bytes = sizeof(double) * 32 * static_cast<size_t>(m->dev.n_tets);
+    const size_t bytes = sizeof(double) * 64 * static_cast<size_t>(m->dev.n_tets);
     if (m->old_vol.ensure(bytes) != hipSuccess) {
         (void)hipGetLastError();
         m->old_vol.release();

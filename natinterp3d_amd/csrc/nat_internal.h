@@ -69,7 +69,7 @@ struct DevMesh {
     // Step-3 subcell volumes, which depend on the mesh alone: per tet, per
     // vertex VI, per choice of vertex order of the three face circumcentres
     // it uses (the face mask's bits of the other three vertices, as the
-    // record's fcmask), at [(t * 4 + VI) * 8 + v]; NULL until built
+    // record's fcmask), at [(t * 16 + fcmask) * 4 + VI]; NULL until built
     // (computed inline then).
     const double *old_vol;
     // Predicates of the query path: 0 = the plain FP64 signs, 1 = FP64 where
@@ -220,7 +220,7 @@ hipError_t launch_grad_geom(const DevMesh &m, int nblocks, const double *d_q, co
                             int n_items, unsigned int *d_work, const Topo &topo,
                             const GradIn &g, int *ovf_list, unsigned int *ovf_count,
                             hipStream_t st);
-// the step-3 table of DevMesh::old_vol (out: n_tets * 32 doubles)
+// the step-3 table of DevMesh::old_vol (out: n_tets * 64 doubles)
 hipError_t launch_old_vol(const DevMesh &m, double *out, hipStream_t st);
 // wave-per-query rows (cavities beyond the group kernel): per-wave scratch
 // of wave_scratch_size() bytes for each of nblocks one-wave blocks

@@ -566,26 +566,27 @@ __device__ __forceinline__ void old_contributions(const DevMesh &m, int t, unsig
 }
 
 // The step-3 table (DevMesh::old_vol): the subcell volume of vertex VI of a
-// tet for each of the 8 choices of vertex order of the three face
-// circumcentres it uses (the faces opposite the other three vertices a, b, c:
-// each in the tet's own ascending order or in the order of the neighbour
-// across it, as old_contributions picks with fcmask), at [t][VI][v] with v =
-// the fcmask bits of faces a, b, c as bits 0, 1, 2 (32 doubles, 256 bytes per
-// tet).  Same functions, same operands: each entry has the bits
-// old_contributions would compute.
+// tet for each choice of vertex order of the three face circumcentres it
+// uses (the faces opposite the other three vertices a, b, c: each in the
+// tet's own ascending order or in the order of the neighbour across it, as
+// old_contributions picks with fcmask), at [t][fcmask][VI]: 16 masks x 4
+// vertices, 64 doubles, 512 bytes per tet, so that the four values one
+// cavity tet contributes are one aligned 32-byte piece (bit VI of the mask
+// does not change vertex VI's value: each value is stored twice).  Same
+// functions, same operands: each entry has the bits old_contributions would
+// compute.
 __device__ __forceinline__ d3 sel3d(bool s, const d3 &a, const d3 &b) {
     return {s ? a.x : b.x, s ? a.y : b.y, s ? a.z : b.z};
 }
 
-// index of vertex vi's value for face mask `mask` (bit vi itself is ignored)
+// index of vertex vi's value for face mask `mask`
 __host__ __device__ __forceinline__ size_t old_vol_index(int t, unsigned mask, int vi) {
-    // the mask with bit vi removed: bits of the faces opposite the other three
-    // vertices, in ascending vertex order
-    const unsigned v = (mask & ((1u << vi) - 1u)) | ((mask >> (vi + 1)) << vi);
-    return static_cast<size_t>(t) * 32 + static_cast<size_t>(vi) * 8 + (v & 7u);
+    return static_cast<size_t>(t) * 64 + static_cast<size_t>(mask & 15u) * 4 +
+           static_cast<size_t>(vi);
 }
 
-// the 8 values of vertex VI
+// vertex VI's value for each of the 16 masks (its own bit VI ignored: the
+// mask's bits of faces a, b, c choose the circumcentres' vertex orders)
 template <int VI>
 __device__ __forceinline__ void old_vol_vertex(const Quad &P, const Quad &Fo, const Quad &Fn,
                                                const d3 &ct, double *out) {
@@ -594,8 +595,13 @@ __device__ __forceinline__ void old_vol_vertex(const Quad &P, const Quad &Fo, co
         const d3 fa = sel3d(v & 1, Fn.get<a>(), Fo.get<a>());
         const d3 fb = sel3d(v & 2, Fn.get<b>(), Fo.get<b>());
         const d3 fc = sel3d(v & 4, Fn.get<c>(), Fo.get<c>());
-        out[VI * 8 + v] =
+        const double vol =
             subcell_volume(P.get<VI>(), P.get<a>(), P.get<b>(), P.get<c>(), ct, fc, fb, fa);
+        // the masks whose bits a, b, c are v's bits 0, 1, 2, with bit VI 0 or 1
+        const unsigned mask = ((v & 1) ? 1u << a : 0u) | ((v & 2) ? 1u << b : 0u) |
+                              ((v & 4) ? 1u << c : 0u);
+        out[mask * 4 + VI] = vol;
+        out[(mask | (1u << VI)) * 4 + VI] = vol;
     }
 }
 
@@ -614,7 +620,7 @@ __global__ void __launch_bounds__(256) old_vol_kernel(DevMesh m, double *__restr
                      step3_face_cc<1>(m, P, tv, tn, tn.z >= 0 ? 2u : 0u),
                      step3_face_cc<2>(m, P, tv, tn, tn.y >= 0 ? 4u : 0u),
                      step3_face_cc<3>(m, P, tv, tn, tn.x >= 0 ? 8u : 0u)};
-    double *o = out + static_cast<size_t>(t) * 32;
+    double *o = out + static_cast<size_t>(t) * 64;
     old_vol_vertex<0>(P, Fo, Fn, ct, o);
     old_vol_vertex<1>(P, Fo, Fn, ct, o);
     old_vol_vertex<2>(P, Fo, Fn, ct, o);
