@@ -46,6 +46,9 @@ typedef struct {
     int64_t device_bytes;    /* device memory held by the mesh (not workspace) */
     int64_t workspace_bytes; /* device workspace currently held */
     int32_t n_problems;      /* 1, or the number of problems of a batch mesh */
+    int32_t builder;         /* 0: host build (the reference's mesh), 1: GPU build */
+    int32_t build_rounds;    /* GPU build: insertion rounds */
+    double build_seconds;    /* wall time of the tetrahedralisation */
 } nat_mesh_info;
 
 /* Replaces buildNewMeshAndVertices (natural.c:114-118) plus the one-time
@@ -58,6 +61,30 @@ typedef struct {
  * orientation, where the reference exits). */
 int nat_mesh_build(const double *data_points, int num_data_points, int device,
                    nat_mesh **out);
+
+/* Opt-in GPU Delaunay build (no reference counterpart; it replaces the
+ * host insertion of buildMesh, delaunay.c:108-161, 561-750).  flags 0 is
+ * nat_mesh_build.  flags NAT_BUILD_GPU: the tetrahedralisation of the same
+ * points and the same super simplex is made on `device` (>= 0) in rounds of
+ * parallel Bowyer-Watson insertions with exact predicates, then packed with a
+ * balanced k-d tree over the tet centroids and uploaded.  For points in
+ * general position the Delaunay tetrahedralisation is unique, so the tets
+ * are the reference's; their order, the vertex order inside each tet and the
+ * k-d tree are not, so rows agree with the reference's as per-row column
+ * sets, with weights equal up to rounding (FP64 operation order).  The GPU
+ * build does not perturb: repeated points or exactly degenerate cavities
+ * return -3 (build with flags 0 instead); -4 when a cavity exceeds 512 tets
+ * or 1024 boundary faces. */
+#define NAT_BUILD_GPU 1
+int nat_mesh_build_ex(const double *data_points, int num_data_points, int device, int flags,
+                      nat_mesh **out);
+
+/* Exact audit of a single-problem mesh's tetrahedralisation on its device
+ * (no reference counterpart): counts[0] = faces with a neighbour tested,
+ * [1] = tets of non-positive orientation, [2] = neighbour links that are not
+ * mutual, [3] = faces whose neighbour's far vertex lies strictly inside the
+ * tet's circumsphere (0 for a Delaunay mesh: local Delaunay on every face). */
+int nat_mesh_delaunay_audit(const nat_mesh *m, int64_t *counts);
 
 /* Replaces freeMeshAndVertices (natural.c:120-123). */
 void nat_mesh_free(nat_mesh *m);

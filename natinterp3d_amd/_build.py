@@ -14,7 +14,8 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB_NAME = "libnatinterp3d_hip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
-SOURCES = ["sibson_kernels.hip", "csr_kernels.hip", "nat_capi.cpp", "mesh_builder.cpp"]
+SOURCES = ["sibson_kernels.hip", "csr_kernels.hip", "delaunay_kernels.hip", "nat_capi.cpp",
+           "mesh_builder.cpp"]
 
 # -ffp-contract=off: every multiply and add rounds separately, as in the
 #   reference's C build, so FP64 results are bit-identical.

@@ -23,7 +23,8 @@ EXPORTS = (
     "nat_mesh_set_policy", "nat_mesh_get_policy", "nat_mesh_autotune",
     "nat_mesh_from_packed", "nat_weights_row_order", "nat_topology_target",
     "nat_weights_spmm", "nat_weights_noalloc", "nat_mesh_set_predicates",
-    "nat_predicate_exact", "nat_mesh_set_mixed_check",
+    "nat_predicate_exact", "nat_mesh_set_mixed_check", "nat_mesh_build_ex",
+    "nat_mesh_delaunay_audit",
 )
 
 
@@ -69,6 +70,8 @@ class MeshInfo(ctypes.Structure):
         ("edge_anomalies", ctypes.c_int32), ("device", ctypes.c_int32),
         ("num_cus", ctypes.c_int32), ("device_bytes", ctypes.c_int64),
         ("workspace_bytes", ctypes.c_int64), ("n_problems", ctypes.c_int32),
+        ("builder", ctypes.c_int32), ("build_rounds", ctypes.c_int32),
+        ("build_seconds", ctypes.c_double),
     ]
 
 
@@ -154,6 +157,11 @@ def lib():
     L.nat_mesh_set_predicates.argtypes = [vp, ctypes.c_int]
     L.nat_mesh_set_mixed_check.restype = ctypes.c_int
     L.nat_mesh_set_mixed_check.argtypes = [vp, ctypes.c_int]
+    L.nat_mesh_build_ex.restype = ctypes.c_int
+    L.nat_mesh_build_ex.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(vp)]
+    L.nat_mesh_delaunay_audit.restype = ctypes.c_int
+    L.nat_mesh_delaunay_audit.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     L.nat_predicate_exact.restype = ctypes.c_int
     L.nat_predicate_exact.argtypes = [dp, ctypes.c_int, ctypes.c_int, ip, ip]
     L.nat_mesh_from_packed.restype = ctypes.c_int

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <array>
 #include <cstring>
+#include <thread>
 
 #include "geom.h"
 
@@ -315,21 +316,9 @@ int Builder::run(const double *pts, PackedMesh *out) {
     X_.resize(static_cast<size_t>(n_) + 4);
     for (int i = 0; i < n_; i++) X_[i] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
 
-    // super simplex, delaunay.c:832-863 and getRange delaunay.c:1086-1111
-    d3 mn = X_[0], mx = X_[0];
-    for (int i = 0; i < n_; i++) {
-        mx.x = mx.x < X_[i].x ? X_[i].x : mx.x;
-        mx.y = mx.y < X_[i].y ? X_[i].y : mx.y;
-        mx.z = mx.z < X_[i].z ? X_[i].z : mx.z;
-        mn.x = mn.x > X_[i].x ? X_[i].x : mn.x;
-        mn.y = mn.y > X_[i].y ? X_[i].y : mn.y;
-        mn.z = mn.z > X_[i].z ? X_[i].z : mn.z;
-    }
-    const d3 r = {mx.x - mn.x, mx.y - mn.y, mx.z - mn.z};
-    X_[n_ + 0] = {mn.x + r.x / 2, mx.y + 3 * r.y, mn.z - r.z};
-    X_[n_ + 1] = {mx.x + 2 * r.x, mn.y - 2 * r.y, mn.z - r.z};
-    X_[n_ + 2] = {mn.x - 2 * r.x, mn.y - 2 * r.y, mn.z - r.z};
-    X_[n_ + 3] = {mn.x + r.x / 2, mn.y + r.y / 2, mx.z + 2 * r.z};
+    double sv[12];
+    super_vertices(pts, n_, sv);
+    for (int k = 0; k < 4; k++) X_[n_ + k] = {sv[3 * k], sv[3 * k + 1], sv[3 * k + 2]};
 
     const size_t reserve = static_cast<size_t>(n_) * 8 + 64;
     R_.reserve(reserve);
@@ -415,7 +404,128 @@ int Builder::run(const double *pts, PackedMesh *out) {
     return 0;
 }
 
+// k-d tree of median splits over cent[idx[lo, hi)]: the subtree's root moves
+// to idx[lo], its left subtree to [lo + 1, lo + 1 + L), its right after.
+void kd_split(std::vector<int> &idx, const std::vector<double> &cent, int lo, int hi, int depth,
+              std::vector<int32_t> &left, std::vector<int32_t> &right, int &depth_max,
+              int spawn_depth) {
+    while (hi > lo) {
+        depth_max = std::max(depth_max, depth + 1);
+        const int dir = depth % 3;
+        const int L = (hi - lo) / 2;
+        auto key = [&](int t) { return cent[3 * static_cast<size_t>(t) + dir]; };
+        std::nth_element(idx.begin() + lo, idx.begin() + lo + L, idx.begin() + hi,
+                         [&](int a, int b) { return key(a) < key(b) || (key(a) == key(b) && a < b); });
+        std::swap(idx[lo], idx[lo + L]);
+        const int l0 = lo + 1, r0 = lo + 1 + L;
+        left[lo] = L > 0 ? l0 : -1;
+        right[lo] = hi > r0 ? r0 : -1;
+        if (depth < spawn_depth && L > 4096) {
+            int dm = 0;
+            std::thread th([&, l0, r0, depth] {
+                kd_split(idx, cent, l0, r0, depth + 1, left, right, dm, spawn_depth);
+            });
+            int dr = 0;
+            kd_split(idx, cent, r0, hi, depth + 1, left, right, dr, spawn_depth);
+            th.join();
+            depth_max = std::max({depth_max, dm, dr});
+            return;
+        }
+        kd_split(idx, cent, l0, r0, depth + 1, left, right, depth_max, spawn_depth);
+        lo = r0;
+        depth++;
+    }
+}
+
 }  // namespace
+
+void super_vertices(const double *pts, int n, double *out) {
+    // super simplex, delaunay.c:832-863 and getRange delaunay.c:1086-1111
+    d3 mn = {pts[0], pts[1], pts[2]}, mx = mn;
+    for (int i = 0; i < n; i++) {
+        const d3 p = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+        mx.x = mx.x < p.x ? p.x : mx.x;
+        mx.y = mx.y < p.y ? p.y : mx.y;
+        mx.z = mx.z < p.z ? p.z : mx.z;
+        mn.x = mn.x > p.x ? p.x : mn.x;
+        mn.y = mn.y > p.y ? p.y : mn.y;
+        mn.z = mn.z > p.z ? p.z : mn.z;
+    }
+    const d3 r = {mx.x - mn.x, mx.y - mn.y, mx.z - mn.z};
+    const d3 s[4] = {{mn.x + r.x / 2, mx.y + 3 * r.y, mn.z - r.z},
+                     {mx.x + 2 * r.x, mn.y - 2 * r.y, mn.z - r.z},
+                     {mn.x - 2 * r.x, mn.y - 2 * r.y, mn.z - r.z},
+                     {mn.x + r.x / 2, mn.y + r.y / 2, mx.z + 2 * r.z}};
+    for (int k = 0; k < 4; k++) {
+        out[3 * k] = s[k].x;
+        out[3 * k + 1] = s[k].y;
+        out[3 * k + 2] = s[k].z;
+    }
+}
+
+void pack_balanced(int n, std::vector<double> verts, const std::vector<int32_t> &tet_v_in,
+                   const std::vector<int32_t> &tet_n_in, int threads, PackedMesh *out) {
+    // drop freed slots (first vertex -1), renumbering the neighbour links
+    const int T0 = static_cast<int>(tet_v_in.size() / 4);
+    std::vector<int> keep(T0, -1);
+    int T = 0;
+    for (int t = 0; t < T0; t++)
+        if (tet_v_in[4 * static_cast<size_t>(t)] >= 0) keep[t] = T++;
+    std::vector<int32_t> tet_v(static_cast<size_t>(T) * 4), tet_n(static_cast<size_t>(T) * 4);
+    for (int t = 0; t < T0; t++) {
+        if (keep[t] < 0) continue;
+        for (int j = 0; j < 4; j++) {
+            tet_v[4 * static_cast<size_t>(keep[t]) + j] = tet_v_in[4 * static_cast<size_t>(t) + j];
+            const int x = tet_n_in[4 * static_cast<size_t>(t) + j];
+            tet_n[4 * static_cast<size_t>(keep[t]) + j] = x >= 0 ? keep[x] : -1;
+        }
+    }
+    // centroids as the reference's k-d tree takes them ((a + b + c + d) / 4)
+    std::vector<double> cent(static_cast<size_t>(T) * 3);
+    for (int t = 0; t < T; t++)
+        for (int d = 0; d < 3; d++) {
+            const int32_t *p = &tet_v[4 * static_cast<size_t>(t)];
+            cent[3 * static_cast<size_t>(t) + d] =
+                (verts[3 * p[0] + d] + verts[3 * p[1] + d] + verts[3 * p[2] + d] +
+                 verts[3 * p[3] + d]) / 4;
+        }
+    std::vector<int> idx(T);
+    for (int t = 0; t < T; t++) idx[t] = t;
+    out->kd_left.assign(T, -1);
+    out->kd_right.assign(T, -1);
+    int depth_max = 0;
+    int spawn = 0;
+    while ((1 << spawn) < threads && spawn < 6) spawn++;
+    kd_split(idx, cent, 0, T, 0, out->kd_left, out->kd_right, depth_max, spawn);
+    std::vector<int> inv(T);
+    for (int i = 0; i < T; i++) inv[idx[i]] = i;
+    out->n_data = n;
+    out->n_tets = T;
+    out->verts = std::move(verts);
+    out->tet_v.resize(static_cast<size_t>(T) * 4);
+    out->tet_n.resize(static_cast<size_t>(T) * 4);
+    out->kd_pos.resize(static_cast<size_t>(T) * 3);
+    for (int d = 0; d < 3; d++) {
+        out->kd_rect[d] = T ? cent[d] : 0.0;
+        out->kd_rect[3 + d] = T ? cent[d] : 0.0;
+    }
+    for (int i = 0; i < T; i++) {
+        const int t = idx[i];
+        for (int j = 0; j < 4; j++) {
+            out->tet_v[4 * static_cast<size_t>(i) + j] = tet_v[4 * static_cast<size_t>(t) + j];
+            const int x = tet_n[4 * static_cast<size_t>(t) + j];
+            out->tet_n[4 * static_cast<size_t>(i) + j] = x >= 0 ? inv[x] : -1;
+        }
+        for (int d = 0; d < 3; d++) {
+            const double c = cent[3 * static_cast<size_t>(t) + d];
+            out->kd_pos[3 * static_cast<size_t>(i) + d] = c;
+            out->kd_rect[d] = std::min(out->kd_rect[d], c);
+            out->kd_rect[3 + d] = std::max(out->kd_rect[3 + d], c);
+        }
+    }
+    out->kd_depth = depth_max;
+    out->coplanar = out->cospherical = out->edge_anomalies = 0;
+}
 
 int build_packed_mesh(const double *points, int n, PackedMesh *out) {
     if (!points || n <= 0 || !out) return -1;

@@ -28,4 +28,19 @@ struct PackedMesh {
 // super simplex, -1 for invalid arguments.
 int build_packed_mesh(const double *points, int n, PackedMesh *out);
 
+// The reference's super simplex for points [n, 3] (delaunay.c:832-863 over
+// the bounding box of getRange, delaunay.c:1086-1111): out[12] = its four
+// vertices, which become slots n .. n + 3.
+void super_vertices(const double *points, int n, double *out);
+
+// A mesh from another builder (same vertex slots, positively oriented tets,
+// neighbour across face i as the reference's face order): tets renumbered in
+// the preorder of a balanced k-d tree over their centroids (median splits,
+// split axis = depth mod 3, node i = tet i, root 0), k-d arrays filled.  The
+// reference's tree is grown in tet-creation order; any tree over the same
+// centroids gives the query path an exact nearest centroid.  Tets whose
+// first vertex is -1 are freed slots and are dropped.
+void pack_balanced(int n, std::vector<double> verts, const std::vector<int32_t> &tet_v,
+                   const std::vector<int32_t> &tet_n, int threads, PackedMesh *out);
+
 }  // namespace nat

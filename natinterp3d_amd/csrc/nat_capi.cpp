@@ -17,6 +17,7 @@
 #include <sys/mman.h>
 #include <vector>
 
+#include "delaunay_build.h"
 #include "exact.h"
 #include "geom.h"
 #include "mesh_builder.h"
@@ -110,6 +111,9 @@ struct nat_mesh {
     bool old_vol_tried = false;
     DevBuf prob, qoff;                    // batch calls: problem of each query
     std::mutex mu;
+    int builder = 0;                      // 0 host build, 1 GPU build
+    int build_rounds = 0;
+    double build_seconds = 0.0;
 
     // row-path policy (nat_policy).  route[p] = visited-tet budget of the
     // group kernel | first thread-per-query level << 8, read by the kernels at
@@ -1414,10 +1418,51 @@ int nat_mesh_build(const double *data_points, int num_data_points, int device, n
     nat_mesh *m = new nat_mesh();
     m->device = device;
     m->parts.resize(1);
+    const auto t0 = std::chrono::steady_clock::now();
     const int rc = nat::build_packed_mesh(data_points, num_data_points, &m->parts[0]);
+    m->build_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (rc) {
         delete m;
         return builder_error(rc);
+    }
+    if (int urc = finish_build(m)) {
+        delete m;
+        return urc;
+    }
+    *out = m;
+    return 0;
+}
+
+int nat_mesh_build_ex(const double *data_points, int num_data_points, int device, int flags,
+                      nat_mesh **out) {
+    g_err.clear();
+    if (flags == 0) return nat_mesh_build(data_points, num_data_points, device, out);
+    if (!out) return fail(-1, "out is NULL");
+    *out = nullptr;
+    if (flags != NAT_BUILD_GPU) return fail(-1, "unknown build flags %d", flags);
+    if (!data_points || num_data_points <= 0)
+        return fail(-1, "need at least one data point (got %d)", num_data_points);
+    if (device < 0) return fail(-1, "the GPU build needs a device (got %d)", device);
+    if (int rc = check_device(device)) return rc;
+    nat_mesh *m = new nat_mesh();
+    m->device = device;
+    m->parts.resize(1);
+    {
+        DeviceGuard g(device);
+        nat::GpuBuild gb;
+        std::string msg;
+        const int rc = nat::gpu_build_tets(data_points, num_data_points, &gb, &msg);
+        if (rc) {
+            delete m;
+            return fail(rc, "%s", msg.c_str());
+        }
+        const unsigned hw = std::thread::hardware_concurrency();
+        nat::pack_balanced(num_data_points, std::move(gb.verts), gb.tet_v, gb.tet_n,
+                           static_cast<int>(std::min(16u, hw ? hw : 1u)), &m->parts[0]);
+        m->builder = 1;
+        m->build_rounds = gb.rounds;
+        m->build_seconds = gb.seconds;
     }
     if (int urc = finish_build(m)) {
         delete m;
@@ -1582,6 +1627,25 @@ int nat_mesh_get_info(const nat_mesh *m, nat_mesh_info *info) {
     info->num_cus = m->num_cus;
     info->device_bytes = m->device_bytes();
     info->workspace_bytes = m->workspace_bytes();
+    info->builder = m->builder;
+    info->build_rounds = m->build_rounds;
+    info->build_seconds = m->build_seconds;
+    return 0;
+}
+
+int nat_mesh_delaunay_audit(const nat_mesh *m, int64_t *counts) {
+    g_err.clear();
+    if (!m || !counts) return fail(-1, "NULL argument");
+    if (m->problems() != 1) return fail(-1, "nat_mesh_delaunay_audit needs a single-problem mesh");
+    if (m->device < 0) return fail(-1, "host-only mesh has no device");
+    DeviceGuard g(m->device);
+    const nat::PackedMesh &h = m->parts[0];
+    long long c[4] = {0, 0, 0, 0};
+    std::string msg;
+    const int rc = nat::gpu_delaunay_audit(h.verts.data(), h.n_data + 4, h.tet_v.data(),
+                                           h.tet_n.data(), h.n_tets, c, &msg);
+    if (rc) return fail(rc, "%s", msg.c_str());
+    for (int i = 0; i < 4; i++) counts[i] = c[i];
     return 0;
 }
 

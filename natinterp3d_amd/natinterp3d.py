@@ -63,17 +63,27 @@ class MeshAndVerticesHIP:
     [num_queries, num_data_points] with float64 data and int32 indices whose
     columns are in natural-neighbour discovery order."""
 
-    def __init__(self, dataPoints, device: Optional[int] = None):
+    def __init__(self, dataPoints, device: Optional[int] = None, builder: str = "host"):
+        """builder "host" (default): the reference's mesh, rebuilt exactly on
+        the host.  builder "gpu": the Delaunay tetrahedralisation made on the
+        device (nat_mesh_build_ex with NAT_BUILD_GPU) -- the same tets for
+        points in general position, in another order, so rows agree with the
+        reference's as per-row column sets; degenerate input raises instead
+        of being perturbed."""
         dataPoints = _as_c_double_2d(dataPoints)
         self.numDataPoints = dataPoints.shape[0]
         if dataPoints.shape[1] != 3:
             raise ValueError("Data points must have shape (num_data_points, 3)")
+        builders = {"host": 0, "gpu": 1}
+        if builder not in builders:
+            raise ValueError(f"builder must be one of {sorted(builders)}")
         L = _lib.lib()
         self._m = ctypes.c_void_p()
         self.device = _default_device() if device is None else int(device)
-        rc = L.nat_mesh_build(dataPoints.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                              self.numDataPoints, self.device, ctypes.byref(self._m))
-        _lib.check(rc, "nat_mesh_build")
+        rc = L.nat_mesh_build_ex(dataPoints.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 self.numDataPoints, self.device, builders[builder],
+                                 ctypes.byref(self._m))
+        _lib.check(rc, "nat_mesh_build_ex")
         self._finalizer = weakref.finalize(self, L.nat_mesh_free, self._m)
 
     @property
@@ -161,6 +171,15 @@ class MeshAndVerticesHIP:
         """Time row-path policies on centred float64 CUDA queries and keep the
         fastest on this mesh; returns the tuner's report (dict)."""
         return autotune_rows(self._m, queries, None, repeats)
+
+    def delaunay_audit(self):
+        """Exact audit of the tetrahedralisation on the device: dict of faces
+        tested, tets of non-positive orientation, non-mutual neighbour links
+        and faces violating the empty-circumsphere property."""
+        c = (ctypes.c_int64 * 4)()
+        _lib.check(_lib.lib().nat_mesh_delaunay_audit(self._m, c), "nat_mesh_delaunay_audit")
+        return {"faces": c[0], "non_positive": c[1], "broken_links": c[2],
+                "not_delaunay": c[3]}
 
     def predicate_audit(self, queries):
         """Re-test the insphere decisions behind the rows of centred float64
