@@ -1148,7 +1148,7 @@ locate_kernel(DevMesh m, Caps caps, Scratch sc,
     }
 }
 
-// Topology of a query, in two kernels.
+// Topology of a query, in three kernels.
 //
 // topo_search_kernel: 16 lanes for a quad of four consecutive queries of the
 // Morton order (4 quads per 64-thread block, tables in LDS).  The queries of
@@ -1165,15 +1165,15 @@ locate_kernel(DevMesh m, Caps caps, Scratch sc,
 // neighbour across each face, 0xff across a face to a tet outside the sphere
 // or to none, i.e. a boundary face) goes to the end of the record.
 //
-// topo_order_kernel (one thread per query, tables in LDS) turns the
-// adjacency into what the reference's order rules fix: the cavity order (the
+// topo_dfs_kernel and topo_order_kernel (below) turn the adjacency into
+// what the reference's order rules fix: the cavity order (the
 // LIFO search, natural_insertionfree.c:498-530, is a depth-first preorder of
 // the in-sphere tets that enters neighbour slots 3, 2, 1, 0 in that order:
 // a tet outside the sphere is never appended or expanded, so it does not
 // change the order of the others), the boundary faces in (cavity position,
 // face) order, the face-circumcentre masks, and the neighbours in order of
 // first appearance in the face-vertex sequence
-// (natural_insertionfree.c:373-382, 559-584); then it completes the record.
+// (natural_insertionfree.c:373-382, 559-584); then they complete the record.
 // A query whose in-sphere tets are not all reachable from its start tet
 // through in-sphere tets (the quad's search can find such tets; a search of
 // its own would not) goes to the larger levels, which search it alone.

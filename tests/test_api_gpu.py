@@ -1,387 +1,304 @@
-"""The reference's API test-suite (tests/test_natinterp3d.py, 39 tests in 18
-classes), run against natinterp3d_amd: same inputs, same assertions.  Every
-weight here comes from the HIP kernels."""
+"""The behaviours the reference's API tests check (tests/test_natinterp3d.py),
+run through natinterp3d_amd's public API as a table: each row names an input
+recipe, the property it must have, its tolerance and the reference lines it
+follows.  Recipes and properties are defined once below; every weight comes
+from the HIP kernels.  Then the golden fixtures (the compiled reference's CSR
+on the same inputs, tests/golden/) are matched bit for bit."""
+import os
+
 import numpy as np
 import pytest
 import scipy.sparse
 
-import natinterp3d_amd as natinterp3d
+import natinterp3d_amd as nat
 
 pytestmark = pytest.mark.gpu
 
 
-def random_interior_queries(keys, n_queries, rng, margin=0.8):
-    center = keys.mean(axis=0)
-    return center + (rng.randn(n_queries, 3)) * keys.std(axis=0) * margin * 0.3
-
-
-class TestPartitionOfUnity:  # :18-42
-    def test_random_queries(self):
-        rng = np.random.RandomState(42)
-        keys = rng.randn(200, 3)
-        queries = random_interior_queries(keys, 50, rng)
-        W = natinterp3d.get_weights(queries, keys)
-        sums = np.asarray(W.sum(axis=1)).ravel()
-        interior = sums > 0.5
-        assert interior.sum() > 30
-        np.testing.assert_allclose(sums[interior], 1.0, atol=1e-10)
-
-    def test_grid_data(self):
-        xs = np.linspace(-1, 1, 6)
-        keys = np.array(np.meshgrid(xs, xs, xs)).reshape(3, -1).T
-        rng = np.random.RandomState(7)
-        queries = rng.uniform(-0.8, 0.8, (30, 3))
-        W = natinterp3d.get_weights(queries, keys)
-        sums = np.asarray(W.sum(axis=1)).ravel()
-        interior = sums > 0.5
-        assert interior.sum() > 20
-        np.testing.assert_allclose(sums[interior], 1.0, atol=1e-10)
-
-
-class TestNonNegativity:  # :45-60
-    def test_random(self):
-        rng = np.random.RandomState(99)
-        keys = rng.randn(300, 3)
-        queries = random_interior_queries(keys, 80, rng)
-        W = natinterp3d.get_weights(queries, keys)
-        assert W.min() >= -1e-12
-
-    def test_serial(self):
-        rng = np.random.RandomState(99)
-        keys = rng.randn(300, 3)
-        queries = random_interior_queries(keys, 80, rng)
-        W = natinterp3d.get_weights(queries, keys, parallel=False)
-        assert W.min() >= -1e-12
-
-
-class TestLinearPrecision:  # :63-99
-    @pytest.fixture()
-    def setup(self):
-        rng = np.random.RandomState(12)
-        keys = rng.randn(500, 3)
-        queries = random_interior_queries(keys, 60, rng)
-        W = natinterp3d.get_weights(queries, keys)
-        sums = np.asarray(W.sum(axis=1)).ravel()
-        return keys, queries, W, sums > 0.5
-
-    def test_constant_function(self, setup):
-        keys, queries, W, interior = setup
-        result = np.asarray(W @ np.full(len(keys), 7.0)[:, None]).ravel()
-        np.testing.assert_allclose(result[interior], 7.0, atol=1e-10)
-
-    def test_linear_xyz(self, setup):
-        keys, queries, W, interior = setup
-        coeffs = np.array([2.0, -3.0, 5.0])
-        values = keys @ coeffs + 1.0
-        expected = queries @ coeffs + 1.0
-        result = np.asarray(W @ values[:, None]).ravel()
-        np.testing.assert_allclose(result[interior], expected[interior], atol=1e-8)
-
-    def test_each_coordinate(self, setup):
-        keys, queries, W, interior = setup
-        for dim in range(3):
-            result = np.asarray(W @ keys[:, dim][:, None]).ravel()
-            np.testing.assert_allclose(result[interior], queries[interior, dim], atol=1e-8)
-
-
-class TestLocalCoordinateProperty:  # :102-114
-    def test_weighted_position(self):
-        rng = np.random.RandomState(55)
-        keys = rng.randn(300, 3)
-        queries = random_interior_queries(keys, 40, rng)
-        W = natinterp3d.get_weights(queries, keys)
-        interior = np.asarray(W.sum(axis=1)).ravel() > 0.5
-        errors = np.linalg.norm(np.asarray(W @ keys)[interior] - queries[interior], axis=1)
-        np.testing.assert_allclose(errors, 0.0, atol=1e-8)
-
-
-class TestConvergence:  # :117-140
-    def test_smooth_function(self):
-        rng = np.random.RandomState(10)
-
-        def func(x):
-            return np.sin(x[:, 0]) + np.cos(x[:, 1]) + 0.5 * x[:, 2]
-
-        queries = rng.randn(100, 3) * 0.3
-        errors = []
-        for n in [200, 2000, 20000]:
-            keys = rng.randn(n, 3)
-            result = natinterp3d.interpolate(queries, keys, func(keys))
-            errors.append(np.mean(np.abs(result - func(queries))))
-        assert errors[1] < errors[0]
-        assert errors[2] < errors[1]
-        assert errors[2] < 2e-3
-
-
-class TestSymmetricConfigurations:  # :145-181
-    def test_cube_center(self):
-        keys = np.array([[-1, -1, -1], [-1, -1, 1], [-1, 1, -1], [-1, 1, 1],
-                         [1, -1, -1], [1, -1, 1], [1, 1, -1], [1, 1, 1]], dtype=np.float64)
-        W = natinterp3d.get_weights(np.array([[0.0, 0.0, 0.0]]), keys)
-        np.testing.assert_allclose(np.asarray(W.todense()).ravel(), 1.0 / 8, atol=1e-4)
-
-    def test_regular_tetrahedron_centroid(self):
-        keys = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], dtype=np.float64)
-        W = natinterp3d.get_weights(keys.mean(axis=0, keepdims=True), keys)
-        np.testing.assert_allclose(np.asarray(W.todense()).ravel(), 0.25, atol=1e-10)
-
-    def test_octahedron_center(self):
-        keys = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]],
-                        dtype=np.float64)
-        W = natinterp3d.get_weights(np.array([[0.0, 0.0, 0.0]]), keys)
-        np.testing.assert_allclose(np.asarray(W.todense()).ravel(), 1.0 / 6, atol=1e-4)
-
-
-class TestQueryAtDataPoint:  # :184-208
-    def test_exact_data_point(self):
-        rng = np.random.RandomState(33)
-        keys = rng.randn(50, 3)
-        for idx in [0, 10, 25, 49]:
-            W = natinterp3d.get_weights(keys[idx:idx + 1], keys)
-            w = np.asarray(W.todense()).ravel()
-            assert w[idx] == pytest.approx(1.0, abs=1e-4)
-            np.testing.assert_allclose(np.delete(w, idx), 0.0, atol=1e-4)
-
-    def test_interpolation_at_data_point(self):
-        rng = np.random.RandomState(34)
-        keys = rng.randn(50, 3)
-        values = rng.randn(50)
-        for idx in [0, 25, 49]:
-            result = natinterp3d.interpolate(keys[idx:idx + 1], keys, values)
-            np.testing.assert_allclose(result[0], values[idx], atol=1e-3)
-
-
-class TestOutsideConvexHull:  # :213-236
-    def test_far_away_query(self):
-        rng = np.random.RandomState(20)
-        keys = rng.randn(100, 3)
-        queries = np.array([[100.0, 100.0, 100.0], [-100.0, -100.0, -100.0]])
-        W = natinterp3d.get_weights(queries, keys)
-        np.testing.assert_allclose(np.asarray(W.sum(axis=1)).ravel(), 0.0, atol=1e-15)
-
-    def test_interpolation_outside(self):
-        rng = np.random.RandomState(21)
-        keys = rng.randn(100, 3)
-        values = rng.randn(100) + 10.0
-        result = natinterp3d.interpolate(np.array([[100.0, 100.0, 100.0]]), keys, values)
-        assert np.all(np.isfinite(result))
-
-
-class TestSmallPointSets:  # :239-264
-    def test_five_points(self):
-        keys = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0.5, 0.5, 0.5]],
-                        dtype=np.float64)
-        w = np.asarray(natinterp3d.get_weights(np.array([[0.2, 0.2, 0.2]]), keys)
-                       .todense()).ravel()
-        assert w.sum() == pytest.approx(1.0, abs=1e-8)
-        assert np.all(w >= -1e-12)
-
-    def test_four_points(self):
-        keys = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float64)
-        w = np.asarray(natinterp3d.get_weights(np.array([[0.1, 0.1, 0.1]]), keys)
-                       .todense()).ravel()
-        assert w.sum() == pytest.approx(1.0, abs=1e-8)
-        assert np.all(w >= -1e-12)
-
-
-class TestSingleQuery:  # :267-276
-    def test_single_query(self):
-        rng = np.random.RandomState(77)
-        keys = rng.randn(50, 3)
-        queries = np.array([[0.0, 0.0, 0.0]])
-        assert natinterp3d.get_weights(queries, keys).shape == (1, 50)
-        result = natinterp3d.interpolate(queries, keys, rng.randn(50))
-        assert result.shape == (1,)
-
-
-class TestLargeCoordinates:  # :279-311
-    @pytest.mark.parametrize("seed,transform", [
-        (88, lambda k: k + np.array([1e6, -2e6, 3e6])),
-        (89, lambda k: k * 1e4),
-        (90, lambda k: k * 1e-4),
-    ])
-    def test_scale(self, seed, transform):
-        rng = np.random.RandomState(seed)
-        keys = transform(rng.randn(200, 3))
-        queries = random_interior_queries(keys, 30, rng)
-        W = natinterp3d.get_weights(queries, keys)
-        sums = np.asarray(W.sum(axis=1)).ravel()
-        interior = sums > 0.5
-        assert interior.sum() > 15
-        np.testing.assert_allclose(sums[interior], 1.0, atol=1e-6)
-
-
-class TestOutputShapes:  # :316-344
-    @pytest.fixture()
-    def data(self):
-        rng = np.random.RandomState(1)
-        return rng.randn(100, 3), rng.randn(50, 3) * 0.3
-
-    def test_1d_values(self, data):
-        keys, queries = data
-        assert natinterp3d.interpolate(queries, keys, np.ones(100)).shape == (50,)
-
-    def test_2d_values(self, data):
-        keys, queries = data
-        assert natinterp3d.interpolate(queries, keys, np.ones((100, 5))).shape == (50, 5)
-
-    def test_weight_matrix_shape(self, data):
-        keys, queries = data
-        assert natinterp3d.get_weights(queries, keys).shape == (50, 100)
-
-    def test_weight_matrix_is_csr(self, data):
-        keys, queries = data
-        assert isinstance(natinterp3d.get_weights(queries, keys), scipy.sparse.csr_matrix)
-
-
-class TestParallelSerialConsistency:  # :347-373
-    def test_weights_match(self):
-        rng = np.random.RandomState(2)
-        keys = rng.randn(200, 3)
-        queries = rng.randn(80, 3) * 0.3
-        W_par = natinterp3d.get_weights(queries, keys, parallel=True)
-        W_ser = natinterp3d.get_weights(queries, keys, parallel=False)
-        np.testing.assert_allclose(W_par.toarray(), W_ser.toarray(), atol=1e-12)
-
-    def test_interpolation_match(self):
-        rng = np.random.RandomState(3)
-        keys = rng.randn(200, 3)
-        queries = rng.randn(80, 3) * 0.3
-        values = rng.randn(200)
-        r_par = natinterp3d.interpolate(queries, keys, values, parallel=True)
-        r_ser = natinterp3d.interpolate(queries, keys, values, parallel=False)
-        np.testing.assert_allclose(r_par, r_ser, atol=1e-12)
-
-    def test_explicit_thread_count(self):
-        rng = np.random.RandomState(4)
-        keys = rng.randn(100, 3)
-        queries = rng.randn(30, 3) * 0.3
-        W1 = natinterp3d.get_weights(queries, keys, parallel=True, num_threads=1)
-        W2 = natinterp3d.get_weights(queries, keys, parallel=True, num_threads=4)
-        W_ser = natinterp3d.get_weights(queries, keys, parallel=False)
-        np.testing.assert_allclose(W1.toarray(), W_ser.toarray(), atol=1e-12)
-        np.testing.assert_allclose(W2.toarray(), W_ser.toarray(), atol=1e-12)
-
-
-class TestInterpolatorReuse:  # :376-399
-    def test_multiple_get_weights(self):
-        rng = np.random.RandomState(5)
-        keys = rng.randn(100, 3)
-        interp = natinterp3d.Interpolator(keys)
-        q1 = rng.randn(20, 3) * 0.3
-        q2 = rng.randn(30, 3) * 0.3
-        W1a = interp.get_weights(q1)
-        W2 = interp.get_weights(q2)
-        W1b = interp.get_weights(q1)
-        np.testing.assert_allclose(W1a.toarray(), W1b.toarray(), atol=1e-15)
-        assert W2.shape == (30, 100)
-
-    def test_multiple_interpolate(self):
-        rng = np.random.RandomState(6)
-        keys = rng.randn(100, 3)
-        values = rng.randn(100)
-        interp = natinterp3d.Interpolator(keys)
-        queries = rng.randn(20, 3) * 0.3
-        np.testing.assert_allclose(interp.interpolate(queries, values),
-                                   interp.interpolate(queries, values), atol=1e-15)
-
-
-class TestDtypeHandling:  # :402-421
-    def test_float32_input(self):
-        rng = np.random.RandomState(50)
-        keys = rng.randn(100, 3).astype(np.float32)
-        queries = (rng.randn(20, 3) * 0.3).astype(np.float32)
-        values = rng.randn(100).astype(np.float32)
-        result = natinterp3d.interpolate(queries, keys, values)
-        assert result.dtype == np.float64 or result.dtype == np.float32
-        assert result.shape == (20,)
-
-    def test_float32_weights(self):
-        rng = np.random.RandomState(51)
-        keys = rng.randn(100, 3).astype(np.float32)
-        queries = (rng.randn(20, 3) * 0.3).astype(np.float32)
-        W = natinterp3d.get_weights(queries, keys)
-        assert W.shape == (20, 100)
-        sums = np.asarray(W.sum(axis=1)).ravel()
-        np.testing.assert_allclose(sums[sums > 0.5], 1.0, atol=1e-6)
-
-
-class TestConvenienceFunctions:  # :424-444
-    def test_interpolate_matches_class(self):
-        rng = np.random.RandomState(60)
-        keys = rng.randn(100, 3)
-        queries = rng.randn(20, 3) * 0.3
-        values = rng.randn(100)
-        r1 = natinterp3d.interpolate(queries, keys, values, parallel=False)
-        r2 = natinterp3d.Interpolator(keys, parallel=False).interpolate(queries, values)
-        np.testing.assert_allclose(r1, r2, atol=1e-15)
-
-    def test_get_weights_matches_class(self):
-        rng = np.random.RandomState(61)
-        keys = rng.randn(100, 3)
-        queries = rng.randn(20, 3) * 0.3
-        W1 = natinterp3d.get_weights(queries, keys, parallel=False)
-        W2 = natinterp3d.Interpolator(keys, parallel=False).get_weights(queries)
-        np.testing.assert_allclose(W1.toarray(), W2.toarray(), atol=1e-15)
-
-
-class TestDeterminism:  # :449-466
-    def test_weights_deterministic(self):
-        rng = np.random.RandomState(70)
-        keys = rng.randn(200, 3)
-        queries = rng.randn(50, 3) * 0.3
-        W1 = natinterp3d.get_weights(queries, keys, parallel=True)
-        W2 = natinterp3d.get_weights(queries, keys, parallel=True)
-        np.testing.assert_array_equal(W1.toarray(), W2.toarray())
-
-    def test_serial_deterministic(self):
-        rng = np.random.RandomState(71)
-        keys = rng.randn(200, 3)
-        queries = rng.randn(50, 3) * 0.3
-        W1 = natinterp3d.get_weights(queries, keys, parallel=False)
-        W2 = natinterp3d.get_weights(queries, keys, parallel=False)
-        np.testing.assert_array_equal(W1.toarray(), W2.toarray())
-
-
-class TestSparsity:  # :471-484
-    def test_few_nonzeros_per_row(self):
-        rng = np.random.RandomState(80)
-        keys = rng.randn(1000, 3)
-        queries = rng.randn(50, 3) * 0.3
-        W = natinterp3d.get_weights(queries, keys)
-        for i in range(W.shape[0]):
-            if W[i].nnz > 0:
-                assert W[i].nnz < 100
-
-
-class TestOriginal:  # :489-503
-    def test_natinterp3d(self):
-        rng = np.random.RandomState(10)
-        keys = rng.randn(10000, 3)
-
-        def func(x):
-            x = x * 0.1
-            return np.sin(x[:, 0]) + np.cos(x[:, 1]) + np.tan(x[:, 2])
-
-        values = func(keys)
-        queries = rng.randn(100, 3)
-        interp_values = natinterp3d.interpolate(queries, keys, values)
-        assert np.mean(np.abs(interp_values - func(queries))) < 1e-3
-
-
-class TestAgainstReferenceFixtures:
+# ------------------------------------------------------------------ recipes
+
+def interior(keys, n, rng):
+    """Queries inside the hull, drawn as the reference's helper draws them
+    (centre + N(0,1) * std * margin 0.8 * 0.3, tests/test_natinterp3d.py:10-13)."""
+    return keys.mean(axis=0) + rng.randn(n, 3) * keys.std(axis=0) * 0.8 * 0.3
+
+
+class Case:
+    """keys, queries and the generator left where the recipe stopped (the
+    properties draw what they need next from it)."""
+
+    def __init__(self, recipe):
+        kind = recipe[0]
+        if kind == "gauss":  # ("gauss", seed, n, scale, offset, queries)
+            _, seed, n, scale, offset, q = recipe
+            self.rng = np.random.RandomState(seed)
+            self.keys = self.rng.randn(n, 3) * scale + np.asarray(offset, float)
+            if q is None:
+                self.queries = None
+            elif q[0] == "interior":
+                self.queries = interior(self.keys, q[1], self.rng)
+            elif q[0] == "fixed":
+                self.queries = np.asarray(q[1], float)
+            else:  # ("normal", n, s)
+                self.queries = self.rng.randn(q[1], 3) * q[2]
+        elif kind == "grid":  # 6^3 grid on [-1, 1]^3, uniform queries
+            xs = np.linspace(-1, 1, 6)
+            self.keys = np.array(np.meshgrid(xs, xs, xs)).reshape(3, -1).T
+            self.rng = np.random.RandomState(recipe[1])
+            self.queries = self.rng.uniform(-0.8, 0.8, (30, 3))
+        else:  # ("fixed", keys, queries)
+            self.keys = np.asarray(recipe[1], float)
+            q = recipe[2]
+            self.queries = self.keys.mean(axis=0, keepdims=True) if q == "centroid" else \
+                np.asarray(q, float)
+            self.rng = None
+
+    def W(self, **kw):
+        return nat.get_weights(self.queries, self.keys, **kw)
+
+
+def row_sums(W):
+    return np.asarray(W.sum(axis=1)).ravel()
+
+
+# --------------------------------------------------------------- properties
+
+def unity(c, atol, min_interior):
+    s = row_sums(c.W())
+    inside = s > 0.5
+    assert inside.sum() > min_interior
+    np.testing.assert_allclose(s[inside], 1.0, atol=atol)
+
+
+def non_negative(c, parallel):
+    assert c.W(parallel=parallel).min() >= -1e-12
+
+
+def reproduces(c, what, atol):
+    """Linear precision on the rows inside the hull: constants, an affine
+    function, each coordinate, the position itself (|sum w p - q|)."""
+    W = c.W()
+    inside = row_sums(W) > 0.5
+    if what == "constant":
+        got, want = np.asarray(W @ np.full(len(c.keys), 7.0)[:, None]).ravel(), 7.0
+        np.testing.assert_allclose(got[inside], want, atol=atol)
+    elif what == "affine":
+        a = np.array([2.0, -3.0, 5.0])
+        got = np.asarray(W @ (c.keys @ a + 1.0)[:, None]).ravel()
+        np.testing.assert_allclose(got[inside], (c.queries @ a + 1.0)[inside], atol=atol)
+    elif what == "coordinates":
+        for d in range(3):
+            got = np.asarray(W @ c.keys[:, d:d + 1]).ravel()
+            np.testing.assert_allclose(got[inside], c.queries[inside, d], atol=atol)
+    else:  # "position"
+        err = np.linalg.norm(np.asarray(W @ c.keys)[inside] - c.queries[inside], axis=1)
+        np.testing.assert_allclose(err, 0.0, atol=atol)
+
+
+def equal_weights(c, value, atol):
+    np.testing.assert_allclose(np.asarray(c.W().todense()).ravel(), value, atol=atol)
+
+
+def at_data_points(c, idx, atol):
+    for i in idx:
+        w = np.asarray(nat.get_weights(c.keys[i:i + 1], c.keys).todense()).ravel()
+        assert w[i] == pytest.approx(1.0, abs=atol)
+        np.testing.assert_allclose(np.delete(w, i), 0.0, atol=atol)
+
+
+def value_at_data_points(c, idx, atol):
+    values = c.rng.randn(len(c.keys))
+    for i in idx:
+        got = nat.interpolate(c.keys[i:i + 1], c.keys, values)
+        np.testing.assert_allclose(got[0], values[i], atol=atol)
+
+
+def exterior(c, what):
+    if what == "zero":
+        np.testing.assert_allclose(row_sums(c.W()), 0.0, atol=1e-15)
+    else:  # the reference only asks for finite values outside the hull
+        values = c.rng.randn(len(c.keys)) + 10.0
+        assert np.all(np.isfinite(nat.interpolate(c.queries, c.keys, values)))
+
+
+def small_set(c):
+    w = np.asarray(c.W().todense()).ravel()
+    assert w.sum() == pytest.approx(1.0, abs=1e-8)
+    assert np.all(w >= -1e-12)
+
+
+def shapes(c):
+    n, m = len(c.keys), len(c.queries)
+    W = c.W()
+    assert isinstance(W, scipy.sparse.csr_matrix) and W.shape == (m, n)
+    assert nat.interpolate(c.queries, c.keys, np.ones(n)).shape == (m,)
+    assert nat.interpolate(c.queries, c.keys, np.ones((n, 5))).shape == (m, 5)
+
+
+def single_query(c):
+    assert c.W().shape == (1, len(c.keys))
+    values = c.rng.randn(len(c.keys))
+    assert nat.interpolate(c.queries, c.keys, values).shape == (1,)
+
+
+def parallel_serial(c, what):
+    if what == "weights":
+        np.testing.assert_allclose(c.W(parallel=True).toarray(),
+                                   c.W(parallel=False).toarray(), atol=1e-12)
+    elif what == "values":
+        values = c.rng.randn(len(c.keys))
+        np.testing.assert_allclose(nat.interpolate(c.queries, c.keys, values, parallel=True),
+                                   nat.interpolate(c.queries, c.keys, values, parallel=False),
+                                   atol=1e-12)
+    else:  # thread counts
+        ser = c.W(parallel=False).toarray()
+        for t in (1, 4):
+            np.testing.assert_allclose(c.W(parallel=True, num_threads=t).toarray(), ser,
+                                       atol=1e-12)
+
+
+def reuse(c, what):
+    it = nat.Interpolator(c.keys)
+    if what == "weights":
+        q1, q2 = c.rng.randn(20, 3) * 0.3, c.rng.randn(30, 3) * 0.3
+        a, b, a2 = it.get_weights(q1), it.get_weights(q2), it.get_weights(q1)
+        np.testing.assert_allclose(a.toarray(), a2.toarray(), atol=1e-15)
+        assert b.shape == (30, len(c.keys))
+    else:
+        values = c.rng.randn(len(c.keys))
+        q = c.rng.randn(20, 3) * 0.3
+        np.testing.assert_allclose(it.interpolate(q, values), it.interpolate(q, values),
+                                   atol=1e-15)
+
+
+def float32_inputs(c, what):
+    keys = c.keys.astype(np.float32)
+    q = (c.rng.randn(20, 3) * 0.3).astype(np.float32)
+    if what == "values":
+        values = c.rng.randn(len(keys)).astype(np.float32)
+        r = nat.interpolate(q, keys, values)
+        assert r.dtype in (np.float32, np.float64) and r.shape == (20,)
+    else:
+        W = nat.get_weights(q, keys)
+        assert W.shape == (20, len(keys))
+        s = row_sums(W)
+        np.testing.assert_allclose(s[s > 0.5], 1.0, atol=1e-6)
+
+
+def module_matches_class(c, what):
+    it = nat.Interpolator(c.keys, parallel=False)
+    if what == "values":
+        values = c.rng.randn(len(c.keys))
+        np.testing.assert_allclose(nat.interpolate(c.queries, c.keys, values, parallel=False),
+                                   it.interpolate(c.queries, values), atol=1e-15)
+    else:
+        np.testing.assert_allclose(c.W(parallel=False).toarray(),
+                                   it.get_weights(c.queries).toarray(), atol=1e-15)
+
+
+def deterministic(c, parallel):
+    np.testing.assert_array_equal(c.W(parallel=parallel).toarray(),
+                                  c.W(parallel=parallel).toarray())
+
+
+def sparse_rows(c, most):
+    W = c.W()
+    counts = np.diff(W.indptr)
+    assert (counts[counts > 0] < most).all()
+
+
+def smooth_function(c, what):
+    """Interpolation of smooth functions: error falls with density
+    (200 -> 2,000 -> 20,000 keys, below 2e-3 at the end), and the first
+    test's 10,000-key case stays below 1e-3 mean error."""
+    rng = c.rng
+    if what == "convergence":
+        f = lambda x: np.sin(x[:, 0]) + np.cos(x[:, 1]) + 0.5 * x[:, 2]  # noqa: E731
+        q = rng.randn(100, 3) * 0.3
+        errs = []
+        for n in (200, 2000, 20000):
+            k = rng.randn(n, 3)
+            errs.append(np.mean(np.abs(nat.interpolate(q, k, f(k)) - f(q))))
+        assert errs[1] < errs[0] and errs[2] < errs[1] and errs[2] < 2e-3
+    else:
+        g = lambda x: np.sin(0.1 * x[:, 0]) + np.cos(0.1 * x[:, 1]) + np.tan(0.1 * x[:, 2])  # noqa
+        k = rng.randn(10000, 3)
+        values = g(k)
+        q = rng.randn(100, 3)
+        assert np.mean(np.abs(nat.interpolate(q, k, values) - g(q))) < 1e-3
+
+
+# ----------------------------------------------------------------- the table
+
+CUBE = [[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)]
+REG_TET = [[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]]
+OCTA = [[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]]
+ORIGIN = [[0.0, 0.0, 0.0]]
+
+SPEC = [
+    # (reference lines, recipe, property, arguments)
+    (":21-30", ("gauss", 42, 200, 1.0, 0.0, ("interior", 50)), unity, (1e-10, 30)),
+    (":32-42", ("grid", 7), unity, (1e-10, 20)),
+    (":48-53", ("gauss", 99, 300, 1.0, 0.0, ("interior", 80)), non_negative, (True,)),
+    (":55-60", ("gauss", 99, 300, 1.0, 0.0, ("interior", 80)), non_negative, (False,)),
+    (":76-79", ("gauss", 12, 500, 1.0, 0.0, ("interior", 60)), reproduces, ("constant", 1e-10)),
+    (":81-89", ("gauss", 12, 500, 1.0, 0.0, ("interior", 60)), reproduces, ("affine", 1e-8)),
+    (":91-99", ("gauss", 12, 500, 1.0, 0.0, ("interior", 60)), reproduces,
+     ("coordinates", 1e-8)),
+    (":105-114", ("gauss", 55, 300, 1.0, 0.0, ("interior", 40)), reproduces, ("position", 1e-8)),
+    (":120-140", ("gauss", 10, 0, 1.0, 0.0, None), smooth_function, ("convergence",)),
+    (":148-161", ("fixed", CUBE, ORIGIN), equal_weights, (1 / 8, 1e-4)),
+    (":163-171", ("fixed", REG_TET, "centroid"), equal_weights, (0.25, 1e-10)),
+    (":173-181", ("fixed", OCTA, ORIGIN), equal_weights, (1 / 6, 1e-4)),
+    (":187-199", ("gauss", 33, 50, 1.0, 0.0, None), at_data_points, ((0, 10, 25, 49), 1e-4)),
+    (":201-208", ("gauss", 34, 50, 1.0, 0.0, None), value_at_data_points, ((0, 25, 49), 1e-3)),
+    (":216-222", ("fixed", np.random.RandomState(20).randn(100, 3), [[100.0] * 3, [-100.0] * 3]),
+     exterior, ("zero",)),
+    (":224-236", ("gauss", 21, 100, 1.0, 0.0, ("fixed", [[100.0] * 3])), exterior, ("finite",)),
+    (":242-252", ("fixed", [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0.5, 0.5, 0.5]],
+                  [[0.2, 0.2, 0.2]]), small_set, ()),
+    (":254-264", ("fixed", [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], [[0.1, 0.1, 0.1]]),
+     small_set, ()),
+    (":268-276", ("gauss", 77, 50, 1.0, 0.0, ("fixed", ORIGIN)), single_query, ()),
+    (":282-291", ("gauss", 88, 200, 1.0, (1e6, -2e6, 3e6), ("interior", 30)), unity, (1e-6, 15)),
+    (":293-301", ("gauss", 89, 200, 1e4, 0.0, ("interior", 30)), unity, (1e-6, 15)),
+    (":303-311", ("gauss", 90, 200, 1e-4, 0.0, ("interior", 30)), unity, (1e-6, 15)),
+    (":318-344", ("gauss", 1, 100, 1.0, 0.0, ("normal", 50, 0.3)), shapes, ()),
+    (":348-354", ("gauss", 2, 200, 1.0, 0.0, ("normal", 80, 0.3)), parallel_serial, ("weights",)),
+    (":356-363", ("gauss", 3, 200, 1.0, 0.0, ("normal", 80, 0.3)), parallel_serial, ("values",)),
+    (":365-373", ("gauss", 4, 100, 1.0, 0.0, ("normal", 30, 0.3)), parallel_serial, ("threads",)),
+    (":379-389", ("gauss", 5, 100, 1.0, 0.0, None), reuse, ("weights",)),
+    (":391-399", ("gauss", 6, 100, 1.0, 0.0, None), reuse, ("values",)),
+    (":403-410", ("gauss", 50, 100, 1.0, 0.0, None), float32_inputs, ("values",)),
+    (":412-421", ("gauss", 51, 100, 1.0, 0.0, None), float32_inputs, ("weights",)),
+    (":427-435", ("gauss", 60, 100, 1.0, 0.0, ("normal", 20, 0.3)), module_matches_class,
+     ("values",)),
+    (":437-444", ("gauss", 61, 100, 1.0, 0.0, ("normal", 20, 0.3)), module_matches_class,
+     ("weights",)),
+    (":452-458", ("gauss", 70, 200, 1.0, 0.0, ("normal", 50, 0.3)), deterministic, (True,)),
+    (":460-466", ("gauss", 71, 200, 1.0, 0.0, ("normal", 50, 0.3)), deterministic, (False,)),
+    (":474-484", ("gauss", 80, 1000, 1.0, 0.0, ("normal", 50, 0.3)), sparse_rows, (100,)),
+    (":492-503", ("gauss", 10, 0, 1.0, 0.0, None), smooth_function, ("original",)),
+]
+
+
+@pytest.mark.parametrize("lines,recipe,prop,args", SPEC,
+                         ids=[f"{s[0]}-{s[2].__name__}" for s in SPEC])
+def test_reference_behaviour(lines, recipe, prop, args):
+    prop(Case(recipe), *args)
+
+
+# ------------------------------------------------------------ golden fixtures
+
+@pytest.mark.parametrize("name", sorted(__import__("tests.golden.cases",
+                                                   fromlist=["CASES"]).CASES))
+def test_golden_fixture_bit_exact(name):
     """Every golden fixture (the compiled reference's CSR on the inputs of its
     own tests) reproduced bit for bit through the public API."""
-
-    @pytest.mark.parametrize("name", sorted(__import__("tests.golden.cases",
-                                                       fromlist=["CASES"]).CASES))
-    def test_fixture(self, name):
-        import os
-        small = np.load(os.path.join(os.path.dirname(__file__), "golden", "small_cases.npz"))
-        keys = small[f"{name}__keys"]
-        q = small[f"{name}__queries"]
-        W = natinterp3d.get_weights(q, keys)
-        assert np.array_equal(W.indptr, small[f"{name}__indptr"])
-        assert np.array_equal(W.indices, small[f"{name}__indices"])
-        d, e = W.data, small[f"{name}__data"]
-        assert ((d == e) | (np.isnan(d) & np.isnan(e))).all()
+    small = np.load(os.path.join(os.path.dirname(__file__), "golden", "small_cases.npz"))
+    keys, q = small[f"{name}__keys"], small[f"{name}__queries"]
+    W = nat.get_weights(q, keys)
+    assert np.array_equal(W.indptr, small[f"{name}__indptr"])
+    assert np.array_equal(W.indices, small[f"{name}__indices"])
+    d, e = W.data, small[f"{name}__data"]
+    assert ((d == e) | (np.isnan(d) & np.isnan(e))).all()
