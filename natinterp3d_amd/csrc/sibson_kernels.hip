@@ -1243,10 +1243,13 @@ __device__ __forceinline__ int *rec_xtet(int4 *r, int C, int B, int room) {
 }
 
 struct alignas(16) QuadMem {
-    int vkey[256];                 // tet index, -1 = empty slot (slot 255 unused)
+    // the hash of the search, then (its keys no longer read) the dense maps
+    union {
+        int vkey[256];                     // tet index, -1 = empty slot (slot 255 unused)
+        unsigned char dense[grp::Q][256];  // per query: dense index of an in-sphere slot
+    };
     unsigned vnb[256];             // visited slots of the 4 neighbours, 8 bits each (0xff none)
     unsigned char vflag[256];      // bit 0 tested; bit 1 + j inside query j's circumsphere
-    unsigned char dense[grp::Q][256];  // per query: dense index of an in-sphere slot
     unsigned char vlist[grp::VMAX];    // visited slots in test order; fronts are ranges
     int vtet[grp::VMAX];           // their tets
     int sslot[grp::Q];             // slot of each query's start tet
