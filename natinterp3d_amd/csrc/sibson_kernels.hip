@@ -1180,15 +1180,15 @@ locate_kernel(DevMesh m, Caps caps, Scratch sc,
 //
 // Record of one query, in 16-byte units at topo.rec[rec_off[qi]]:
 //   [0]            (C, B, NB, 0): cavity tets, boundary faces, neighbours
-//   [1 .. C]       per cavity tet in dense order: (tet, slot(p0) | slot(p1)
+//   [1 .. C]       per cavity tet in cavity order: (tet, slot(p0) | slot(p1)
 //                  << 16, slot(p2) | slot(p3) << 16, fcmask) -- slot 0xffff
 //                  = none; fcmask bit k: the face opposite p[k] is held by a
 //                  cavity tet earlier in cavity order (its circumcentre uses
 //                  that tet's vertex order)
 //   next ceil(C/8)   the cavity order, 16 bits per position: its dense index
 //                    | the fcmask of that tet << 6
-//   next ceil(B/16)  boundary faces in sequence order, a byte each: dense
-//                    index << 2 | face
+//   next ceil(B/16)  boundary faces in sequence order, a byte each: cavity
+//                    position << 2 | face
 //   next ceil(NB/4)  neighbour vertex indices in slot order (room is reserved
 //                    for min(32, B/2 + 4): a cavity bounded by one closed
 //                    surface has B/2 + 2 boundary vertices)
@@ -1196,7 +1196,9 @@ locate_kernel(DevMesh m, Caps caps, Scratch sc,
 //                    index (int each)
 // C = 0 rows: NB = vertices coinciding with the query (weight 1/NB each, in
 // unit [1]), or NB = 0 for an empty row.  Between the kernels the header is
-// (C, B, neighbour room, 1) and entry d holds the tet's vertex slots.
+// (C, B, neighbour room, 1) and entry d holds the vertex slots of the tet of
+// dense index d; the order kernel rewrites the entries in cavity order, so
+// the geometry reads them in sequence.
 namespace grp {
 constexpr int G = 16;             // lanes per quad
 constexpr int Q = 4;              // queries per quad
@@ -1661,7 +1663,7 @@ topo_dfs_kernel(const int *__restrict__ items, int n_items, Topo topo, unsigned 
         for (int k = 0; k < 4; k++) {
             if (((a >> (8 * k)) & 0xffu) == 0xffu) {
                 if (nb < B) {
-                    bw |= static_cast<unsigned>((c << 2) | k) << (8 * (nb & 3));
+                    bw |= static_cast<unsigned>((n << 2) | k) << (8 * (nb & 3));
                     if ((nb & 3) == 3) {
                         rb[nb >> 2] = bw;
                         bw = 0;
@@ -1759,11 +1761,13 @@ topo_order_kernel(DevMesh m, const int *__restrict__ items, int n_items, Topo to
     const int *xtet = rec_xtet(r, C, B, room);
     const unsigned short *ro = rec_order(r, hd);
     const unsigned char *rb = rec_bnd(r, hd);
-    for (int d = gl; d < C; d += kOrderLanes) {
-        g.dtet[d] = xtet[d];
-        g.cv[d] = r[1 + d];
-        const unsigned o = ro[d];  // position d: its dense index and mask
-        g.fcm[o & 63u] = static_cast<unsigned char>(o >> 6);
+    // the cavity tets in cavity order (the entries are still in dense order)
+    for (int p = gl; p < C; p += kOrderLanes) {
+        const unsigned o = ro[p];  // position p: its dense index and mask
+        const int d = static_cast<int>(o & 63u);
+        g.dtet[p] = xtet[d];
+        g.cv[p] = r[1 + d];
+        g.fcm[p] = static_cast<unsigned char>(o >> 6);
     }
     for (int e = gl; e < B; e += kOrderLanes) g.bnd[e] = rb[e];
     for (int h = gl; h < grp::NT; h += kOrderLanes) {
@@ -1829,7 +1833,7 @@ topo_order_kernel(DevMesh m, const int *__restrict__ items, int n_items, Topo to
         if (gl == 0) r[0] = make_int4(0, 0, 0, 0);
         return;
     }
-    // the cavity entries: tet, vertex slots, face-circumcentre mask
+    // the cavity entries in cavity order: tet, vertex slots, face-circumcentre mask
     for (int d = gl; d < C; d += kOrderLanes) {
         const int4 tv = g.cv[d];
         int sl[4];
@@ -2622,12 +2626,11 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
                 const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                                q[3 * static_cast<size_t>(qi) + 2]};
                 const unsigned char *rf = rec_bnd(r, hd);
-                const unsigned short *ro = rec_order(r, hd);
                 double absum = 0.0;
                 auto run = [&](bool fp32) {
                     for (int j = 0; j < nnb; j++) acc[j * kGeomThreads] = 0.0;
                     for (int ci = 0; ci < ncav; ci++) {
-                        const int4 cr = r[1 + (ro[ci] & 63u)];
+                        const int4 cr = r[1 + ci];
                         auto slot_of = [&](int vi, int) { return rec_slot(cr, vi); };
                         auto add = [&](int slot, double vol) {
                             acc[slot * kGeomThreads] += vol;
@@ -2789,14 +2792,13 @@ __global__ void __launch_bounds__(kGeomRowsThreads) geom_rows_kernel(
                                q[3 * static_cast<size_t>(qi) + 1],
                                q[3 * static_cast<size_t>(qi) + 2]};
                 const unsigned char *rf = rec_bnd(r, hd);
-                const unsigned short *ro = rec_order(r, hd);
                 double absum = 0.0;  // this lane's share of sum |vol| (MIXED)
                 auto run = [&](bool fp32) {
                     for (int j = g; j < nnb; j += G) acc[j][rr] = 0.0;
                     // step 3 (the exact table in both modes), cavity order
                     if (g < 4) {
                         for (int ci = 0; ci < ncav; ci++) {
-                            const int4 cr = r[1 + (ro[ci] & 63u)];
+                            const int4 cr = r[1 + ci];
 #pragma unroll
                             for (int vi = g; vi < 4; vi += G) {
                                 const double v = m.old_vol[old_vol_index(
