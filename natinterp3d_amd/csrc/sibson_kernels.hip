@@ -1159,11 +1159,11 @@ locate_kernel(DevMesh m, Caps caps, Scratch sc,
 // in parallel; a tet is expanded when it is inside any query's sphere.  A
 // query's decisions are the same FP64 insphere tests the reference makes, and
 // every tet its own search pops is among those tested.  Per query, the tets
-// inside its sphere get dense indices (its start tet 0) and its record is
-// reserved; entry d holds, for now, the vertex slots of the tet, and a
-// handoff pair (tet, in-sphere adjacency: the dense index of the in-sphere
-// neighbour across each face, 0xff across a face to a tet outside the sphere
-// or to none, i.e. a boundary face) goes to the end of the record.
+// inside its sphere get dense indices (its start tet 0), its record is
+// reserved, and a handoff pair per dense index (tet, in-sphere adjacency: the
+// dense index of the in-sphere neighbour across each face, 0xff across a
+// face to a tet outside the sphere or to none, i.e. a boundary face) goes to
+// the end of the record.
 //
 // topo_dfs_kernel and topo_order_kernel (below) turn the adjacency into
 // what the reference's order rules fix: the cavity order (the
@@ -1196,9 +1196,9 @@ locate_kernel(DevMesh m, Caps caps, Scratch sc,
 //                    index (int each)
 // C = 0 rows: NB = vertices coinciding with the query (weight 1/NB each, in
 // unit [1]), or NB = 0 for an empty row.  Between the kernels the header is
-// (C, B, neighbour room, 1) and entry d holds the vertex slots of the tet of
-// dense index d; the order kernel rewrites the entries in cavity order, so
-// the geometry reads them in sequence.
+// (C, B, neighbour room, 1) and the entries are not written yet; the order
+// kernel writes them in cavity order, so the geometry reads them in
+// sequence.
 namespace grp {
 constexpr int G = 16;             // lanes per quad
 constexpr int Q = 4;              // queries per quad
@@ -1580,13 +1580,13 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
                 }
             }
         }
-        // entries and handoff pairs of the queries with a cavity
+        // handoff pairs of the queries with a cavity (the order kernel reads
+        // the tets' vertices itself)
         for (int i = gl; i < nvis; i += grp::G) {
             const int s = g.vlist[i];
             const unsigned fl = g.vflag[s];
             if (!(fl & 0x1eu)) continue;
             const int t = g.vtet[i];
-            const int4 tv = m.fat[t].v;
 #pragma unroll
             for (int j = 0; j < grp::Q; j++) {
                 if (qi[j] < 0 || !ok[j] || !fits[j] || ncav[j] == 0 || !(fl & (2u << j)))
@@ -1594,7 +1594,6 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
                 int4 *r = topo.rec + off[j];
                 const int d = g.dense[j][s];
                 const int room = nb_room(nbnd[j]);
-                r[1 + d] = tv;
                 rec_xadj(r, ncav[j], nbnd[j], room)[d] = static_cast<int>(adjacency(g, s, j));
                 rec_xtet(r, ncav[j], nbnd[j], room)[d] = t;
             }
@@ -1761,12 +1760,12 @@ topo_order_kernel(DevMesh m, const int *__restrict__ items, int n_items, Topo to
     const int *xtet = rec_xtet(r, C, B, room);
     const unsigned short *ro = rec_order(r, hd);
     const unsigned char *rb = rec_bnd(r, hd);
-    // the cavity tets in cavity order (the entries are still in dense order)
+    // the cavity tets in cavity order
     for (int p = gl; p < C; p += kOrderLanes) {
         const unsigned o = ro[p];  // position p: its dense index and mask
-        const int d = static_cast<int>(o & 63u);
-        g.dtet[p] = xtet[d];
-        g.cv[p] = r[1 + d];
+        const int t = xtet[o & 63u];
+        g.dtet[p] = t;
+        g.cv[p] = m.tet_v[t];
         g.fcm[p] = static_cast<unsigned char>(o >> 6);
     }
     for (int e = gl; e < B; e += kOrderLanes) g.bnd[e] = rb[e];
