@@ -1345,35 +1345,47 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
             }
             if (gl == 0) g.cnt[3] = 0;
             group_sync();
-            // the first front: the start tets and their neighbours
-            if (gl == 0) {
-                bool isnew;
-                int n = 0;
-                for (int j = 0; j < grp::Q; j++) {
-                    g.sslot[j] = -1;
-                    if (!act[j]) continue;
-                    const int s0 = vis_probe(g, st[j], vt_hash(st[j]), isnew);
-                    g.sslot[j] = s0;
-                    if (isnew) {
-                        g.vlist[n] = static_cast<unsigned char>(s0);
-                        g.vtet[n++] = st[j];
-                    }
-                }
-                for (int j = 0; j < grp::Q; j++) {
-                    if (!act[j]) continue;
-                    const int4 tn0 = m.tet_n[st[j]];
+            // the first front: the start tets (lanes 0-3), then their
+            // neighbours (lane 4 j + k: neighbour k of query j's start tet),
+            // each group placed in lane order
+            {
+                const int jq = gl >> 2, kq = gl & 3;
+                int stj = -1;
 #pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        const int nb = sel4(tn0, k);
-                        if (nb < 0) continue;
-                        const int s2 = vis_probe(g, nb, vt_hash(nb), isnew);
-                        if (s2 >= 0 && isnew) {
-                            g.vlist[n] = static_cast<unsigned char>(s2);
-                            g.vtet[n++] = nb;
-                        }
+                for (int j = 0; j < grp::Q; j++)
+                    if (jq == j && act[j]) stj = st[j];
+                const int nbk = stj >= 0 ? sel4(m.tet_n[stj], kq) : -1;
+                bool isnew = false;
+                int s0 = -1;
+                if (gl < grp::Q) {
+                    int stl = -1;
+#pragma unroll
+                    for (int j = 0; j < grp::Q; j++)
+                        if (gl == j && act[j]) stl = st[j];
+                    if (stl >= 0) s0 = vis_probe(g, stl, vt_hash(stl), isnew);
+                    g.sslot[gl] = s0;
+                    const unsigned mk = static_cast<unsigned>(__ballot(isnew) >> (16 * gw)) &
+                                        0xfu;
+                    if (isnew) {
+                        const int p = __popc(mk & ((1u << gl) - 1u));
+                        g.vlist[p] = static_cast<unsigned char>(s0);
+                        g.vtet[p] = stl;
                     }
                 }
-                g.cnt[0] = n;
+                const int n0 = __popc(static_cast<unsigned>(__ballot(gl < grp::Q && isnew) >>
+                                                            (16 * gw)) & 0xffffu);
+                group_sync();
+                bool isnew2 = false;
+                int s2 = -1;
+                if (nbk >= 0) s2 = vis_probe(g, nbk, vt_hash(nbk), isnew2);
+                const bool add = nbk >= 0 && s2 >= 0 && isnew2;
+                const unsigned mk2 = static_cast<unsigned>(__ballot(add) >> (16 * gw)) & 0xffffu;
+                if (add) {
+                    const int p = n0 + __popc(mk2 & ((1u << gl) - 1u));
+                    g.vlist[p] = static_cast<unsigned char>(s2);
+                    g.vtet[p] = nbk;
+                }
+                if (gl == 0) g.cnt[0] = n0 + __popc(mk2);
             }
             group_sync();
             int f0 = 0, f1 = g.cnt[0];
