@@ -1315,19 +1315,25 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
         d3 qq[grp::Q];
         bool act[grp::Q], ok[grp::Q];
         int budget = 0;
+        // lane j of the quad loads query j; the quad's lanes share them
+        int qil = -1, stl = -1, bl = 0;
+        d3 ql = {0, 0, 0};
+        if (gl < grp::Q) {
+            const int item = grp::Q * gw + gl;
+            qil = item < cnt ? items[first + item] : -1;
+            if (qil >= 0) {
+                stl = start_of[qil];
+                ql = {q[3 * static_cast<size_t>(qil)], q[3 * static_cast<size_t>(qil) + 1],
+                      q[3 * static_cast<size_t>(qil) + 2]};
+                bl = min(grp::VQ, static_cast<int>(route_of(m, qil) & 0xffu));
+            }
+        }
 #pragma unroll
         for (int j = 0; j < grp::Q; j++) {
-            const int item = grp::Q * gw + j;
-            qi[j] = item < cnt ? items[first + item] : -1;
-            st[j] = -1;
-            qq[j] = {0, 0, 0};
-            int b = 0;
-            if (qi[j] >= 0) {
-                st[j] = start_of[qi[j]];
-                qq[j] = {q[3 * static_cast<size_t>(qi[j])], q[3 * static_cast<size_t>(qi[j]) + 1],
-                         q[3 * static_cast<size_t>(qi[j]) + 2]};
-                b = min(grp::VQ, static_cast<int>(route_of(m, qi[j]) & 0xffu));
-            }
+            qi[j] = __shfl(qil, j, grp::G);
+            st[j] = __shfl(stl, j, grp::G);
+            qq[j] = {__shfl(ql.x, j, grp::G), __shfl(ql.y, j, grp::G), __shfl(ql.z, j, grp::G)};
+            const int b = __shfl(bl, j, grp::G);
             // a query outside the super simplex (st -1) has an empty row; k-d
             // overflow (st -2) and budget 0 go to the larger levels
             act[j] = qi[j] >= 0 && st[j] >= 0 && b > 0;
@@ -1339,9 +1345,9 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
         int ncav[grp::Q] = {0, 0, 0, 0}, nbnd[grp::Q] = {0, 0, 0, 0}, nnb[grp::Q] = {0, 0, 0, 0};
         int nvis = 0;
         if (any) {
-            for (int i = gl; i < 256; i += grp::G) {
-                g.vkey[i] = -1;
-                g.vflag[i] = 0;
+            for (int i = gl; i < 64; i += grp::G) {
+                reinterpret_cast<int4 *>(g.vkey)[i] = make_int4(-1, -1, -1, -1);
+                reinterpret_cast<unsigned *>(g.vflag)[i] = 0u;
             }
             if (gl == 0) g.cnt[3] = 0;
             group_sync();
