@@ -242,7 +242,8 @@ void nat_graph_free(nat_graph *g);
  *
  * nat_topology_save copies what the last nat_weights_compute on this mesh
  * learned about its queries -- the cavity, boundary faces and neighbour
- * order of every row the level-0 kernels made, and the query order -- into
+ * order of every row the level-0 kernels made, the start tet of every
+ * query, and the query order -- into
  * caller-owned device memory, so that the backward pass recomputes the
  * geometry from it instead of searching again.  With d_dst NULL it only
  * reports the size in *bytes.  Returns -7 when capacity is too small. */

@@ -103,7 +103,7 @@ struct nat_mesh {
     int kd_depth_max = 0;
     std::vector<nat::LocParams> lph;      // [P]
     nat::DevMesh dev{};
-    DevBuf verts, tet_v, tet_n, fat, kd, cell_start, cell_pts, lp;
+    DevBuf verts, tet_v, tet_n, fat, sph, kd, cell_start, cell_pts, lp;
     DevBuf vorig;                         // [N+4P] device vertex -> key index
     DevBuf vals_dev;                      // backward: values in device vertex order
     DevBuf vals_fwd;                      // nat_weights_spmm: values in device vertex order
@@ -181,7 +181,8 @@ struct nat_mesh {
     nat_timings times{};
 
     int64_t device_bytes() const {
-        return static_cast<int64_t>(verts.bytes + tet_v.bytes + tet_n.bytes + fat.bytes + kd.bytes +
+        return static_cast<int64_t>(verts.bytes + tet_v.bytes + tet_n.bytes + fat.bytes +
+                                    sph.bytes + kd.bytes +
                                     cell_start.bytes + cell_pts.bytes + lp.bytes +
                                     old_vol.bytes + vorig.bytes);
     }
@@ -199,7 +200,8 @@ struct nat_mesh {
         return static_cast<int64_t>(s);
     }
     void release_all() {
-        DevBuf *all[] = {&verts,    &tet_v,     &tet_n,     &fat,      &kd,       &cell_start, &cell_pts,
+        DevBuf *all[] = {&verts,    &tet_v,     &tet_n,     &fat,      &sph,      &kd,
+                         &cell_start, &cell_pts,
                          &q,        &order,     &geom_order, &sortw,   &scan,
                          &row_off,  &row_cnt,   &rowptr64,  &row_total, &ovf[0],    &ovf[1],
                          &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,   &csr_rowptr,
@@ -555,6 +557,7 @@ int upload_mesh(nat_mesh *m) {
     d.tet_v = m->tet_v.as<int4>();
     d.tet_n = m->tet_n.as<int4>();
     d.fat = m->fat.as<nat::FatTet>();
+    d.sph = nullptr;
     d.kd = m->kd.as<double4>();
     d.n_data = static_cast<int>(NT);
     d.n_tets = static_cast<int>(TT);
@@ -562,6 +565,11 @@ int upload_mesh(nat_mesh *m) {
     d.cell_pts = m->cell_pts.as<double4>();
     d.lp = m->lp.as<nat::LocParams>();
     d.prob = nullptr;
+    // the cavity search's circumspheres, from the fat tets (64 B per tet)
+    HIP_OR_FAIL(m->sph.ensure(64 * std::max<size_t>(1, tv.size())));
+    HIP_OR_FAIL(nat::launch_sphere(d, m->sph.as<nat::SphereTet>(), nullptr));
+    HIP_OR_FAIL(hipStreamSynchronize(nullptr));
+    d.sph = m->sph.as<nat::SphereTet>();
     HIP_OR_FAIL(m->route.ensure(sizeof(unsigned short) * P));
     d.route = m->route.as<unsigned short>();
     m->budget.assign(P, static_cast<unsigned char>(kGroupBudget));
@@ -919,14 +927,17 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
 
 // Saved topology: int64 header {magic, M, record units, N, P, 0, 0, 0},
 // rec_off int64[M], the query order int32[M], the problem of each query
-// int32[M] (filled for batches, P > 1), the row totals float64[M], then the
-// records (16-byte aligned).
-constexpr long long kTopoMagic = 0x4e41545450304c33LL;
+// int32[M] (filled for batches, P > 1), the start tet of each query int32[M]
+// (the forward's locate, so that the backward does not locate the rows
+// without a record again), the row totals float64[M], then the records
+// (16-byte aligned).
+constexpr long long kTopoMagic = 0x4e41545450304c34LL;
 constexpr size_t kTopoHeader = 64;
 size_t topo_order_off(long long M) { return kTopoHeader + 8 * static_cast<size_t>(M); }
 size_t topo_prob_off(long long M) { return topo_order_off(M) + 4 * static_cast<size_t>(M); }
+size_t topo_start_off(long long M) { return topo_prob_off(M) + 4 * static_cast<size_t>(M); }
 size_t topo_total_off(long long M) {
-    return (topo_prob_off(M) + 4 * static_cast<size_t>(M) + 7) & ~static_cast<size_t>(7);
+    return (topo_start_off(M) + 4 * static_cast<size_t>(M) + 7) & ~static_cast<size_t>(7);
 }
 size_t topo_rec_off(long long M) {
     return (topo_total_off(M) + 8 * static_cast<size_t>(M) + 15) & ~static_cast<size_t>(15);
@@ -988,19 +999,15 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
     m->times.grad_wave_ms = 0.0;
     m->times.grad_wave_items = 0;
     if (n_items > 0 && m->wave_rows) {
-        // start tets of the rows without a record, then one wave per row
-        HIP_OR_FAIL(m->start.ensure(sizeof(int) * Mz));
-        int nthr0 = 0, nw = 0;
-        if (int rc = ensure_level_scratch(m, 0, n_items, &nthr0)) return rc;
+        // one wave per row without a record, from the forward's start tet
+        // (the same locate with the same capacities: a k-d overflow, -2, goes
+        // on to the thread-per-query levels, which locate with larger ones)
+        int nw = 0;
         if (int rc = ensure_wave_scratch(m, n_items, &nw)) return rc;
-        nat::Scratch sc0;
-        nat::scratch_carve(m->scratch[0].p, m->caps[0], m->scap[0], &sc0);
+        const int *start = reinterpret_cast<const int *>(blob + topo_start_off(M));
         mark(m, 2, st);
-        HIP_OR_FAIL(nat::launch_locate(m->dev, m->caps[0], sc0, (nthr0 + 255) / 256, d_q, items,
-                                       n_items, m->start.as<int>(), st));
         HIP_OR_FAIL(nat::launch_wave_grad(m->dev, m->wscr.as<char>(), nw, d_q, items, n_items,
-                                          m->start.as<int>(), ctr + 6, g, m->ovf[1].as<int>(),
-                                          ctr + 12, st));
+                                          start, ctr + 6, g, m->ovf[1].as<int>(), ctr + 12, st));
         mark(m, 3, st);
         unsigned nnext = 0;
         HIP_OR_FAIL(hipMemcpyAsync(&nnext, ctr + 12, sizeof(unsigned), hipMemcpyDeviceToHost, st));
@@ -1861,6 +1868,8 @@ int nat_topology_save(nat_mesh *m, void *d_dst, int64_t capacity, int64_t *bytes
         // the backward keeps the Morton order: its gathers (values, dL/dy)
         // favour spatial locality over grouping by cavity size
         HIP_OR_FAIL(hipMemcpyAsync(dst + topo_order_off(M), m->order.p,
+                                   4 * static_cast<size_t>(M), hipMemcpyDeviceToDevice, st));
+        HIP_OR_FAIL(hipMemcpyAsync(dst + topo_start_off(M), m->start.p,
                                    4 * static_cast<size_t>(M), hipMemcpyDeviceToDevice, st));
         HIP_OR_FAIL(hipMemcpyAsync(dst + topo_total_off(M), m->row_total.p,
                                    8 * static_cast<size_t>(M), hipMemcpyDeviceToDevice, st));

@@ -44,12 +44,16 @@ struct alignas(128) FatTet {
 };
 static_assert(sizeof(FatTet) == 128, "one line per tet");
 
+struct SphereTet;  // sphere.h: circumsphere and certificate constants of a tet
+
 struct DevMesh {
     const double4 *verts;  // [N+4P] x,y,z,0; slots N.. are the super vertices
     const int4 *tet_v;     // [T] vertex slots p0..p3; bit 31 of .w set when
                            //     orient3d(p0,p1,p2,p3) <= 0
     const int4 *tet_n;     // [T] neighbour across face 0..3, -1 = none
     const FatTet *fat;     // [T] the same tets with their vertex coordinates inline
+    const SphereTet *sph;  // [T] circumsphere, certificate constants and neighbours per
+                           //     tet: the cavity search's in-sphere shortcut (sphere.h)
     const double4 *kd;     // [T] k-d node i = tet i: centroid x,y,z and
                            //     .w = bit pattern of int2(left, right)
     int n_data;            // N (all problems)
@@ -220,6 +224,8 @@ hipError_t launch_grad_geom(const DevMesh &m, int nblocks, const double *d_q, co
                             int n_items, unsigned int *d_work, const Topo &topo,
                             const GradIn &g, int *ovf_list, unsigned int *ovf_count,
                             hipStream_t st);
+// the cavity search's per-tet circumspheres (out: n_tets records), from fat
+hipError_t launch_sphere(const DevMesh &m, SphereTet *out, hipStream_t st);
 // the step-3 table of DevMesh::old_vol (out: n_tets * 64 doubles)
 hipError_t launch_old_vol(const DevMesh &m, double *out, hipStream_t st);
 // wave-per-query rows (cavities beyond the group kernel): per-wave scratch

@@ -24,6 +24,7 @@
 #include "geom.h"
 #include "geom32.h"
 #include "nat_internal.h"
+#include "sphere.h"
 
 namespace nat {
 
@@ -603,6 +604,17 @@ __device__ __forceinline__ void old_vol_vertex(const Quad &P, const Quad &Fo, co
         out[mask * 4 + VI] = vol;
         out[(mask | (1u << VI)) * 4 + VI] = vol;
     }
+}
+
+// The cavity search's circumsphere of each tet (sphere.h), from its fat
+// record, with the vertex order the search passes to insphere.
+__global__ void __launch_bounds__(256) sphere_kernel(DevMesh m, SphereTet *__restrict__ out) {
+    const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= m.n_tets) return;
+    const FatTet &ft = m.fat[t];
+    d3 p0, p1, p2, p3;
+    fat_points(ft, p0, p1, p2, p3);
+    out[t] = ft.v.w < 0 ? sphere_of(p1, p0, p2, p3, ft.n) : sphere_of(p0, p1, p2, p3, ft.n);
 }
 
 __global__ void __launch_bounds__(256) old_vol_kernel(DevMesh m, double *__restrict__ out) {
@@ -1297,9 +1309,10 @@ __device__ __forceinline__ unsigned adjacency(const QuadMem &g, int s, int j) {
     return a;
 }
 
-// 64-thread blocks of 4 quads; 17 KB of LDS per block allow 9 per CU
+// 64-thread blocks of 4 quads; 13.2 KB of LDS per block allow 12 per CU, and
+// at most 168 VGPRs keep 3 waves per SIMD
 template <bool ROBUST>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
 topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restrict__ items,
                    int n_items, const int *__restrict__ start_of, unsigned int *work, Topo topo,
                    unsigned int *ovf_count, int *ovf_list) {
@@ -1405,21 +1418,34 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
                     if (i < f1) {
                         const int slot = g.vlist[i];
                         const int t = g.vtet[i];
-                        const FatTet &ft = m.fat[t];
-                        const bool neg = ft.v.w < 0;
-                        const int4 tn = ft.n;
-                        d3 p0, p1, p2, p3;
-                        fat_points(ft, p0, p1, p2, p3);
-                        if (neg) {
-                            const d3 tmp = p0;
-                            p0 = p1;
-                            p1 = tmp;
-                        }
-                        unsigned in4 = 0;
+                        // the certified circumsphere test first; the
+                        // determinant only where it cannot decide
+                        // (one copy of the record: every query's test
+                        // reads it, so its loads are issued once, together)
+                        const SphereTet sp = m.sph[t];
+                        const int4 tn = sp.n;
+                        unsigned in4 = 0, unc = 0;
 #pragma unroll
-                        for (int j = 0; j < grp::Q; j++)
-                            if (act[j] && inside_sphere<ROBUST>(m, p0, p1, p2, p3, qq[j]))
-                                in4 |= 1u << j;
+                        for (int j = 0; j < grp::Q; j++) {
+                            const int sd = sphere_side(sp, qq[j]);
+                            in4 |= (act[j] && sd > 0 ? 1u : 0u) << j;
+                            unc |= (act[j] && sd < 0 ? 1u : 0u) << j;
+                        }
+                        if (unc) {
+                            const FatTet &ft = m.fat[t];
+                            d3 p0, p1, p2, p3;
+                            fat_points(ft, p0, p1, p2, p3);
+                            if (ft.v.w < 0) {
+                                const d3 tmp = p0;
+                                p0 = p1;
+                                p1 = tmp;
+                            }
+#pragma unroll
+                            for (int j = 0; j < grp::Q; j++)
+                                if (((unc >> j) & 1u) &&
+                                    inside_sphere<ROBUST>(m, p0, p1, p2, p3, qq[j]))
+                                    in4 |= 1u << j;
+                        }
                         unsigned packed = 0xffffffffu;
                         if (in4) {
                             packed = 0;
@@ -3261,6 +3287,12 @@ hipError_t launch_rows(const DevMesh &m, const Caps &caps, const Scratch &sc, in
     else
         hipLaunchKernelGGL(sibson_rows_kernel<false>, dim3(nblocks), dim3(256), 0, st, m, caps,
                            sc, level, d_q, d_items, n_items, d_n_items, d_work, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sphere(const DevMesh &m, SphereTet *out, hipStream_t st) {
+    if (m.n_tets > 0)
+        hipLaunchKernelGGL(sphere_kernel, dim3((m.n_tets + 255) / 256), dim3(256), 0, st, m, out);
     return hipGetLastError();
 }
 
