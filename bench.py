@@ -64,22 +64,25 @@ def kernel_bytes_per_query(w, D):
     topo_search_kernel: order + start (8), query (24), vertex and neighbour
       ids of every tested tet (32 T), their vertices (24 V), the record it
       writes: header, C entries and the handoff (tet, adjacency) (16 + 24 C);
-    topo_dfs_kernel + topo_order_kernel: the handoff and entries read back
-      (16 + 24 C), the cavity order (2 C) and boundary faces (B) written and
-      read, the entries, neighbour list and header written (16 + 16 C + 4 nnz);
+    topo_dfs_kernel + topo_order_kernel: the header and handoff read back
+      (16 + 8 C), the cavity order (2 C) and boundary faces (B) written and
+      read back, the cavity tets' vertex ids (16 C), the entries, neighbour
+      list and header written (16 + 16 C + 4 nnz);
     geom_rows_kernel: order + record offset (12), query (24), the record
-      sections it reads (16 + 18 C + B + 4 nnz), tet vertex ids of cavity tets
-      (16 C), vertices (24 V), row written (12 nnz) and its offset/length (12);
+      sections it reads (header, entries, boundary faces, neighbour list:
+      16 + 16 C + B + 4 nnz), the step-3 table's slice per cavity tet (32 C),
+      tet vertex ids of cavity tets (16 C), vertices (24 V), row written
+      (12 nnz) and its offset/length (12);
     grad_geom_kernel: as geom_rows_kernel without the row, plus dL/dy (8 D),
       the neighbours' values (8 D nnz) and dL/dq written (24)."""
     C, B, T, V, nnz = w["cavity"], w["boundary"], w["tets_read"], w["verts_read"], w["nnz"]
-    rec_read = 16 + 18 * C + B + 4 * nnz
+    rec_read = 16 + 16 * C + B + 4 * nnz
     geo = 12 + 24 + rec_read + 16 * C + 24 * V
     return {
         "topo_search_kernel": 8 + 24 + 32 * T + 24 * V + 16 + 24 * C,
-        "topo_dfs_kernel+topo_order_kernel": (16 + 24 * C) + 2 * (2 * C + B)
+        "topo_dfs_kernel+topo_order_kernel": (16 + 8 * C) + 2 * (2 * C + B) + 16 * C
                                              + (16 + 16 * C + 4 * nnz),
-        "geom_rows_kernel": geo + 12 * nnz + 12,
+        "geom_rows_kernel": geo + 32 * C + 12 * nnz + 12,
         "grad_geom_kernel": geo + 8 * D + 8 * D * nnz + 24,
     }
 

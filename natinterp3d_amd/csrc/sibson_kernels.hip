@@ -276,6 +276,21 @@ __device__ __forceinline__ bool inside_sphere(const DevMesh &m, const d3 &a, con
     }
     return ins > 0;
 }
+// The search's decision for tet t (record sp = m.sph[t]): the certified
+// circumsphere test, the tet's determinant where that cannot decide.
+template <bool ROBUST>
+__device__ __forceinline__ bool tet_inside(const DevMesh &m, const SphereTet &sp, int t,
+                                           const d3 &q) {
+    const int sd = sphere_side(sp, q);
+    if (sd >= 0) return sd > 0;
+    bool neg;
+    const int4 tv = ld_tet_v(m, t, neg);
+    const d3 p0 = ld_vert(m, tv.x), p1 = ld_vert(m, tv.y);
+    const d3 p2 = ld_vert(m, tv.z), p3 = ld_vert(m, tv.w);
+    return neg ? inside_sphere<ROBUST>(m, p1, p0, p2, p3, q)
+               : inside_sphere<ROBUST>(m, p0, p1, p2, p3, q);
+}
+
 // q on the outer side of face (a, b, c)
 template <bool ROBUST>
 __device__ __forceinline__ bool separated(const DevMesh &m, const d3 &a, const d3 &b,
@@ -399,18 +414,13 @@ __device__ int cavity_search(const DevMesh &m, const Caps &caps, const Scratch &
         const unsigned h = vis.insert(cur, found);
         if (found) continue;
         if (++nvis >= vis_max) return -1;
-        bool neg;
-        const int4 tv = ld_tet_v(m, cur, neg);
-        const d3 p0 = ld_vert(m, tv.x), p1 = ld_vert(m, tv.y);
-        const d3 p2 = ld_vert(m, tv.z), p3 = ld_vert(m, tv.w);
-        const bool ins = neg ? inside_sphere<ROBUST>(m, p1, p0, p2, p3, q)
-                             : inside_sphere<ROBUST>(m, p0, p1, p2, p3, q);
-        if (ins) {
+        const SphereTet sp = m.sph[cur];
+        if (tet_inside<ROBUST>(m, sp, cur, q)) {
             if (ncav >= caps.cav) return -1;
             sc.cav[il(ncav, S, tid)] = cur;
             vis.set(h, ncav);
             ncav++;
-            const int4 tn = m.tet_n[cur];
+            const int4 tn = sp.n;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int nb = sel4(tn, i);
@@ -2138,13 +2148,9 @@ __device__ bool wave_topology(const DevMesh &m, const WaveScr &W, WaveLds &L, in
             if (i < f1) {
                 const int slot = W.vlist[i];
                 const int t = W.vtet[i];
-                bool neg;
-                const int4 tv = ld_tet_v(m, t, neg);
-                const int4 tn = m.tet_n[t];
-                const d3 p0 = ld_vert(m, tv.x), p1 = ld_vert(m, tv.y);
-                const d3 p2 = ld_vert(m, tv.z), p3 = ld_vert(m, tv.w);
-                const bool ins = neg ? inside_sphere<ROBUST>(m, p1, p0, p2, p3, qq)
-                                     : inside_sphere<ROBUST>(m, p0, p1, p2, p3, qq);
+                const SphereTet sp = m.sph[t];
+                const int4 tn = sp.n;
+                const bool ins = tet_inside<ROBUST>(m, sp, t, qq);
                 if (ins) {
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
