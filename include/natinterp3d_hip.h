@@ -283,7 +283,10 @@ int nat_topology_row_order(const void *d_topo, int num_query_points, const int32
  * in d_row_order when it is given (NULL: 0..M-1); a spatial order such as
  * nat_topology_row_order's keeps the atomic additions of neighbouring rows
  * on the same cache lines.  Float64 atomic accumulation: the order of the
- * additions into one output is not fixed. */
+ * additions into one output is not fixed.  When the output is small (at most
+ * 512 KB: few columns under many entries), 8 to 64 private copies of it (at
+ * most 32 MB) are allocated and freed in stream order (hipMallocAsync) so
+ * that the atomics spread over more addresses; they are summed into d_out. */
 int nat_spmm_t_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
                    const int32_t *d_cols, const double *d_vals, const double *d_gy,
                    const int32_t *d_row_order, double *d_out, void *stream);

@@ -2165,8 +2165,29 @@ int nat_spmm_t_csr(int num_rows, int num_cols, int dim, const int32_t *d_rowptr,
     if (num_rows < 0 || num_cols < 0 || dim < 0) return fail(-1, "negative size");
     if (num_cols == 0 || dim == 0) return 0;
     if (!d_out || (num_rows > 0 && (!d_rowptr || !d_gy))) return fail(-1, "NULL argument");
+    // few columns (at most 512 KB of output): 8 to 64 private copies of the
+    // output, 32 MB at most, stream-ordered, so that the blocks' atomics do
+    // not all queue on the same addresses
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t out_bytes = sizeof(double) * static_cast<size_t>(num_cols) * dim;
+    int P = 1;
+    double *priv = nullptr;
+    if (num_rows > 0 && out_bytes <= (512u << 10)) {
+        P = static_cast<int>(std::min<size_t>(64, (32u << 20) / out_bytes));
+        while (P & (P - 1)) P &= P - 1;  // a power of two
+        if (P < 8 || hipMallocAsync(reinterpret_cast<void **>(&priv), out_bytes * P, st) !=
+                         hipSuccess) {
+            (void)hipGetLastError();
+            P = 1;
+            priv = nullptr;
+        }
+    }
     hipError_t e = nat::launch_spmm_t(num_rows, num_cols, dim, d_rowptr, d_cols, d_vals, d_gy,
-                                      d_row_order, d_out, static_cast<hipStream_t>(stream));
+                                      d_row_order, d_out, priv, P, st);
+    if (priv) {
+        const hipError_t ef = hipFreeAsync(priv, st);
+        if (e == hipSuccess) e = ef;
+    }
     if (e != hipSuccess) return fail(-10, "spmm_t launch failed: %s", hipGetErrorString(e));
     return 0;
 }

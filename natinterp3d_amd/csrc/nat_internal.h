@@ -247,9 +247,11 @@ hipError_t launch_grad_rows(const DevMesh &m, const Caps &caps, const Scratch &s
 // insphere decisions behind saved rows re-tested under the static error bound
 hipError_t launch_predicate_audit(const DevMesh &m, const double *d_q, int n, const Topo &topo,
                                   unsigned long long *d_counts, hipStream_t st);
+// W^T g into out [N, D]; with P > 1 and priv (P x N x D doubles) the blocks
+// accumulate into P private copies, summed into out at the end
 hipError_t launch_spmm_t(int M, int N, int D, const int *rowptr, const int *cols,
                          const double *vals, const double *gy, const int *order, double *out,
-                         hipStream_t st);
+                         double *priv, int P, hipStream_t st);
 // error statement of mixed rows: save the sampled rows (order[i * stride],
 // i < n) aside, then, after an FP64 geometry pass over them, compare and put
 // the mixed rows back; out[0..2] = max |dw| bits, rows, entries

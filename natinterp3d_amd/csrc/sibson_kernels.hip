@@ -3143,7 +3143,11 @@ __global__ void __launch_bounds__(256) spmm_t_kernel(int M, int D, const int *__
 // of a wave meet on a table slot at once), sums their contributions per
 // column in an LDS hash table of 2^hlog2 columns x D, and adds each distinct
 // column to out once, with float64 atomics.  An entry whose column finds no
-// table slot within 16 probes goes to out directly.
+// table slot within 16 probes goes to out directly.  With P > 1, block b adds
+// into private copy b mod P of the [N, D] output instead (priv_sum_kernel
+// then sums the copies): when few columns take many rows' entries -- 20k keys
+// under 2M rows of 459 entries -- the float64 atomics of every block would
+// otherwise queue on the same addresses.
 constexpr int kAggRows = 256;
 constexpr int kAggLanes = 16;
 
@@ -3153,8 +3157,10 @@ __global__ void __launch_bounds__(256) spmm_t_agg_kernel(int M, int D, int hlog2
                                                          const double *__restrict__ vals,
                                                          const double *__restrict__ gy,
                                                          const int *__restrict__ order,
-                                                         double *__restrict__ out) {
+                                                         double *__restrict__ out_all,
+                                                         int P, long long nd) {
     extern __shared__ double agg_lds[];
+    double *__restrict__ out = out_all + (P > 1 ? static_cast<long long>(blockIdx.x % P) * nd : 0);
     const int H = 1 << hlog2;
     double *hv = agg_lds;                                // [H * D]
     int *hk = reinterpret_cast<int *>(agg_lds + static_cast<size_t>(H) * D);  // [H]
@@ -3203,6 +3209,16 @@ __global__ void __launch_bounds__(256) spmm_t_agg_kernel(int M, int D, int hlog2
 __global__ void __launch_bounds__(256) fill_zero_kernel(double *p, long long n) {
     const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t < n) p[t] = 0.0;
+}
+
+// out[t] = sum of the P private copies (each nd values) at t
+__global__ void __launch_bounds__(256) priv_sum_kernel(const double *__restrict__ priv, int P,
+                                                       long long nd, double *__restrict__ out) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nd) return;
+    double v = 0.0;
+    for (int p = 0; p < P; p++) v += priv[static_cast<long long>(p) * nd + t];
+    out[t] = v;
 }
 
 // ------------------------------------------------------------ host helpers
@@ -3467,23 +3483,32 @@ hipError_t launch_predicate_audit(const DevMesh &m, const double *d_q, int n, co
 
 hipError_t launch_spmm_t(int M, int N, int D, const int *rowptr, const int *cols,
                          const double *vals, const double *gy, const int *order, double *out,
-                         hipStream_t st) {
+                         double *priv, int P, hipStream_t st) {
     const long long nz = static_cast<long long>(N) * D;
-    if (nz > 0)
-        hipLaunchKernelGGL(fill_zero_kernel, dim3(static_cast<unsigned>((nz + 255) / 256)),
-                           dim3(256), 0, st, out, nz);
     const long long nt = static_cast<long long>(M) * D;
-    if (nt <= 0) return hipGetLastError();
     // the largest power-of-two table of at most 2048 columns in 48 KB of LDS
     int hlog2 = 11;
     while (hlog2 > 0 && (static_cast<size_t>(1) << hlog2) * (4 + 8 * static_cast<size_t>(D)) >
                             48 * 1024)
         hlog2--;
-    if (hlog2 >= 7) {
+    const bool agg = hlog2 >= 7;
+    if (P < 2 || !priv || !agg || nt <= 0) P = 1;
+    // the accumulators: out itself, or P private copies summed into out at the end
+    double *acc = P > 1 ? priv : out;
+    const long long nacc = nz * P;
+    if (nacc > 0)
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(static_cast<unsigned>((nacc + 255) / 256)),
+                           dim3(256), 0, st, acc, nacc);
+    if (nt <= 0) return hipGetLastError();
+    if (agg) {
         const size_t lds = (static_cast<size_t>(1) << hlog2) * (4 + 8 * static_cast<size_t>(D));
         hipLaunchKernelGGL(spmm_t_agg_kernel,
                            dim3(static_cast<unsigned>((M + kAggRows - 1) / kAggRows)),
-                           dim3(256), lds, st, M, D, hlog2, rowptr, cols, vals, gy, order, out);
+                           dim3(256), lds, st, M, D, hlog2, rowptr, cols, vals, gy, order, acc,
+                           P, nz);
+        if (P > 1)
+            hipLaunchKernelGGL(priv_sum_kernel, dim3(static_cast<unsigned>((nz + 255) / 256)),
+                               dim3(256), 0, st, acc, P, nz, out);
     } else {
         hipLaunchKernelGGL(spmm_t_kernel, dim3(static_cast<unsigned>((nt + 255) / 256)),
                            dim3(256), 0, st, M, D, rowptr, cols, vals, gy, order, out);
