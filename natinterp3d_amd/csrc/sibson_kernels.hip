@@ -3143,7 +3143,11 @@ __global__ void __launch_bounds__(256) spmm_t_kernel(int M, int D, const int *__
 // of a wave meet on a table slot at once), sums their contributions per
 // column in an LDS hash table of 2^hlog2 columns x D, and adds each distinct
 // column to out once, with float64 atomics.  An entry whose column finds no
-// table slot within 16 probes goes to out directly.  With P > 1, block b adds
+// table slot within 8 probes goes to out directly.  The rows per block are
+// the power of two (16 to 256) that puts about 4 table sizes of entries in a
+// block, from the mean row length (rowptr[M] / M, read by each block): long
+// rows -- 459 entries per row on a hollow shell -- would otherwise fill the
+// table early and send most entries through failed probes.  With P > 1, block b adds
 // into private copy b mod P of the [N, D] output instead (priv_sum_kernel
 // then sums the copies): when few columns take many rows' entries -- 20k keys
 // under 2M rows of 459 entries -- the float64 atomics of every block would
@@ -3164,13 +3168,17 @@ __global__ void __launch_bounds__(256) spmm_t_agg_kernel(int M, int D, int hlog2
     const int H = 1 << hlog2;
     double *hv = agg_lds;                                // [H * D]
     int *hk = reinterpret_cast<int *>(agg_lds + static_cast<size_t>(H) * D);  // [H]
+    // rows of this block: R (a power of two, 16..256) from the mean row length
+    const long long nnz = rowptr[M];
+    int R = kAggRows;
+    while (R > 16 && static_cast<long long>(R) * nnz > 4LL * H * M) R >>= 1;
+    const long long base = static_cast<long long>(blockIdx.x) * R;
+    if (base >= M) return;
     for (int i = threadIdx.x; i < H; i += blockDim.x) hk[i] = -1;
     for (int i = threadIdx.x; i < H * D; i += blockDim.x) hv[i] = 0.0;
     __syncthreads();
     const int lane = static_cast<int>(threadIdx.x) & (kAggLanes - 1);
-    const long long base = static_cast<long long>(blockIdx.x) * kAggRows;
-    for (int k = static_cast<int>(threadIdx.x) / kAggLanes; k < kAggRows;
-         k += 256 / kAggLanes) {
+    for (int k = static_cast<int>(threadIdx.x) / kAggLanes; k < R; k += 256 / kAggLanes) {
         const long long ii = base + k;
         if (ii >= M) break;
         const int row = order ? order[ii] : static_cast<int>(ii);
@@ -3182,7 +3190,7 @@ __global__ void __launch_bounds__(256) spmm_t_agg_kernel(int M, int D, int hlog2
             const double w = vals[j];
             unsigned h = hash_key(c, hlog2);
             int slot = -1;
-            for (int n = 0; n < 16; n++) {
+            for (int n = 0; n < 8; n++) {
                 const int old = atomicCAS(&hk[h], -1, c);
                 if (old == -1 || old == c) {
                     slot = static_cast<int>(h);
@@ -3502,8 +3510,8 @@ hipError_t launch_spmm_t(int M, int N, int D, const int *rowptr, const int *cols
     if (nt <= 0) return hipGetLastError();
     if (agg) {
         const size_t lds = (static_cast<size_t>(1) << hlog2) * (4 + 8 * static_cast<size_t>(D));
-        hipLaunchKernelGGL(spmm_t_agg_kernel,
-                           dim3(static_cast<unsigned>((M + kAggRows - 1) / kAggRows)),
+        // enough blocks for 16 rows each; blocks beyond the rows leave at once
+        hipLaunchKernelGGL(spmm_t_agg_kernel, dim3(static_cast<unsigned>((M + 15) / 16)),
                            dim3(256), lds, st, M, D, hlog2, rowptr, cols, vals, gy, order, acc,
                            P, nz);
         if (P > 1)
