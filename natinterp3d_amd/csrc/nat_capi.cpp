@@ -701,6 +701,8 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
     // the records go straight into a caller's saved-topology buffer when one
     // was set for this many queries and they fit; else into the workspace
     char *ext_base = m->ext_topo && m->ext_M == M ? m->ext_topo : nullptr;
+    // a topology will be saved: the wave-per-query rows keep big records too
+    const bool save_big = ext_base != nullptr;
     long long ext_units = 0;
     if (ext_base) {
         ext_units = (m->ext_capacity - static_cast<long long>(topo_rec_off(M))) /
@@ -855,17 +857,32 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         if (n_items > 0 && m->wave_rows) {
             int nw = 0;
             if (int rc = ensure_wave_scratch(m, n_items, &nw)) return rc;
+            nat::RowOut wout = out;
+            if (save_big) wout.save = topo;
             mark(m, 2, st);
             HIP_OR_FAIL(nat::launch_wave_rows(m->dev, m->wscr.as<char>(), nw, d_q, items, n_items,
-                                              nullptr, m->start.as<int>(), ctr + 6, out,
+                                              nullptr, m->start.as<int>(), ctr + 6, wout,
                                               m->ovf[1].as<int>(), ctr + 12, st));
             mark(m, 3, st);
             unsigned nnext = 0;
+            unsigned long long used_rec = 0;
             HIP_OR_FAIL(
                 hipMemcpyAsync(&nnext, ctr + 12, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+            HIP_OR_FAIL(hipMemcpyAsync(&used_rec, rec_used, sizeof(used_rec),
+                                       hipMemcpyDeviceToHost, st));
             HIP_OR_FAIL(hipStreamSynchronize(st));
             m->times.wave_ms = elapsed(m, 2, 3);
             m->times.wave_items = n_items;
+            if (save_big) {
+                const long long u2 = static_cast<long long>(used_rec);
+                if (u2 > topo.rec_cap) {  // the big records did not fit: again, larger
+                    want_rec = u2 + u2 / 8 + 1024;
+                    ext_base = nullptr;
+                    continue;
+                }
+                m->units_per_row = std::max(8.0, 1.125 * static_cast<double>(u2) / M);
+                m->last_units = u2;
+            }
             n_items = static_cast<int>(nnext);
             cur = 1;
             items = m->ovf[1].as<int>();
@@ -979,7 +996,8 @@ int grad_rows(nat_mesh *m, const char *blob, const double *d_q, int M, const int
     HIP_OR_FAIL(nat::launch_gather_rows(d_values, m->vorig.as<int>(), nd, D,
                                         m->vals_dev.as<double>(), st));
     nat::GradIn g{d_gy, m->vals_dev.as<double>(), D, d_dq, d_rowptr, d_w,
-                  reinterpret_cast<const double *>(blob + topo_total_off(M))};
+                  reinterpret_cast<const double *>(blob + topo_total_off(M)), topo.rec,
+                  topo.rec_off};
     for (int l = 0; l < kLevels; l++) {
         m->times.grad_ms[l] = 0.0;
         m->times.grad_items[l] = 0;

@@ -130,6 +130,15 @@ struct Scratch {
     int nthr;
 };
 
+// Per-query topology records written by the topology kernel and consumed by
+// the geometry kernel (16-byte units, one contiguous record per query).
+struct Topo {
+    int4 *rec;                   // record pool
+    long long rec_cap;           // in int4 units
+    unsigned long long *rec_used;
+    long long *rec_off;          // [M] record offset, -1 = row made by another level
+};
+
 struct RowOut {
     double *pool_vals;           // row entries in completion order
     int *pool_cols;
@@ -140,15 +149,9 @@ struct RowOut {
     double *row_total;           // [M] total stolen volume of each row (level 0 rows)
     int *ovf_list;               // queries that exceeded the level's capacities
     unsigned int *ovf_count;
-};
-
-// Per-query topology records written by the topology kernel and consumed by
-// the geometry kernel (16-byte units, one contiguous record per query).
-struct Topo {
-    int4 *rec;                   // record pool
-    long long rec_cap;           // in int4 units
-    unsigned long long *rec_used;
-    long long *rec_off;          // [M] record offset, -1 = row made by another level
+    // when a topology will be saved (save.rec set): where the wave-per-query
+    // rows put their big records (header .w = 2) for the backward
+    Topo save{};
 };
 
 // Backward inputs: dL/dy rows and the interpolated values, both float64;
@@ -161,6 +164,8 @@ struct GradIn {
     const int *rowptr;        // [M + 1]
     const double *w;          // [nnz] weights
     const double *row_total;  // [M] total stolen volume (rows with a record)
+    const int4 *rec;          // the saved records (big records of the wave-per-query rows)
+    const long long *rec_off; // [M]
 };
 
 size_t scratch_bytes_per_thread(const Caps &c);

@@ -1248,6 +1248,13 @@ __device__ __forceinline__ int rec_x_off(int C, int B, int room) {
 __device__ __forceinline__ int rec_units(int C, int B, int room) {
     return rec_x_off(C, B, room) + 2 * ((C + 3) / 4);
 }
+// neighbour slot of vertex vi of a record entry cr (-1: none)
+__device__ __forceinline__ int rec_slot(const int4 &cr, int vi) {
+    const int w = vi < 2 ? cr.y : cr.z;
+    const int s = (vi & 1) ? ((w >> 16) & 0xffff) : (w & 0xffff);
+    return s == 0xffff ? -1 : s;
+}
+
 // sections of a complete record whose header is hd
 __device__ __forceinline__ const unsigned short *rec_order(const int4 *r, const int4 &hd) {
     return reinterpret_cast<const unsigned short *>(r + rec_ord_off(hd.x));
@@ -2382,6 +2389,22 @@ __device__ bool wave_topology(const DevMesh &m, const WaveScr &W, WaveLds &L, in
     return true;
 }
 
+// Big record of a wave-per-query row (header .w = 2), kept only when the
+// forward's topology is saved: the backward's gradient then reads the cavity
+// instead of searching again.  In 16-byte units: [0] (C, B, NB, 2); [1..C]
+// the cavity entries as in a level-0 record (tet, vertex slots, face mask);
+// ceil(B/8) units of boundary faces, 16 bits each (cavity position << 2 |
+// face); ceil(NB/4) units of neighbour vertex indices in slot order.
+__device__ __forceinline__ int big_units(int C, int B, int NB) {
+    return 1 + C + (B + 7) / 8 + (NB + 3) / 4;
+}
+__device__ __forceinline__ const unsigned short *big_bnd(const int4 *r, int C) {
+    return reinterpret_cast<const unsigned short *>(r + 1 + C);
+}
+__device__ __forceinline__ const int *big_nbv(const int4 *r, int C, int B) {
+    return reinterpret_cast<const int *>(r + 1 + C + (B + 7) / 8);
+}
+
 // Forward (GRAD = false): the row into the pool, bit-identical to the
 // reference.  Gradient: dL/dq from the same cavity (rows left empty by the
 // reference get 0).  Queries that exceed a capacity go to ovf_list.
@@ -2407,6 +2430,56 @@ __global__ void __launch_bounds__(64) wave_rows_kernel(DevMesh m, char *wscr,
         const int st = start_of[qi];
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
+        if constexpr (GRAD) {
+            // a big record of the forward: dL/dq from its boundary faces, as
+            // grad_geom_kernel does from a level-0 record, reduced over the wave
+            const long long roff = g.rec_off ? g.rec_off[qi] : -1;
+            const int4 *r = roff >= 0 ? g.rec + roff : nullptr;
+            const int4 hd = r ? r[0] : make_int4(0, 0, 0, 0);
+            if (hd.w == 2) {
+                const int Cr = hd.x, Br = hd.y, NBr = hd.z;
+                const int e0 = g.rowptr[qi];
+                if (g.rowptr[qi + 1] - e0 != NBr) {
+                    if (lane == 0) store_dq(g, qi, {0.0, 0.0, 0.0});
+                    continue;
+                }
+                const int *rn = big_nbv(r, Cr, Br);
+                double abar = 0.0;
+                for (int j = lane; j < NBr; j += 64) {
+                    const double a = grad_dot(g, qi, rn[j]);
+                    W.acc[j] = a;
+                    W.val[j] = facet_scale(ld_vert(m, rn[j]), qq);
+                    abar += a * g.w[e0 + j];
+                }
+                wave_sync();
+                d3 ag = {0.0, 0.0, 0.0}, tg = {0.0, 0.0, 0.0};
+                const unsigned short *bb = big_bnd(r, Cr);
+                for (int b = lane; b < Br; b += 64) {
+                    const int code = bb[b];
+                    const int4 cr = r[1 + (code >> 2)];
+                    new_contributions<false>(
+                        m, cr.x, code & 3, qq, [&](int vi, int) { return rec_slot(cr, vi); },
+                        [](int, double) {},
+                        moment6_fn([&](int slot, const d3 &g6) {
+                            const double sc = W.val[slot];
+                            const d3 gm = {g6.x * sc, g6.y * sc, g6.z * sc};
+                            const double a = W.acc[slot];
+                            ag = {ag.x + a * gm.x, ag.y + a * gm.y, ag.z + a * gm.z};
+                            tg = {tg.x + gm.x, tg.y + gm.y, tg.z + gm.z};
+                        }));
+                }
+                abar = wave_sum_f64(abar);
+                ag = {wave_sum_f64(ag.x), wave_sum_f64(ag.y), wave_sum_f64(ag.z)};
+                tg = {wave_sum_f64(tg.x), wave_sum_f64(tg.y), wave_sum_f64(tg.z)};
+                if (lane == 0) {
+                    const double inv = 1.0 / g.row_total[qi];
+                    store_dq(g, qi, {(ag.x - abar * tg.x) * inv, (ag.y - abar * tg.y) * inv,
+                                     (ag.z - abar * tg.z) * inv});
+                }
+                __syncthreads();  // W.acc and W.val are rewritten by the next row
+                continue;
+            }
+        }
         int C = 0, B = 0, NB = 0;
         if (st < 0 || !wave_topology<!GRAD, ROBUST>(m, W, L, st, qq, C, B, NB)) {
             if (lane == 0) ovf_list[atomicAdd(ovf_count, 1u)] = qi;
@@ -2580,6 +2653,31 @@ __global__ void __launch_bounds__(64) wave_rows_kernel(DevMesh m, char *wscr,
                 out.row_cnt[qi] = cnt;
                 out.row_total[qi] = total;
             }
+            // the big record, when the topology will be saved
+            if (out.save.rec && cnt > 0 && off + cnt <= out.pool_cap) {
+                const int units = big_units(C, B, NB);
+                unsigned long long rb = 0;
+                if (lane == 0)
+                    rb = atomicAdd(out.save.rec_used, static_cast<unsigned long long>(units));
+                rb = __shfl(rb, 0, 64);
+                const long long ro = static_cast<long long>(rb);
+                if (ro + units <= out.save.rec_cap) {
+                    int4 *r = out.save.rec + ro;
+                    for (int ci = lane; ci < C; ci += 64) {
+                        const uint2 cs = W.cslot[ci];
+                        r[1 + ci] = make_int4(W.dtet[W.cav[ci]], static_cast<int>(cs.x),
+                                              static_cast<int>(cs.y), W.cmask[ci]);
+                    }
+                    unsigned short *bb = reinterpret_cast<unsigned short *>(r + 1 + C);
+                    for (int b = lane; b < B; b += 64) bb[b] = static_cast<unsigned short>(W.bnd[b]);
+                    int *nv = reinterpret_cast<int *>(r + 1 + C + (B + 7) / 8);
+                    for (int j = lane; j < NB; j += 64) nv[j] = W.nbv[j];
+                    if (lane == 0) {
+                        r[0] = make_int4(C, B, NB, 2);
+                        out.save.rec_off[qi] = ro;
+                    }
+                }
+            }
             __syncthreads();
         }
     }
@@ -2630,11 +2728,6 @@ __global__ void __launch_bounds__(256) work_sort_kernel(const int *__restrict__ 
     }
 }
 
-__device__ __forceinline__ int rec_slot(const int4 &cr, int vi) {
-    const int w = vi < 2 ? cr.y : cr.z;
-    const int s = (vi & 1) ? ((w >> 16) & 0xffff) : (w & 0xffff);
-    return s == 0xffff ? -1 : s;
-}
 
 // Steps 3-5 from the topology record, one thread per query; the row's
 // per-neighbour accumulators live in LDS (slot j of thread t at
@@ -2665,6 +2758,7 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
             roff = topo.rec_off[qi];
         }
         const int4 *r = roff >= 0 ? topo.rec + roff : nullptr;
+        if (r && r[0].w == 2) r = nullptr;  // a wave-per-query row's big record
         if (r) {
             const int4 hd = r[0];
             const int ncav = hd.x, nbnd = hd.y;
@@ -2831,6 +2925,7 @@ __global__ void __launch_bounds__(kGeomRowsThreads) geom_rows_kernel(
             roff = topo.rec_off[qi];
         }
         const int4 *r = roff >= 0 ? topo.rec + roff : nullptr;
+        if (r && r[0].w == 2) r = nullptr;  // a wave-per-query row's big record
         if (r) {
             const int4 hd = r[0];
             const int ncav = hd.x, nbnd = hd.y, nnb = hd.z;
@@ -3022,7 +3117,7 @@ __global__ void __launch_bounds__(kGradThreads) grad_geom_kernel(
         const int4 *r = topo.rec + roff;
         const int4 hd = r[0];
         const int ncav = hd.x, nbnd = hd.y, nnb = hd.z;
-        if (nnb > kLevel0Row) {  // not produced by the topology kernel; recompute
+        if (nnb > kLevel0Row || hd.w == 2) {  // a wave-per-query row: that level
             if (gl == 0) ovf_list[atomicAdd(ovf_count, 1u)] = qi;
             continue;
         }
@@ -3092,6 +3187,7 @@ __global__ void __launch_bounds__(256) predicate_audit_kernel(DevMesh m,
         if (roff < 0) continue;
         const int4 *r = topo.rec + roff;
         const int4 hd = r[0];
+        if (hd.w == 2) continue;  // a wave-per-query row's big record: not audited
         const int ncav = hd.x, nbnd = hd.y;
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
