@@ -108,6 +108,7 @@ struct nat_mesh {
     DevBuf vals_dev;                      // backward: values in device vertex order
     DevBuf vals_fwd;                      // nat_weights_spmm: values in device vertex order
     DevBuf old_vol;                       // step-3 table (built on the first compute)
+    DevBuf itab;                          // the search's item entries (Topo::itab)
     bool old_vol_tried = false;
     DevBuf prob, qoff;                    // batch calls: problem of each query
     std::mutex mu;
@@ -194,7 +195,8 @@ struct nat_mesh {
                    ovf[0].bytes +
                    ovf[1].bytes + counters.bytes + pool_vals.bytes + pool_cols.bytes +
                    csr_vals.bytes + csr_cols.bytes + csr_rowptr.bytes +
-                   rec.bytes + rec_off.bytes + start.bytes + wscr.bytes + vals_dev.bytes +
+                   rec.bytes + rec_off.bytes + start.bytes + wscr.bytes + itab.bytes +
+                   vals_dev.bytes +
                    vals_fwd.bytes + chk.bytes;
         for (int l = 0; l < kLevels; l++) s += scratch[l].bytes;
         return static_cast<int64_t>(s);
@@ -206,7 +208,7 @@ struct nat_mesh {
                          &row_off,  &row_cnt,   &rowptr64,  &row_total, &ovf[0],    &ovf[1],
                          &counters, &pool_vals, &pool_cols, &csr_vals, &csr_cols,   &csr_rowptr,
                          &rec,      &rec_off,   &start,     &lp,       &prob,       &qoff,
-                         &route,    &wscr,      &old_vol,   &vorig,    &vals_dev,
+                         &route,    &wscr,      &itab,      &old_vol,   &vorig,    &vals_dev,
                          &vals_fwd, &chk};
         for (DevBuf *b : all) b->release();
         for (int l = 0; l < kLevels; l++) scratch[l].release();
@@ -759,6 +761,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
 
     HIP_OR_FAIL(m->rec_off.ensure(sizeof(long long) * Mz));
     HIP_OR_FAIL(m->start.ensure(sizeof(int) * Mz));
+    HIP_OR_FAIL(m->itab.ensure(sizeof(int4) * Mz));
     const long long check_room =
         n_check ? static_cast<long long>(std::ceil(2.0 * m->nnz_per_row * n_check)) + 1024 : 0;
     long long want = static_cast<long long>(std::ceil(m->nnz_per_row * M)) + 1024 + check_room;
@@ -802,6 +805,7 @@ int compute_rows(nat_mesh *m, const double *d_q, int M, const int *d_prob, hipSt
         topo.rec_cap = ext_base ? ext_units : m->rec_cap;
         topo.rec_used = rec_used;
         topo.rec_off = m->rec_off.as<long long>();
+        topo.itab = m->itab.as<int4>();
         mark(m, 2, st);
         HIP_OR_FAIL(nat::launch_locate(m->dev, m->caps[0], sc0, (nthr0 + 255) / 256, d_q,
                                        m->order.as<int>(), M, m->start.as<int>(), st));

@@ -137,6 +137,10 @@ struct Topo {
     long long rec_cap;           // in int4 units
     unsigned long long *rec_used;
     long long *rec_off;          // [M] record offset, -1 = row made by another level
+    // [items] per item of the search: (query, record offset low, high,
+    // C | B << 8 | room << 16), .w = -1 when the order kernels have nothing
+    // to do for it; NULL: they read the offset and the header instead
+    int4 *itab = nullptr;
 };
 
 struct RowOut {

@@ -1624,6 +1624,12 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
                                 : 1 + (nnb[j] + 3) / 4;
             fits[j] = off[j] + u <= topo.rec_cap;
             if (gl == j && qi[j] >= 0) {
+                const bool pending = ok[j] && fits[j] && ncav[j] > 0;
+                if (topo.itab)
+                    topo.itab[first + grp::Q * gw + j] = make_int4(
+                        qi[j], static_cast<int>(off[j] & 0xffffffffLL),
+                        static_cast<int>(off[j] >> 32),
+                        pending ? ncav[j] | (nbnd[j] << 8) | (nb_room(nbnd[j]) << 16) : -1);
                 if (!ok[j] || !fits[j]) {
                     topo.rec_off[qi[j]] = -1;
                     if (!ok[j]) ovf_list[atomicAdd(ovf_count, 1u)] = qi[j];
@@ -1691,13 +1697,28 @@ topo_dfs_kernel(const int *__restrict__ items, int n_items, Topo topo, unsigned 
     const int hi = static_cast<int>(static_cast<long long>(n_items) * (part + 1) / kParts);
     const int i = lo + static_cast<int>(blockIdx.x) / kParts * kDfsThreads + t;
     if (i >= hi) return;
-    const int qi = items ? items[i] : i;
-    const long long roff = topo.rec_off[qi];
-    if (roff < 0) return;
-    int4 *r = topo.rec + roff;
-    const int4 hd = r[0];
-    if (hd.w != 1) return;  // coincident or empty row: complete
-    const int C = hd.x, B = hd.y, room = hd.z;
+    int qi, C, B, room;
+    int4 *r;
+    if (topo.itab) {  // the search's item entry: no offset or header loads first
+        const int4 e = topo.itab[i];
+        if (e.w < 0) return;
+        qi = e.x;
+        r = topo.rec + (static_cast<long long>(static_cast<unsigned>(e.y)) |
+                        (static_cast<long long>(e.z) << 32));
+        C = e.w & 0xff;
+        B = (e.w >> 8) & 0xff;
+        room = (e.w >> 16) & 0xff;
+    } else {
+        qi = items ? items[i] : i;
+        const long long roff = topo.rec_off[qi];
+        if (roff < 0) return;
+        r = topo.rec + roff;
+        const int4 hd = r[0];
+        if (hd.w != 1) return;  // coincident or empty row: complete
+        C = hd.x;
+        B = hd.y;
+        room = hd.z;
+    }
     const int *xadj = rec_xadj(r, C, B, room);
     const int4 *xa4 = reinterpret_cast<const int4 *>(xadj);
     const int cl = min(C, kDfsLds);
@@ -1767,6 +1788,7 @@ topo_dfs_kernel(const int *__restrict__ items, int n_items, Topo topo, unsigned 
     }
     if (n != C || nb != B) {
         topo.rec_off[qi] = -1;
+        if (topo.itab) topo.itab[i].w = -1;
         ovf_list[atomicAdd(ovf_count, 1u)] = qi;
         return;
     }
@@ -1809,13 +1831,29 @@ topo_order_kernel(DevMesh m, const int *__restrict__ items, int n_items, Topo to
     const int hi = static_cast<int>(static_cast<long long>(n_items) * (part + 1) / kParts);
     const int i = lo + static_cast<int>(blockIdx.x) / kParts * kOrderQueries + gw;
     if (i >= hi) return;  // whole groups leave: nothing below spans groups
-    const int qi = items ? items[i] : i;
-    const long long roff = topo.rec_off[qi];
-    if (roff < 0) return;
-    int4 *r = topo.rec + roff;
-    const int4 hd = r[0];
-    if (hd.w != 1) return;  // coincident or empty row: complete
-    const int C = hd.x, B = hd.y, room = hd.z;
+    int qi, C, B, room;
+    int4 *r;
+    if (topo.itab) {  // the search's item entry: no offset or header loads first
+        const int4 e = topo.itab[i];
+        if (e.w < 0) return;
+        qi = e.x;
+        r = topo.rec + (static_cast<long long>(static_cast<unsigned>(e.y)) |
+                        (static_cast<long long>(e.z) << 32));
+        C = e.w & 0xff;
+        B = (e.w >> 8) & 0xff;
+        room = (e.w >> 16) & 0xff;
+    } else {
+        qi = items ? items[i] : i;
+        const long long roff = topo.rec_off[qi];
+        if (roff < 0) return;
+        r = topo.rec + roff;
+        const int4 hd0 = r[0];
+        if (hd0.w != 1) return;  // coincident or empty row: complete
+        C = hd0.x;
+        B = hd0.y;
+        room = hd0.z;
+    }
+    const int4 hd = make_int4(C, B, room, 1);
     const int N = m.n_data;
     const unsigned gmask = 0xffffu;
     const int *xtet = rec_xtet(r, C, B, room);
