@@ -278,3 +278,59 @@ def test_topology_target_in_place_and_fallbacks():
     same(run(1e-3), base, "after another-M target")
     assert L.nat_topology_target(mesh.handle, ctypes.c_void_p(buf.data_ptr() + 8), 64, 5) < 0
     say("[grad] topology target: in place, too small and other-M targets agree")
+
+
+def test_shell_gradient_from_big_records_equals_recompute():
+    """Rows of the wave-per-query level keep a big record when the forward
+    saves its topology (autograd), and their gradient comes from it; a
+    topology saved without them (a plain forward, then nat_topology_save)
+    makes the backward search those cavities again.  Both give the same
+    dL/dq (to rounding: the record path divides by the forward's row total,
+    the other by its own), and the record path's topology is the larger."""
+    import ctypes
+
+    import torch
+
+    from natinterp3d_amd import _lib
+    keys, queries, values, _ = workloads.shell(n_keys=20_000, n_queries=3_000)
+    interp = natinterp3d_amd.Interpolator(np.ascontiguousarray(keys, np.float64))
+    mesh = interp.mesh_and_vertices
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(19)
+    g = rng.standard_normal((len(queries), 3))
+    # the record path (autograd)
+    qt = torch.tensor(queries, device=dev, requires_grad=True)
+    vt = torch.tensor(values, device=dev)
+    y = interp.interpolate_torch(qt, vt)
+    (y * torch.tensor(g, device=dev)).sum().backward()
+    dq_rec = qt.grad.cpu().numpy()
+    assert mesh.timings()["wave_items"] >= 0.9 * len(queries)  # rows of the wave level
+    # the search-again path: a plain forward, its topology saved afterwards
+    L = _lib.lib()
+    qc = torch.tensor(np.ascontiguousarray(queries - interp._centroid), device=dev)
+    crow, col, val = mesh.weights_device(qc)
+    nb = ctypes.c_int64()
+    assert L.nat_topology_save(mesh.handle, None, 0, ctypes.byref(nb), None) == 0
+    topo = torch.empty(int(nb.value), dtype=torch.uint8, device=dev)
+    assert L.nat_topology_save(mesh.handle, ctypes.c_void_p(topo.data_ptr()), topo.numel(),
+                               ctypes.byref(nb), None) == 0
+    vals = torch.tensor(values, device=dev, dtype=torch.float64)
+    gy = torch.tensor(g, device=dev, dtype=torch.float64)
+    dq = torch.zeros((len(queries), 3), dtype=torch.float64, device=dev)
+    rc = L.nat_grad_queries(mesh.handle, ctypes.c_void_p(topo.data_ptr()),
+                            ctypes.c_void_p(qc.data_ptr()), len(queries),
+                            ctypes.c_void_p(crow.data_ptr()), ctypes.c_void_p(val.data_ptr()),
+                            ctypes.c_void_p(gy.data_ptr()), ctypes.c_void_p(vals.data_ptr()), 3,
+                            ctypes.c_void_p(dq.data_ptr()), None)
+    assert rc == 0, _lib.last_error()
+    torch.cuda.synchronize()
+    dq_again = dq.cpu().numpy()
+    scale = np.maximum(np.abs(dq_again).max(1), np.median(np.abs(dq_again).max(1)))
+    rel = np.abs(dq_rec - dq_again).max(1) / scale
+    say(f"[big records] max relative difference {rel.max():.2e}; saved topology "
+        f"{mesh.timings()['wave_items']} wave rows, {int(nb.value):,} bytes without records")
+    assert rel.max() <= 1e-9
+    # the autograd topology held the big records: far more bytes per query
+    from natinterp3d_amd import autograd
+    per_query_rec = autograd._TOPO_BYTES_PER_QUERY[mesh.handle.value]
+    assert per_query_rec > 4 * int(nb.value) / len(queries)
