@@ -755,6 +755,72 @@ __device__ __forceinline__ void new_contributions(const DevMesh &m, int t, int f
     }
 }
 
+// The facet moments of boundary face fi of cavity tet t -- the gradient's
+// part of step 4 -- in the query's frame.  With P_k = F_k - q, l_k = |P_k|^2
+// and X_k = P_k x P_k+1, the virtual tet's circumcentre and the three edge
+// triangles' share the cross products and squares:
+//   cv - q = (l0 X1 + l1 X2 + l2 X0) / (2 P0.X1),
+//   circumcentre of (F_k, q, F_k+1) - q = ((l_k P_k+1 - l_k+1 P_k) x X_k) / (2 |X_k|^2)
+// (the centroid below the same 1e-30 denominators as cc_tet and cc_tri).  The
+// gradient has no prescribed rounding, so nothing here has to match the
+// forward's circumcentres bit for bit.  f(slot, m6) receives the unscaled
+// moment of each face vertex that has a neighbour slot (facet_moment6 with
+// every point taken relative to q).
+template <typename SlotOf, typename F>
+__device__ __forceinline__ void face_moments_rel(const DevMesh &m, int t, int fi, const d3 &q,
+                                                 SlotOf slot_of, F f) {
+    bool neg;
+    const int4 tv = ld_tet_v(m, t, neg);
+    const int s[3] = {face_slot(fi, 0), face_slot(fi, 1), face_slot(fi, 2)};
+    d3 P[3];
+    double l[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        P[k] = sub3(ld_vert(m, sel4(tv, s[k])), q);
+        l[k] = dot3(P[k], P[k]);
+    }
+    const d3 X[3] = {cross3(P[0], P[1]), cross3(P[1], P[2]), cross3(P[2], P[0])};
+    d3 V;
+    const double den = 2.0 * dot3(P[0], X[1]);
+    if (fabs(den) < 1e-30) {
+        V = {(P[0].x + P[1].x + P[2].x) * 0.25, (P[0].y + P[1].y + P[2].y) * 0.25,
+             (P[0].z + P[1].z + P[2].z) * 0.25};
+    } else {
+        const double inv = 1.0 / den;
+        V = {(l[0] * X[1].x + l[1] * X[2].x + l[2] * X[0].x) * inv,
+             (l[0] * X[1].y + l[1] * X[2].y + l[2] * X[0].y) * inv,
+             (l[0] * X[1].z + l[1] * X[2].z + l[2] * X[0].z) * inv};
+    }
+    d3 T[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const d3 &a = P[k], &b = P[(k + 1) % 3];
+        const double x2 = dot3(X[k], X[k]);
+        if (x2 < 1e-30) {
+            T[k] = {(a.x + b.x) / 3.0, (a.y + b.y) / 3.0, (a.z + b.z) / 3.0};
+        } else {
+            const double la = l[k], lb = l[(k + 1) % 3];
+            const d3 c = cross3({la * b.x - lb * a.x, la * b.y - lb * a.y, la * b.z - lb * a.z},
+                                X[k]);
+            const double inv = 0.5 / x2;
+            T[k] = {c.x * inv, c.y * inv, c.z * inv};
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int slot = slot_of(s[k], sel4(tv, s[k]));
+        if (slot < 0) continue;
+        const d3 &w = P[k], &A = T[k], &B = T[(k + 2) % 3];
+        const d3 h = {w.x * 0.5, w.y * 0.5, w.z * 0.5};
+        const d3 u = sub3(A, h), v = sub3(V, h), x = sub3(B, h);
+        const double a1 = dot3(cross3(u, v), w);
+        const double a2 = dot3(cross3(v, x), w);
+        f(slot, d3{a1 * (h.x + A.x + V.x) + a2 * (h.x + V.x + B.x),
+                   a1 * (h.y + A.y + V.y) + a2 * (h.y + V.y + B.y),
+                   a1 * (h.z + A.z + V.z) + a2 * (h.z + V.z + B.z)});
+    }
+}
+
 // ---- mixed precision: steps 3 and 4 in FP32 relative to the query, with
 // FP64 (the functions above, bit-identical to the plain path) for tets whose
 // condition estimate is below tau and for tets with a super vertex (points
@@ -2495,16 +2561,15 @@ __global__ void __launch_bounds__(64) wave_rows_kernel(DevMesh m, char *wscr,
                 for (int b = lane; b < Br; b += 64) {
                     const int code = bb[b];
                     const int4 cr = r[1 + (code >> 2)];
-                    new_contributions<false>(
+                    face_moments_rel(
                         m, cr.x, code & 3, qq, [&](int vi, int) { return rec_slot(cr, vi); },
-                        [](int, double) {},
-                        moment6_fn([&](int slot, const d3 &g6) {
+                        [&](int slot, const d3 &g6) {
                             const double sc = W.val[slot];
                             const d3 gm = {g6.x * sc, g6.y * sc, g6.z * sc};
                             const double a = W.acc[slot];
                             ag = {ag.x + a * gm.x, ag.y + a * gm.y, ag.z + a * gm.z};
                             tg = {tg.x + gm.x, tg.y + gm.y, tg.z + gm.z};
-                        }));
+                        });
                 }
                 abar = wave_sum_f64(abar);
                 ag = {wave_sum_f64(ag.x), wave_sum_f64(ag.y), wave_sum_f64(ag.z)};
@@ -3181,16 +3246,15 @@ __global__ void __launch_bounds__(kGradThreads) grad_geom_kernel(
         for (int bi = gl; bi < nbnd; bi += kGradLanes) {
             const int code = rf[bi];
             const int4 cr = r[1 + (code >> 2)];
-            new_contributions<false>(
+            face_moments_rel(
                 m, cr.x, code & 3, qq, [&](int vi, int) { return rec_slot(cr, vi); },
-                [](int, double) {},
-                moment6_fn([&](int slot, const d3 &g6) {
+                [&](int slot, const d3 &g6) {
                     const double sc = nb_s[slot];
                     const d3 gm = {g6.x * sc, g6.y * sc, g6.z * sc};
                     const double a = nb_a[slot];
                     ag = {ag.x + a * gm.x, ag.y + a * gm.y, ag.z + a * gm.z};
                     tg = {tg.x + gm.x, tg.y + gm.y, tg.z + gm.z};
-                }));
+                });
         }
 #pragma unroll
         for (int d = kGradLanes / 2; d >= 1; d >>= 1) {
