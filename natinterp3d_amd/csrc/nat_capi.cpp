@@ -635,7 +635,9 @@ int ensure_old_vol(nat_mesh *m, hipStream_t st) {
     }
     nat::DevMesh d = m->dev;
     d.old_vol = nullptr;
-    HIP_OR_FAIL(nat::launch_old_vol(d, m->old_vol.as<double>(), st));
+    // the kernel also puts each tet's table face mask into its circumsphere
+    // record (sphere_dm), where the search hands it on to the order kernel
+    HIP_OR_FAIL(nat::launch_old_vol(d, m->old_vol.as<double>(), m->sph.as<nat::SphereTet>(), st));
     m->dev.old_vol = m->old_vol.as<double>();
     return 0;
 }

@@ -73,8 +73,11 @@ struct DevMesh {
     // Step-3 subcell volumes, which depend on the mesh alone: per tet, per
     // vertex VI, per choice of vertex order of the three face circumcentres
     // it uses (the face mask's bits of the other three vertices, as the
-    // record's fcmask), at [(t * 16 + fcmask) * 4 + VI]; NULL until built
-    // (computed inline then).
+    // record's fcmask), at [(t * 16 + table_pos(fcmask, dm)) * 4 + VI] with dm
+    // = sphere_dm(sph[t]): bit K set when the circumcentre of face K in the
+    // neighbour's vertex order differs from the one in the tet's own order
+    // (only those bits of a mask change a value); NULL until built (computed
+    // inline then).
     const double *old_vol;
     // Predicates of the query path: 0 = the plain FP64 signs, 1 = FP64 where
     // the static error bound certifies the sign, else the exact sign.  With
@@ -236,7 +239,7 @@ hipError_t launch_grad_geom(const DevMesh &m, int nblocks, const double *d_q, co
 // the cavity search's per-tet circumspheres (out: n_tets records), from fat
 hipError_t launch_sphere(const DevMesh &m, SphereTet *out, hipStream_t st);
 // the step-3 table of DevMesh::old_vol (out: n_tets * 64 doubles)
-hipError_t launch_old_vol(const DevMesh &m, double *out, hipStream_t st);
+hipError_t launch_old_vol(const DevMesh &m, double *out, SphereTet *sph, hipStream_t st);
 // wave-per-query rows (cavities beyond the group kernel): per-wave scratch
 // of wave_scratch_size() bytes for each of nblocks one-wave blocks
 size_t wave_scratch_size();

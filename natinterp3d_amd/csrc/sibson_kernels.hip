@@ -25,6 +25,7 @@
 #include "geom32.h"
 #include "nat_internal.h"
 #include "sphere.h"
+#include "step3_table.h"
 
 namespace nat {
 
@@ -576,44 +577,30 @@ __device__ __forceinline__ void old_contributions(const DevMesh &m, int t, unsig
     step3_vertex<3>(P, F, ct, tv, slot_of, add);
 }
 
-// The step-3 table (DevMesh::old_vol): the subcell volume of vertex VI of a
-// tet for each choice of vertex order of the three face circumcentres it
-// uses (the faces opposite the other three vertices a, b, c: each in the
-// tet's own ascending order or in the order of the neighbour across it, as
-// old_contributions picks with fcmask), at [t][fcmask][VI]: 16 masks x 4
-// vertices, 64 doubles, 512 bytes per tet, so that the four values one
-// cavity tet contributes are one aligned 32-byte piece (bit VI of the mask
-// does not change vertex VI's value: each value is stored twice).  Same
-// functions, same operands: each entry has the bits old_contributions would
-// compute.
+// The step-3 table (DevMesh::old_vol, addressed as step3_table.h says):
+// per tet, the old subcell volume of each vertex for each reduced face mask.
+// Same functions, same operands: each entry has the bits old_contributions
+// would compute.
 __device__ __forceinline__ d3 sel3d(bool s, const d3 &a, const d3 &b) {
     return {s ? a.x : b.x, s ? a.y : b.y, s ? a.z : b.z};
 }
 
-// index of vertex vi's value for face mask `mask`
-__host__ __device__ __forceinline__ size_t old_vol_index(int t, unsigned mask, int vi) {
-    return static_cast<size_t>(t) * 64 + static_cast<size_t>(mask & 15u) * 4 +
-           static_cast<size_t>(vi);
+// vertex VI's value for the reduced mask m (its bits of faces a, b, c choose
+// the circumcentres' vertex orders)
+template <int VI>
+__device__ __forceinline__ double old_vol_value(const Quad &P, const Quad &Fo, const Quad &Fn,
+                                                const d3 &ct, unsigned m) {
+    constexpr int a = FaceOf<VI>::a, b = FaceOf<VI>::b, c = FaceOf<VI>::c;
+    const d3 fa = sel3d((m >> a) & 1u, Fn.get<a>(), Fo.get<a>());
+    const d3 fb = sel3d((m >> b) & 1u, Fn.get<b>(), Fo.get<b>());
+    const d3 fc = sel3d((m >> c) & 1u, Fn.get<c>(), Fo.get<c>());
+    return subcell_volume(P.get<VI>(), P.get<a>(), P.get<b>(), P.get<c>(), ct, fc, fb, fa);
 }
 
-// vertex VI's value for each of the 16 masks (its own bit VI ignored: the
-// mask's bits of faces a, b, c choose the circumcentres' vertex orders)
-template <int VI>
-__device__ __forceinline__ void old_vol_vertex(const Quad &P, const Quad &Fo, const Quad &Fn,
-                                               const d3 &ct, double *out) {
-    constexpr int a = FaceOf<VI>::a, b = FaceOf<VI>::b, c = FaceOf<VI>::c;
-    for (int v = 0; v < 8; v++) {
-        const d3 fa = sel3d(v & 1, Fn.get<a>(), Fo.get<a>());
-        const d3 fb = sel3d(v & 2, Fn.get<b>(), Fo.get<b>());
-        const d3 fc = sel3d(v & 4, Fn.get<c>(), Fo.get<c>());
-        const double vol =
-            subcell_volume(P.get<VI>(), P.get<a>(), P.get<b>(), P.get<c>(), ct, fc, fb, fa);
-        // the masks whose bits a, b, c are v's bits 0, 1, 2, with bit VI 0 or 1
-        const unsigned mask = ((v & 1) ? 1u << a : 0u) | ((v & 2) ? 1u << b : 0u) |
-                              ((v & 4) ? 1u << c : 0u);
-        out[mask * 4 + VI] = vol;
-        out[(mask | (1u << VI)) * 4 + VI] = vol;
-    }
+__device__ __forceinline__ bool same_bits(const d3 &a, const d3 &b) {
+    return __double_as_longlong(a.x) == __double_as_longlong(b.x) &&
+           __double_as_longlong(a.y) == __double_as_longlong(b.y) &&
+           __double_as_longlong(a.z) == __double_as_longlong(b.z);
 }
 
 // The cavity search's circumsphere of each tet (sphere.h), from its fat
@@ -627,7 +614,8 @@ __global__ void __launch_bounds__(256) sphere_kernel(DevMesh m, SphereTet *__res
     out[t] = ft.v.w < 0 ? sphere_of(p1, p0, p2, p3, ft.n) : sphere_of(p0, p1, p2, p3, ft.n);
 }
 
-__global__ void __launch_bounds__(256) old_vol_kernel(DevMesh m, double *__restrict__ out) {
+__global__ void __launch_bounds__(256) old_vol_kernel(DevMesh m, double *__restrict__ out,
+                                                      SphereTet *__restrict__ sph) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m.n_tets) return;
     bool neg;
@@ -642,22 +630,35 @@ __global__ void __launch_bounds__(256) old_vol_kernel(DevMesh m, double *__restr
                      step3_face_cc<1>(m, P, tv, tn, tn.z >= 0 ? 2u : 0u),
                      step3_face_cc<2>(m, P, tv, tn, tn.y >= 0 ? 4u : 0u),
                      step3_face_cc<3>(m, P, tv, tn, tn.x >= 0 ? 8u : 0u)};
-    double *o = out + static_cast<size_t>(t) * 64;
-    old_vol_vertex<0>(P, Fo, Fn, ct, o);
-    old_vol_vertex<1>(P, Fo, Fn, ct, o);
-    old_vol_vertex<2>(P, Fo, Fn, ct, o);
-    old_vol_vertex<3>(P, Fo, Fn, ct, o);
+    const unsigned dm = (same_bits(Fo.p0, Fn.p0) ? 0u : 1u) | (same_bits(Fo.p1, Fn.p1) ? 0u : 2u) |
+                        (same_bits(Fo.p2, Fn.p2) ? 0u : 4u) | (same_bits(Fo.p3, Fn.p3) ? 0u : 8u);
+    {  // the mask into the low bits of the circumsphere record's d1 (sphere_dm)
+        unsigned long long b = __double_as_longlong(sph[t].d1);
+        b = (b & ~15ull) | dm;
+        sph[t].d1 = __longlong_as_double(static_cast<long long>(b));
+    }
+    double4 *o = reinterpret_cast<double4 *>(out + static_cast<size_t>(t) * 64);
+    for (unsigned mk = 0; mk < 16; mk++) {
+        if (mk & ~dm) continue;  // not a reduced mask of this tet
+        o[table_pos(mk, dm)] = make_double4(old_vol_value<0>(P, Fo, Fn, ct, mk),
+                                            old_vol_value<1>(P, Fo, Fn, ct, mk),
+                                            old_vol_value<2>(P, Fo, Fn, ct, mk),
+                                            old_vol_value<3>(P, Fo, Fn, ct, mk));
+    }
+    for (unsigned p = 1u << __builtin_popcount(dm); p < 16; p++)
+        o[p] = make_double4(0.0, 0.0, 0.0, 0.0);  // unused positions
 }
 
 // Step 3 for one cavity tet from the table; the same calls of add as
 // old_contributions (vertex order, slots from slot_of(VI, .)).
+// pos: table_pos(fcmask, sphere_dm(m.sph[t])).
 template <typename SlotOf, typename Add>
-__device__ __forceinline__ void old_contributions_table(const DevMesh &m, int t, unsigned fcmask,
+__device__ __forceinline__ void old_contributions_table(const DevMesh &m, int t, unsigned pos,
                                                         SlotOf slot_of, Add add) {
 #pragma unroll
     for (int vi = 0; vi < 4; vi++) {
         const int s = slot_of(vi, -1);
-        if (s >= 0) add(s, m.old_vol[old_vol_index(t, fcmask, vi)]);
+        if (s >= 0) add(s, m.old_vol[old_vol_index(t, pos, vi)]);
     }
 }
 
@@ -1269,10 +1270,12 @@ locate_kernel(DevMesh m, Caps caps, Scratch sc,
 // Record of one query, in 16-byte units at topo.rec[rec_off[qi]]:
 //   [0]            (C, B, NB, 0): cavity tets, boundary faces, neighbours
 //   [1 .. C]       per cavity tet in cavity order: (tet, slot(p0) | slot(p1)
-//                  << 16, slot(p2) | slot(p3) << 16, fcmask) -- slot 0xffff
-//                  = none; fcmask bit k: the face opposite p[k] is held by a
-//                  cavity tet earlier in cavity order (its circumcentre uses
-//                  that tet's vertex order)
+//                  << 16, slot(p2) | slot(p3) << 16, fcmask | table position
+//                  << 4) -- slot 0xffff = none; fcmask bit k: the face
+//                  opposite p[k] is held by a cavity tet earlier in cavity
+//                  order (its circumcentre uses that tet's vertex order); the
+//                  position is table_pos(fcmask, sphere_dm(sph[tet])) (0
+//                  without the step-3 table)
 //   next ceil(C/8)   the cavity order, 16 bits per position: its dense index
 //                    | the fcmask of that tet << 6
 //   next ceil(B/16)  boundary faces in sequence order, a byte each: cavity
@@ -1280,7 +1283,8 @@ locate_kernel(DevMesh m, Caps caps, Scratch sc,
 //   next ceil(NB/4)  neighbour vertex indices in slot order (room is reserved
 //                    for min(32, B/2 + 4): a cavity bounded by one closed
 //                    surface has B/2 + 2 boundary vertices)
-//   next 2 ceil(C/4) handoff: the in-sphere adjacency, then the tet, per dense
+//   next 2 ceil(C/4) handoff: the in-sphere adjacency, then the tet (and
+//                    its table mask: handoff_tet, handoff_dm), per dense
 //                    index (int each)
 // C = 0 rows: NB = vertices coinciding with the query (weight 1/NB each, in
 // unit [1]), or NB = 0 for an empty row.  Between the kernels the header is
@@ -1337,6 +1341,17 @@ __device__ __forceinline__ int *rec_xadj(int4 *r, int C, int B, int room) {
 }
 __device__ __forceinline__ int *rec_xtet(int4 *r, int C, int B, int room) {
     return reinterpret_cast<int *>(r + rec_x_off(C, B, room) + (C + 3) / 4);
+}
+
+// The handoff's tet entries (rec_xtet) carry the tet's step-3 table mask
+// (sphere_dm) in bits 27..30 when every tet index fits in 27 bits; the order
+// kernel then needs no load for it.
+__device__ __forceinline__ bool dm_packs(const DevMesh &m) { return m.n_tets <= (1 << 27); }
+__device__ __forceinline__ int handoff_tet(const DevMesh &m, int xt) {
+    return dm_packs(m) ? (xt & ((1 << 27) - 1)) : xt;
+}
+__device__ __forceinline__ unsigned handoff_dm(const DevMesh &m, int xt) {
+    return dm_packs(m) ? static_cast<unsigned>(xt) >> 27 : sphere_dm(m.sph[xt]);
 }
 
 struct alignas(16) QuadMem {
@@ -1567,6 +1582,9 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
                             }
                         }
                         g.vnb[slot] = packed;
+                        // an in-sphere tet carries its step-3 table mask to
+                        // the handoff
+                        if (in4 && dm_packs(m)) g.vtet[i] = t | static_cast<int>(sphere_dm(sp) << 27);
                         g.vflag[slot] = static_cast<unsigned char>(1u | (in4 << 1));
                     }
                     // positions in the next front: exclusive scan over the 16 lanes
@@ -1719,7 +1737,7 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
             const int s = g.vlist[i];
             const unsigned fl = g.vflag[s];
             if (!(fl & 0x1eu)) continue;
-            const int t = g.vtet[i];
+            const int t = g.vtet[i];  // with the table mask (handoff_tet, handoff_dm)
 #pragma unroll
             for (int j = 0; j < grp::Q; j++) {
                 if (qi[j] < 0 || !ok[j] || !fits[j] || ncav[j] == 0 || !(fl & (2u << j)))
@@ -1874,7 +1892,7 @@ struct alignas(16) OrderMem {
     int dtet[grp::CMAX];           // tet per dense index
     int nkey[grp::NT];             // vertex table: keys, -1 = empty
     int nmin[grp::NT];             // first appearance (face-vertex sequence number)
-    unsigned char fcm[grp::CMAX];  // face-circumcentre mask per dense index
+    unsigned char fcm[grp::CMAX];  // face-circumcentre mask | table position << 4, per position
     unsigned char nslot[grp::NT];  // neighbour slot per vertex-table slot
     unsigned char bnd[grp::BMAX];  // boundary faces in sequence order, dense << 2 | face
 };
@@ -1928,10 +1946,12 @@ topo_order_kernel(DevMesh m, const int *__restrict__ items, int n_items, Topo to
     // the cavity tets in cavity order
     for (int p = gl; p < C; p += kOrderLanes) {
         const unsigned o = ro[p];  // position p: its dense index and mask
-        const int t = xtet[o & 63u];
+        const int xt = xtet[o & 63u];
+        const int t = handoff_tet(m, xt);
         g.dtet[p] = t;
         g.cv[p] = m.tet_v[t];
-        g.fcm[p] = static_cast<unsigned char>(o >> 6);
+        const unsigned fm = (o >> 6) & 15u;
+        g.fcm[p] = static_cast<unsigned char>(fm | (table_pos(fm, handoff_dm(m, xt)) << 4));
     }
     for (int e = gl; e < B; e += kOrderLanes) g.bnd[e] = rb[e];
     for (int h = gl; h < grp::NT; h += kOrderLanes) {
@@ -2686,14 +2706,15 @@ __global__ void __launch_bounds__(64) wave_rows_kernel(DevMesh m, char *wscr,
             // (negated: stolen -= vol is stolen + -vol), each at its position
             for (int ci = lane; ci < C; ci += 64) {
                 const uint2 cs = W.cslot[ci];
+                const int ct = W.dtet[W.cav[ci]];
+                const unsigned cm = static_cast<unsigned>(W.cmask[ci]);
                 auto old_fn = [&](auto... a) {
                     if (m.old_vol)
-                        old_contributions_table(m, a...);
+                        old_contributions_table(m, ct, table_pos(cm, sphere_dm(m.sph[ct])), a...);
                     else
-                        old_contributions(m, a...);
+                        old_contributions(m, ct, cm, a...);
                 };
-                old_fn(W.dtet[W.cav[ci]], static_cast<unsigned>(W.cmask[ci]),
-                                  slot_of_ci(ci), [&](int slot, double vol) {
+                old_fn(slot_of_ci(ci), [&](int slot, double vol) {
                                       const int k = slot == static_cast<int>(u16_of(cs, 0))
                                                         ? 0
                                                         : slot == static_cast<int>(u16_of(cs, 1))
@@ -2883,7 +2904,7 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
                             if constexpr (MIXED) absum += fabs(vol);
                         };
                         if (m.old_vol) {  // exact FP64 values in both modes
-                            old_contributions_table(m, cr.x, static_cast<unsigned>(cr.w),
+                            old_contributions_table(m, cr.x, static_cast<unsigned>(cr.w) >> 4,
                                                     slot_of, add);
                             continue;
                         }
@@ -2896,7 +2917,8 @@ __global__ void __launch_bounds__(kGeomThreads) geom_kernel(DevMesh m,
                                 n64++;
                             }
                         }
-                        old_contributions(m, cr.x, static_cast<unsigned>(cr.w), slot_of, add);
+                        old_contributions(m, cr.x, static_cast<unsigned>(cr.w) & 15u, slot_of,
+                                          add);
                     }
                     for (int bi = 0; bi < nbnd; bi++) {
                         const int code = rf[bi];
@@ -3049,7 +3071,7 @@ __global__ void __launch_bounds__(kGeomRowsThreads) geom_rows_kernel(
 #pragma unroll
                             for (int vi = g; vi < 4; vi += G) {
                                 const double v = m.old_vol[old_vol_index(
-                                    cr.x, static_cast<unsigned>(cr.w) & 15u, vi)];
+                                    cr.x, static_cast<unsigned>(cr.w) >> 4, vi)];
                                 const int slot = rec_slot(cr, vi);
                                 if (slot >= 0) {
                                     acc[slot][rr] += v;
@@ -3524,10 +3546,10 @@ hipError_t launch_sphere(const DevMesh &m, SphereTet *out, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_old_vol(const DevMesh &m, double *out, hipStream_t st) {
+hipError_t launch_old_vol(const DevMesh &m, double *out, SphereTet *sph, hipStream_t st) {
     if (m.n_tets > 0)
         hipLaunchKernelGGL(old_vol_kernel, dim3((m.n_tets + 255) / 256), dim3(256), 0, st, m,
-                           out);
+                           out, sph);
     return hipGetLastError();
 }
 

@@ -45,7 +45,7 @@ namespace nat {
 struct alignas(64) SphereTet {
     double cx, cy, cz;  // FP64 circumcentre c~
     double r2;          // |a - c~|^2, a the first vertex of the oriented tet
-    double d1;          // 160 delta^2
+    double d1;          // 160 delta^2, rounded up; low 4 bits: sphere_dm
     double w;           // 1e21 O_lb^2; 0: never certified
     int4 n;             // neighbour across face 0..3 (tet_n)
 };
@@ -88,9 +88,24 @@ __host__ __device__ inline SphereTet sphere_of(const d3 &a, const d3 &b, const d
     if (!(delta * delta <= 1e-12 * s.r2)) return s;
     const double w = 1e21 * olb * olb;
     if (!(w < 1e300)) return s;
-    s.d1 = 160.0 * delta * delta;
+    // rounded up to a multiple of 16 units in the last place: its low four
+    // bits are free for the step-3 table's face mask (sphere_dm)
+    const double d1 = 160.0 * delta * delta;
+    unsigned long long bits;
+    __builtin_memcpy(&bits, &d1, sizeof(bits));
+    bits = (bits + 15ull) & ~15ull;
+    __builtin_memcpy(&s.d1, &bits, sizeof(bits));
     s.w = w;
     return s;
+}
+
+// the face mask a tet's step-3 table was built with (DevMesh::old_vol), kept
+// in the low bits of d1 (0 before the table is built); raising d1 by at most
+// 15 units in the last place only makes the certificate stricter
+__host__ __device__ __forceinline__ unsigned sphere_dm(const SphereTet &s) {
+    unsigned long long bits;
+    __builtin_memcpy(&bits, &s.d1, sizeof(bits));
+    return static_cast<unsigned>(bits & 15ull);
 }
 
 // 1: e inside the tet's circumsphere, 0: outside (both certified to be the
