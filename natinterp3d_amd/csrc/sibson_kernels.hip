@@ -1766,55 +1766,86 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
 // before it).  Both go out as packed streams, a 32-bit store per two
 // positions or four boundary faces.
 constexpr int kDfsThreads = 64;
-constexpr int kDfsLds = 32;  // adjacency words kept in LDS; the rest read from the record
+constexpr int kDfsPool = 2048;  // adjacency words of a wave kept in LDS (mean 64 x 20)
 
 __global__ void __launch_bounds__(kDfsThreads)
 topo_dfs_kernel(const int *__restrict__ items, int n_items, Topo topo, unsigned int *ovf_count,
                 int *ovf_list) {
-    // adjacency (dense < kDfsLds) and depth-first parent per dense index, one
-    // column per thread ([k][t]: the lanes of a wave hit distinct banks)
-    __shared__ unsigned adjl[kDfsLds][kDfsThreads];
-    __shared__ unsigned char parl[grp::CMAX][kDfsThreads];
+    // the wave's adjacency words: each query's C words at its offset in one
+    // pool (a prefix sum over the lanes); a query past the pool's end reads
+    // and writes its words in the record instead
+    __shared__ unsigned pool[kDfsPool];
     const int t = static_cast<int>(threadIdx.x);
     const int part = static_cast<int>(blockIdx.x) % kParts;
     const int lo = static_cast<int>(static_cast<long long>(n_items) * part / kParts);
     const int hi = static_cast<int>(static_cast<long long>(n_items) * (part + 1) / kParts);
     const int i = lo + static_cast<int>(blockIdx.x) / kParts * kDfsThreads + t;
-    if (i >= hi) return;
-    int qi, C, B, room;
-    int4 *r;
-    if (topo.itab) {  // the search's item entry: no offset or header loads first
-        const int4 e = topo.itab[i];
-        if (e.w < 0) return;
-        qi = e.x;
-        r = topo.rec + (static_cast<long long>(static_cast<unsigned>(e.y)) |
-                        (static_cast<long long>(e.z) << 32));
-        C = e.w & 0xff;
-        B = (e.w >> 8) & 0xff;
-        room = (e.w >> 16) & 0xff;
-    } else {
-        qi = items ? items[i] : i;
-        const long long roff = topo.rec_off[qi];
-        if (roff < 0) return;
-        r = topo.rec + roff;
-        const int4 hd = r[0];
-        if (hd.w != 1) return;  // coincident or empty row: complete
-        C = hd.x;
-        B = hd.y;
-        room = hd.z;
+    int qi = -1, C = 0, B = 0, room = 0;
+    int4 *r = nullptr;
+    if (i < hi) {
+        if (topo.itab) {  // the search's item entry: no offset or header loads first
+            const int4 e = topo.itab[i];
+            if (e.w >= 0) {
+                qi = e.x;
+                r = topo.rec + (static_cast<long long>(static_cast<unsigned>(e.y)) |
+                                (static_cast<long long>(e.z) << 32));
+                C = e.w & 0xff;
+                B = (e.w >> 8) & 0xff;
+                room = (e.w >> 16) & 0xff;
+            }
+        } else {
+            qi = items ? items[i] : i;
+            const long long roff = topo.rec_off[qi];
+            if (roff >= 0) {
+                r = topo.rec + roff;
+                const int4 hd = r[0];
+                if (hd.w == 1) {  // else a coincident or empty row: complete
+                    C = hd.x;
+                    B = hd.y;
+                    room = hd.z;
+                } else {
+                    r = nullptr;
+                }
+            }
+        }
     }
-    const int *xadj = rec_xadj(r, C, B, room);
-    const int4 *xa4 = reinterpret_cast<const int4 *>(xadj);
-    const int cl = min(C, kDfsLds);
-    for (int d0 = 0; d0 < cl; d0 += 4) {
-        const int4 v = xa4[d0 >> 2];
-        adjl[d0][t] = static_cast<unsigned>(v.x);
-        if (d0 + 1 < cl) adjl[d0 + 1][t] = static_cast<unsigned>(v.y);
-        if (d0 + 2 < cl) adjl[d0 + 2][t] = static_cast<unsigned>(v.z);
-        if (d0 + 3 < cl) adjl[d0 + 3][t] = static_cast<unsigned>(v.w);
+    int total;
+    const int off = wave_excl_scan(r ? C : 0, total);  // every lane of the wave takes part
+    if (!r) return;
+    int *xadj = rec_xadj(r, C, B, room);
+    const bool in_pool = off + C <= kDfsPool;
+    unsigned *adj = in_pool ? pool + off : nullptr;
+    if (in_pool) {
+        const int4 *xa4 = reinterpret_cast<const int4 *>(xadj);
+        for (int d0 = 0; d0 < C; d0 += 4) {
+            const int4 v = xa4[d0 >> 2];
+            adj[d0] = static_cast<unsigned>(v.x);
+            if (d0 + 1 < C) adj[d0 + 1] = static_cast<unsigned>(v.y);
+            if (d0 + 2 < C) adj[d0 + 2] = static_cast<unsigned>(v.z);
+            if (d0 + 3 < C) adj[d0 + 3] = static_cast<unsigned>(v.w);
+        }
     }
     auto adj_of = [&](int d) {
-        return d < kDfsLds ? adjl[d][t] : static_cast<unsigned>(xadj[d]);
+        return in_pool ? adj[d] : static_cast<unsigned>(xadj[d]);
+    };
+    // The depth-first parent of a tet is one of its neighbours: once the tet
+    // is appended, the byte of its adjacency word that holds the parent gets
+    // bit 6 (a dense index is below 64 and 0xff marks a boundary face, so
+    // the marked byte is neither; the child scan skips it as >= C)
+    auto set_adj = [&](int d, unsigned a) {
+        if (in_pool)
+            adj[d] = a;
+        else
+            xadj[d] = static_cast<int>(a);
+    };
+    auto parent_of = [&](unsigned a) {
+        int p = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const unsigned b = (a >> (8 * k)) & 0xffu;
+            if ((b & 0xc0u) == 0x40u) p = static_cast<int>(b & 0x3fu);
+        }
+        return p;
     };
     unsigned *ro = reinterpret_cast<unsigned *>(r + rec_ord_off(C));
     unsigned *rb = reinterpret_cast<unsigned *>(r + rec_bnd_off(C));
@@ -1859,14 +1890,18 @@ topo_dfs_kernel(const int *__restrict__ items, int n_items, Topo topo, unsigned 
             if (ck < static_cast<unsigned>(C) && !((vis >> ck) & 1ull)) c = static_cast<int>(ck);
         }
         if (c >= 0) {
-            parl[c][t] = static_cast<unsigned char>(u);
             a = adj_of(c);
             append(c, a);  // the mask sees the tets appended before c
+            unsigned mark = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (((a >> (8 * k)) & 0xffu) == static_cast<unsigned>(u)) mark = 0x40u << (8 * k);
+            set_adj(c, a | mark);
             vis |= 1ull << c;
             u = c;
         } else {
             if (u == 0) break;
-            u = parl[u][t];
+            u = parent_of(adj_of(u));
             a = adj_of(u);
         }
     }
@@ -1888,13 +1923,14 @@ constexpr int kOrderLanes = 16;
 constexpr int kOrderQueries = 4;
 
 struct alignas(16) OrderMem {
-    int4 cv[grp::CMAX];            // vertex slots per dense index
-    int dtet[grp::CMAX];           // tet per dense index
+    int4 cv[grp::CMAX];            // vertex slots per position
+    int dtet[grp::CMAX];           // tet per position
     int nkey[grp::NT];             // vertex table: keys, -1 = empty
     int nmin[grp::NT];             // first appearance (face-vertex sequence number)
     unsigned char fcm[grp::CMAX];  // face-circumcentre mask | table position << 4, per position
     unsigned char nslot[grp::NT];  // neighbour slot per vertex-table slot
-    unsigned char bnd[grp::BMAX];  // boundary faces in sequence order, dense << 2 | face
+    unsigned char bnd[grp::BMAX];  // boundary faces in sequence order, position << 2 | face
+    unsigned char posof[grp::CMAX];  // position of each dense index
 };
 
 __device__ __forceinline__ int order_vertex(const OrderMem &g, int e) {
@@ -1943,21 +1979,51 @@ topo_order_kernel(DevMesh m, const int *__restrict__ items, int n_items, Topo to
     const int *xtet = rec_xtet(r, C, B, room);
     const unsigned short *ro = rec_order(r, hd);
     const unsigned char *rb = rec_bnd(r, hd);
-    // the cavity tets in cavity order
+    // the cavity tets: the tets by dense index (with their vertex slots, read
+    // as soon as the tets arrive) and the cavity order (position -> dense
+    // index and face mask) are read side by side, then put together in
+    // position order: two dependent loads after the item entry, not three
+    static_assert(grp::CMAX <= 4 * kOrderLanes, "four dense indices per lane");
+    // four named sets, not arrays: they stay in registers
+    int t0 = -1, t1 = -1, t2 = -1, t3 = -1;  // -1: no such dense index
+    unsigned dm0 = 0, dm1 = 0, dm2 = 0, dm3 = 0;
+    int4 v0 = make_int4(0, 0, 0, 0), v1 = v0, v2 = v0, v3 = v0;
+#define NAT_LOAD_TET(K)                                  \
+    if (gl + (K) * kOrderLanes < C) {                    \
+        const int xt = xtet[gl + (K) * kOrderLanes];     \
+        t##K = handoff_tet(m, xt);                       \
+        dm##K = handoff_dm(m, xt);                       \
+        v##K = m.tet_v[t##K];                            \
+    }
+    NAT_LOAD_TET(0)
+    NAT_LOAD_TET(1)
+    NAT_LOAD_TET(2)
+    NAT_LOAD_TET(3)
+#undef NAT_LOAD_TET
     for (int p = gl; p < C; p += kOrderLanes) {
         const unsigned o = ro[p];  // position p: its dense index and mask
-        const int xt = xtet[o & 63u];
-        const int t = handoff_tet(m, xt);
-        g.dtet[p] = t;
-        g.cv[p] = m.tet_v[t];
-        const unsigned fm = (o >> 6) & 15u;
-        g.fcm[p] = static_cast<unsigned char>(fm | (table_pos(fm, handoff_dm(m, xt)) << 4));
+        g.posof[o & 63u] = static_cast<unsigned char>(p);
+        g.fcm[p] = static_cast<unsigned char>((o >> 6) & 15u);
     }
     for (int e = gl; e < B; e += kOrderLanes) g.bnd[e] = rb[e];
     for (int h = gl; h < grp::NT; h += kOrderLanes) {
         g.nkey[h] = -1;
         g.nmin[h] = 0x7fffffff;
     }
+    group_sync();
+#define NAT_PLACE_TET(K)                                                         \
+    if (t##K >= 0) {                                                             \
+        const int p = g.posof[gl + (K) * kOrderLanes];                           \
+        g.dtet[p] = t##K;                                                        \
+        g.cv[p] = v##K;                                                          \
+        const unsigned fm = g.fcm[p];                                            \
+        g.fcm[p] = static_cast<unsigned char>(fm | (table_pos(fm, dm##K) << 4)); \
+    }
+    NAT_PLACE_TET(0)
+    NAT_PLACE_TET(1)
+    NAT_PLACE_TET(2)
+    NAT_PLACE_TET(3)
+#undef NAT_PLACE_TET
     group_sync();
     // first appearance of each data vertex in the face-vertex sequence
     const int nent = 3 * B;
