@@ -1814,30 +1814,20 @@ topo_dfs_kernel(const int *__restrict__ items, int n_items, Topo topo, unsigned 
     if (!r) return;
     int *xadj = rec_xadj(r, C, B, room);
     const bool in_pool = off + C <= kDfsPool;
-    unsigned *adj = in_pool ? pool + off : nullptr;
     if (in_pool) {
         const int4 *xa4 = reinterpret_cast<const int4 *>(xadj);
         for (int d0 = 0; d0 < C; d0 += 4) {
             const int4 v = xa4[d0 >> 2];
-            adj[d0] = static_cast<unsigned>(v.x);
-            if (d0 + 1 < C) adj[d0 + 1] = static_cast<unsigned>(v.y);
-            if (d0 + 2 < C) adj[d0 + 2] = static_cast<unsigned>(v.z);
-            if (d0 + 3 < C) adj[d0 + 3] = static_cast<unsigned>(v.w);
+            pool[off + d0] = static_cast<unsigned>(v.x);
+            if (d0 + 1 < C) pool[off + d0 + 1] = static_cast<unsigned>(v.y);
+            if (d0 + 2 < C) pool[off + d0 + 2] = static_cast<unsigned>(v.z);
+            if (d0 + 3 < C) pool[off + d0 + 3] = static_cast<unsigned>(v.w);
         }
     }
-    auto adj_of = [&](int d) {
-        return in_pool ? adj[d] : static_cast<unsigned>(xadj[d]);
-    };
     // The depth-first parent of a tet is one of its neighbours: once the tet
     // is appended, the byte of its adjacency word that holds the parent gets
     // bit 6 (a dense index is below 64 and 0xff marks a boundary face, so
     // the marked byte is neither; the child scan skips it as >= C)
-    auto set_adj = [&](int d, unsigned a) {
-        if (in_pool)
-            adj[d] = a;
-        else
-            xadj[d] = static_cast<int>(a);
-    };
     auto parent_of = [&](unsigned a) {
         int p = 0;
 #pragma unroll
@@ -1853,58 +1843,79 @@ topo_dfs_kernel(const int *__restrict__ items, int n_items, Topo topo, unsigned 
     int n = 0, nb = 0;
     unsigned ow = 0, bw = 0;
     // append dense index c at position n: its order entry, boundary faces, mask
-    auto append = [&](int c, unsigned a) {
+    auto append = [&](bool on, int c, unsigned a) {
+        // bytes 0xff (boundary faces): bit 7 of each byte of bm (an exact
+        // zero-byte test of ~a, whose other bytes are >= 0xc0)
+        const unsigned na = ~a;
+        unsigned bm = on ? (na - 0x01010101u) & ~na & 0x80808080u : 0u;
+        while (bm) {  // in face order
+            const int k = __builtin_ctz(bm) >> 3;
+            bm &= bm - 1u;
+            if (nb < B) {
+                bw |= static_cast<unsigned>((n << 2) | k) << (8 * (nb & 3));
+                if ((nb & 3) == 3) {
+                    rb[nb >> 2] = bw;
+                    bw = 0;
+                }
+            }
+            nb++;
+        }
         unsigned fc = 0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            if (((a >> (8 * k)) & 0xffu) == 0xffu) {
-                if (nb < B) {
-                    bw |= static_cast<unsigned>((n << 2) | k) << (8 * (nb & 3));
-                    if ((nb & 3) == 3) {
-                        rb[nb >> 2] = bw;
-                        bw = 0;
-                    }
-                }
-                nb++;
-            }
             const unsigned ns = (a >> (8 * (3 - k))) & 0xffu;
             if (ns != 0xffu && ((vis >> ns) & 1ull)) fc |= 1u << k;
         }
-        if (n < C) {
+        if (on && n < C) {
             ow |= (static_cast<unsigned>(c) | (fc << 6)) << (16 * (n & 1));
             if (n & 1) {
                 ro[n >> 1] = ow;
                 ow = 0;
             }
         }
-        n++;
+        n += on ? 1 : 0;
     };
-    int u = 0;
-    unsigned a = adj_of(0);
-    append(0, a);
-    for (;;) {
-        int c = -1;
+    // the walk with the words in the pool or (a query past its end) in the
+    // record: two copies of the loop, so the pool's accesses stay LDS
+    // accesses (one loop over a pointer to either would make them generic)
+    auto walk = [&](auto adj_of, auto set_adj) {
+        int u = 0;
+        unsigned a = adj_of(0);
+        append(true, 0, a);
+        // one step per iteration, down to the highest free slot or back up to
+        // the parent; a holds u's word with its parent mark
+        for (;;) {
+            int c = -1;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {  // the highest free slot wins
-            const unsigned ck = (a >> (8 * k)) & 0xffu;
-            if (ck < static_cast<unsigned>(C) && !((vis >> ck) & 1ull)) c = static_cast<int>(ck);
-        }
-        if (c >= 0) {
-            a = adj_of(c);
-            append(c, a);  // the mask sees the tets appended before c
+            for (int k = 0; k < 4; k++) {  // the highest free slot wins
+                const unsigned ck = (a >> (8 * k)) & 0xffu;
+                if (ck < static_cast<unsigned>(C) && !((vis >> ck) & 1ull)) c = static_cast<int>(ck);
+            }
+            // both kinds of step in one straight path (predicated, not
+            // branched: a branch lets the compiler split the up-steps into a
+            // loop of their own, which the other lanes of the wave wait out)
+            const bool down = c >= 0;
+            if (!down && u == 0) break;
+            const int nx = down ? c : parent_of(a);
+            unsigned an = adj_of(nx);
+            append(down, c, an);  // the mask sees the tets appended before c
             unsigned mark = 0;
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                if (((a >> (8 * k)) & 0xffu) == static_cast<unsigned>(u)) mark = 0x40u << (8 * k);
-            set_adj(c, a | mark);
-            vis |= 1ull << c;
-            u = c;
-        } else {
-            if (u == 0) break;
-            u = parent_of(adj_of(u));
-            a = adj_of(u);
+                if (down && ((an >> (8 * k)) & 0xffu) == static_cast<unsigned>(u))
+                    mark = 0x40u << (8 * k);
+            an |= mark;
+            if (down) set_adj(c, an);
+            vis |= down ? 1ull << c : 0ull;
+            u = nx;
+            a = an;
         }
-    }
+    };
+    if (in_pool)
+        walk([&](int d) { return pool[off + d]; }, [&](int d, unsigned w) { pool[off + d] = w; });
+    else
+        walk([&](int d) { return static_cast<unsigned>(xadj[d]); },
+             [&](int d, unsigned w) { xadj[d] = static_cast<int>(w); });
     if (n != C || nb != B) {
         topo.rec_off[qi] = -1;
         if (topo.itab) topo.itab[i].w = -1;
