@@ -996,6 +996,35 @@ __device__ __forceinline__ bool part_batch(unsigned int *ctr, int n, int batch, 
     return false;
 }
 
+// part_batch in two halves, so that the claim's atomic is in flight while
+// the wave works on its current batch: part_claim_issue takes the range
+// (lane 0's old counter value; meaningful for the part it names),
+// part_claim_finish reads it and, if that part is spent, goes on with
+// part_batch over the remaining parts.
+__device__ __forceinline__ unsigned part_claim_issue(unsigned int *ctr, int batch, int k) {
+    unsigned b = 0;
+    if (k < kParts && __lane_id() == 0) {
+        const int p = static_cast<int>((blockIdx.x + static_cast<unsigned>(k)) % kParts);
+        b = atomicAdd(&ctr[p], static_cast<unsigned>(batch));
+    }
+    return b;
+}
+__device__ __forceinline__ bool part_claim_finish(unsigned int *ctr, int n, int batch, unsigned b,
+                                                  int &k, int &first, int &cnt) {
+    if (k >= kParts) return false;
+    b = __shfl(b, 0, 64);
+    const int p = static_cast<int>((blockIdx.x + static_cast<unsigned>(k)) % kParts);
+    const int lo = static_cast<int>(static_cast<long long>(n) * p / kParts);
+    const int hi = static_cast<int>(static_cast<long long>(n) * (p + 1) / kParts);
+    if (static_cast<long long>(lo) + b < hi) {
+        first = lo + static_cast<int>(b);
+        cnt = min(batch, hi - first);
+        return true;
+    }
+    k++;
+    return part_batch(ctr, n, batch, k, first, cnt);
+}
+
 // The wave's next 64 work items; returns false when the list is exhausted.
 __device__ __forceinline__ bool next_batch(unsigned int *work, int n_items, int &item) {
     unsigned base = 0;
@@ -1392,16 +1421,32 @@ __device__ __forceinline__ int vis_probe(QuadMem &g, int t, unsigned h, bool &is
     return -1;
 }
 
-// query j's in-sphere adjacency word of visited slot s: byte k = its dense
-// index of the neighbour across face k when that is inside its sphere, else 0xff
-__device__ __forceinline__ unsigned adjacency(const QuadMem &g, int s, int j) {
-    const unsigned nbs = g.vnb[s];
+// The four neighbours of visited slot s: their slots (0xff none) and flags
+// (0 for none), read once for all the quad's queries.
+struct SlotNbrs {
+    unsigned nbs;  // vnb[s]
+    unsigned fl[4];
+};
+__device__ __forceinline__ SlotNbrs slot_nbrs(const QuadMem &g, int s) {
+    SlotNbrs n;
+    n.nbs = g.vnb[s];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const unsigned ns = (n.nbs >> (8 * k)) & 0xffu;
+        n.fl[k] = ns != 0xffu ? g.vflag[ns] : 0u;
+    }
+    return n;
+}
+// Query j's in-sphere adjacency word of visited slot s: byte k = the dense
+// index of the neighbour across face k when that is inside query j's
+// circumsphere, else 0xff.
+__device__ __forceinline__ unsigned adjacency_of(const QuadMem &g, const SlotNbrs &n, int j) {
     const unsigned bit = 2u << j;
     unsigned a = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const unsigned ns = (nbs >> (8 * k)) & 0xffu;
-        const unsigned d = (ns != 0xffu && (g.vflag[ns] & bit)) ? g.dense[j][ns] : 0xffu;
+        const unsigned ns = (n.nbs >> (8 * k)) & 0xffu;
+        const unsigned d = (n.fl[k] & bit) ? g.dense[j][ns] : 0xffu;
         a |= d << (8 * k);
     }
     return a;
@@ -1419,26 +1464,43 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
     const int gl = lane & (grp::G - 1);
     const int gw = lane >> 4;
     QuadMem &g = gmem[gw];
-    int parts_done = 0, first = 0, cnt = 0;
-    while (part_batch(work, n_items, grp::Q * grp::QPB, parts_done, first, cnt)) {
+    constexpr int kBatch = grp::Q * grp::QPB;
+    // Software pipeline over the wave's batches: the next batch is claimed
+    // when the current one starts, its items are read after the current
+    // search, its queries and start tets before the current handoff, so
+    // their latencies pass while the current batch is worked on.  Lane j
+    // of a quad holds query j's values; the quad's lanes share them.
+    int parts_done = 0, nfirst = 0, ncnt = 0;
+    bool have = part_batch(work, n_items, kBatch, parts_done, nfirst, ncnt);
+    int nqil = -1, nstl = -1, nbl = 0;
+    d3 nql = {0, 0, 0};
+    auto load_items = [&]() {
+        const int item = grp::Q * gw + gl;
+        nqil = have && gl < grp::Q && item < ncnt ? items[nfirst + item] : -1;
+    };
+    auto load_queries = [&]() {
+        nstl = -1;
+        nbl = 0;
+        nql = {0, 0, 0};
+        if (nqil >= 0) {
+            nstl = start_of[nqil];
+            nql = {q[3 * static_cast<size_t>(nqil)], q[3 * static_cast<size_t>(nqil) + 1],
+                   q[3 * static_cast<size_t>(nqil) + 2]};
+            nbl = min(grp::VQ, static_cast<int>(route_of(m, nqil) & 0xffu));
+        }
+    };
+    load_items();
+    load_queries();
+    while (have) {
         // query j of this quad: item first + 4 gw + j
+        const int first = nfirst;
+        const int qil = nqil, stl = nstl, bl = nbl;
+        const d3 ql = nql;
+        const unsigned claim = part_claim_issue(work, kBatch, parts_done);
         int qi[grp::Q], st[grp::Q];
         d3 qq[grp::Q];
         bool act[grp::Q], ok[grp::Q];
         int budget = 0;
-        // lane j of the quad loads query j; the quad's lanes share them
-        int qil = -1, stl = -1, bl = 0;
-        d3 ql = {0, 0, 0};
-        if (gl < grp::Q) {
-            const int item = grp::Q * gw + gl;
-            qil = item < cnt ? items[first + item] : -1;
-            if (qil >= 0) {
-                stl = start_of[qil];
-                ql = {q[3 * static_cast<size_t>(qil)], q[3 * static_cast<size_t>(qil) + 1],
-                      q[3 * static_cast<size_t>(qil) + 2]};
-                bl = min(grp::VQ, static_cast<int>(route_of(m, qil) & 0xffu));
-            }
-        }
 #pragma unroll
         for (int j = 0; j < grp::Q; j++) {
             qi[j] = __shfl(qil, j, grp::G);
@@ -1661,13 +1723,15 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
                 for (int i = gl; i < nvis; i += grp::G) {
                     const int s = g.vlist[i];
                     const unsigned fl = g.vflag[s];
+                    if (!(fl & (cav_q << 1))) continue;
+                    const SlotNbrs nb = slot_nbrs(g, s);
 #pragma unroll
                     for (int j = 0; j < grp::Q; j++) {
-                        if (!((cav_q >> j) & 1u) || !(fl & (2u << j)) || ncav[j] > grp::CMAX)
-                            continue;
-                        const unsigned a = adjacency(g, s, j);
+                        const unsigned bit = 2u << j;
+                        if (!((cav_q >> j) & 1u) || !(fl & bit) || ncav[j] > grp::CMAX) continue;
+                        // faces to a neighbour outside the circumsphere (or none)
 #pragma unroll
-                        for (int k = 0; k < 4; k++) b[j] += ((a >> (8 * k)) & 0xffu) == 0xffu;
+                        for (int k = 0; k < 4; k++) b[j] += !(nb.fl[k] & bit);
                     }
                 }
 #pragma unroll
@@ -1687,6 +1751,10 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
             }
         }
         // reserve the records: lane j of the quad reserves query j's
+        // the next batch: its claim is back by now; read its items (every
+        // lane of the wave takes part: the claim is the wave's)
+        have = part_claim_finish(work, n_items, kBatch, claim, parts_done, nfirst, ncnt);
+        load_items();
         int units = 0;
 #pragma unroll
         for (int j = 0; j < grp::Q; j++) {
@@ -1731,6 +1799,7 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
                 }
             }
         }
+        load_queries();  // of the next batch
         // handoff pairs of the queries with a cavity (the order kernel reads
         // the tets' vertices itself)
         for (int i = gl; i < nvis; i += grp::G) {
@@ -1738,6 +1807,7 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
             const unsigned fl = g.vflag[s];
             if (!(fl & 0x1eu)) continue;
             const int t = g.vtet[i];  // with the table mask (handoff_tet, handoff_dm)
+            const SlotNbrs nb = slot_nbrs(g, s);
 #pragma unroll
             for (int j = 0; j < grp::Q; j++) {
                 if (qi[j] < 0 || !ok[j] || !fits[j] || ncav[j] == 0 || !(fl & (2u << j)))
@@ -1745,7 +1815,7 @@ topo_search_kernel(DevMesh m, const double *__restrict__ q, const int *__restric
                 int4 *r = topo.rec + off[j];
                 const int d = g.dense[j][s];
                 const int room = nb_room(nbnd[j]);
-                rec_xadj(r, ncav[j], nbnd[j], room)[d] = static_cast<int>(adjacency(g, s, j));
+                rec_xadj(r, ncav[j], nbnd[j], room)[d] = static_cast<int>(adjacency_of(g, nb, j));
                 rec_xtet(r, ncav[j], nbnd[j], room)[d] = t;
             }
         }
