@@ -756,6 +756,15 @@ __device__ __forceinline__ void new_contributions(const DevMesh &m, int t, int f
     }
 }
 
+// 1 / x for the gradient (no prescribed rounding): the hardware estimate and
+// one Newton step, r (2 - x r), which squares its relative error (about
+// 2^-52 after it) -- 3 instructions where the correctly rounded division
+// takes about 10.  x is a denominator already checked against 1e-30.
+__device__ __forceinline__ double grad_rcp(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
 // The facet moments of boundary face fi of cavity tet t -- the gradient's
 // part of step 4 -- in the query's frame.  With P_k = F_k - q, l_k = |P_k|^2
 // and X_k = P_k x P_k+1, the virtual tet's circumcentre and the three edge
@@ -764,7 +773,7 @@ __device__ __forceinline__ void new_contributions(const DevMesh &m, int t, int f
 //   circumcentre of (F_k, q, F_k+1) - q = ((l_k P_k+1 - l_k+1 P_k) x X_k) / (2 |X_k|^2)
 // (the centroid below the same 1e-30 denominators as cc_tet and cc_tri).  The
 // gradient has no prescribed rounding, so nothing here has to match the
-// forward's circumcentres bit for bit.  f(slot, m6) receives the unscaled
+// forward's circumcentres bit for bit, and the divisions are grad_rcp.  f(slot, m6) receives the unscaled
 // moment of each face vertex that has a neighbour slot (facet_moment6 with
 // every point taken relative to q).
 template <typename SlotOf, typename F>
@@ -787,7 +796,7 @@ __device__ __forceinline__ void face_moments_rel(const DevMesh &m, int t, int fi
         V = {(P[0].x + P[1].x + P[2].x) * 0.25, (P[0].y + P[1].y + P[2].y) * 0.25,
              (P[0].z + P[1].z + P[2].z) * 0.25};
     } else {
-        const double inv = 1.0 / den;
+        const double inv = grad_rcp(den);
         V = {(l[0] * X[1].x + l[1] * X[2].x + l[2] * X[0].x) * inv,
              (l[0] * X[1].y + l[1] * X[2].y + l[2] * X[0].y) * inv,
              (l[0] * X[1].z + l[1] * X[2].z + l[2] * X[0].z) * inv};
@@ -803,7 +812,7 @@ __device__ __forceinline__ void face_moments_rel(const DevMesh &m, int t, int fi
             const double la = l[k], lb = l[(k + 1) % 3];
             const d3 c = cross3({la * b.x - lb * a.x, la * b.y - lb * a.y, la * b.z - lb * a.z},
                                 X[k]);
-            const double inv = 0.5 / x2;
+            const double inv = 0.5 * grad_rcp(x2);
             T[k] = {c.x * inv, c.y * inv, c.z * inv};
         }
     }
@@ -3363,12 +3372,11 @@ constexpr int kGradBatch = 32;
 __global__ void __launch_bounds__(kGradThreads) grad_geom_kernel(
     DevMesh m, const double *__restrict__ q, const int *__restrict__ items, int n_items,
     unsigned int *work, Topo topo, GradIn g, int *ovf_list, unsigned int *ovf_count) {
-    __shared__ double a_lds[kGradThreads / kGradLanes][kLevel0Row];
-    __shared__ double s_lds[kGradThreads / kGradLanes][kLevel0Row];  // 1 / (6 |p_j - q|^2)
+    // (a_j - abar) / (6 |p_j - q|^2) per neighbour slot of each row in flight
+    __shared__ double c_lds[kGradThreads / kGradLanes][kLevel0Row];
     const int gl = __lane_id() & (kGradLanes - 1);
     const int gw = __lane_id() / kGradLanes;
-    double *nb_a = a_lds[threadIdx.x / kGradLanes];
-    double *nb_s = s_lds[threadIdx.x / kGradLanes];
+    double *nb_c = c_lds[threadIdx.x / kGradLanes];
     int parts_done = 0, first = 0, nbatch = 0, sub = kGradBatch;
     for (;;) {
         // a wave claims kGradBatch consecutive items and works through them
@@ -3401,16 +3409,32 @@ __global__ void __launch_bounds__(kGradThreads) grad_geom_kernel(
         }
         const d3 qq = {q[3 * static_cast<size_t>(qi)], q[3 * static_cast<size_t>(qi) + 1],
                        q[3 * static_cast<size_t>(qi) + 2]};
+        // per neighbour c_j = (a_j - abar) / (6 |p_j - q|^2), abar = sum_j a_j
+        // lambda_j: then dq S = sum over the faces' vertices of c_j times
+        // their unscaled facet moments, one accumulator
         const int *rn = rec_nbv(r, hd);
         double abar = 0.0;
-        for (int j = gl; j < nnb; j += kGradLanes) {
-            const double a = grad_dot(g, qi, rn[j]);
-            nb_a[j] = a;
-            nb_s[j] = facet_scale(ld_vert(m, rn[j]), qq);
-            abar += a * g.w[e0 + j];
+        double a_own[kLevel0Row / kGradLanes], s_own[kLevel0Row / kGradLanes];
+#pragma unroll
+        for (int k = 0; k < kLevel0Row / kGradLanes; k++) {
+            const int j = gl + k * kGradLanes;
+            a_own[k] = 0.0;
+            s_own[k] = 0.0;
+            if (j < nnb) {
+                a_own[k] = grad_dot(g, qi, rn[j]);
+                s_own[k] = facet_scale(ld_vert(m, rn[j]), qq);
+                abar += a_own[k] * g.w[e0 + j];
+            }
+        }
+#pragma unroll
+        for (int d = kGradLanes / 2; d >= 1; d >>= 1) abar += __shfl_xor(abar, d, kGradLanes);
+#pragma unroll
+        for (int k = 0; k < kLevel0Row / kGradLanes; k++) {
+            const int j = gl + k * kGradLanes;
+            if (j < nnb) nb_c[j] = (a_own[k] - abar) * s_own[k];
         }
         group_sync();
-        d3 ag = {0.0, 0.0, 0.0}, tg = {0.0, 0.0, 0.0};
+        d3 gq = {0.0, 0.0, 0.0};
         const unsigned char *rf = rec_bnd(r, hd);
         for (int bi = gl; bi < nbnd; bi += kGradLanes) {
             const int code = rf[bi];
@@ -3418,29 +3442,21 @@ __global__ void __launch_bounds__(kGradThreads) grad_geom_kernel(
             face_moments_rel(
                 m, cr.x, code & 3, qq, [&](int vi, int) { return rec_slot(cr, vi); },
                 [&](int slot, const d3 &g6) {
-                    const double sc = nb_s[slot];
-                    const d3 gm = {g6.x * sc, g6.y * sc, g6.z * sc};
-                    const double a = nb_a[slot];
-                    ag = {ag.x + a * gm.x, ag.y + a * gm.y, ag.z + a * gm.z};
-                    tg = {tg.x + gm.x, tg.y + gm.y, tg.z + gm.z};
+                    const double c = nb_c[slot];
+                    gq = {gq.x + c * g6.x, gq.y + c * g6.y, gq.z + c * g6.z};
                 });
         }
 #pragma unroll
         for (int d = kGradLanes / 2; d >= 1; d >>= 1) {
-            abar += __shfl_xor(abar, d, kGradLanes);
-            ag.x += __shfl_xor(ag.x, d, kGradLanes);
-            ag.y += __shfl_xor(ag.y, d, kGradLanes);
-            ag.z += __shfl_xor(ag.z, d, kGradLanes);
-            tg.x += __shfl_xor(tg.x, d, kGradLanes);
-            tg.y += __shfl_xor(tg.y, d, kGradLanes);
-            tg.z += __shfl_xor(tg.z, d, kGradLanes);
+            gq.x += __shfl_xor(gq.x, d, kGradLanes);
+            gq.y += __shfl_xor(gq.y, d, kGradLanes);
+            gq.z += __shfl_xor(gq.z, d, kGradLanes);
         }
         if (gl == 0) {
             const double inv = 1.0 / g.row_total[qi];
-            store_dq(g, qi, {(ag.x - abar * tg.x) * inv, (ag.y - abar * tg.y) * inv,
-                             (ag.z - abar * tg.z) * inv});
+            store_dq(g, qi, {gq.x * inv, gq.y * inv, gq.z * inv});
         }
-        group_sync();  // nb_a is rewritten by the group's next row
+        group_sync();  // nb_c is rewritten by the group's next row
     }
 }
 
