@@ -20,6 +20,8 @@
 // with -ffp-contract=off.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "exact.h"
 #include "geom.h"
 #include "geom32.h"
@@ -1849,7 +1851,7 @@ constexpr int kDfsPool = 2048;  // adjacency words of a wave kept in LDS (mean 6
 
 __global__ void __launch_bounds__(kDfsThreads)
 topo_dfs_kernel(const int *__restrict__ items, int n_items, Topo topo, unsigned int *ovf_count,
-                int *ovf_list) {
+                int *ovf_list, int pool_words) {
     // the wave's adjacency words: each query's C words at its offset in one
     // pool (a prefix sum over the lanes); a query past the pool's end reads
     // and writes its words in the record instead
@@ -1892,7 +1894,7 @@ topo_dfs_kernel(const int *__restrict__ items, int n_items, Topo topo, unsigned 
     const int off = wave_excl_scan(r ? C : 0, total);  // every lane of the wave takes part
     if (!r) return;
     int *xadj = rec_xadj(r, C, B, room);
-    const bool in_pool = off + C <= kDfsPool;
+    const bool in_pool = off + C <= pool_words;
     if (in_pool) {
         const int4 *xa4 = reinterpret_cast<const int4 *>(xadj);
         for (int d0 = 0; d0 < C; d0 += 4) {
@@ -3783,6 +3785,19 @@ hipError_t launch_topo_search(const DevMesh &m, int nblocks, const double *d_q,
     return hipGetLastError();
 }
 
+// Words of the DFS kernel's LDS pool a wave may use: kDfsPool, or less under
+// the test hook NATINTERP3D_TEST_DFS_POOL (a non-negative integer, read at
+// each launch), so that tests can send the walks to their record-resident
+// copy; the rows do not change.
+static int dfs_pool_words() {
+    const char *s = std::getenv("NATINTERP3D_TEST_DFS_POOL");
+    if (!s || !*s) return kDfsPool;
+    char *end = nullptr;
+    const long v = std::strtol(s, &end, 10);
+    if (!end || *end != '\0' || v < 0) return kDfsPool;
+    return v < kDfsPool ? static_cast<int>(v) : kDfsPool;
+}
+
 hipError_t launch_topo_order(const DevMesh &m, const int *d_items, int n_items, const Topo &topo,
                              unsigned int *ovf_count, int *ovf_list, hipStream_t st) {
     if (n_items <= 0) return hipSuccess;
@@ -3790,7 +3805,7 @@ hipError_t launch_topo_order(const DevMesh &m, const int *d_items, int n_items, 
     const int per_part = (n_items + kParts - 1) / kParts;
     const int nb1 = kParts * ((per_part + kDfsThreads - 1) / kDfsThreads);
     hipLaunchKernelGGL(topo_dfs_kernel, dim3(nb1), dim3(kDfsThreads), 0, st, d_items, n_items,
-                       topo, ovf_count, ovf_list);
+                       topo, ovf_count, ovf_list, dfs_pool_words());
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int nb2 = kParts * ((per_part + kOrderQueries - 1) / kOrderQueries);
