@@ -396,10 +396,13 @@ int nat_mesh_set_mixed_check(nat_mesh *m, int stride);
  * Which kernel makes a row changes speed, never the row: every policy gives
  * the same CSR, bit for bit.
  *
- * A query first goes to the group kernel (16 lanes of a wave per query,
- * cavity tables in LDS) unless its problem's budget is 0; the group kernel
- * gives up on a query once it has visited more tets than the budget (at most
- * 96, the LDS tables' size), or when the row has more than 32 neighbours.
+ * A query first goes to the group kernel -- the topology search, 16 lanes of
+ * a wave for a group of four consecutive queries of the Morton order, its
+ * tables in LDS, then the order kernels -- unless its problem's budget is 0;
+ * the group gives up on a query once the group has visited more tets than
+ * its queries' budgets (at most 96 each, 192 in all, the LDS tables' size),
+ * or when the row has more than 64 cavity tets, 136 boundary faces or 32
+ * neighbours.
  * Those queries go next, when wave_rows is set, to the wave-per-query level
  * (a wave of 64 lanes per query, tables in global memory, up to 2048 cavity
  * tets, 8192 boundary faces and 2048 neighbours), and then to the
